@@ -1,0 +1,244 @@
+"""fastem_amd -- Gaussian-mixture EM engine for one AMD Instinct MI355X.
+
+Python bindings (ctypes) of the C ABI in include/fastem_abi.h.  The compute
+path is the HIP library libfastem.so built in-tree for gfx950; there is no CPU
+fallback: if the library is missing, loading it raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libfastem.so")
+EM2_PATH = os.path.join(_PKG, "em2")
+
+PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
+FEM_OK, FEM_ERR_ARG, FEM_ERR_HIP, FEM_ERR_NOT_PD, FEM_ERR_STATE = 0, -1, -2, -3, -4
+
+_lib = None
+_dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+
+
+class FemError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"fastem error {code}: {msg}")
+        self.code = code
+
+
+_SIGS = {
+    "fem_create": (ctypes.c_int, [ctypes.POINTER(_vp), _i64, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_int, ctypes.c_int]),
+    "fem_destroy": (ctypes.c_int, [_vp]),
+    "fem_upload_data": (ctypes.c_int, [_vp, _fp, _i64]),
+    "fem_upload_data_device": (ctypes.c_int, [_vp, _vp, _i64]),
+    "fem_set_total_rows": (ctypes.c_int, [_vp, _i64]),
+    "fem_set_regularization": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "fem_set_params": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
+    "fem_get_params": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
+    "fem_factor": (ctypes.c_int, [_vp, _dp, _ip]),
+    "fem_estep": (ctypes.c_int, [_vp, _dp]),
+    "fem_mstep": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
+    "fem_get_posteriors": (ctypes.c_int, [_vp, _dp, _i64]),
+    "fem_last_times": (ctypes.c_int, [_vp, _fp, _fp, _fp, _fp]),
+    "fem_last_error": (ctypes.c_char_p, [_vp]),
+    "fem_version": (ctypes.c_char_p, []),
+    "fem_gengmm": (ctypes.c_int, [_i64, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                  ctypes.c_double, ctypes.c_double, ctypes.c_uint64, _fp, _ip,
+                                  _dp, _dp, _dp]),
+    "fem_kmeanspp_init": (ctypes.c_int, [_i64, ctypes.c_int, ctypes.c_int, _fp, ctypes.c_uint64,
+                                         ctypes.c_int, _dp, _dp, _dp]),
+}
+
+
+def lib():
+    """Loads libfastem.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run `python -m fastem_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+class Context:
+    """One device-resident dataset plus mixture parameters (fem_ctx)."""
+
+    def __init__(self, n, d, K, precision=PREC_FP64, device=0):
+        self.n, self.d, self.K = int(n), int(d), int(K)
+        self.precision = precision
+        h = _vp()
+        rc = lib().fem_create(ctypes.byref(h), self.n, self.d, self.K, precision, device)
+        if rc != 0:
+            raise FemError(rc, lib().fem_last_error(None).decode())
+        self._h = h
+
+    def _check(self, rc):
+        if rc != 0:
+            raise FemError(rc, lib().fem_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().fem_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        assert X.shape == (self.n, self.d)
+        self._check(lib().fem_upload_data(self._h, X.ctypes.data_as(_fp), self.d))
+
+    def upload_device(self, ptr, ld=None):
+        self._check(lib().fem_upload_data_device(self._h, _vp(ptr), ld or self.d))
+
+    def set_total_rows(self, ntotal):
+        self._check(lib().fem_set_total_rows(self._h, int(ntotal)))
+
+    def set_regularization(self, g):
+        self._check(lib().fem_set_regularization(self._h, float(g)))
+
+    def set_params(self, w, mu, cov):
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        mu = np.ascontiguousarray(mu, dtype=np.float64)
+        cov = np.ascontiguousarray(cov, dtype=np.float64)
+        assert w.shape == (self.K,) and mu.shape == (self.K, self.d)
+        assert cov.shape == (self.K, self.d, self.d)
+        self._check(lib().fem_set_params(self._h, _d(w), _d(mu), _d(cov)))
+
+    def get_params(self):
+        w = np.empty(self.K)
+        mu = np.empty((self.K, self.d))
+        cov = np.empty((self.K, self.d, self.d))
+        self._check(lib().fem_get_params(self._h, _d(w), _d(mu), _d(cov)))
+        return w, mu, cov
+
+    def factor(self):
+        """Returns (logdet, info); raises FemError(FEM_ERR_NOT_PD) on failure
+        after filling info."""
+        ld = np.empty(self.K)
+        info = np.zeros(self.K, dtype=np.int32)
+        rc = lib().fem_factor(self._h, _d(ld), info.ctypes.data_as(_ip))
+        if rc != 0:
+            err = FemError(rc, lib().fem_last_error(self._h).decode())
+            err.info = info
+            raise err
+        return ld, info
+
+    def estep(self):
+        ll = ctypes.c_double()
+        self._check(lib().fem_estep(self._h, ctypes.byref(ll)))
+        return ll.value
+
+    def mstep(self, copy=True):
+        if not copy:
+            self._check(lib().fem_mstep(self._h, None, None, None))
+            return None
+        w = np.empty(self.K)
+        mu = np.empty((self.K, self.d))
+        cov = np.empty((self.K, self.d, self.d))
+        self._check(lib().fem_mstep(self._h, _d(w), _d(mu), _d(cov)))
+        return w, mu, cov
+
+    def posteriors(self):
+        R = np.empty((self.n, self.K))
+        self._check(lib().fem_get_posteriors(self._h, _d(R), self.K))
+        return R
+
+    def times(self):
+        t = [ctypes.c_float() for _ in range(4)]
+        self._check(lib().fem_last_times(self._h, *[ctypes.byref(x) for x in t]))
+        return {"estep_ms": t[0].value, "means_ms": t[1].value, "cov_ms": t[2].value,
+                "factor_ms": t[3].value}
+
+
+def regularize(cov, g):
+    """EM2/EMAlgorithm.cpp:237-244 on host arrays (used before the first
+    factorisation, as TrainNoCopy does at :228)."""
+    cov = np.array(cov, dtype=np.float64, copy=True)
+    d = cov.shape[-1]
+    for k in range(cov.shape[0]):
+        tr = np.trace(cov[k])
+        cov[k] = cov[k] * (1.0 - g) + (g * tr / d) * np.eye(d)
+    return cov
+
+
+def train(ctx, w, mu, cov, maxsweeps, eps=1e-5, g=0.0, ll_trace=None):
+    """EMAlgorithm::TrainNoCopy (EM2/EMAlgorithm.cpp:222-274) over the C ABI,
+    without the condition-number check.  Returns (iter_counter, LL, (w, mu,
+    cov)) with the reference's IterCounter semantics."""
+    ctx.set_regularization(g)
+    ctx.set_params(w, mu, regularize(cov, g))
+    ctx.factor()
+    prev = -np.finfo(np.float64).max
+    ll = prev
+    it = 1
+    while it <= maxsweeps:
+        ll = ctx.estep()
+        if ll_trace is not None:
+            ll_trace.append(ll)
+        ctx.mstep(copy=False)
+        ctx.factor()
+        e = abs((ll - prev) / ll)
+        if ll < prev:
+            e = 0.0
+        prev = ll
+        if e < eps:
+            it += 1
+            break
+        it += 1
+    return it, ll, ctx.get_params()
+
+
+def gengmm(n, d, K, mean_lo, mean_hi, diag, seed, labels=False):
+    """Seeded synthetic mixture (include/fastem_tools.h).  Returns X (FP32)
+    and the generating (w, mu, cov) [and labels]."""
+    X = np.empty((n, d), dtype=np.float32)
+    lab = np.empty(n, dtype=np.int32) if labels else None
+    w = np.empty(K)
+    mu = np.empty((K, d))
+    cov = np.empty((K, d, d))
+    rc = lib().fem_gengmm(n, d, K, mean_lo, mean_hi, diag, seed, X.ctypes.data_as(_fp),
+                          None if lab is None else lab.ctypes.data_as(_ip), _d(w), _d(mu),
+                          _d(cov))
+    if rc != 0:
+        raise FemError(rc, "fem_gengmm failed")
+    return (X, w, mu, cov, lab) if labels else (X, w, mu, cov)
+
+
+def kmeanspp_init(X, K, seed, lloyd_iters=29):
+    """k-means++ seeding, up to lloyd_iters Lloyd sweeps, mixture from the
+    clusters (include/fastem_tools.h)."""
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    n, d = X.shape
+    w = np.empty(K)
+    mu = np.empty((K, d))
+    cov = np.empty((K, d, d))
+    rc = lib().fem_kmeanspp_init(n, d, K, X.ctypes.data_as(_fp), seed, lloyd_iters, _d(w),
+                                 _d(mu), _d(cov))
+    if rc != 0:
+        raise FemError(rc, "k-means++ init left a cluster with fewer than two points")
+    return w, mu, cov

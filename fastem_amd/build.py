@@ -1,0 +1,56 @@
+"""Builds libfastem.so (HIP kernels + C ABI + C++ host) and the em2 driver
+in-tree for gfx950 with hipcc.  Usage: python -m fastem_amd.build"""
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+LIB = os.path.join(PKG, "libfastem.so")
+
+HIP_SOURCES = ["fem_kernels.hip", "fem_abi.hip"]
+HOST_SOURCES = ["fem_tools.cpp"]
+
+# Packed FP32 VALU ops are disabled for all device code.
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+
+
+def hipcc():
+    for c in ("/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def _run(cmd):
+    print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+
+
+def build(verbose=True):
+    objdir = os.path.join(PKG, "_objs")
+    os.makedirs(objdir, exist_ok=True)
+    objs = []
+    for s in HIP_SOURCES + HOST_SOURCES:
+        src = os.path.join(CSRC, s)
+        obj = os.path.join(objdir, os.path.splitext(s)[0] + ".o")
+        flags = list(HIP_FLAGS)
+        if s.endswith(".cpp"):
+            flags += ["-fopenmp"]
+        _run([hipcc()] + flags + ["-I", os.path.join(ROOT, "include"), "-c", src, "-o", obj])
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-fopenmp", "-o", tmp] + objs)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

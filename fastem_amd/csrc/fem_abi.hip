@@ -1,0 +1,312 @@
+// fem_abi.hip -- the C ABI of libfastem (include/fastem_abi.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "fem_internal.h"
+
+namespace {
+
+thread_local std::string g_create_err;
+
+int fail(fem_ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    else g_create_err = msg;
+    return code;
+}
+
+#define FEM_HIP(c, call)                                                              \
+    do {                                                                              \
+        hipError_t _e = (call);                                                       \
+        if (_e != hipSuccess)                                                         \
+            return fail((c), FEM_ERR_HIP,                                             \
+                        std::string(#call) + ": " + hipGetErrorString(_e));           \
+    } while (0)
+
+template <typename T>
+hipError_t dmalloc(T **p, size_t count) {
+    return hipMalloc((void **)p, count * sizeof(T) > 0 ? count * sizeof(T) : 16);
+}
+
+void free_all(fem_ctx *c) {
+    void *ptrs[] = {c->X, c->Rt, c->Rt32, c->w, c->mu, c->cov, c->nk, c->logdet, c->info, c->coef,
+                    c->U, c->ufrag, c->ufrag32, c->minit, c->mus32, c->ll_part, c->ll_dev,
+                    c->mpart, c->cpart};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    for (auto &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+int check_ctx(fem_ctx *c) {
+    if (!c) return fail(nullptr, FEM_ERR_ARG, "null context");
+    return FEM_OK;
+}
+
+int set_device(fem_ctx *c) {
+    FEM_HIP(c, hipSetDevice(c->device));
+    return FEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *fem_version(void) { return "fastem-mi355x 0.1 (gfx950)"; }
+
+const char *fem_last_error(const fem_ctx *c) {
+    return c ? c->err.c_str() : g_create_err.c_str();
+}
+
+int fem_create(fem_ctx **out, int64_t n, int d, int K, int precision, int device) {
+    if (!out) return fail(nullptr, FEM_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (n < 1 || d < 1 || K < 1)
+        return fail(nullptr, FEM_ERR_ARG, "n, d and K must be positive");
+    if (d > fem::kMaxDim)
+        return fail(nullptr, FEM_ERR_ARG, "dimension above 128 is not supported");
+    if (precision != FEM_PREC_FP64)
+        return fail(nullptr, FEM_ERR_ARG, "precision not available in this build");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(nullptr, FEM_ERR_HIP, "no HIP device");
+    if (device < 0 || device >= ndev) return fail(nullptr, FEM_ERR_ARG, "bad device index");
+
+    fem_ctx *c = new fem_ctx();
+    c->device = device;
+    c->n = n;
+    c->ntotal = n;
+    c->d = d;
+    c->NB = (d + 15) / 16;
+    c->dpad = 16 * c->NB;
+    c->K = K;
+    c->KP = 16 * ((K + 15) / 16);
+    c->prec = precision;
+    c->npad = ((n + fem::kRowAlign - 1) / fem::kRowAlign) * fem::kRowAlign;
+    const int npairs = fem::estep_pairs(c->NB);
+    const int ub = fem::upper_blocks(c->NB);
+    const int P = (c->NB <= 4) ? 2 : 1;
+    c->est_blocks = (int)(c->npad / (64 * P));
+    c->m_chunks = (int)(c->npad / 4096);
+    // covariance chunks: at most 256 chunks of a multiple of 4096 rows
+    int64_t units = c->npad / 4096;
+    int64_t per = (units + 255) / 256;
+    c->c_chunk_rows = per * 4096;
+    c->c_chunks = (int)((c->npad + c->c_chunk_rows - 1) / c->c_chunk_rows);
+
+    auto bail = [&](hipError_t e, const char *what) {
+        std::string m = std::string(what) + ": " + hipGetErrorString(e);
+        free_all(c);
+        delete c;
+        return fail(nullptr, FEM_ERR_HIP, m);
+    };
+    hipError_t e;
+    if ((e = hipSetDevice(device)) != hipSuccess) return bail(e, "hipSetDevice");
+    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
+        return bail(e, "hipStreamCreate");
+    for (auto &ev : c->ev)
+        if ((e = hipEventCreate(&ev)) != hipSuccess) return bail(e, "hipEventCreate");
+    const size_t nX = (size_t)c->npad * c->dpad;
+    const size_t nR = (size_t)c->KP * c->npad;
+    if ((e = dmalloc(&c->X, nX)) != hipSuccess) return bail(e, "hipMalloc(X)");
+    if ((e = dmalloc(&c->Rt, nR)) != hipSuccess) return bail(e, "hipMalloc(R)");
+    if ((e = dmalloc(&c->w, K)) != hipSuccess ||
+        (e = dmalloc(&c->mu, (size_t)K * d)) != hipSuccess ||
+        (e = dmalloc(&c->cov, (size_t)K * d * d)) != hipSuccess ||
+        (e = dmalloc(&c->nk, K)) != hipSuccess || (e = dmalloc(&c->logdet, K)) != hipSuccess ||
+        (e = dmalloc(&c->info, K)) != hipSuccess || (e = dmalloc(&c->coef, K)) != hipSuccess)
+        return bail(e, "hipMalloc(params)");
+    if ((e = dmalloc(&c->U, (size_t)K * c->dpad * c->dpad)) != hipSuccess ||
+        (e = dmalloc(&c->ufrag, (size_t)K * npairs * 64)) != hipSuccess ||
+        (e = dmalloc(&c->minit, (size_t)K * c->dpad)) != hipSuccess)
+        return bail(e, "hipMalloc(factors)");
+    if ((e = dmalloc(&c->ll_part, c->est_blocks)) != hipSuccess ||
+        (e = dmalloc(&c->ll_dev, 1)) != hipSuccess ||
+        (e = dmalloc(&c->mpart, (size_t)c->m_chunks * c->KP * (c->dpad + 1))) != hipSuccess ||
+        (e = dmalloc(&c->cpart, (size_t)c->c_chunks * c->KP * ub * 256)) != hipSuccess)
+        return bail(e, "hipMalloc(partials)");
+    if ((e = hipMemsetAsync(c->X, 0, nX * sizeof(float), c->stream)) != hipSuccess ||
+        (e = hipMemsetAsync(c->Rt, 0, nR * sizeof(double), c->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(c->stream)) != hipSuccess)
+        return bail(e, "initial clear");
+    *out = c;
+    return FEM_OK;
+}
+
+int fem_destroy(fem_ctx *c) {
+    if (!c) return FEM_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    free_all(c);
+    delete c;
+    return FEM_OK;
+}
+
+static int upload(fem_ctx *c, const float *X, int64_t ld, hipMemcpyKind kind) {
+    if (int r = check_ctx(c)) return r;
+    if (!X || ld < c->d) return fail(c, FEM_ERR_ARG, "bad data pointer or row stride");
+    if (int r = set_device(c)) return r;
+    FEM_HIP(c, hipMemcpy2DAsync(c->X, (size_t)c->dpad * sizeof(float), X, (size_t)ld * sizeof(float),
+                                (size_t)c->d * sizeof(float), (size_t)c->n, kind, c->stream));
+    FEM_HIP(c, hipStreamSynchronize(c->stream));
+    return FEM_OK;
+}
+
+int fem_upload_data(fem_ctx *c, const float *X, int64_t ld) {
+    return upload(c, X, ld, hipMemcpyHostToDevice);
+}
+
+int fem_upload_data_device(fem_ctx *c, const float *dX, int64_t ld) {
+    return upload(c, dX, ld, hipMemcpyDeviceToDevice);
+}
+
+int fem_set_total_rows(fem_ctx *c, int64_t ntotal) {
+    if (int r = check_ctx(c)) return r;
+    if (ntotal < 1) return fail(c, FEM_ERR_ARG, "ntotal must be positive");
+    c->ntotal = ntotal;
+    return FEM_OK;
+}
+
+int fem_set_regularization(fem_ctx *c, double g) {
+    if (int r = check_ctx(c)) return r;
+    if (!(g >= 0.0 && g <= 1.0)) return fail(c, FEM_ERR_ARG, "regularisation must be in [0,1]");
+    c->g = g;
+    return FEM_OK;
+}
+
+int fem_set_params(fem_ctx *c, const double *w, const double *mu, const double *cov) {
+    if (int r = check_ctx(c)) return r;
+    if (!w || !mu || !cov) return fail(c, FEM_ERR_ARG, "null parameter pointer");
+    if (int r = set_device(c)) return r;
+    const size_t K = c->K, d = c->d;
+    FEM_HIP(c, hipMemcpyAsync(c->w, w, K * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    FEM_HIP(c, hipMemcpyAsync(c->mu, mu, K * d * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    FEM_HIP(c, hipMemcpyAsync(c->cov, cov, K * d * d * sizeof(double), hipMemcpyHostToDevice,
+                              c->stream));
+    FEM_HIP(c, hipStreamSynchronize(c->stream));
+    c->have_params = true;
+    c->factored = false;
+    return FEM_OK;
+}
+
+int fem_get_params(fem_ctx *c, double *w, double *mu, double *cov) {
+    if (int r = check_ctx(c)) return r;
+    if (int r = set_device(c)) return r;
+    const size_t K = c->K, d = c->d;
+    if (w) FEM_HIP(c, hipMemcpyAsync(w, c->w, K * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (mu)
+        FEM_HIP(c, hipMemcpyAsync(mu, c->mu, K * d * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (cov)
+        FEM_HIP(c, hipMemcpyAsync(cov, c->cov, K * d * d * sizeof(double), hipMemcpyDeviceToHost,
+                                  c->stream));
+    FEM_HIP(c, hipStreamSynchronize(c->stream));
+    return FEM_OK;
+}
+
+int fem_factor(fem_ctx *c, double *logdet, int *info) {
+    if (int r = check_ctx(c)) return r;
+    if (!c->have_params) return fail(c, FEM_ERR_STATE, "fem_factor before fem_set_params");
+    if (int r = set_device(c)) return r;
+    FEM_HIP(c, hipEventRecord(c->ev[6], c->stream));
+    FEM_HIP(c, fem::launch_factor(c));
+    FEM_HIP(c, hipEventRecord(c->ev[7], c->stream));
+    int *hinfo = info;
+    std::string tmp;
+    int *buf = nullptr;
+    if (!hinfo) {
+        buf = new int[c->K];
+        hinfo = buf;
+    }
+    hipError_t e = hipMemcpyAsync(hinfo, c->info, c->K * sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && logdet)
+        e = hipMemcpyAsync(logdet, c->logdet, c->K * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        delete[] buf;
+        return fail(c, FEM_ERR_HIP, std::string("fem_factor: ") + hipGetErrorString(e));
+    }
+    (void)hipEventElapsedTime(&c->t_factor, c->ev[6], c->ev[7]);
+    int bad = -1;
+    for (int k = 0; k < c->K; k++)
+        if (hinfo[k] != 0) {
+            bad = k;
+            break;
+        }
+    if (bad >= 0) {
+        char m[160];
+        snprintf(m, sizeof m, "Cholesky decomposition failed for component %d (column %d)", bad,
+                 hinfo[bad] - 1);
+        delete[] buf;
+        c->factored = false;
+        return fail(c, FEM_ERR_NOT_PD, m);
+    }
+    delete[] buf;
+    c->factored = true;
+    return FEM_OK;
+}
+
+int fem_estep(fem_ctx *c, double *loglik) {
+    if (int r = check_ctx(c)) return r;
+    if (!c->factored) return fail(c, FEM_ERR_STATE, "fem_estep needs a successful fem_factor");
+    if (int r = set_device(c)) return r;
+    FEM_HIP(c, hipEventRecord(c->ev[0], c->stream));
+    FEM_HIP(c, fem::launch_estep(c));
+    FEM_HIP(c, hipEventRecord(c->ev[1], c->stream));
+    double ll = 0.0;
+    FEM_HIP(c, hipMemcpyAsync(&ll, c->ll_dev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    FEM_HIP(c, hipStreamSynchronize(c->stream));
+    (void)hipEventElapsedTime(&c->t_estep, c->ev[0], c->ev[1]);
+    if (loglik) *loglik = ll;
+    c->have_post = true;
+    return FEM_OK;
+}
+
+int fem_mstep(fem_ctx *c, double *w, double *mu, double *cov) {
+    if (int r = check_ctx(c)) return r;
+    if (!c->have_post) return fail(c, FEM_ERR_STATE, "fem_mstep needs posteriors from fem_estep");
+    if (int r = set_device(c)) return r;
+    FEM_HIP(c, hipEventRecord(c->ev[2], c->stream));
+    FEM_HIP(c, fem::launch_means(c));
+    FEM_HIP(c, hipEventRecord(c->ev[3], c->stream));
+    FEM_HIP(c, fem::launch_cov(c));
+    FEM_HIP(c, hipEventRecord(c->ev[4], c->stream));
+    c->factored = false;
+    if (int r = fem_get_params(c, w, mu, cov)) return r;  // synchronises
+    (void)hipEventElapsedTime(&c->t_means, c->ev[2], c->ev[3]);
+    (void)hipEventElapsedTime(&c->t_cov, c->ev[3], c->ev[4]);
+    return FEM_OK;
+}
+
+int fem_get_posteriors(fem_ctx *c, double *R, int64_t ld) {
+    if (int r = check_ctx(c)) return r;
+    if (!R || ld < c->K) return fail(c, FEM_ERR_ARG, "bad posterior pointer or stride");
+    if (!c->have_post) return fail(c, FEM_ERR_STATE, "no posteriors yet");
+    if (int r = set_device(c)) return r;
+    double *tmp = nullptr;
+    FEM_HIP(c, hipMalloc((void **)&tmp, (size_t)c->n * c->K * sizeof(double)));
+    hipError_t e = fem::launch_transpose_post(c, tmp, c->K);
+    if (e == hipSuccess)
+        e = hipMemcpy2DAsync(R, (size_t)ld * sizeof(double), tmp, (size_t)c->K * sizeof(double),
+                             (size_t)c->K * sizeof(double), (size_t)c->n, hipMemcpyDeviceToHost,
+                             c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess)
+        return fail(c, FEM_ERR_HIP, std::string("fem_get_posteriors: ") + hipGetErrorString(e));
+    return FEM_OK;
+}
+
+int fem_last_times(fem_ctx *c, float *estep_ms, float *means_ms, float *cov_ms, float *factor_ms) {
+    if (int r = check_ctx(c)) return r;
+    if (estep_ms) *estep_ms = c->t_estep;
+    if (means_ms) *means_ms = c->t_means;
+    if (cov_ms) *cov_ms = c->t_cov;
+    if (factor_ms) *factor_ms = c->t_factor;
+    return FEM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,83 @@
+// fem_internal.h -- device context and kernel launchers of libfastem.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/fastem_abi.h"
+
+namespace fem {
+
+// Points are padded to a multiple of this; padded rows of X are zero and
+// their posteriors are written as zero, so they never contribute.
+constexpr int64_t kRowAlign = 4096;
+// Largest supported dimension (the factor kernel keeps L in LDS).
+constexpr int kMaxDim = 128;
+
+// Number of (jb, lb) fragment pairs the E-step consumes for NB 16-feature
+// blocks: lb runs over 4-feature blocks, jb >= lb/4 (U is upper triangular).
+inline int estep_pairs(int NB) {
+    int p = 0;
+    for (int lb = 0; lb < 4 * NB; lb++) p += NB - lb / 4;
+    return p;
+}
+inline int upper_blocks(int NB) { return NB * (NB + 1) / 2; }
+
+struct Launch {
+    hipStream_t stream;
+};
+
+}  // namespace fem
+
+struct fem_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t n = 0, npad = 0, ntotal = 0;
+    int d = 0, dpad = 0, NB = 0, K = 0, KP = 0, prec = 0;
+    double g = 0.0;
+    bool have_params = false, factored = false, have_post = false;
+
+    // dataset and posteriors
+    float *X = nullptr;        // npad x dpad, FP32, zero padded
+    double *Rt = nullptr;      // KP x npad FP64 posteriors (component-major)
+    float *Rt32 = nullptr;     // KP x npad FP32 posteriors (mixed precision)
+
+    // parameters (dense d)
+    double *w = nullptr;       // K
+    double *mu = nullptr;      // K x d
+    double *cov = nullptr;     // K x d x d
+    double *nk = nullptr;      // K   posterior sums of the last M-step
+
+    // factorisation products
+    double *logdet = nullptr;  // K
+    int *info = nullptr;       // K
+    double *coef = nullptr;    // K   log w_k - d/2 log 2pi - logdet_k/2
+    double *U = nullptr;       // K x dpad x dpad  U_k = L_k^-T (zero padded)
+    double *ufrag = nullptr;   // K x pairs x 64   E-step A fragments (FP64)
+    float *ufrag32 = nullptr;  // K x pairs x 64   (FP32)
+    double *minit = nullptr;   // K x dpad         U_k^T mu_k
+    float *mus32 = nullptr;    // K x 2 x dpad     mu split hi/lo (FP32)
+
+    // reductions
+    int est_blocks = 0;
+    double *ll_part = nullptr;  // per E-step workgroup
+    double *ll_dev = nullptr;   // 1
+    int m_chunks = 0, c_chunks = 0;
+    int64_t c_chunk_rows = 0;
+    double *mpart = nullptr;    // m_chunks x KP x (dpad+1)
+    double *cpart = nullptr;    // c_chunks x KP x ublocks x 256
+
+    hipEvent_t ev[8] = {};
+    float t_estep = 0, t_means = 0, t_cov = 0, t_factor = 0;
+    std::string err;
+};
+
+namespace fem {
+hipError_t launch_factor(fem_ctx *c);
+hipError_t launch_estep(fem_ctx *c);
+hipError_t launch_means(fem_ctx *c);
+hipError_t launch_cov(fem_ctx *c);
+hipError_t launch_zero(double *p, int64_t n, hipStream_t s);
+hipError_t launch_transpose_post(const fem_ctx *c, double *out_dev, int64_t ld);
+}  // namespace fem

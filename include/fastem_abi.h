@@ -1,0 +1,117 @@
+/*
+ * fastem_abi.h -- C ABI of the MI355X Gaussian-mixture EM engine (libfastem.so).
+ *
+ * Plain C types only.  One context owns the device copy of one dataset, the
+ * mixture parameters, every scratch buffer and one HIP stream.  Host arrays
+ * passed in are borrowed for the duration of the call.  Every call is
+ * synchronous at return unless stated otherwise.  Calls on one context are not
+ * thread-safe.  Return value: FEM_OK (0) or a negative FEM_ERR_* code, with a
+ * message from fem_last_error().
+ *
+ * Layouts (row-major, FP64 unless stated): weights[K]; means[K][d];
+ * covariances[K][d][d]; data X[n][ld] FP32; posteriors R[n][ld] FP64.
+ *
+ * Each entry point names the reference hook it replaces (paths relative to
+ * the fastem source tree).
+ */
+#ifndef FASTEM_ABI_H
+#define FASTEM_ABI_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fem_ctx fem_ctx;
+
+enum fem_status {
+    FEM_OK = 0,
+    FEM_ERR_ARG = -1,     /* bad argument or unsupported shape */
+    FEM_ERR_HIP = -2,     /* HIP runtime error */
+    FEM_ERR_NOT_PD = -3,  /* a covariance is not positive definite (see info[]) */
+    FEM_ERR_STATE = -4    /* call out of order (e.g. E-step before factor) */
+};
+
+/* Arithmetic of the E-step and M-step contractions.  Cholesky, log-det and
+ * the final reduction of the sufficient statistics are FP64 in every mode. */
+enum fem_precision {
+    FEM_PREC_FP64 = 0,    /* FP64 MFMA everywhere: the parity form */
+    FEM_PREC_FP32 = 1,    /* FP32 MFMA contractions, FP32 posteriors */
+    FEM_PREC_BF16X3 = 2   /* BF16x3 split products, FP32 accumulation */
+};
+
+/* Allocates a context for n points of dimension d and K components on HIP
+ * device `device`.  Replaces the constructors EMAlgorithm(ncols,nrows,ncl)
+ * (EM2/EMAlgorithm.cpp:115-149) and MatrixEMBase/MatrixEM/SimpleMatrixEM
+ * (EM2/MatrixEM.cpp:10-30,161-174,361-374): Posteriors, Dataset and the
+ * per-thread reducer buffers become device buffers.  1 <= d <= 128, K >= 1. */
+int fem_create(fem_ctx **ctx, int64_t n, int d, int K, int precision, int device);
+int fem_destroy(fem_ctx *ctx);
+
+/* Copies the FP32 dataset (n rows, row stride ld floats) from host memory to
+ * HBM.  Replaces EMAlgorithm::CopyDataset / MatrixEM::CopyDataset
+ * (EM2/EMAlgorithm.cpp:225-235, EM2/MatrixEM.cpp:102-121); the FP32->FP64
+ * widening happens in registers inside the kernels. */
+int fem_upload_data(fem_ctx *ctx, const float *X, int64_t ld);
+/* Same, from a device pointer already in HBM (no PCIe transfer). */
+int fem_upload_data_device(fem_ctx *ctx, const float *dX, int64_t ld);
+
+/* Number of rows the weights are normalised by (EMAlgorithm::TotalRowCount,
+ * EM2/EMAlgorithm.cpp:134; MatrixEM.cpp:375).  Defaults to n. */
+int fem_set_total_rows(fem_ctx *ctx, int64_t ntotal);
+
+/* Covariance regularisation coefficient g in [0,1] applied by every M-step:
+ * C <- C(1-g) + g tr(C)/d I  (EMAlgorithm::RegularizeCovariances,
+ * EM2/EMAlgorithm.cpp:237-244, called after MStep at :158). */
+int fem_set_regularization(fem_ctx *ctx, double g);
+
+/* Uploads mixture parameters (EMAlgorithm::DecodeMixture,
+ * EM2/EMAlgorithm.cpp:125-131).  Covariances are read from their lower
+ * triangle when factorised, as NormalDensity::Init does. */
+int fem_set_params(fem_ctx *ctx, const double *w, const double *mu, const double *cov);
+/* Downloads the current parameters; any pointer may be NULL. */
+int fem_get_params(fem_ctx *ctx, double *w, double *mu, double *cov);
+
+/* Stage (d): batched FP64 Cholesky, log-determinant and U_k = (L_k^-1)^T for
+ * every component.  Replaces EMAlgorithm::InitDensities ->
+ * NormalDensity::Init (EM2/EMAlgorithm.cpp:75-87, EM2/NormalDensity.cpp:43-65)
+ * and MatrixEMBase::CalcLogCoeffs (EM2/MatrixEM.cpp:83-87).  logdet[K] and
+ * info[K] may be NULL; info[k] = 0, or j+1 if the pivot of column j was not
+ * positive.  Returns FEM_ERR_NOT_PD if any component failed. */
+int fem_factor(fem_ctx *ctx, double *logdet, int *info);
+
+/* Stage (a): fused E-step -- log-densities through U_k on MFMA, log-sum-exp
+ * over K, posteriors -- and the log-likelihood of the current parameters.
+ * Replaces MatrixEM/SimpleMatrixEM::PrecomputeDensityMatrix +
+ * ComputeLogLike (EM2/MatrixEM.cpp:39-81,126-157,275-307; MatrixEM.h:88). */
+int fem_estep(fem_ctx *ctx, double *loglik);
+
+/* Stages (b)+(c): N_k and means (R^T X), then covariances about the new
+ * means (weighted SYRK), weights N_k/ntotal, then regularisation.  Replaces
+ * MatrixEM/SimpleMatrixEM::MStep/doMStep (EM2/MatrixEM.cpp:176-249,309-378)
+ * including both OpenMP reducers (EM2/OpenMPEMReducer.cpp, Util/OMPReducer.h).
+ * The outputs may be NULL (parameters stay on the device). */
+int fem_mstep(fem_ctx *ctx, double *w, double *mu, double *cov);
+
+/* Copies the posteriors of the last E-step to host memory (n rows, stride ld
+ * doubles).  The reference's EMAlgorithm::Posteriors matrix
+ * (EM2/EMAlgorithm.h:41) read by RetrieveClasses (EM2/EMAlgorithm.cpp:95-110). */
+int fem_get_posteriors(fem_ctx *ctx, double *R, int64_t ld);
+
+/* Device time in milliseconds of the last fem_estep / fem_mstep (means and
+ * covariance passes) / fem_factor, from HIP events on the context's stream.
+ * Any pointer may be NULL. */
+int fem_last_times(fem_ctx *ctx, float *estep_ms, float *means_ms, float *cov_ms,
+                   float *factor_ms);
+
+/* Message of the last failed call on ctx (or of the last failed fem_create
+ * when ctx is NULL). */
+const char *fem_last_error(const fem_ctx *ctx);
+
+/* Library version string. */
+const char *fem_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,195 @@
+"""Parity of the HIP engine (through the C ABI) with the FP64 oracle.
+
+Plain FP64 form: per-iteration log-likelihood, weights, means and covariances
+within 1e-10 relative of the oracle on the same data and the same initial
+parameters.  Needs an MI355X.
+"""
+import numpy as np
+import pytest
+
+import fastem_amd as fem
+
+pytestmark = pytest.mark.gpu
+
+LL_RTOL = 1e-10
+PARAM_RTOL = 1e-10
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def rel_err(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def mixture(n, d, K, seed, lo=-10.0, hi=10.0, diag=0.1):
+    X, w, mu, cov = fem.gengmm(n, d, K, lo, hi, diag, seed)
+    return X, w, mu, cov
+
+
+def jittered_init(X, K, seed):
+    """Initial parameters: K distinct data rows as means, the data covariance
+    for every component, equal weights."""
+    rng = np.random.default_rng(seed)
+    n, d = X.shape
+    idx = rng.choice(n, size=K, replace=False)
+    mu = X[idx].astype(np.float64)
+    C = np.cov(X.astype(np.float64), rowvar=False).reshape(d, d) + 1e-3 * np.eye(d)
+    return np.full(K, 1.0 / K), mu, np.stack([C] * K)
+
+
+@pytest.mark.parametrize("d", [1, 3, 16, 17, 64, 128])
+def test_factor_logdet(orc, d):
+    say(f"factor d={d}")
+    K = 7
+    rng = np.random.default_rng(d)
+    A = rng.standard_normal((K, d, d))
+    cov = A @ A.transpose(0, 2, 1) / d + 0.1 * np.eye(d)
+    with fem.Context(100, d, K) as ctx:
+        ctx.set_params(np.full(K, 1.0 / K), rng.standard_normal((K, d)), cov)
+        ld, info = ctx.factor()
+    assert np.all(info == 0)
+    _, ld_ref = orc.factor_all(cov)
+    assert rel_err(ld, ld_ref) < 1e-12
+
+
+def test_factor_not_pd(orc):
+    say("factor not-PD")
+    d, K = 4, 3
+    cov = np.stack([np.eye(d)] * K)
+    cov[1, 2, 2] = -1.0
+    with fem.Context(64, d, K) as ctx:
+        ctx.set_params(np.full(K, 1.0 / K), np.zeros((K, d)), cov)
+        with pytest.raises(fem.FemError) as ei:
+            ctx.factor()
+        assert ei.value.code == fem.FEM_ERR_NOT_PD
+        assert list(ei.value.info) == [0, 3, 0]
+        with pytest.raises(fem.FemError) as ei:
+            ctx.estep()  # no E-step on a failed factorisation
+        assert ei.value.code == fem.FEM_ERR_STATE
+
+
+@pytest.mark.parametrize("n,d,K", [(1000, 3, 2), (5003, 17, 7), (4096, 16, 8), (3000, 64, 5),
+                                   (2500, 128, 3), (777, 1, 1), (9000, 8, 64), (4099, 2, 16)])
+def test_estep_mstep_parity(orc, n, d, K):
+    say(f"estep/mstep n={n} d={d} K={K}")
+    X, _, _, _ = mixture(n, d, K, seed=n + d + K)
+    w, mu, cov = jittered_init(X, K, seed=1)
+    U, ld = orc.factor_all(cov)
+    ll_ref, R_ref = orc.estep(X, w, mu, U, ld)
+    w_ref, mu_ref, cov_ref = orc.mstep(X, R_ref)
+    with fem.Context(n, d, K) as ctx:
+        ctx.upload(X)
+        ctx.set_params(w, mu, cov)
+        ctx.factor()
+        ll = ctx.estep()
+        R = ctx.posteriors()
+        w2, mu2, cov2 = ctx.mstep()
+    assert abs(ll - ll_ref) <= LL_RTOL * abs(ll_ref)
+    np.testing.assert_allclose(R, R_ref, rtol=1e-9, atol=1e-13)
+    np.testing.assert_allclose(R.sum(1), 1.0, rtol=1e-13)
+    assert rel_err(w2, w_ref) < PARAM_RTOL
+    assert rel_err(mu2, mu_ref) < PARAM_RTOL
+    assert rel_err(cov2, cov_ref) < PARAM_RTOL
+    for k in range(K):
+        assert np.array_equal(cov2[k], cov2[k].T)
+
+
+@pytest.mark.parametrize("g", [0.0, 0.01])
+def test_trajectory_config1(orc, g):
+    """N=100,000, d=16, K=8, means U[-10,10], cov AA^T/d + 0.1 I, k-means++
+    init; every iteration's LL and the final parameters within 1e-10."""
+    n, d, K, iters = 100000, 16, 8, 40
+    say(f"trajectory config1 g={g}")
+    X, _, _, _ = mixture(n, d, K, seed=11)
+    w0, mu0, cov0 = fem.kmeanspp_init(X, K, seed=5)
+    ref = orc.train(X, w0, mu0, cov0, maxsweeps=iters, eps=1e-300, g=g, svd=False)
+    assert ref.rc == iters + 1
+    trace = []
+    with fem.Context(n, d, K) as ctx:
+        ctx.upload(X)
+        it, ll, (w, mu, cov) = fem.train(ctx, w0, mu0, cov0, maxsweeps=iters, eps=1e-300, g=g,
+                                         ll_trace=trace)
+    assert it == ref.rc
+    errs = np.abs(np.array(trace) - ref.ll_trace) / np.abs(ref.ll_trace)
+    say("max per-iteration LL rel err", errs.max())
+    assert errs.max() < LL_RTOL
+    assert rel_err(w, ref.w) < PARAM_RTOL
+    assert rel_err(mu, ref.mu) < PARAM_RTOL
+    assert rel_err(cov, ref.cov) < PARAM_RTOL
+
+
+def test_convergence_stop_matches(orc):
+    n, d, K = 20000, 5, 4
+    say("convergence stop")
+    X, _, _, _ = mixture(n, d, K, seed=3)
+    w0, mu0, cov0 = jittered_init(X, K, seed=2)
+    ref = orc.train(X, w0, mu0, cov0, maxsweeps=500, eps=1e-7, svd=False)
+    with fem.Context(n, d, K) as ctx:
+        ctx.upload(X)
+        it, ll, _ = fem.train(ctx, w0, mu0, cov0, maxsweeps=500, eps=1e-7)
+    assert it == ref.rc
+    assert abs(ll - ref.ll) <= LL_RTOL * abs(ref.ll)
+
+
+def test_posteriors_deterministic():
+    n, d, K = 50000, 16, 8
+    say("determinism")
+    X, w, mu, cov = mixture(n, d, K, seed=4)
+    outs = []
+    for _ in range(2):
+        with fem.Context(n, d, K) as ctx:
+            ctx.upload(X)
+            ctx.set_params(w, mu, cov)
+            ctx.factor()
+            ll = ctx.estep()
+            p = ctx.mstep()
+            outs.append((ll, p))
+    assert outs[0][0] == outs[1][0]
+    for a, b in zip(outs[0][1], outs[1][1]):
+        assert np.array_equal(a, b)
+
+
+def test_full_size_properties(orc):
+    """N=4,194,304, d=64, K=64 (means U[-20,20], cov AA^T/d + 0.05 I): the LL of
+    the whole set equals the sum over two halves run separately, posteriors
+    of sampled rows equal the oracle's for those rows, weights sum to one and
+    the LL does not decrease."""
+    n, d, K = 4194304, 64, 64
+    say("full size generate")
+    X, w, mu, cov = mixture(n, d, K, seed=2024, lo=-20.0, hi=20.0, diag=0.05)
+    rng = np.random.default_rng(0)
+    mu0 = mu + 0.5 * rng.standard_normal(mu.shape)
+    cov0 = cov * 1.3
+    w0 = np.full(K, 1.0 / K)
+    with fem.Context(n, d, K) as ctx:
+        ctx.upload(X)
+        ctx.set_params(w0, mu0, cov0)
+        ctx.factor()
+        say("full size estep")
+        ll_full = ctx.estep()
+        R = ctx.posteriors()
+        sample = rng.choice(n, 512, replace=False)
+        R_s = R[sample]
+        del R
+        lls = [ll_full]
+        for _ in range(3):
+            w1, mu1, cov1 = ctx.mstep()
+            assert abs(w1.sum() - 1.0) < 1e-12
+            ctx.factor()
+            lls.append(ctx.estep())
+    assert np.all(np.diff(lls) >= -1e-12 * np.abs(lls[1:]))
+    halves = []
+    for a, b in ((0, n // 2), (n // 2, n)):
+        say("full size half", a)
+        with fem.Context(b - a, d, K) as ctx:
+            ctx.upload(X[a:b])
+            ctx.set_params(w0, mu0, cov0)
+            ctx.factor()
+            halves.append(ctx.estep())
+    assert abs(sum(halves) - ll_full) <= 1e-12 * abs(ll_full)
+    U, ld = orc.factor_all(cov0)
+    _, R_ref = orc.estep(X[sample], w0, mu0, U, ld)
+    np.testing.assert_allclose(R_s, R_ref, rtol=1e-9, atol=1e-13)
