@@ -14,7 +14,9 @@
 // Finalisers reduce the per-chunk partials in a fixed order in FP64.
 //
 // MFMA 16x16x4 operand layout (wave64, lane l): A[i][k] from lane i + 16k,
-// B[k][j] from lane j + 16k, D[4*(l/16)+v][l%16] in accumulator element v.
+// B[k][j] from lane j + 16k.  Accumulator element v of lane l holds
+// D[4*v + l/16][l%16] for v_mfma_f64_16x16x4f64 and D[4*(l/16) + v][l%16] for
+// v_mfma_f32_16x16x4f32 (checked on gfx950 with a lane-layout probe).
 #include "fem_internal.h"
 
 #include <cmath>
@@ -191,10 +193,10 @@ __global__ __launch_bounds__(256) void k_estep64(const float *__restrict__ X, in
 #pragma unroll
         for (int jb = 0; jb < NB; jb++) {
             v4d init;
-            init[0] = -lm[16 * jb + 4 * g + 0];
-            init[1] = -lm[16 * jb + 4 * g + 1];
-            init[2] = -lm[16 * jb + 4 * g + 2];
-            init[3] = -lm[16 * jb + 4 * g + 3];
+            init[0] = -lm[16 * jb + 0 + g];
+            init[1] = -lm[16 * jb + 4 + g];
+            init[2] = -lm[16 * jb + 8 + g];
+            init[3] = -lm[16 * jb + 12 + g];
 #pragma unroll
             for (int p = 0; p < P; p++) acc[p][jb] = init;
         }
@@ -326,7 +328,7 @@ __global__ __launch_bounds__(256) void k_means64(const float *__restrict__ X,
         }
 #pragma unroll
         for (int v = 0; v < 4; v++)
-            out[(size_t)(16 * kb + 4 * g + v) * (DP + 1) + 16 * jb + c] = acc[v];
+            out[(size_t)(16 * kb + 4 * v + g) * (DP + 1) + 16 * jb + c] = acc[v];
         if (jb == 0) {
             ns = wave_xor_sum_g(ns);
             if (g == 0) out[(size_t)(16 * kb + c) * (DP + 1) + DP] = ns;
@@ -432,7 +434,7 @@ __global__ __launch_bounds__(256) void k_cov64(const float *__restrict__ X,
 #pragma unroll
     for (int u = 0; u < UB; u++)
 #pragma unroll
-        for (int v = 0; v < 4; v++) out[(size_t)u * 256 + (4 * g + v) * 16 + c] = acc[u][v];
+        for (int v = 0; v < 4; v++) out[(size_t)u * 256 + (4 * v + g) * 16 + c] = acc[u][v];
 }
 
 // Sum the chunk partials of component k (fixed order), mirror the upper
