@@ -61,6 +61,18 @@ def cpu_baseline(X, w0, mu0, cov0, rows, iters):
                       f"by {N}/{rows}; {t:.1f} s measured"}
 
 
+def calib_copy():
+    """A plain device-to-device copy of exactly 1 GiB, three times: the
+    reference point that scales the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"calib_copy_bytes": x.numel() * 4, "launches": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,7 +81,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=262144)
     ap.add_argument("--cpu-iters", type=int, default=2)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
+    if args.calib_copy:
+        calib_copy()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this engine runs a fit on a single MI355X "
               "and has no multi-GPU path; run with --gpus 1", file=sys.stderr)
