@@ -11,7 +11,11 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libfastem.so")
 
 HIP_SOURCES = ["fem_kernels.hip", "fem_abi.hip"]
-HOST_SOURCES = ["fem_tools.cpp"]
+HOST_SOURCES = ["fem_tools.cpp", "host/linalg.cpp", "host/rand.cpp", "host/block_manager.cpp",
+                "host/normal_density.cpp", "host/gaussian_mixture.cpp", "host/dataset.cpp",
+                "host/em_algorithm.cpp", "host/em_initializer.cpp"]
+EM2_SOURCE = "host/em2_main.cpp"
+EM2 = os.path.join(PKG, "em2")
 
 # Packed FP32 VALU ops are disabled for all device code.
 HIP_FLAGS = [
@@ -39,7 +43,7 @@ def build(verbose=True):
     objs = []
     for s in HIP_SOURCES + HOST_SOURCES:
         src = os.path.join(CSRC, s)
-        obj = os.path.join(objdir, os.path.splitext(s)[0] + ".o")
+        obj = os.path.join(objdir, os.path.splitext(s.replace("/", "_"))[0] + ".o")
         flags = list(HIP_FLAGS)
         if s.endswith(".cpp"):
             flags += ["-fopenmp"]
@@ -48,6 +52,14 @@ def build(verbose=True):
     tmp = LIB + ".tmp"
     _run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-fopenmp", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    # the em2 driver, linked against the library next to it
+    src = os.path.join(CSRC, EM2_SOURCE)
+    obj = os.path.join(objdir, "em2_main.o")
+    _run([hipcc(), "-O3", "-std=c++17", "-fopenmp", "-I", os.path.join(ROOT, "include"), "-c",
+          src, "-o", obj])
+    _run([hipcc(), "-fopenmp", "-o", EM2 + ".tmp", obj, "-L", PKG, "-lfastem",
+          "-Wl,-rpath,$ORIGIN"])
+    os.replace(EM2 + ".tmp", EM2)
     return LIB
 
 

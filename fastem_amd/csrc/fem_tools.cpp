@@ -131,14 +131,32 @@ extern "C" int fem_gengmm(int64_t n, int d, int K, double mean_lo, double mean_h
     return 0;
 }
 
-extern "C" int fem_kmeanspp_init(int64_t n, int d, int K, const float *X, uint64_t seed,
-                                 int lloyd_iters, double *w, double *mu, double *cov) {
+namespace fem_host {
+
+// Seeding (Forgy random partition, Clust/KMeansInitializer.cpp:33-49, or
+// k-means++), up to max_sweeps Lloyd sweeps (Clust/KMAlgorithm.cpp:44-87 with
+// MinRel 1e-4; the reference's MaxIter = max_sweeps + 1), then the mixture of
+// the final clustering (EM2/EMInitializer.cpp:103-134).  rnd(rs) is the
+// reference's Rand().
+int kmeans_mixture(int64_t n, int d, int K, const float *X, double (*rnd)(void *), void *rs,
+                   int seeding, int max_sweeps, double *w, double *mu, double *cov) {
     if (n < 1 || d < 1 || K < 1 || !X || !w || !mu || !cov) return -1;
-    std::mt19937_64 rng(seed);
     std::vector<double> C((size_t)K * d);
+    if (seeding == 0) {
+        std::vector<int64_t> cnt(K, 0);
+        for (int64_t i = 0; i < n; i++) {
+            const int part = (int)(rnd(rs) * K);
+            cnt[part]++;
+            for (int j = 0; j < d; j++) C[(size_t)part * d + j] += X[(size_t)i * d + j];
+        }
+        for (int k = 0; k < K; k++) {
+            const double f = 1.0 / (double)cnt[k];
+            for (int j = 0; j < d; j++) C[(size_t)k * d + j] *= f;
+        }
+    } else {
     std::vector<double> cd(n, std::numeric_limits<double>::infinity()), agg(n);
     // Clust/PlusPlusInitializer/PlusPlusInitializer.cpp:33-84
-    const int64_t r0 = (int64_t)(canonical(rng) * (double)n);
+    const int64_t r0 = (int64_t)(rnd(rs) * (double)n);
     for (int j = 0; j < d; j++) C[j] = (double)X[(size_t)r0 * d + j];
     for (int c = 1; c < K; c++) {
         const double *pc = C.data() + (size_t)(c - 1) * d;
@@ -154,12 +172,13 @@ extern "C" int fem_kmeanspp_init(int64_t n, int d, int K, const float *X, uint64
         }
 #pragma omp parallel for schedule(static)
         for (int64_t i = 0; i < n; i++) agg[i] /= sum;
-        const double r = canonical(rng);
+        const double r = rnd(rs);
         for (int64_t i = 0; i < n; i++)
             if (r <= agg[i]) {
                 for (int j = 0; j < d; j++) C[(size_t)c * d + j] = (double)X[(size_t)i * d + j];
                 break;
             }
+    }
     }
     std::vector<int> cls(n);
     std::vector<double> psum((size_t)kParts * K * d);
@@ -197,7 +216,7 @@ extern "C" int fem_kmeanspp_init(int64_t n, int d, int K, const float *X, uint64
     };
     // Clust/KMAlgorithm.cpp:44-87,205-255 (empty clusters repaired from Rand())
     double best = std::numeric_limits<double>::max();
-    for (int it = 1; it <= lloyd_iters; it++) {
+    for (int it = 1; it <= max_sweeps; it++) {
         const double fit = assign(true);
         for (int p = 1; p < kParts; p++) {
             for (size_t q = 0; q < (size_t)K * d; q++) psum[q] += psum[(size_t)p * K * d + q];
@@ -208,7 +227,7 @@ extern "C" int fem_kmeanspp_init(int64_t n, int d, int K, const float *X, uint64
                 const double f = 1.0 / (double)pcnt[k];
                 for (int j = 0; j < d; j++) C[(size_t)k * d + j] = psum[(size_t)k * d + j] * f;
             } else {
-                const int64_t src = (int64_t)(canonical(rng) * (double)n);
+                const int64_t src = (int64_t)(rnd(rs) * (double)n);
                 for (int j = 0; j < d; j++) C[(size_t)k * d + j] = (double)X[(size_t)src * d + j];
             }
         }
@@ -216,7 +235,7 @@ extern "C" int fem_kmeanspp_init(int64_t n, int d, int K, const float *X, uint64
         if (fit < std::nextafter(fit, best)) best = fit;
         else break;
         if (rel < 1e-4) break;
-        if (it == lloyd_iters) break;
+        if (it == max_sweeps) break;
     }
     assign(false);
     // EM2/EMInitializer.cpp:103-134
@@ -249,4 +268,14 @@ extern "C" int fem_kmeanspp_init(int64_t n, int d, int K, const float *X, uint64
         for (int j = 0; j < d; j++) mu[(size_t)k * d + j] = C[(size_t)k * d + j];
     }
     return rc;
+}
+
+}  // namespace fem_host
+
+static double mt_canonical(void *p) { return canonical(*static_cast<std::mt19937_64 *>(p)); }
+
+extern "C" int fem_kmeanspp_init(int64_t n, int d, int K, const float *X, uint64_t seed,
+                                 int lloyd_iters, double *w, double *mu, double *cov) {
+    std::mt19937_64 rng(seed);
+    return fem_host::kmeans_mixture(n, d, K, X, mt_canonical, &rng, 1, lloyd_iters, w, mu, cov);
 }
