@@ -25,6 +25,7 @@ sys.path.insert(0, ROOT)
 N, D, K = 4194304, 64, 64
 SEED = 2024
 PEAK_FP64_MFMA = 78.6e12   # MI355X dense FP64 matrix, FLOP/s
+PEAK_FP32_MFMA = 157.3e12  # MI355X dense FP32 matrix, FLOP/s
 PEAK_HBM = 8.0e12          # B/s
 
 
@@ -79,6 +80,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the FP32-form measurement")
     ap.add_argument("--cpu-rows", type=int, default=262144)
     ap.add_argument("--cpu-iters", type=int, default=2)
     ap.add_argument("--calib-copy", action="store_true",
@@ -102,46 +105,57 @@ def main():
     log("k-means++ init")
     w0, mu0, cov0 = fem.kmeanspp_init(X, K, seed=1, lloyd_iters=0)
 
-    ctx = fem.Context(N, D, K, fem.PREC_FP64, 0)
-    ctx.upload(X)
-    ctx.set_params(w0, mu0, cov0)
-    ctx.factor()
-
-    def step():
-        ll = ctx.estep()
-        t = ctx.times()
-        ctx.mstep(copy=False)
-        t2 = ctx.times()
+    def run_form(prec, steps, warmup):
+        ctx = fem.Context(N, D, K, prec, 0)
+        ctx.upload(X)
+        ctx.set_params(w0, mu0, cov0)
         ctx.factor()
-        t3 = ctx.times()
-        return ll, t["estep_ms"], t2["means_ms"], t2["cov_ms"], t3["factor_ms"]
 
-    log("warmup")
-    for _ in range(args.warmup):
-        step()
-    log("timed")
-    recs = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        recs.append(step())
-    wall = time.perf_counter() - t0
-    lls = [r[0] for r in recs]
-    est = np.mean([r[1] for r in recs])
-    mea = np.mean([r[2] for r in recs])
-    cov = np.mean([r[3] for r in recs])
-    fac = np.mean([r[4] for r in recs])
-    ctx.close()
+        def step():
+            ll = ctx.estep()
+            t = ctx.times()
+            ctx.mstep(copy=False)
+            t2 = ctx.times()
+            ctx.factor()
+            t3 = ctx.times()
+            return ll, t["estep_ms"], t2["means_ms"], t2["cov_ms"], t3["factor_ms"]
+
+        for _ in range(warmup):
+            step()
+        recs = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            recs.append(step())
+        wall = time.perf_counter() - t0
+        ctx.close()
+        ph = {name: float(np.mean([r[i] for r in recs]))
+              for i, name in ((1, "estep"), (2, "means"), (3, "cov"), (4, "factor"))}
+        return {"iters_per_s": steps / wall, "ms_per_step": 1e3 * wall / steps,
+                "phases_ms": ph, "ll": [recs[0][0], recs[-1][0]]}
+
+    log("fp64 form")
+    r64 = run_form(fem.PREC_FP64, args.steps, args.warmup)
+    variants = {}
+    if not args.no_variants:
+        log("fp32 form")
+        r32 = run_form(fem.PREC_FP32, args.steps, args.warmup)
+        variants["fp32"] = {
+            "iters_per_s": r32["iters_per_s"], "phases_ms": r32["phases_ms"],
+            "estep_frac_of_fp32_mfma_peak":
+                algorithmic_flops(N, D, K)["estep"] / (r32["phases_ms"]["estep"] * 1e-3)
+                / PEAK_FP32_MFMA}
 
     fl = algorithmic_flops(N, D, K)
-    achieved = fl["estep"] / (est * 1e-3)
+    ph = r64["phases_ms"]
+    achieved = fl["estep"] / (ph["estep"] * 1e-3)
     out = {
         "metric": "EM iters/s (plain FP64-statistics GMM EM, N=4194304 d=64 K=64)",
-        "value": args.steps / wall,
+        "value": r64["iters_per_s"],
         "unit": "EM iters/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * wall / args.steps,
+        "ms_per_step": r64["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -152,10 +166,10 @@ def main():
         "roofline": {"bound": "mfma", "kernel": "k_estep64<4,2>", "achieved": achieved / 1e12,
                      "peak": PEAK_FP64_MFMA / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_MFMA, "traffic": None},
-        "phases_ms": {"estep": est, "means": mea, "cov": cov, "factor": fac},
-        "phase_tflops": {"estep": fl["estep"] / est / 1e9, "means": fl["means"] / mea / 1e9,
-                         "cov": fl["cov"] / cov / 1e9},
-        "loglik_first_last": [lls[0], lls[-1]],
+        "phases_ms": ph,
+        "phase_tflops": {k2: fl[k2] / ph[k2] / 1e9 for k2 in ("estep", "means", "cov")},
+        "loglik_first_last": r64["ll"],
+        "variants": variants,
     }
     if not args.no_cpu_baseline:
         log("cpu baseline")

@@ -69,7 +69,7 @@ int fem_create(fem_ctx **out, int64_t n, int d, int K, int precision, int device
         return fail(nullptr, FEM_ERR_ARG, "n, d and K must be positive");
     if (d > fem::kMaxDim)
         return fail(nullptr, FEM_ERR_ARG, "dimension above 128 is not supported");
-    if (precision != FEM_PREC_FP64)
+    if (precision != FEM_PREC_FP64 && precision != FEM_PREC_FP32)
         return fail(nullptr, FEM_ERR_ARG, "precision not available in this build");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
@@ -90,7 +90,9 @@ int fem_create(fem_ctx **out, int64_t n, int d, int K, int precision, int device
     const int npairs = fem::estep_pairs(c->NB);
     const int ub = fem::upper_blocks(c->NB);
     const int P = (c->NB <= 4) ? 2 : 1;
-    c->est_blocks = (int)(c->npad / (64 * P));
+    c->est_blocks = (precision == FEM_PREC_FP64)
+                        ? (int)(c->npad / (64 * P))
+                        : (int)(c->npad / fem::estep32_points_per_block(c->NB));
     c->m_chunks = (int)(c->npad / 4096);
     // covariance chunks: at most 256 chunks of a multiple of 4096 rows
     int64_t units = c->npad / 4096;
@@ -113,7 +115,12 @@ int fem_create(fem_ctx **out, int64_t n, int d, int K, int precision, int device
     const size_t nX = (size_t)c->npad * c->dpad;
     const size_t nR = (size_t)c->KP * c->npad;
     if ((e = dmalloc(&c->X, nX)) != hipSuccess) return bail(e, "hipMalloc(X)");
-    if ((e = dmalloc(&c->Rt, nR)) != hipSuccess) return bail(e, "hipMalloc(R)");
+    const bool f64 = precision == FEM_PREC_FP64;
+    if (f64 && (e = dmalloc(&c->Rt, nR)) != hipSuccess) return bail(e, "hipMalloc(R)");
+    if (!f64 && (e = dmalloc(&c->Rt32, nR)) != hipSuccess) return bail(e, "hipMalloc(R32)");
+    if (!f64 && ((e = dmalloc(&c->ufrag32, (size_t)K * fem::estep_pairs(c->NB) * 64)) != hipSuccess ||
+                 (e = dmalloc(&c->mus32, (size_t)K * 2 * c->dpad)) != hipSuccess))
+        return bail(e, "hipMalloc(fp32 factors)");
     if ((e = dmalloc(&c->w, K)) != hipSuccess ||
         (e = dmalloc(&c->mu, (size_t)K * d)) != hipSuccess ||
         (e = dmalloc(&c->cov, (size_t)K * d * d)) != hipSuccess ||
@@ -130,7 +137,8 @@ int fem_create(fem_ctx **out, int64_t n, int d, int K, int precision, int device
         (e = dmalloc(&c->cpart, (size_t)c->c_chunks * c->KP * ub * 256)) != hipSuccess)
         return bail(e, "hipMalloc(partials)");
     if ((e = hipMemsetAsync(c->X, 0, nX * sizeof(float), c->stream)) != hipSuccess ||
-        (e = hipMemsetAsync(c->Rt, 0, nR * sizeof(double), c->stream)) != hipSuccess ||
+        (f64 && (e = hipMemsetAsync(c->Rt, 0, nR * sizeof(double), c->stream)) != hipSuccess) ||
+        (!f64 && (e = hipMemsetAsync(c->Rt32, 0, nR * sizeof(float), c->stream)) != hipSuccess) ||
         (e = hipStreamSynchronize(c->stream)) != hipSuccess)
         return bail(e, "initial clear");
     *out = c;

@@ -80,4 +80,10 @@ hipError_t launch_means(fem_ctx *c);
 hipError_t launch_cov(fem_ctx *c);
 hipError_t launch_zero(double *p, int64_t n, hipStream_t s);
 hipError_t launch_transpose_post(const fem_ctx *c, double *out_dev, int64_t ld);
+// FP32 form (fem_kernels32.hip)
+int estep32_points_per_block(int NB);
+hipError_t launch_estep32(fem_ctx *c);
+hipError_t launch_means32_partials(fem_ctx *c);
+hipError_t launch_cov32_partials(fem_ctx *c);
+hipError_t launch_transpose_post32(const fem_ctx *c, double *out_dev, int64_t ld);
 }  // namespace fem

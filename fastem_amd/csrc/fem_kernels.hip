@@ -282,6 +282,7 @@ static hipError_t estep64_t(fem_ctx *c) {
 }
 
 hipError_t launch_estep(fem_ctx *c) {
+    if (c->prec != FEM_PREC_FP64) return launch_estep32(c);
     switch (c->NB) {
         case 1: return estep64_t<1, 2>(c);
         case 2: return estep64_t<2, 2>(c);
@@ -361,12 +362,10 @@ template <int NB>
 static hipError_t means_t(fem_ctx *c) {
     hipLaunchKernelGGL(k_means64<NB>, dim3(c->m_chunks), dim3(256), 0, c->stream, c->X, c->Rt,
                        c->npad, c->KP, c->mpart);
-    hipLaunchKernelGGL(k_means_finalize, dim3(c->K), dim3(256), 0, c->stream, c->K, c->d, c->dpad,
-                       c->KP, c->m_chunks, c->mpart, c->mu, c->nk);
     return hipGetLastError();
 }
 
-hipError_t launch_means(fem_ctx *c) {
+static hipError_t means64_partials(fem_ctx *c) {
     switch (c->NB) {
         case 1: return means_t<1>(c);
         case 2: return means_t<2>(c);
@@ -378,6 +377,14 @@ hipError_t launch_means(fem_ctx *c) {
         case 8: return means_t<8>(c);
     }
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_means(fem_ctx *c) {
+    hipError_t e = (c->prec == FEM_PREC_FP64) ? means64_partials(c) : launch_means32_partials(c);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_means_finalize, dim3(c->K), dim3(256), 0, c->stream, c->K, c->d, c->dpad,
+                       c->KP, c->m_chunks, c->mpart, c->mu, c->nk);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -479,23 +486,27 @@ static hipError_t cov_t(fem_ctx *c) {
     const int blocks = (c->KP / 4) * c->c_chunks;
     hipLaunchKernelGGL(k_cov64<NB>, dim3(blocks), dim3(256), 0, c->stream, c->X, c->Rt, c->npad,
                        c->K, c->KP, c->d, c->mu, c->nk, c->c_chunk_rows, c->cpart);
-    hipLaunchKernelGGL(k_cov_finalize, dim3(c->K), dim3(256), 0, c->stream, c->K, c->d, c->NB,
-                       c->KP, c->c_chunks, c->cpart, c->nk, c->g, (double)c->ntotal, c->cov, c->w);
     return hipGetLastError();
 }
 
 hipError_t launch_cov(fem_ctx *c) {
-    switch (c->NB) {
-        case 1: return cov_t<1>(c);
-        case 2: return cov_t<2>(c);
-        case 3: return cov_t<3>(c);
-        case 4: return cov_t<4>(c);
-        case 5: return cov_t<5>(c);
-        case 6: return cov_t<6>(c);
-        case 7: return cov_t<7>(c);
-        case 8: return cov_t<8>(c);
-    }
-    return hipErrorInvalidValue;
+    hipError_t e = (c->prec == FEM_PREC_FP64) ? [&]() -> hipError_t {
+        switch (c->NB) {
+            case 1: return cov_t<1>(c);
+            case 2: return cov_t<2>(c);
+            case 3: return cov_t<3>(c);
+            case 4: return cov_t<4>(c);
+            case 5: return cov_t<5>(c);
+            case 6: return cov_t<6>(c);
+            case 7: return cov_t<7>(c);
+            case 8: return cov_t<8>(c);
+        }
+        return hipErrorInvalidValue;
+    }() : launch_cov32_partials(c);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_cov_finalize, dim3(c->K), dim3(256), 0, c->stream, c->K, c->d, c->NB,
+                       c->KP, c->c_chunks, c->cpart, c->nk, c->g, (double)c->ntotal, c->cov, c->w);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -518,6 +529,7 @@ __global__ void k_transpose_post(const double *__restrict__ Rt, int64_t npad, in
 }
 
 hipError_t launch_transpose_post(const fem_ctx *c, double *out_dev, int64_t ld) {
+    if (c->prec != FEM_PREC_FP64) return launch_transpose_post32(c, out_dev, ld);
     hipLaunchKernelGGL(k_transpose_post, dim3((unsigned)((c->n + 255) / 256)), dim3(256), 0,
                        c->stream, c->Rt, c->npad, c->n, c->K, out_dev, ld);
     return hipGetLastError();

@@ -1,0 +1,370 @@
+// fem_kernels32.hip -- FP32 precision form of the E-step and M-step
+// contractions on v_mfma_f32_16x16x4f32 (gfx950).  Posteriors are FP32; the
+// per-chunk partial sums are flushed from FP32 accumulators into FP64 at fixed
+// intervals and all cross-chunk reductions, the Cholesky and the log-det stay
+// FP64 (shared with the FP64 form).
+//
+// Accumulator element v of lane l of v_mfma_f32_16x16x4f32 holds
+// D[4*(l/16) + v][l%16]; A[i][k] comes from lane i + 16k, B[k][j] from lane
+// j + 16k.
+//
+// Centring: the E-step and the covariance pass form x - mu as
+// (x - mu_hi) - mu_lo with mu = mu_hi + mu_lo split into two floats, so the
+// centred value is accurate to FP32 rounding of the centred value rather than
+// of |mu| (means reach |mu| ~ 20 sigma).
+#include "fem_internal.h"
+
+#include <cmath>
+
+namespace fem {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__device__ __host__ constexpr int pairs32(int NB) {
+    int p = 0;
+    for (int lb = 0; lb < 4 * NB; lb++) p += NB - lb / 4;
+    return p;
+}
+
+__device__ inline float xor_sum_g(float v) {
+    v += __shfl_xor(v, 16);
+    v += __shfl_xor(v, 32);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Stage (a), FP32.  Workgroup = 4 waves x P groups of 16 points.  Component
+// operands (U_k fragments, mu_k hi/lo) are double-buffered in LDS: the loads of
+// component k+1 are issued into registers before the MFMAs of component k and
+// written to the idle buffer after them, one barrier per component.
+template <int NB, int P>
+__global__ __launch_bounds__(256) void k_estep32(const float *__restrict__ X, int64_t n, int64_t npad,
+                                                 int K, const float *__restrict__ ufrag,
+                                                 const float *__restrict__ mus,
+                                                 const double *__restrict__ coef,
+                                                 float *__restrict__ Rt, double *__restrict__ ll_part) {
+    constexpr int DP = 16 * NB;
+    constexpr int NPAIR = pairs32(NB);
+    constexpr int STAGE = NPAIR * 64 + 2 * DP;
+    constexpr int PER = (STAGE + 255) / 256;
+    extern __shared__ float lds32[];  // 2 * STAGE
+    __shared__ double wsum[4];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int g = lane >> 4, c = lane & 15;
+    const int64_t base = (int64_t)blockIdx.x * (64 * P) + wave * (16 * P);
+
+    float xr[P][4 * NB];
+#pragma unroll
+    for (int p = 0; p < P; p++)
+#pragma unroll
+        for (int lb = 0; lb < 4 * NB; lb++) xr[p][lb] = X[(base + 16 * p + c) * DP + 4 * lb + g];
+
+    auto stage_src = [&](int k, int e) -> float {
+        return (e < NPAIR * 64) ? ufrag[(size_t)k * NPAIR * 64 + e]
+                                : mus[(size_t)k * 2 * DP + (e - NPAIR * 64)];
+    };
+    {
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int e = tid + 256 * q;
+            if (e < STAGE) lds32[e] = stage_src(0, e);
+        }
+    }
+    __syncthreads();
+
+    float mx[P];
+#pragma unroll
+    for (int p = 0; p < P; p++) mx[p] = -INFINITY;
+
+    for (int k = 0; k < K; k++) {
+        const float *lu = lds32 + (k & 1) * STAGE;
+        const float *lmh = lu + NPAIR * 64;
+        const float *lml = lmh + DP;
+        float pre[PER];
+        const bool more = k + 1 < K;
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int e = tid + 256 * q;
+            pre[q] = (more && e < STAGE) ? stage_src(k + 1, e) : 0.0f;
+        }
+        const float ck = (float)coef[k];
+        v4f acc[P][NB];
+#pragma unroll
+        for (int p = 0; p < P; p++)
+#pragma unroll
+            for (int jb = 0; jb < NB; jb++) acc[p][jb] = (v4f){0.f, 0.f, 0.f, 0.f};
+        int pr = 0;
+#pragma unroll
+        for (int lb = 0; lb < 4 * NB; lb++) {
+            const float mh = lmh[4 * lb + g], ml = lml[4 * lb + g];
+            float b[P];
+#pragma unroll
+            for (int p = 0; p < P; p++) b[p] = (xr[p][lb] - mh) - ml;
+#pragma unroll
+            for (int jb = lb / 4; jb < NB; jb++) {
+                const float a = lu[pr * 64 + lane];
+                pr++;
+#pragma unroll
+                for (int p = 0; p < P; p++)
+                    acc[p][jb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b[p], acc[p][jb], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            float q = 0.f;
+#pragma unroll
+            for (int jb = 0; jb < NB; jb++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) q += acc[p][jb][v] * acc[p][jb][v];
+            q = xor_sum_g(q);
+            const float s = -0.5f * q + ck;
+            mx[p] = fmaxf(mx[p], s);
+            if ((k & 3) == g) Rt[(size_t)k * npad + base + 16 * p + c] = s;
+        }
+        if (more) {
+            float *dst = lds32 + ((k + 1) & 1) * STAGE;
+#pragma unroll
+            for (int q = 0; q < PER; q++) {
+                const int e = tid + 256 * q;
+                if (e < STAGE) dst[e] = pre[q];
+            }
+        }
+        __syncthreads();
+    }
+
+    double llsum = 0.0;
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        const int64_t pt = base + 16 * p + c;
+        float se = 0.f;
+        for (int k = g; k < K; k += 4) se += expf(Rt[(size_t)k * npad + pt] - mx[p]);
+        se = xor_sum_g(se);
+        const float lse = logf(se) + mx[p];
+        const bool valid = pt < n;
+        for (int k = g; k < K; k += 4) {
+            float *r = Rt + (size_t)k * npad + pt;
+            *r = valid ? expf(*r - lse) : 0.0f;
+        }
+        if (g == 0 && valid) llsum += (double)lse;
+    }
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) llsum += __shfl_xor(llsum, o);
+    if (lane == 0) wsum[wave] = llsum;
+    __syncthreads();
+    if (tid == 0) ll_part[blockIdx.x] = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
+}
+
+__global__ __launch_bounds__(256) void k_reduce_sum32(const double *__restrict__ parts, int64_t n,
+                                                      double *__restrict__ out) {
+    __shared__ double s[256];
+    double a = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 256) a += parts[i];
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = s[0];
+}
+
+template <int NB, int P>
+static hipError_t estep32_t(fem_ctx *c) {
+    constexpr int STAGE = pairs32(NB) * 64 + 2 * 16 * NB;
+    const size_t lds = (size_t)2 * STAGE * sizeof(float);
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_estep32<NB, P>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    const int blocks = (int)(c->npad / (64 * P));
+    hipLaunchKernelGGL((k_estep32<NB, P>), dim3(blocks), dim3(256), lds, c->stream, c->X, c->n, c->npad,
+                       c->K, c->ufrag32, c->mus32, c->coef, c->Rt32, c->ll_part);
+    hipLaunchKernelGGL(k_reduce_sum32, dim3(1), dim3(256), 0, c->stream, c->ll_part, (int64_t)blocks,
+                       c->ll_dev);
+    return hipGetLastError();
+}
+
+int estep32_points_per_block(int NB) { return 64 * (NB <= 4 ? 4 : 2); }
+
+hipError_t launch_estep32(fem_ctx *c) {
+    switch (c->NB) {
+        case 1: return estep32_t<1, 4>(c);
+        case 2: return estep32_t<2, 4>(c);
+        case 3: return estep32_t<3, 4>(c);
+        case 4: return estep32_t<4, 4>(c);
+        case 5: return estep32_t<5, 2>(c);
+        case 6: return estep32_t<6, 2>(c);
+        case 7: return estep32_t<7, 2>(c);
+        case 8: return estep32_t<8, 2>(c);
+    }
+    return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------
+// Stage (b), FP32 products: as k_means64 with FP32 operands; the FP32
+// accumulator is flushed into FP64 every 1024 points.
+template <int NB>
+__global__ __launch_bounds__(256) void k_means32(const float *__restrict__ X, const float *__restrict__ Rt,
+                                                 int64_t npad, int KP, double *__restrict__ mpart) {
+    constexpr int DP = 16 * NB;
+    constexpr int CH = 4096, FLUSH = 1024;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int g = lane >> 4, c = lane & 15;
+    const int chunk = blockIdx.x;
+    const int64_t i_begin = (int64_t)chunk * CH;
+    const int KB = KP / 16;
+    double *out = mpart + (size_t)chunk * KP * (DP + 1);
+    for (int blk = wave; blk < KB * NB; blk += 4) {
+        const int kb = blk / NB, jb = blk % NB;
+        const float *rrow = Rt + (size_t)(16 * kb + c) * npad + i_begin + g;
+        const float *xcol = X + (i_begin + g) * DP + 16 * jb + c;
+        double accd[4] = {0.0, 0.0, 0.0, 0.0};
+        double nsd = 0.0;
+        for (int s0 = 0; s0 < CH; s0 += FLUSH) {
+            v4f acc = {0.f, 0.f, 0.f, 0.f};
+            float ns = 0.f;
+#pragma unroll 8
+            for (int s = s0; s < s0 + FLUSH; s += 4) {
+                const float a = rrow[s];
+                const float b = xcol[(size_t)s * DP];
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+                ns += a;
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++) accd[v] += (double)acc[v];
+            nsd += (double)ns;
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) out[(size_t)(16 * kb + 4 * g + v) * (DP + 1) + 16 * jb + c] = accd[v];
+        if (jb == 0) {
+            nsd += __shfl_xor(nsd, 16);
+            nsd += __shfl_xor(nsd, 32);
+            if (g == 0) out[(size_t)(16 * kb + c) * (DP + 1) + DP] = nsd;
+        }
+    }
+}
+
+template <int NB>
+static hipError_t means32_t(fem_ctx *c) {
+    hipLaunchKernelGGL(k_means32<NB>, dim3(c->m_chunks), dim3(256), 0, c->stream, c->X, c->Rt32, c->npad,
+                       c->KP, c->mpart);
+    return hipGetLastError();
+}
+
+hipError_t launch_means32_partials(fem_ctx *c) {
+    switch (c->NB) {
+        case 1: return means32_t<1>(c);
+        case 2: return means32_t<2>(c);
+        case 3: return means32_t<3>(c);
+        case 4: return means32_t<4>(c);
+        case 5: return means32_t<5>(c);
+        case 6: return means32_t<6>(c);
+        case 7: return means32_t<7>(c);
+        case 8: return means32_t<8>(c);
+    }
+    return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------
+// Stage (c), FP32 products: T = ((x - mu_hi) - mu_lo) * sqrt(r) / sqrt(N_k) in
+// FP32, D += T_ib^T T_jb on FP32 MFMA, accumulators flushed into FP64 every
+// 2048 points.  Same grid and XCD remapping as k_cov64.
+template <int NB>
+__global__ __launch_bounds__(256) void k_cov32(const float *__restrict__ X, const float *__restrict__ Rt,
+                                               int64_t npad, int K, int KP, int d,
+                                               const double *__restrict__ mu,
+                                               const double *__restrict__ nk, int64_t chunk_rows,
+                                               double *__restrict__ cpart) {
+    constexpr int DP = 16 * NB;
+    constexpr int UB = NB * (NB + 1) / 2;
+    constexpr int FLUSH = 2048;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int g = lane >> 4, c = lane & 15;
+    const int KG = KP / 4;
+    const int nblk = gridDim.x;
+    const int b = blockIdx.x;
+    const int logical = (nblk % 8 == 0) ? (b % 8) * (nblk / 8) + b / 8 : b;
+    const int cg = logical % KG, chunk = logical / KG;
+    const int k = 4 * cg + wave;
+    if (k >= K) return;
+    const int64_t i0 = (int64_t)chunk * chunk_rows;
+    const int64_t i1 = (i0 + chunk_rows < npad) ? i0 + chunk_rows : npad;
+    float mh[NB], ml[NB];
+#pragma unroll
+    for (int fb = 0; fb < NB; fb++) {
+        const double m = (16 * fb + c < d) ? mu[(size_t)k * d + 16 * fb + c] : 0.0;
+        mh[fb] = (float)m;
+        ml[fb] = (float)(m - (double)mh[fb]);
+    }
+    const float inv = (float)(1.0 / sqrt(nk[k]));
+    // Each wave owns its partial slice, so flushes are plain read-add-writes.
+    double *out = cpart + ((size_t)chunk * KP + k) * UB * 256;
+    const float *rrow = Rt + (size_t)k * npad;
+    bool first = true;
+    for (int64_t s0 = i0; s0 < i1; s0 += FLUSH) {
+        const int64_t s1 = (s0 + FLUSH < i1) ? s0 + FLUSH : i1;
+        v4f acc[UB];
+#pragma unroll
+        for (int u = 0; u < UB; u++) acc[u] = (v4f){0.f, 0.f, 0.f, 0.f};
+        for (int64_t i = s0; i < s1; i += 4) {
+            const float sw = sqrtf(rrow[i + g]) * inv;
+            float t[NB];
+#pragma unroll
+            for (int fb = 0; fb < NB; fb++) t[fb] = ((X[(i + g) * DP + 16 * fb + c] - mh[fb]) - ml[fb]) * sw;
+            int u = 0;
+#pragma unroll
+            for (int ib = 0; ib < NB; ib++)
+#pragma unroll
+                for (int jb = ib; jb < NB; jb++) {
+                    acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(t[ib], t[jb], acc[u], 0, 0, 0);
+                    u++;
+                }
+        }
+#pragma unroll
+        for (int u = 0; u < UB; u++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                double *o = out + (size_t)u * 256 + (4 * g + v) * 16 + c;
+                *o = (first ? 0.0 : *o) + (double)acc[u][v];
+            }
+        first = false;
+    }
+}
+
+template <int NB>
+static hipError_t cov32_t(fem_ctx *c) {
+    const int blocks = (c->KP / 4) * c->c_chunks;
+    hipLaunchKernelGGL(k_cov32<NB>, dim3(blocks), dim3(256), 0, c->stream, c->X, c->Rt32, c->npad, c->K,
+                       c->KP, c->d, c->mu, c->nk, c->c_chunk_rows, c->cpart);
+    return hipGetLastError();
+}
+
+hipError_t launch_cov32_partials(fem_ctx *c) {
+    switch (c->NB) {
+        case 1: return cov32_t<1>(c);
+        case 2: return cov32_t<2>(c);
+        case 3: return cov32_t<3>(c);
+        case 4: return cov32_t<4>(c);
+        case 5: return cov32_t<5>(c);
+        case 6: return cov32_t<6>(c);
+        case 7: return cov32_t<7>(c);
+        case 8: return cov32_t<8>(c);
+    }
+    return hipErrorInvalidValue;
+}
+
+__global__ void k_transpose_post32(const float *__restrict__ Rt, int64_t npad, int64_t n, int K,
+                                   double *__restrict__ out, int64_t ld) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int k = 0; k < K; k++) out[i * ld + k] = (double)Rt[(size_t)k * npad + i];
+}
+
+hipError_t launch_transpose_post32(const fem_ctx *c, double *out_dev, int64_t ld) {
+    hipLaunchKernelGGL(k_transpose_post32, dim3((unsigned)((c->n + 255) / 256)), dim3(256), 0, c->stream,
+                       c->Rt32, c->npad, c->n, c->K, out_dev, ld);
+    return hipGetLastError();
+}
+
+}  // namespace fem
