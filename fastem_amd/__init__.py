@@ -45,6 +45,10 @@ _SIGS = {
     "fem_mstep": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
     "fem_get_posteriors": (ctypes.c_int, [_vp, _dp, _i64]),
     "fem_last_times": (ctypes.c_int, [_vp, _fp, _fp, _fp, _fp]),
+    "fem_kmeans_init": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, _dp, _dp,
+                                       _dp]),
+    "fem_kmeans_init_rng": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _dp, _dp,
+                                           _dp]),
     "fem_last_error": (ctypes.c_char_p, [_vp]),
     "fem_version": (ctypes.c_char_p, []),
     "fem_gengmm": (ctypes.c_int, [_i64, ctypes.c_int, ctypes.c_int, ctypes.c_double,
@@ -168,6 +172,16 @@ class Context:
         self._check(lib().fem_get_posteriors(self._h, _d(R), self.K))
         return R
 
+    def kmeans_init(self, seed, seeding=1, max_sweeps=29):
+        """k-means++ (seeding=1) or Forgy (0) seeding, Lloyd sweeps and the
+        mixture of the clusters on the device data; becomes the parameters."""
+        w = np.empty(self.K)
+        mu = np.empty((self.K, self.d))
+        cov = np.empty((self.K, self.d, self.d))
+        self._check(lib().fem_kmeans_init(self._h, seed, seeding, max_sweeps, _d(w), _d(mu),
+                                          _d(cov)))
+        return w, mu, cov
+
     def times(self):
         t = [ctypes.c_float() for _ in range(4)]
         self._check(lib().fem_last_times(self._h, *[ctypes.byref(x) for x in t]))
@@ -231,7 +245,7 @@ def gengmm(n, d, K, mean_lo, mean_hi, diag, seed, labels=False):
 
 def kmeanspp_init(X, K, seed, lloyd_iters=29):
     """k-means++ seeding, up to lloyd_iters Lloyd sweeps, mixture from the
-    clusters (include/fastem_tools.h)."""
+    clusters, on a temporary device context (include/fastem_tools.h)."""
     X = np.ascontiguousarray(X, dtype=np.float32)
     n, d = X.shape
     w = np.empty(K)

@@ -68,6 +68,13 @@ struct fem_ctx {
     double *mpart = nullptr;    // m_chunks x KP x (dpad+1)
     double *cpart = nullptr;    // c_chunks x KP x ublocks x 256
 
+    // k-means scratch (allocated on first use)
+    double *kC = nullptr;       // K x d centroids
+    double *dmin = nullptr;     // npad  k-means++ closest squared distances
+    int *cls = nullptr;         // npad  cluster of each point
+    double *kfit = nullptr;     // per-workgroup fit partials
+    bool owns_rt64 = false;     // Rt allocated for k-means in a non-FP64 context
+
     hipEvent_t ev[8] = {};
     float t_estep = 0, t_means = 0, t_cov = 0, t_factor = 0;
     std::string err;
@@ -80,6 +87,10 @@ hipError_t launch_means(fem_ctx *c);
 hipError_t launch_cov(fem_ctx *c);
 hipError_t launch_zero(double *p, int64_t n, hipStream_t s);
 hipError_t launch_transpose_post(const fem_ctx *c, double *out_dev, int64_t ld);
+hipError_t launch_means64_partials(fem_ctx *c);
+hipError_t launch_means_finalize(fem_ctx *c, double *mu_out, double *nk_out, int raw);
+hipError_t launch_cov64_partials(fem_ctx *c);
+hipError_t launch_cov_finalize(fem_ctx *c, double g);
 // FP32 form (fem_kernels32.hip)
 int estep32_points_per_block(int NB);
 hipError_t launch_estep32(fem_ctx *c);

@@ -338,10 +338,11 @@ __global__ __launch_bounds__(256) void k_means64(const float *__restrict__ X,
 }
 
 // mu_k = M_k / N_k.  One workgroup per component; fixed-order chunk sums.
+// raw != 0 writes the sums M_k themselves (k-means centroid sums).
 __global__ __launch_bounds__(256) void k_means_finalize(int K, int d, int dpad, int KP, int chunks,
                                                         const double *__restrict__ mpart,
                                                         double *__restrict__ mu,
-                                                        double *__restrict__ nk) {
+                                                        double *__restrict__ nk, int raw) {
     __shared__ double sn;
     const int k = blockIdx.x, tid = threadIdx.x;
     if (tid == 0) {
@@ -354,7 +355,7 @@ __global__ __launch_bounds__(256) void k_means_finalize(int K, int d, int dpad, 
     for (int j = tid; j < d; j += blockDim.x) {
         double s = 0.0;
         for (int ch = 0; ch < chunks; ch++) s += mpart[((size_t)ch * KP + k) * (dpad + 1) + j];
-        mu[(size_t)k * d + j] = s / sn;
+        mu[(size_t)k * d + j] = raw ? s : s / sn;
     }
 }
 
@@ -365,7 +366,7 @@ static hipError_t means_t(fem_ctx *c) {
     return hipGetLastError();
 }
 
-static hipError_t means64_partials(fem_ctx *c) {
+hipError_t launch_means64_partials(fem_ctx *c) {
     switch (c->NB) {
         case 1: return means_t<1>(c);
         case 2: return means_t<2>(c);
@@ -379,12 +380,16 @@ static hipError_t means64_partials(fem_ctx *c) {
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_means(fem_ctx *c) {
-    hipError_t e = (c->prec == FEM_PREC_FP64) ? means64_partials(c) : launch_means32_partials(c);
-    if (e != hipSuccess) return e;
+hipError_t launch_means_finalize(fem_ctx *c, double *mu_out, double *nk_out, int raw) {
     hipLaunchKernelGGL(k_means_finalize, dim3(c->K), dim3(256), 0, c->stream, c->K, c->d, c->dpad,
-                       c->KP, c->m_chunks, c->mpart, c->mu, c->nk);
+                       c->KP, c->m_chunks, c->mpart, mu_out, nk_out, raw);
     return hipGetLastError();
+}
+
+hipError_t launch_means(fem_ctx *c) {
+    hipError_t e = (c->prec == FEM_PREC_FP64) ? launch_means64_partials(c) : launch_means32_partials(c);
+    if (e != hipSuccess) return e;
+    return launch_means_finalize(c, c->mu, c->nk, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -489,24 +494,30 @@ static hipError_t cov_t(fem_ctx *c) {
     return hipGetLastError();
 }
 
-hipError_t launch_cov(fem_ctx *c) {
-    hipError_t e = (c->prec == FEM_PREC_FP64) ? [&]() -> hipError_t {
-        switch (c->NB) {
-            case 1: return cov_t<1>(c);
-            case 2: return cov_t<2>(c);
-            case 3: return cov_t<3>(c);
-            case 4: return cov_t<4>(c);
-            case 5: return cov_t<5>(c);
-            case 6: return cov_t<6>(c);
-            case 7: return cov_t<7>(c);
-            case 8: return cov_t<8>(c);
-        }
-        return hipErrorInvalidValue;
-    }() : launch_cov32_partials(c);
-    if (e != hipSuccess) return e;
+hipError_t launch_cov64_partials(fem_ctx *c) {
+    switch (c->NB) {
+        case 1: return cov_t<1>(c);
+        case 2: return cov_t<2>(c);
+        case 3: return cov_t<3>(c);
+        case 4: return cov_t<4>(c);
+        case 5: return cov_t<5>(c);
+        case 6: return cov_t<6>(c);
+        case 7: return cov_t<7>(c);
+        case 8: return cov_t<8>(c);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_cov_finalize(fem_ctx *c, double g) {
     hipLaunchKernelGGL(k_cov_finalize, dim3(c->K), dim3(256), 0, c->stream, c->K, c->d, c->NB,
-                       c->KP, c->c_chunks, c->cpart, c->nk, c->g, (double)c->ntotal, c->cov, c->w);
+                       c->KP, c->c_chunks, c->cpart, c->nk, g, (double)c->ntotal, c->cov, c->w);
     return hipGetLastError();
+}
+
+hipError_t launch_cov(fem_ctx *c) {
+    hipError_t e = (c->prec == FEM_PREC_FP64) ? launch_cov64_partials(c) : launch_cov32_partials(c);
+    if (e != hipSuccess) return e;
+    return launch_cov_finalize(c, c->g);
 }
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,274 @@
+// fem_kmeans.hip -- k-means initialisation on the device data of a context:
+// k-means++ seeding (Clust/PlusPlusInitializer/PlusPlusInitializer.cpp:33-84),
+// Lloyd sweeps (Clust/KMAlgorithm.cpp:44-87,205-255) and the mixture of the
+// final clustering (EM2/EMInitializer.cpp:103-134).
+//
+// Squared distances are FP64 sums in ascending feature order with no
+// multiply-add contraction (double centroid minus float row widened), the
+// argmin keeps the lowest index on ties, and the D^2 prefix sum and draw of
+// k-means++ run on the host in sequential FP64 order, so seeds and assignment
+// counts equal the FP64 oracle's on inputs without near-ties.  Centroid sums
+// and counts come from the MFMA means pass applied to the one-hot assignment
+// (products by 1.0 are exact; sums in FP64, fixed order).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "fem_internal.h"
+
+namespace fem {
+
+#pragma clang fp contract(off)
+__device__ inline double sqdist_nc(const double *c, const float *x, int d) {
+    double s = 0.0;
+    for (int j = 0; j < d; j++) {
+        const double t = c[j] - (double)x[j];
+        s = s + t * t;
+    }
+    return s;
+}
+
+__global__ __launch_bounds__(256) void k_kpp_update(const float *__restrict__ X, int64_t n, int dpad, int d,
+                                                    const double *__restrict__ cen, double *__restrict__ dmin,
+                                                    int reset) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double s = sqdist_nc(cen, X + i * dpad, d);
+    if (reset || dmin[i] > s) dmin[i] = s;
+}
+
+// One point per thread; centroids staged in LDS.  Writes the one-hot
+// assignment into the component-major Rt (all KP rows), cls[i], and a
+// fixed-order per-workgroup sum of the minimum squared distances.
+__global__ __launch_bounds__(256) void k_assign(const float *__restrict__ X, int64_t n, int64_t npad, int dpad,
+                                                int d, int K, int KP, const double *__restrict__ C,
+                                                double *__restrict__ Rt, int *__restrict__ cls,
+                                                double *__restrict__ fit_part) {
+    extern __shared__ double lc[];  // K * d
+    __shared__ double red[256];
+    for (int e = threadIdx.x; e < K * d; e += 256) lc[e] = C[e];
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    double best = 0.0;
+    if (i < npad) {
+        int bj = -1;
+        if (i < n) {
+            best = std::numeric_limits<double>::max();
+            const float *x = X + i * dpad;
+            for (int k = 0; k < K; k++) {
+                const double s = sqdist_nc(lc + (size_t)k * d, x, d);
+                if (s < best) {
+                    best = s;
+                    bj = k;
+                }
+            }
+            cls[i] = bj;
+        }
+        for (int k = 0; k < KP; k++) Rt[(size_t)k * npad + i] = (k == bj) ? 1.0 : 0.0;
+    }
+    red[threadIdx.x] = (i < n) ? best : 0.0;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) fit_part[blockIdx.x] = red[0];
+}
+#pragma clang fp contract(on)
+
+}  // namespace fem
+
+namespace {
+
+inline double canonical(std::mt19937_64 &g) {
+    const double r = (double)g() / 18446744073709551616.0;
+    return r < 1.0 ? r : std::nextafter(1.0, 0.0);
+}
+
+struct KmErr {
+    int code;
+    std::string msg;
+};
+
+#define KM_HIP(call)                                                                       \
+    do {                                                                                   \
+        hipError_t _e = (call);                                                            \
+        if (_e != hipSuccess) throw KmErr{FEM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(_e)}; \
+    } while (0)
+
+void ensure_scratch(fem_ctx *c) {
+    if (!c->kC) KM_HIP(hipMalloc((void **)&c->kC, sizeof(double) * c->K * c->d));
+    if (!c->dmin) KM_HIP(hipMalloc((void **)&c->dmin, sizeof(double) * c->npad));
+    if (!c->cls) KM_HIP(hipMalloc((void **)&c->cls, sizeof(int) * c->npad));
+    if (!c->kfit) KM_HIP(hipMalloc((void **)&c->kfit, sizeof(double) * (c->npad / 256)));
+    if (!c->Rt) {
+        KM_HIP(hipMalloc((void **)&c->Rt, sizeof(double) * c->KP * c->npad));
+        c->owns_rt64 = true;
+    }
+}
+
+void fetch_row(fem_ctx *c, int64_t i, double *dst) {
+    std::vector<float> row(c->d);
+    KM_HIP(hipMemcpyAsync(row.data(), c->X + i * c->dpad, sizeof(float) * c->d, hipMemcpyDeviceToHost,
+                          c->stream));
+    KM_HIP(hipStreamSynchronize(c->stream));
+    for (int j = 0; j < c->d; j++) dst[j] = (double)row[j];
+}
+
+// assignment with centroids C (host, K x d): returns the mean squared
+// distance; leaves the one-hot in Rt and counts/sums computed when asked.
+double assign(fem_ctx *c, const std::vector<double> &C, std::vector<double> *sums, std::vector<double> *counts) {
+    KM_HIP(hipMemcpyAsync(c->kC, C.data(), sizeof(double) * c->K * c->d, hipMemcpyHostToDevice, c->stream));
+    const int blocks = (int)(c->npad / 256);
+    const size_t lds = sizeof(double) * c->K * c->d;
+    if (lds > 64 * 1024)
+        KM_HIP(hipFuncSetAttribute((const void *)fem::k_assign, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds));
+    hipLaunchKernelGGL(fem::k_assign, dim3(blocks), dim3(256), lds, c->stream, c->X, c->n, c->npad, c->dpad,
+                       c->d, c->K, c->KP, c->kC, c->Rt, c->cls, c->kfit);
+    KM_HIP(hipGetLastError());
+    std::vector<double> fp(blocks);
+    KM_HIP(hipMemcpyAsync(fp.data(), c->kfit, sizeof(double) * blocks, hipMemcpyDeviceToHost, c->stream));
+    if (sums) {
+        KM_HIP(fem::launch_means64_partials(c));
+        KM_HIP(fem::launch_means_finalize(c, c->kC, c->nk, 1));
+        KM_HIP(hipMemcpyAsync(sums->data(), c->kC, sizeof(double) * c->K * c->d, hipMemcpyDeviceToHost,
+                              c->stream));
+        KM_HIP(hipMemcpyAsync(counts->data(), c->nk, sizeof(double) * c->K, hipMemcpyDeviceToHost, c->stream));
+    }
+    KM_HIP(hipStreamSynchronize(c->stream));
+    double fit = 0.0;
+    for (double v : fp) fit += v;
+    return fit / (double)c->n;
+}
+
+}  // namespace
+
+extern "C" int fem_kmeans_init_rng(fem_ctx *c, double (*rnd)(void *), void *rnd_state, int seeding,
+                                   int max_sweeps, double *w, double *mu, double *cov);
+
+static double mt_rand(void *p) { return canonical(*static_cast<std::mt19937_64 *>(p)); }
+
+extern "C" int fem_kmeans_init(fem_ctx *c, uint64_t seed, int seeding, int max_sweeps, double *w, double *mu,
+                               double *cov) {
+    std::mt19937_64 rng(seed);
+    return fem_kmeans_init_rng(c, mt_rand, &rng, seeding, max_sweeps, w, mu, cov);
+}
+
+extern "C" int fem_kmeans_init_rng(fem_ctx *c, double (*rnd)(void *), void *rnd_state, int seeding,
+                                   int max_sweeps, double *w, double *mu, double *cov) {
+    if (!c) return FEM_ERR_ARG;
+    if (!rnd || !w || !mu || !cov || max_sweeps < 0 || (seeding != 0 && seeding != 1)) {
+        c->err = "fem_kmeans_init: bad argument";
+        return FEM_ERR_ARG;
+    }
+    try {
+        KM_HIP(hipSetDevice(c->device));
+        ensure_scratch(c);
+        const int K = c->K, d = c->d;
+        const int64_t n = c->n;
+        auto draw = [&]() { return rnd(rnd_state); };
+        std::vector<double> C((size_t)K * d, 0.0);
+        if (seeding == 1) {
+            // Clust/PlusPlusInitializer/PlusPlusInitializer.cpp:33-84
+            const int64_t r0 = (int64_t)(draw() * (double)n);
+            fetch_row(c, r0, C.data());
+            std::vector<double> dm(n), agg(n);
+            const int blocks = (int)((n + 255) / 256);
+            for (int k = 1; k < K; k++) {
+                KM_HIP(hipMemcpyAsync(c->kC, C.data() + (size_t)(k - 1) * d, sizeof(double) * d,
+                                      hipMemcpyHostToDevice, c->stream));
+                hipLaunchKernelGGL(fem::k_kpp_update, dim3(blocks), dim3(256), 0, c->stream, c->X, n, c->dpad, d,
+                                   c->kC, c->dmin, k == 1 ? 1 : 0);
+                KM_HIP(hipGetLastError());
+                KM_HIP(hipMemcpyAsync(dm.data(), c->dmin, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+                KM_HIP(hipStreamSynchronize(c->stream));
+                double sum = 0.0;
+                for (int64_t i = 0; i < n; i++) {
+                    sum += dm[i] * 1.0;
+                    agg[i] = sum;
+                }
+                for (int64_t i = 0; i < n; i++) agg[i] /= sum;
+                const double r = draw();
+                for (int64_t i = 0; i < n; i++)
+                    if (r <= agg[i]) {
+                        fetch_row(c, i, C.data() + (size_t)k * d);
+                        break;
+                    }
+            }
+        } else {
+            // Forgy random partition (Clust/KMeansInitializer.cpp:33-49): one
+            // Rand() per row in row order; partition sums by the means pass.
+            std::vector<int> part(n);
+            for (int64_t i = 0; i < n; i++) part[i] = (int)(draw() * K);
+            std::vector<double> onehot_col(c->npad);
+            for (int k = 0; k < c->KP; k++) {
+                for (int64_t i = 0; i < c->npad; i++) onehot_col[i] = (i < n && part[i] == k) ? 1.0 : 0.0;
+                KM_HIP(hipMemcpyAsync(c->Rt + (size_t)k * c->npad, onehot_col.data(), sizeof(double) * c->npad,
+                                      hipMemcpyHostToDevice, c->stream));
+                KM_HIP(hipStreamSynchronize(c->stream));
+            }
+            std::vector<double> sums((size_t)K * d), counts(K);
+            KM_HIP(fem::launch_means64_partials(c));
+            KM_HIP(fem::launch_means_finalize(c, c->kC, c->nk, 1));
+            KM_HIP(hipMemcpyAsync(sums.data(), c->kC, sizeof(double) * K * d, hipMemcpyDeviceToHost, c->stream));
+            KM_HIP(hipMemcpyAsync(counts.data(), c->nk, sizeof(double) * K, hipMemcpyDeviceToHost, c->stream));
+            KM_HIP(hipStreamSynchronize(c->stream));
+            for (int k = 0; k < K; k++) {
+                const double f = 1.0 / counts[k];
+                for (int j = 0; j < d; j++) C[(size_t)k * d + j] = sums[(size_t)k * d + j] * f;
+            }
+        }
+        // Lloyd sweeps (NaiveKMA::RunKMeans, MinRel 1e-4)
+        double best = std::numeric_limits<double>::max();
+        std::vector<double> sums((size_t)K * d), counts(K);
+        for (int it = 1; it <= max_sweeps; it++) {
+            const double fit = assign(c, C, &sums, &counts);
+            for (int k = 0; k < K; k++) {
+                if (counts[k] > 0) {
+                    const double f = 1.0 / counts[k];
+                    for (int j = 0; j < d; j++) C[(size_t)k * d + j] = sums[(size_t)k * d + j] * f;
+                } else {
+                    const int64_t src = (int64_t)(draw() * (double)n);
+                    fetch_row(c, src, C.data() + (size_t)k * d);
+                }
+            }
+            const double rel = (best - fit) / best;
+            if (fit < std::nextafter(fit, best)) best = fit;
+            else break;
+            if (rel < 1e-4) break;
+            if (it == max_sweeps) break;
+        }
+        // mixture of the final clustering: counts, then covariances about the
+        // centroids divided by (n_k - 1), weights n_k / n
+        assign(c, C, &sums, &counts);
+        int rc = FEM_OK;
+        std::vector<double> nk1(K);
+        for (int k = 0; k < K; k++) {
+            nk1[k] = counts[k] - 1.0;
+            if (counts[k] < 2) rc = FEM_ERR_ARG;
+        }
+        KM_HIP(hipMemcpyAsync(c->mu, C.data(), sizeof(double) * K * d, hipMemcpyHostToDevice, c->stream));
+        KM_HIP(hipMemcpyAsync(c->nk, nk1.data(), sizeof(double) * K, hipMemcpyHostToDevice, c->stream));
+        KM_HIP(fem::launch_cov64_partials(c));
+        KM_HIP(fem::launch_cov_finalize(c, 0.0));
+        for (int k = 0; k < K; k++) w[k] = counts[k] / (double)n;
+        KM_HIP(hipMemcpyAsync(c->w, w, sizeof(double) * K, hipMemcpyHostToDevice, c->stream));
+        KM_HIP(hipMemcpyAsync(cov, c->cov, sizeof(double) * K * d * d, hipMemcpyDeviceToHost, c->stream));
+        KM_HIP(hipStreamSynchronize(c->stream));
+        std::memcpy(mu, C.data(), sizeof(double) * K * d);
+        c->have_params = true;
+        c->factored = false;
+        c->have_post = false;
+        if (rc != FEM_OK) c->err = "k-means left a cluster with fewer than two points";
+        return rc;
+    } catch (KmErr &e) {
+        c->err = e.msg;
+        return e.code;
+    }
+}

@@ -1,5 +1,5 @@
-// fem_tools.cpp -- synthetic mixture generator and k-means++ initializer
-// (include/fastem_tools.h).  Host code, OpenMP.
+// fem_tools.cpp -- synthetic mixture generator and the k-means++ initializer
+// entry on host data (include/fastem_tools.h).  Host code, OpenMP.
 #include <omp.h>
 
 #include <cmath>
@@ -9,6 +9,7 @@
 #include <random>
 #include <vector>
 
+#include "../../include/fastem_abi.h"
 #include "../../include/fastem_tools.h"
 
 namespace {
@@ -50,23 +51,6 @@ bool chol_lower(int d, const double *C, double *L) {
         }
     }
     return true;
-}
-
-inline double sqdist(int d, const double *c, const float *x) {
-    double s = 0.0;
-    for (int j = 0; j < d; j++) {
-        const double t = c[j] - (double)x[j];
-        s += t * t;
-    }
-    return s;
-}
-
-constexpr int kParts = 64;  // fixed virtual partition for deterministic sums
-
-inline void part_range(int64_t n, int p, int64_t &b, int64_t &e) {
-    const int64_t q = n / kParts, r = n % kParts;
-    b = p * q + (p < r ? p : r);
-    e = b + q + (p < r ? 1 : 0);
 }
 
 }  // namespace
@@ -131,151 +115,16 @@ extern "C" int fem_gengmm(int64_t n, int d, int K, double mean_lo, double mean_h
     return 0;
 }
 
-namespace fem_host {
-
-// Seeding (Forgy random partition, Clust/KMeansInitializer.cpp:33-49, or
-// k-means++), up to max_sweeps Lloyd sweeps (Clust/KMAlgorithm.cpp:44-87 with
-// MinRel 1e-4; the reference's MaxIter = max_sweeps + 1), then the mixture of
-// the final clustering (EM2/EMInitializer.cpp:103-134).  rnd(rs) is the
-// reference's Rand().
-int kmeans_mixture(int64_t n, int d, int K, const float *X, double (*rnd)(void *), void *rs,
-                   int seeding, int max_sweeps, double *w, double *mu, double *cov) {
-    if (n < 1 || d < 1 || K < 1 || !X || !w || !mu || !cov) return -1;
-    std::vector<double> C((size_t)K * d);
-    if (seeding == 0) {
-        std::vector<int64_t> cnt(K, 0);
-        for (int64_t i = 0; i < n; i++) {
-            const int part = (int)(rnd(rs) * K);
-            cnt[part]++;
-            for (int j = 0; j < d; j++) C[(size_t)part * d + j] += X[(size_t)i * d + j];
-        }
-        for (int k = 0; k < K; k++) {
-            const double f = 1.0 / (double)cnt[k];
-            for (int j = 0; j < d; j++) C[(size_t)k * d + j] *= f;
-        }
-    } else {
-    std::vector<double> cd(n, std::numeric_limits<double>::infinity()), agg(n);
-    // Clust/PlusPlusInitializer/PlusPlusInitializer.cpp:33-84
-    const int64_t r0 = (int64_t)(rnd(rs) * (double)n);
-    for (int j = 0; j < d; j++) C[j] = (double)X[(size_t)r0 * d + j];
-    for (int c = 1; c < K; c++) {
-        const double *pc = C.data() + (size_t)(c - 1) * d;
-#pragma omp parallel for schedule(static)
-        for (int64_t i = 0; i < n; i++) {
-            const double s = sqdist(d, pc, X + (size_t)i * d);
-            if (cd[i] > s) cd[i] = s;
-        }
-        double sum = 0.0;
-        for (int64_t i = 0; i < n; i++) {
-            sum += cd[i] * 1.0;
-            agg[i] = sum;
-        }
-#pragma omp parallel for schedule(static)
-        for (int64_t i = 0; i < n; i++) agg[i] /= sum;
-        const double r = rnd(rs);
-        for (int64_t i = 0; i < n; i++)
-            if (r <= agg[i]) {
-                for (int j = 0; j < d; j++) C[(size_t)c * d + j] = (double)X[(size_t)i * d + j];
-                break;
-            }
-    }
-    }
-    std::vector<int> cls(n);
-    std::vector<double> psum((size_t)kParts * K * d);
-    std::vector<int64_t> pcnt((size_t)kParts * K);
-    std::vector<double> pfit(kParts);
-    auto assign = [&](bool accumulate) {
-        std::fill(psum.begin(), psum.end(), 0.0);
-        std::fill(pcnt.begin(), pcnt.end(), 0);
-#pragma omp parallel for schedule(dynamic, 1)
-        for (int p = 0; p < kParts; p++) {
-            int64_t b, e;
-            part_range(n, p, b, e);
-            double f = 0.0;
-            for (int64_t i = b; i < e; i++) {
-                const float *x = X + (size_t)i * d;
-                double best = std::numeric_limits<double>::max();
-                int bj = -1;
-                for (int k = 0; k < K; k++) {
-                    const double s = sqdist(d, C.data() + (size_t)k * d, x);
-                    if (s < best) { best = s; bj = k; }
-                }
-                cls[i] = bj;
-                f += best;
-                if (accumulate) {
-                    pcnt[(size_t)p * K + bj]++;
-                    double *ce = psum.data() + ((size_t)p * K + bj) * d;
-                    for (int j = 0; j < d; j++) ce[j] += (double)x[j];
-                }
-            }
-            pfit[p] = f;
-        }
-        double fit = 0.0;
-        for (int p = 0; p < kParts; p++) fit += pfit[p];
-        return fit / (double)n;
-    };
-    // Clust/KMAlgorithm.cpp:44-87,205-255 (empty clusters repaired from Rand())
-    double best = std::numeric_limits<double>::max();
-    for (int it = 1; it <= max_sweeps; it++) {
-        const double fit = assign(true);
-        for (int p = 1; p < kParts; p++) {
-            for (size_t q = 0; q < (size_t)K * d; q++) psum[q] += psum[(size_t)p * K * d + q];
-            for (int k = 0; k < K; k++) pcnt[k] += pcnt[(size_t)p * K + k];
-        }
-        for (int k = 0; k < K; k++) {
-            if (pcnt[k] > 0) {
-                const double f = 1.0 / (double)pcnt[k];
-                for (int j = 0; j < d; j++) C[(size_t)k * d + j] = psum[(size_t)k * d + j] * f;
-            } else {
-                const int64_t src = (int64_t)(rnd(rs) * (double)n);
-                for (int j = 0; j < d; j++) C[(size_t)k * d + j] = (double)X[(size_t)src * d + j];
-            }
-        }
-        const double rel = (best - fit) / best;
-        if (fit < std::nextafter(fit, best)) best = fit;
-        else break;
-        if (rel < 1e-4) break;
-        if (it == max_sweeps) break;
-    }
-    assign(false);
-    // EM2/EMInitializer.cpp:103-134
-    std::vector<double> pc((size_t)kParts * K * d * d, 0.0);
-    std::fill(pcnt.begin(), pcnt.end(), 0);
-#pragma omp parallel for schedule(dynamic, 1)
-    for (int p = 0; p < kParts; p++) {
-        int64_t b, e;
-        part_range(n, p, b, e);
-        std::vector<double> row(d);
-        for (int64_t i = b; i < e; i++) {
-            const int c = cls[i];
-            pcnt[(size_t)p * K + c]++;
-            for (int j = 0; j < d; j++) row[j] = (double)X[(size_t)i * d + j] - C[(size_t)c * d + j];
-            double *S = pc.data() + ((size_t)p * K + c) * d * d;
-            for (int a = 0; a < d; a++)
-                for (int bb = 0; bb < d; bb++) S[(size_t)a * d + bb] += row[a] * row[bb];
-        }
-    }
-    for (int p = 1; p < kParts; p++) {
-        for (size_t q = 0; q < (size_t)K * d * d; q++) pc[q] += pc[(size_t)p * K * d * d + q];
-        for (int k = 0; k < K; k++) pcnt[k] += pcnt[(size_t)p * K + k];
-    }
-    int rc = 0;
-    for (int k = 0; k < K; k++) {
-        if (pcnt[k] < 2) rc = -1;
-        const double f = 1.0 / ((double)pcnt[k] - 1.0);
-        for (size_t q = 0; q < (size_t)d * d; q++) cov[(size_t)k * d * d + q] = pc[(size_t)k * d * d + q] * f;
-        w[k] = (double)pcnt[k] / (double)n;
-        for (int j = 0; j < d; j++) mu[(size_t)k * d + j] = C[(size_t)k * d + j];
-    }
-    return rc;
-}
-
-}  // namespace fem_host
-
-static double mt_canonical(void *p) { return canonical(*static_cast<std::mt19937_64 *>(p)); }
-
+// k-means++ initializer on host data: a temporary FP64 context runs the
+// device k-means (fem_kmeans_init) on a copy of X.
 extern "C" int fem_kmeanspp_init(int64_t n, int d, int K, const float *X, uint64_t seed,
                                  int lloyd_iters, double *w, double *mu, double *cov) {
-    std::mt19937_64 rng(seed);
-    return fem_host::kmeans_mixture(n, d, K, X, mt_canonical, &rng, 1, lloyd_iters, w, mu, cov);
+    if (n < 1 || d < 1 || K < 1 || !X || !w || !mu || !cov || lloyd_iters < 0) return FEM_ERR_ARG;
+    fem_ctx *c = nullptr;
+    int rc = fem_create(&c, n, d, K, FEM_PREC_FP64, 0);
+    if (rc != FEM_OK) return rc;
+    rc = fem_upload_data(c, X, d);
+    if (rc == FEM_OK) rc = fem_kmeans_init(c, seed, 1, lloyd_iters, w, mu, cov);
+    fem_destroy(c);
+    return rc;
 }

@@ -1,12 +1,13 @@
 // em_initializer.cpp -- EM2/EMInitializer.cpp:20-177 semantics.
 #include <cstring>
+#include <string>
 #include <stdexcept>
 #include <vector>
 
 #include "../../../include/fastem/EMException.h"
 #include "../../../include/fastem/EMInitializer.h"
 #include "../../../include/fastem/Rand.h"
-#include "../fem_host.h"
+#include "../../../include/fastem_abi.h"
 
 EMInitializer::EMInitializer(MultithreadedDataset &D, int nC)
     : ncols(D.GetColCount()), Components(nC), Means(nC, D.GetColCount()), Data(D) {
@@ -76,8 +77,15 @@ void KMeansEMInitializer::Init(GaussianMixture &G) {
     const int seeding = dynamic_cast<KMeansPPEMInitializer *>(this) ? 1 : 0;
     std::vector<double> w(Components), mu((size_t)Components * ncols),
         cov((size_t)Components * ncols * ncols);
-    fem_host::kmeans_mixture(Data.GetRowCount(), ncols, Components, Data.GetData(), global_rand, nullptr,
-                             seeding, 29, w.data(), mu.data(), cov.data());
+    fem_ctx *c = nullptr;
+    if (fem_create(&c, Data.GetRowCount(), ncols, Components, FEM_PREC_FP64, 0) != FEM_OK)
+        throw std::runtime_error(std::string("k-means initializer: ") + fem_last_error(nullptr));
+    int rc = fem_upload_data(c, Data.GetData(), ncols);
+    if (rc == FEM_OK)
+        rc = fem_kmeans_init_rng(c, global_rand, nullptr, seeding, 29, w.data(), mu.data(), cov.data());
+    const std::string msg = fem_last_error(c);
+    fem_destroy(c);
+    if (rc != FEM_OK && rc != FEM_ERR_ARG) throw std::runtime_error("k-means initializer: " + msg);
     for (int i = 0; i < Components; i++) {
         Weights[i] = w[i];
         for (int j = 0; j < ncols; j++) Means(i, j) = mu[(size_t)i * ncols + j];

@@ -98,6 +98,26 @@ int fem_mstep(fem_ctx *ctx, double *w, double *mu, double *cov);
  * (EM2/EMAlgorithm.h:41) read by RetrieveClasses (EM2/EMAlgorithm.cpp:95-110). */
 int fem_get_posteriors(fem_ctx *ctx, double *R, int64_t ld);
 
+/* k-means initialisation on the context's data, the step before the EM path:
+ * seeding (1 = k-means++, Clust/PlusPlusInitializer/PlusPlusInitializer.cpp:33-84;
+ * 0 = Forgy random partition, Clust/KMeansInitializer.cpp:33-49), up to
+ * max_sweeps Lloyd sweeps with MinRel 1e-4 (Clust/KMAlgorithm.cpp:44-87,205-255;
+ * the reference's MaxIter = max_sweeps + 1), then the mixture of the final
+ * clustering (EM2/EMInitializer.cpp:103-134: covariances about the centroids
+ * divided by n_k - 1, weights n_k / n).  Replaces KMeansEMInitializer::Init.
+ * Distances are FP64 in ascending feature order without multiply-add
+ * contraction, ties go to the lower index, the k-means++ prefix sum and draw
+ * are sequential FP64 on the host.  The random stream is
+ * std::mt19937_64(seed) through generate_canonical (Util/Rand.cpp:7-14), or
+ * rnd(rnd_state) for the _rng variant.  The result becomes the context's
+ * parameters and is copied to w[K], mu[K][d], cov[K][d][d].  Returns
+ * FEM_ERR_ARG (after writing the result) if a cluster has fewer than two
+ * points.  Overwrites the posteriors. */
+int fem_kmeans_init(fem_ctx *ctx, uint64_t seed, int seeding, int max_sweeps, double *w, double *mu,
+                    double *cov);
+int fem_kmeans_init_rng(fem_ctx *ctx, double (*rnd)(void *), void *rnd_state, int seeding,
+                        int max_sweeps, double *w, double *mu, double *cov);
+
 /* Device time in milliseconds of the last fem_estep / fem_mstep (means and
  * covariance passes) / fem_factor, from HIP events on the context's stream.
  * Any pointer may be NULL. */

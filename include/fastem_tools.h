@@ -21,14 +21,13 @@ extern "C" {
 int fem_gengmm(int64_t n, int d, int K, double mean_lo, double mean_hi, double diag,
                uint64_t seed, float *X, int *labels, double *w, double *mu, double *cov);
 
-/* k-means++ mixture initializer on host data.  Seeds K centres with the
- * D^2 rule of Clust/PlusPlusInitializer/PlusPlusInitializer.cpp:33-84 on the
- * reference's Rand() stream (std::mt19937_64(seed), sequential FP64 prefix
- * sums), runs `lloyd_iters` Lloyd sweeps with the stopping rule of
- * Clust/KMAlgorithm.cpp:44-87 (MinRel 1e-4; 0 sweeps = seeding only), then
- * builds the mixture as EM2/EMInitializer.cpp:103-134 does (per-cluster
- * covariance / (n_k - 1), weight n_k / n).  Returns 0, or -1 if a cluster
- * ends with fewer than two points. */
+/* k-means++ mixture initializer on host data: copies X to a temporary FP64
+ * device context and runs fem_kmeans_init (include/fastem_abi.h) with
+ * k-means++ seeding, std::mt19937_64(seed) as the Rand() stream and at most
+ * lloyd_iters Lloyd sweeps (the reference's KMeansEMInitializer uses MaxIter
+ * 30, i.e. 29 sweeps; 0 = seeding only).  Needs a GPU.  Returns 0, or a
+ * negative FEM_ERR_* code (FEM_ERR_ARG if a cluster ends with fewer than two
+ * points). */
 int fem_kmeanspp_init(int64_t n, int d, int K, const float *X, uint64_t seed, int lloyd_iters,
                       double *w, double *mu, double *cov);
 

@@ -63,25 +63,3 @@ def test_generator_fixed_bytes():
     h = hashlib.sha256(X.tobytes()).hexdigest()
     ref = open(os.path.join(ROOT, "tests", "golden", "gengmm_1000_4_3_7.sha256")).read().split()[0]
     assert h == ref
-
-
-def test_kmeanspp_init_matches_oracle(orc):
-    X = fem.gengmm(30000, 6, 5, -10.0, 10.0, 0.1, 9)[0]
-    w, mu, cov = fem.kmeanspp_init(X, 5, seed=3, lloyd_iters=29)
-    rng = orc.Rand(3)
-    C = orc.kmeanspp(X, 5, rng)
-    C, _, _ = orc.lloyd(X, C, rng, minrel=1e-4, maxiter=30)
-    cls = orc.classify(X, C)
-    w_ref, mu_ref, cov_ref = orc.gmm_from_clusters(X, C, cls)
-    np.testing.assert_array_equal(w, w_ref)  # counts are exact
-    np.testing.assert_allclose(mu, mu_ref, rtol=1e-12)
-    np.testing.assert_allclose(cov, cov_ref, rtol=1e-10)
-
-
-def test_kmeanspp_seeding_only(orc):
-    X = fem.gengmm(5000, 3, 4, -10.0, 10.0, 0.1, 5)[0]
-    w, mu, cov = fem.kmeanspp_init(X, 4, seed=8, lloyd_iters=0)
-    C = orc.kmeanspp(X, 4, orc.Rand(8))
-    np.testing.assert_array_equal(mu, C)
-    cls = orc.classify(X, C)
-    np.testing.assert_array_equal(w, np.bincount(cls, minlength=4) / 5000)
