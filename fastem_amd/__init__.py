@@ -37,6 +37,7 @@ _SIGS = {
     "fem_upload_data": (ctypes.c_int, [_vp, _fp, _i64]),
     "fem_upload_data_device": (ctypes.c_int, [_vp, _vp, _i64]),
     "fem_set_total_rows": (ctypes.c_int, [_vp, _i64]),
+    "fem_set_precision": (ctypes.c_int, [_vp, ctypes.c_int]),
     "fem_set_regularization": (ctypes.c_int, [_vp, ctypes.c_double]),
     "fem_set_params": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
     "fem_get_params": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
@@ -118,6 +119,10 @@ class Context:
 
     def upload_device(self, ptr, ld=None):
         self._check(lib().fem_upload_data_device(self._h, _vp(ptr), ld or self.d))
+
+    def set_precision(self, precision):
+        self._check(lib().fem_set_precision(self._h, precision))
+        self.precision = precision
 
     def set_total_rows(self, ntotal):
         self._check(lib().fem_set_total_rows(self._h, int(ntotal)))

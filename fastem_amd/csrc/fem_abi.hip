@@ -3,6 +3,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 
@@ -89,10 +90,8 @@ int fem_create(fem_ctx **out, int64_t n, int d, int K, int precision, int device
     c->npad = ((n + fem::kRowAlign - 1) / fem::kRowAlign) * fem::kRowAlign;
     const int npairs = fem::estep_pairs(c->NB);
     const int ub = fem::upper_blocks(c->NB);
-    const int P = (c->NB <= 4) ? 2 : 1;
-    c->est_blocks = (precision == FEM_PREC_FP64)
-                        ? (int)(c->npad / (64 * P))
-                        : (int)(c->npad / fem::estep32_points_per_block(c->NB));
+    c->est_blocks = (int)(c->npad / 64);  // >= workgroups of any E-step variant
+    if (const char *e = std::getenv("FASTEM_ESTEP64_GROUPS")) c->estep_groups = std::atoi(e) == 4 ? 4 : 2;
     c->m_chunks = (int)(c->npad / 4096);
     // covariance chunks: at most 256 chunks of a multiple of 4096 rows
     int64_t units = c->npad / 4096;
@@ -170,6 +169,33 @@ int fem_upload_data(fem_ctx *c, const float *X, int64_t ld) {
 
 int fem_upload_data_device(fem_ctx *c, const float *dX, int64_t ld) {
     return upload(c, dX, ld, hipMemcpyDeviceToDevice);
+}
+
+int fem_set_precision(fem_ctx *c, int precision) {
+    if (int r = check_ctx(c)) return r;
+    if (precision != FEM_PREC_FP64 && precision != FEM_PREC_FP32)
+        return fail(c, FEM_ERR_ARG, "precision not available in this build");
+    if (int r = set_device(c)) return r;
+    const size_t nR = (size_t)c->KP * c->npad;
+    if (precision == FEM_PREC_FP64 && !c->Rt) {
+        FEM_HIP(c, hipMalloc((void **)&c->Rt, nR * sizeof(double)));
+        FEM_HIP(c, hipMemsetAsync(c->Rt, 0, nR * sizeof(double), c->stream));
+    }
+    if (precision == FEM_PREC_FP32) {
+        if (!c->Rt32) {
+            FEM_HIP(c, hipMalloc((void **)&c->Rt32, nR * sizeof(float)));
+            FEM_HIP(c, hipMemsetAsync(c->Rt32, 0, nR * sizeof(float), c->stream));
+        }
+        if (!c->ufrag32)
+            FEM_HIP(c, hipMalloc((void **)&c->ufrag32,
+                                 sizeof(float) * c->K * fem::estep_pairs(c->NB) * 64));
+        if (!c->mus32) FEM_HIP(c, hipMalloc((void **)&c->mus32, sizeof(float) * c->K * 2 * c->dpad));
+    }
+    FEM_HIP(c, hipStreamSynchronize(c->stream));
+    c->prec = precision;
+    c->factored = false;
+    c->have_post = false;
+    return FEM_OK;
 }
 
 int fem_set_total_rows(fem_ctx *c, int64_t ntotal) {

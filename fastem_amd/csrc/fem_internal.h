@@ -60,7 +60,8 @@ struct fem_ctx {
     float *mus32 = nullptr;    // K x 2 x dpad     mu split hi/lo (FP32)
 
     // reductions
-    int est_blocks = 0;
+    int est_blocks = 0;         // capacity of ll_part (one per E-step workgroup)
+    int estep_groups = 2;       // 16-point groups per wave in the FP64 E-step (2 or 4)
     double *ll_part = nullptr;  // per E-step workgroup
     double *ll_dev = nullptr;   // 1
     int m_chunks = 0, c_chunks = 0;

@@ -147,9 +147,12 @@ hipError_t launch_factor(fem_ctx *c) {
 // ---------------------------------------------------------------------------
 // Stage (a), FP64.  Workgroup = 4 waves; each wave owns P groups of 16 points.
 // Lane (g = l/16, c = l%16) holds, for point c of each group, features
-// 4*lb + g (lb = 0..4NB-1) widened to FP64: the B operand of every MFMA.
-// For component k the A operands (fragments of U_k^T) and m_k = U_k^T mu_k
-// are staged in LDS; accumulators start at -m_k so that D = U^T (x - mu).
+// 4*lb + g (lb = 0..4NB-1) as FP32, widened to FP64 per use: the B operand of
+// every MFMA.  For component k the A operands (fragments of U_k^T) and
+// m_k = U_k^T mu_k are double-buffered in LDS: the global loads of component
+// k+1 are issued into registers before the MFMAs of component k and stored to
+// the idle buffer after them (one barrier per component).  Accumulators start
+// at -m_k so that D = U^T (x - mu).
 template <int NB, int P>
 __global__ __launch_bounds__(256) void k_estep64(const float *__restrict__ X, int64_t n,
                                                  int64_t npad, int K,
@@ -160,34 +163,45 @@ __global__ __launch_bounds__(256) void k_estep64(const float *__restrict__ X, in
                                                  double *__restrict__ ll_part) {
     constexpr int DP = 16 * NB;
     constexpr int NPAIR = pairs_of(NB);
-    extern __shared__ double lds[];
-    double *lu = lds;               // NPAIR * 64
-    double *lm = lds + NPAIR * 64;  // DP
+    constexpr int STAGE = NPAIR * 64 + DP;
+    constexpr int PER = (STAGE + 255) / 256;
+    extern __shared__ double lds[];  // 2 * STAGE
     __shared__ double wsum[4];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int g = lane >> 4, c = lane & 15;
     const int64_t base = (int64_t)blockIdx.x * (64 * P) + wave * (16 * P);
 
-    double xb[P][4 * NB];
+    float xr[P][4 * NB];
 #pragma unroll
     for (int p = 0; p < P; p++)
 #pragma unroll
-        for (int lb = 0; lb < 4 * NB; lb++)
-            xb[p][lb] = (double)X[(base + 16 * p + c) * DP + 4 * lb + g];
+        for (int lb = 0; lb < 4 * NB; lb++) xr[p][lb] = X[(base + 16 * p + c) * DP + 4 * lb + g];
+
+    auto stage_src = [&](int k, int e) -> double {
+        return (e < NPAIR * 64) ? ufrag[(size_t)k * NPAIR * 64 + e]
+                                : minit[(size_t)k * DP + (e - NPAIR * 64)];
+    };
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int e = tid + 256 * q;
+        if (e < STAGE) lds[e] = stage_src(0, e);
+    }
+    __syncthreads();
 
     double mx[P];
 #pragma unroll
     for (int p = 0; p < P; p++) mx[p] = -INFINITY;
 
     for (int k = 0; k < K; k++) {
-        __syncthreads();
-        {
-            const double2 *src = reinterpret_cast<const double2 *>(ufrag + (size_t)k * NPAIR * 64);
-            double2 *dst = reinterpret_cast<double2 *>(lu);
-            for (int e = tid; e < NPAIR * 32; e += 256) dst[e] = src[e];
-            for (int e = tid; e < DP; e += 256) lm[e] = minit[(size_t)k * DP + e];
+        const double *lu = lds + (k & 1) * STAGE;
+        const double *lm = lu + NPAIR * 64;
+        const bool more = k + 1 < K;
+        double pre[PER];
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int e = tid + 256 * q;
+            pre[q] = (more && e < STAGE) ? stage_src(k + 1, e) : 0.0;
         }
-        __syncthreads();
         const double ck = coef[k];
         v4d acc[P][NB];
 #pragma unroll
@@ -203,13 +217,16 @@ __global__ __launch_bounds__(256) void k_estep64(const float *__restrict__ X, in
         int pr = 0;
 #pragma unroll
         for (int lb = 0; lb < 4 * NB; lb++) {
+            double b[P];
+#pragma unroll
+            for (int p = 0; p < P; p++) b[p] = (double)xr[p][lb];
 #pragma unroll
             for (int jb = lb / 4; jb < NB; jb++) {
                 const double a = lu[pr * 64 + lane];
                 pr++;
 #pragma unroll
                 for (int p = 0; p < P; p++)
-                    acc[p][jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, xb[p][lb], acc[p][jb], 0, 0, 0);
+                    acc[p][jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[p], acc[p][jb], 0, 0, 0);
             }
         }
 #pragma unroll
@@ -224,6 +241,15 @@ __global__ __launch_bounds__(256) void k_estep64(const float *__restrict__ X, in
             mx[p] = fmax(mx[p], s);
             if ((k & 3) == g) Rt[(size_t)k * npad + base + 16 * p + c] = s;
         }
+        if (more) {
+            double *dst = lds + ((k + 1) & 1) * STAGE;
+#pragma unroll
+            for (int q = 0; q < PER; q++) {
+                const int e = tid + 256 * q;
+                if (e < STAGE) dst[e] = pre[q];
+            }
+        }
+        __syncthreads();
     }
 
     double llsum = 0.0;
@@ -267,7 +293,7 @@ __global__ __launch_bounds__(256) void k_reduce_sum(const double *__restrict__ p
 template <int NB, int P>
 static hipError_t estep64_t(fem_ctx *c) {
     constexpr int NPAIR = pairs_of(NB);
-    const size_t lds = (size_t)(NPAIR * 64 + 16 * NB) * sizeof(double);
+    const size_t lds = (size_t)2 * (NPAIR * 64 + 16 * NB) * sizeof(double);
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void *)k_estep64<NB, P>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -281,17 +307,19 @@ static hipError_t estep64_t(fem_ctx *c) {
     return hipGetLastError();
 }
 
+// Point groups per wave of the FP64 E-step: 2 or 4 (4 only for d <= 64).
 hipError_t launch_estep(fem_ctx *c) {
     if (c->prec != FEM_PREC_FP64) return launch_estep32(c);
+    const bool p4 = c->estep_groups == 4;
     switch (c->NB) {
-        case 1: return estep64_t<1, 2>(c);
-        case 2: return estep64_t<2, 2>(c);
-        case 3: return estep64_t<3, 2>(c);
-        case 4: return estep64_t<4, 2>(c);
-        case 5: return estep64_t<5, 1>(c);
-        case 6: return estep64_t<6, 1>(c);
-        case 7: return estep64_t<7, 1>(c);
-        case 8: return estep64_t<8, 1>(c);
+        case 1: return p4 ? estep64_t<1, 4>(c) : estep64_t<1, 2>(c);
+        case 2: return p4 ? estep64_t<2, 4>(c) : estep64_t<2, 2>(c);
+        case 3: return p4 ? estep64_t<3, 4>(c) : estep64_t<3, 2>(c);
+        case 4: return p4 ? estep64_t<4, 4>(c) : estep64_t<4, 2>(c);
+        case 5: return estep64_t<5, 2>(c);
+        case 6: return estep64_t<6, 2>(c);
+        case 7: return estep64_t<7, 2>(c);
+        case 8: return estep64_t<8, 2>(c);
     }
     return hipErrorInvalidValue;
 }

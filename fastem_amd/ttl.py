@@ -1,0 +1,92 @@
+"""Time-to-log-likelihood harness.
+
+LL* is the converged FP64 log-likelihood of plain EM from the k-means++
+initialisation (run until |dLL/LL| < 1e-12 or a sweep cap).  A run reaches
+LL* at the first FP64 E-step whose log-likelihood is at least
+LL* - tol*|LL*|.  Times are wall-clock from the start of the initialisation
+(the dataset is already in HBM) to the E-step that meets the criterion.
+
+Forms:
+* plain: k-means++ init, then FP64 EM (the parity form).
+* mixed: the same init, FP32-form EM until the relative LL change falls below
+  `switch_eps`, then FP64 iterations on the same context until the target is
+  met; the LL that decides the criterion is always an FP64 E-step.
+"""
+import time
+
+import numpy as np
+
+import fastem_amd as fem
+
+
+def kmeanspp(ctx, seed, sweeps=29):
+    return ctx.kmeans_init(seed=seed, seeding=1, max_sweeps=sweeps)
+
+
+def run_plain(ctx, seed, tol=1e-6, cap=500, conv=1e-12, ll_star=None, init=None):
+    """FP64 EM; returns dict(trace=[(t, LL)], ll_final, t_reach, iters)."""
+    ctx.set_precision(fem.PREC_FP64)
+    trace = []
+    t0 = time.perf_counter()
+    if init is None:
+        kmeanspp(ctx, seed)
+    else:
+        ctx.set_params(*init)
+    t_init = time.perf_counter() - t0
+    ctx.factor()
+    prev = -np.finfo(np.float64).max
+    for _ in range(cap):
+        ll = ctx.estep()
+        trace.append((time.perf_counter() - t0, ll))
+        ctx.mstep(copy=False)
+        ctx.factor()
+        e = abs((ll - prev) / ll)
+        prev = ll
+        if e < conv:
+            break
+    ll_final = trace[-1][1]
+    target = ll_star if ll_star is not None else ll_final
+    hit = next((i for i, (t, v) in enumerate(trace) if v >= target - tol * abs(target)), None)
+    return {"trace": trace, "ll_final": ll_final, "t_init": t_init,
+            "t_reach": None if hit is None else trace[hit][0],
+            "iters_to_reach": None if hit is None else hit + 1, "iters": len(trace)}
+
+
+def run_mixed(ctx, seed, ll_star, tol=1e-6, switch_eps=1e-7, cap=500, init=None):
+    """FP32-form iterations, then FP64 polish on the same context."""
+    target = ll_star - tol * abs(ll_star)
+    trace = []
+    t0 = time.perf_counter()
+    ctx.set_precision(fem.PREC_FP64)
+    if init is None:
+        kmeanspp(ctx, seed)
+    else:
+        ctx.set_params(*init)
+    t_init = time.perf_counter() - t0
+    ctx.set_precision(fem.PREC_FP32)
+    ctx.factor()
+    prev = -np.finfo(np.float64).max
+    n32 = 0
+    for _ in range(cap):
+        ll = ctx.estep()
+        trace.append((time.perf_counter() - t0, ll, "fp32"))
+        ctx.mstep(copy=False)
+        ctx.factor()
+        n32 += 1
+        e = abs((ll - prev) / ll)
+        prev = ll
+        if e < switch_eps:
+            break
+    ctx.set_precision(fem.PREC_FP64)
+    ctx.factor()
+    for _ in range(cap):
+        ll = ctx.estep()
+        t = time.perf_counter() - t0
+        trace.append((t, ll, "fp64"))
+        if ll >= target:
+            return {"trace": trace, "t_reach": t, "t_init": t_init, "fp32_iters": n32,
+                    "fp64_iters": len(trace) - n32}
+        ctx.mstep(copy=False)
+        ctx.factor()
+    return {"trace": trace, "t_reach": None, "t_init": t_init, "fp32_iters": n32,
+            "fp64_iters": len(trace) - n32}

@@ -56,6 +56,12 @@ int fem_upload_data(fem_ctx *ctx, const float *X, int64_t ld);
 /* Same, from a device pointer already in HBM (no PCIe transfer). */
 int fem_upload_data_device(fem_ctx *ctx, const float *dX, int64_t ld);
 
+/* Switches the arithmetic of later E-/M-steps (allocating the posterior
+ * buffer of the new form on first use; the dataset and parameters stay in
+ * HBM).  A fem_factor must follow before the next fem_estep.  Used to finish
+ * a mixed-precision fit in FP64. */
+int fem_set_precision(fem_ctx *ctx, int precision);
+
 /* Number of rows the weights are normalised by (EMAlgorithm::TotalRowCount,
  * EM2/EMAlgorithm.cpp:134; MatrixEM.cpp:375).  Defaults to n. */
 int fem_set_total_rows(fem_ctx *ctx, int64_t ntotal);
