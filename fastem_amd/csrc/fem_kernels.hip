@@ -325,10 +325,11 @@ hipError_t launch_estep(fem_ctx *c) {
 }
 
 // ---------------------------------------------------------------------------
-// Stage (b), FP64.  Workgroup = one chunk of kMeansChunk points; its 4 waves
-// share the (KP/16) x NB output blocks of R^T X.  Step = 4 points:
-// A[i][k] = R[16kb+i][i0+k], B[k][j] = X[i0+k][16jb+j].  The jb = 0 waves also
-// sum the A values for N_k.  Partials: mpart[chunk][KP][DP+1].
+// Stage (b), FP64.  Workgroup = one chunk of kMeansChunk points.  Wave w owns
+// component blocks kb = w, w+4, ...; per step of 4 points it loads the R
+// fragment once (A[i][k] = R[16kb+i][i0+k]) and one X fragment per feature
+// block (B[k][j] = X[i0+k][16jb+j]), issuing NB MFMAs; the A values also sum
+// to N_k.  Partials: mpart[chunk][KP][DP+1].
 constexpr int kMeansChunk = 4096;
 
 template <int NB>
@@ -342,47 +343,60 @@ __global__ __launch_bounds__(256) void k_means64(const float *__restrict__ X,
     const int64_t i_begin = (int64_t)chunk * kMeansChunk;
     const int KB = KP / 16;
     double *out = mpart + (size_t)chunk * KP * (DP + 1);
-    for (int blk = wave; blk < KB * NB; blk += 4) {
-        const int kb = blk / NB, jb = blk % NB;
+    for (int kb = wave; kb < KB; kb += 4) {
         const double *rrow = Rt + (size_t)(16 * kb + c) * npad + i_begin + g;
-        const float *xcol = X + (i_begin + g) * DP + 16 * jb + c;
-        v4d acc = {0.0, 0.0, 0.0, 0.0};
+        const float *xrow = X + (i_begin + g) * DP + c;
+        v4d acc[NB];
+#pragma unroll
+        for (int jb = 0; jb < NB; jb++) acc[jb] = (v4d){0.0, 0.0, 0.0, 0.0};
         double ns = 0.0;
-#pragma unroll 8
+#pragma unroll 4
         for (int s = 0; s < kMeansChunk; s += 4) {
             const double a = rrow[s];
-            const double b = (double)xcol[(size_t)s * DP];
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
             ns += a;
+#pragma unroll
+            for (int jb = 0; jb < NB; jb++) {
+                const double bv = (double)xrow[(size_t)s * DP + 16 * jb];
+                acc[jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc[jb], 0, 0, 0);
+            }
         }
 #pragma unroll
-        for (int v = 0; v < 4; v++)
-            out[(size_t)(16 * kb + 4 * v + g) * (DP + 1) + 16 * jb + c] = acc[v];
-        if (jb == 0) {
-            ns = wave_xor_sum_g(ns);
-            if (g == 0) out[(size_t)(16 * kb + c) * (DP + 1) + DP] = ns;
-        }
+        for (int jb = 0; jb < NB; jb++)
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                out[(size_t)(16 * kb + 4 * v + g) * (DP + 1) + 16 * jb + c] = acc[jb][v];
+        ns = wave_xor_sum_g(ns);
+        if (g == 0) out[(size_t)(16 * kb + c) * (DP + 1) + DP] = ns;
     }
 }
 
-// mu_k = M_k / N_k.  One workgroup per component; fixed-order chunk sums.
+// mu_k = M_k / N_k.  Grid (K, column tiles); each thread sums one column over
+// the chunks with 4 interleaved accumulators combined in a fixed order.
+__device__ inline double chunk_sum4(const double *__restrict__ p, size_t stride, int chunks) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int ch = 0;
+    for (; ch + 4 <= chunks; ch += 4) {
+        s0 += p[(size_t)ch * stride];
+        s1 += p[(size_t)(ch + 1) * stride];
+        s2 += p[(size_t)(ch + 2) * stride];
+        s3 += p[(size_t)(ch + 3) * stride];
+    }
+    for (; ch < chunks; ch++) s0 += p[(size_t)ch * stride];
+    return (s0 + s1) + (s2 + s3);
+}
+
 // raw != 0 writes the sums M_k themselves (k-means centroid sums).
-__global__ __launch_bounds__(256) void k_means_finalize(int K, int d, int dpad, int KP, int chunks,
+__global__ __launch_bounds__(128) void k_means_finalize(int K, int d, int dpad, int KP, int chunks,
                                                         const double *__restrict__ mpart,
                                                         double *__restrict__ mu,
                                                         double *__restrict__ nk, int raw) {
-    __shared__ double sn;
     const int k = blockIdx.x, tid = threadIdx.x;
-    if (tid == 0) {
-        double s = 0.0;
-        for (int ch = 0; ch < chunks; ch++) s += mpart[((size_t)ch * KP + k) * (dpad + 1) + dpad];
-        sn = s;
-        nk[k] = s;
-    }
-    __syncthreads();
-    for (int j = tid; j < d; j += blockDim.x) {
-        double s = 0.0;
-        for (int ch = 0; ch < chunks; ch++) s += mpart[((size_t)ch * KP + k) * (dpad + 1) + j];
+    const size_t stride = (size_t)KP * (dpad + 1);
+    const double sn = chunk_sum4(mpart + (size_t)k * (dpad + 1) + dpad, stride, chunks);
+    if (blockIdx.y == 0 && tid == 0) nk[k] = sn;
+    const int j = blockIdx.y * 128 + tid;
+    if (j < d) {
+        const double s = chunk_sum4(mpart + (size_t)k * (dpad + 1) + j, stride, chunks);
         mu[(size_t)k * d + j] = raw ? s : s / sn;
     }
 }
@@ -409,8 +423,8 @@ hipError_t launch_means64_partials(fem_ctx *c) {
 }
 
 hipError_t launch_means_finalize(fem_ctx *c, double *mu_out, double *nk_out, int raw) {
-    hipLaunchKernelGGL(k_means_finalize, dim3(c->K), dim3(256), 0, c->stream, c->K, c->d, c->dpad,
-                       c->KP, c->m_chunks, c->mpart, mu_out, nk_out, raw);
+    hipLaunchKernelGGL(k_means_finalize, dim3(c->K, (c->d + 127) / 128), dim3(128), 0, c->stream, c->K,
+                       c->d, c->dpad, c->KP, c->m_chunks, c->mpart, mu_out, nk_out, raw);
     return hipGetLastError();
 }
 
@@ -455,20 +469,31 @@ __global__ __launch_bounds__(256) void k_cov64(const float *__restrict__ X,
 #pragma unroll
     for (int u = 0; u < UB; u++) acc[u] = (v4d){0.0, 0.0, 0.0, 0.0};
     const double *rrow = Rt + (size_t)k * npad;
-    for (int64_t i = i0; i < i1; i += 4) {
-        const double sw = sqrt(rrow[i + g]) * inv;
-        double t[NB];
+    for (int64_t i = i0; i < i1; i += 8) {
+        // issue the loads of two steps (8 points) before any arithmetic
+        double rv[2];
+        float xv[2][NB];
 #pragma unroll
-        for (int fb = 0; fb < NB; fb++)
-            t[fb] = ((double)X[(i + g) * DP + 16 * fb + c] - muv[fb]) * sw;
-        int u = 0;
+        for (int q = 0; q < 2; q++) {
+            rv[q] = rrow[i + 4 * q + g];
 #pragma unroll
-        for (int ib = 0; ib < NB; ib++)
+            for (int fb = 0; fb < NB; fb++) xv[q][fb] = X[(i + 4 * q + g) * DP + 16 * fb + c];
+        }
 #pragma unroll
-            for (int jb = ib; jb < NB; jb++) {
-                acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(t[ib], t[jb], acc[u], 0, 0, 0);
-                u++;
-            }
+        for (int q = 0; q < 2; q++) {
+            const double sw = sqrt(rv[q]) * inv;
+            double t[NB];
+#pragma unroll
+            for (int fb = 0; fb < NB; fb++) t[fb] = ((double)xv[q][fb] - muv[fb]) * sw;
+            int u = 0;
+#pragma unroll
+            for (int ib = 0; ib < NB; ib++)
+#pragma unroll
+                for (int jb = ib; jb < NB; jb++) {
+                    acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(t[ib], t[jb], acc[u], 0, 0, 0);
+                    u++;
+                }
+        }
     }
     double *out = cpart + ((size_t)chunk * KP + k) * UB * 256;
 #pragma unroll
@@ -477,29 +502,32 @@ __global__ __launch_bounds__(256) void k_cov64(const float *__restrict__ X,
         for (int v = 0; v < 4; v++) out[(size_t)u * 256 + (4 * v + g) * 16 + c] = acc[u][v];
 }
 
-// Sum the chunk partials of component k (fixed order), mirror the upper
-// triangle, regularise, and set w_k = N_k / ntotal.  One workgroup per k.
-__global__ __launch_bounds__(256) void k_cov_finalize(int K, int d, int NB, int KP, int chunks,
-                                                      const double *__restrict__ cpart,
-                                                      const double *__restrict__ nk, double g,
-                                                      double ntotal, double *__restrict__ cov,
-                                                      double *__restrict__ w) {
+// Sum the chunk partials (fixed order, 4 interleaved accumulators) of one
+// 16x16 upper block per workgroup and write it to both triangles of cov.
+__global__ __launch_bounds__(256) void k_cov_reduce(int K, int d, int NB, int KP, int chunks,
+                                                    const double *__restrict__ cpart,
+                                                    double *__restrict__ cov) {
+    const int k = blockIdx.x, u = blockIdx.y, tid = threadIdx.x;
+    const int UB = NB * (NB + 1) / 2;
+    int ib = 0, rem = u;
+    while (rem >= NB - ib) { rem -= NB - ib; ib++; }
+    const int jb = ib + rem;
+    const int a = 16 * ib + (tid >> 4), bb = 16 * jb + (tid & 15);
+    if (a >= d || bb >= d || a > bb) return;
+    const double s = chunk_sum4(cpart + ((size_t)k * UB + u) * 256 + tid, (size_t)KP * UB * 256, chunks);
+    double *C = cov + (size_t)k * d * d;
+    C[(size_t)a * d + bb] = s;
+    C[(size_t)bb * d + a] = s;
+}
+
+// Regularise component k (EM2/EMAlgorithm.cpp:237-244) and set
+// w_k = N_k / ntotal.  One workgroup per k.
+__global__ __launch_bounds__(256) void k_cov_finish(int K, int d, const double *__restrict__ nk, double g,
+                                                    double ntotal, double *__restrict__ cov,
+                                                    double *__restrict__ w) {
     __shared__ double str;
     const int k = blockIdx.x, tid = threadIdx.x;
-    const int UB = NB * (NB + 1) / 2;
     double *C = cov + (size_t)k * d * d;
-    for (int e = tid; e < d * d; e += blockDim.x) {
-        const int a = e / d, bb = e % d;
-        if (a > bb) continue;
-        const int ib = a >> 4, jb = bb >> 4;
-        const int u = ib * NB - ib * (ib - 1) / 2 + (jb - ib);
-        const size_t off = (size_t)u * 256 + (a & 15) * 16 + (bb & 15);
-        double s = 0.0;
-        for (int ch = 0; ch < chunks; ch++) s += cpart[((size_t)ch * KP + k) * UB * 256 + off];
-        C[(size_t)a * d + bb] = s;
-        C[(size_t)bb * d + a] = s;
-    }
-    __syncthreads();
     if (tid == 0) {
         double tr = 0.0;
         for (int i = 0; i < d; i++) tr += C[(size_t)i * d + i];
@@ -537,8 +565,11 @@ hipError_t launch_cov64_partials(fem_ctx *c) {
 }
 
 hipError_t launch_cov_finalize(fem_ctx *c, double g) {
-    hipLaunchKernelGGL(k_cov_finalize, dim3(c->K), dim3(256), 0, c->stream, c->K, c->d, c->NB,
-                       c->KP, c->c_chunks, c->cpart, c->nk, g, (double)c->ntotal, c->cov, c->w);
+    const int UB = c->NB * (c->NB + 1) / 2;
+    hipLaunchKernelGGL(k_cov_reduce, dim3(c->K, UB), dim3(256), 0, c->stream, c->K, c->d, c->NB, c->KP,
+                       c->c_chunks, c->cpart, c->cov);
+    hipLaunchKernelGGL(k_cov_finish, dim3(c->K), dim3(256), 0, c->stream, c->K, c->d, c->nk, g,
+                       (double)c->ntotal, c->cov, c->w);
     return hipGetLastError();
 }
 

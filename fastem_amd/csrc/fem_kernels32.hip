@@ -202,8 +202,8 @@ hipError_t launch_estep32(fem_ctx *c) {
 }
 
 // ---------------------------------------------------------------------------
-// Stage (b), FP32 products: as k_means64 with FP32 operands; the FP32
-// accumulator is flushed into FP64 every 1024 points.
+// Stage (b), FP32 products: as k_means64 (one R fragment per step, NB
+// MFMAs), FP32 accumulators flushed into FP64 every 1024 points.
 template <int NB>
 __global__ __launch_bounds__(256) void k_means32(const float *__restrict__ X, const float *__restrict__ Rt,
                                                  int64_t npad, int KP, double *__restrict__ mpart) {
@@ -215,33 +215,42 @@ __global__ __launch_bounds__(256) void k_means32(const float *__restrict__ X, co
     const int64_t i_begin = (int64_t)chunk * CH;
     const int KB = KP / 16;
     double *out = mpart + (size_t)chunk * KP * (DP + 1);
-    for (int blk = wave; blk < KB * NB; blk += 4) {
-        const int kb = blk / NB, jb = blk % NB;
+    for (int kb = wave; kb < KB; kb += 4) {
         const float *rrow = Rt + (size_t)(16 * kb + c) * npad + i_begin + g;
-        const float *xcol = X + (i_begin + g) * DP + 16 * jb + c;
-        double accd[4] = {0.0, 0.0, 0.0, 0.0};
+        const float *xrow = X + (i_begin + g) * DP + c;
+        double accd[NB][4];
+#pragma unroll
+        for (int jb = 0; jb < NB; jb++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) accd[jb][v] = 0.0;
         double nsd = 0.0;
         for (int s0 = 0; s0 < CH; s0 += FLUSH) {
-            v4f acc = {0.f, 0.f, 0.f, 0.f};
+            v4f acc[NB];
+#pragma unroll
+            for (int jb = 0; jb < NB; jb++) acc[jb] = (v4f){0.f, 0.f, 0.f, 0.f};
             float ns = 0.f;
-#pragma unroll 8
+#pragma unroll 4
             for (int s = s0; s < s0 + FLUSH; s += 4) {
                 const float a = rrow[s];
-                const float b = xcol[(size_t)s * DP];
-                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
                 ns += a;
+#pragma unroll
+                for (int jb = 0; jb < NB; jb++)
+                    acc[jb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, xrow[(size_t)s * DP + 16 * jb], acc[jb], 0,
+                                                                 0, 0);
             }
 #pragma unroll
-            for (int v = 0; v < 4; v++) accd[v] += (double)acc[v];
+            for (int jb = 0; jb < NB; jb++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) accd[jb][v] += (double)acc[jb][v];
             nsd += (double)ns;
         }
 #pragma unroll
-        for (int v = 0; v < 4; v++) out[(size_t)(16 * kb + 4 * g + v) * (DP + 1) + 16 * jb + c] = accd[v];
-        if (jb == 0) {
-            nsd += __shfl_xor(nsd, 16);
-            nsd += __shfl_xor(nsd, 32);
-            if (g == 0) out[(size_t)(16 * kb + c) * (DP + 1) + DP] = nsd;
-        }
+        for (int jb = 0; jb < NB; jb++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) out[(size_t)(16 * kb + 4 * g + v) * (DP + 1) + 16 * jb + c] = accd[jb][v];
+        nsd += __shfl_xor(nsd, 16);
+        nsd += __shfl_xor(nsd, 32);
+        if (g == 0) out[(size_t)(16 * kb + c) * (DP + 1) + DP] = nsd;
     }
 }
 
@@ -269,7 +278,9 @@ hipError_t launch_means32_partials(fem_ctx *c) {
 // ---------------------------------------------------------------------------
 // Stage (c), FP32 products: T = ((x - mu_hi) - mu_lo) * sqrt(r) / sqrt(N_k) in
 // FP32, D += T_ib^T T_jb on FP32 MFMA, accumulators flushed into FP64 every
-// 2048 points.  Same grid and XCD remapping as k_cov64.
+// 2048 points (into registers for d <= 64, else into the wave's own partial
+// slice in global memory).  The loads of the next 16 points are issued before
+// the MFMAs of the current 16.  Same grid and XCD remapping as k_cov64.
 template <int NB>
 __global__ __launch_bounds__(256) void k_cov32(const float *__restrict__ X, const float *__restrict__ Rt,
                                                int64_t npad, int K, int KP, int d,
@@ -279,6 +290,7 @@ __global__ __launch_bounds__(256) void k_cov32(const float *__restrict__ X, cons
     constexpr int DP = 16 * NB;
     constexpr int UB = NB * (NB + 1) / 2;
     constexpr int FLUSH = 2048;
+    constexpr bool REG = NB <= 4;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int g = lane >> 4, c = lane & 15;
     const int KG = KP / 4;
@@ -298,37 +310,78 @@ __global__ __launch_bounds__(256) void k_cov32(const float *__restrict__ X, cons
         ml[fb] = (float)(m - (double)mh[fb]);
     }
     const float inv = (float)(1.0 / sqrt(nk[k]));
-    // Each wave owns its partial slice, so flushes are plain read-add-writes.
     double *out = cpart + ((size_t)chunk * KP + k) * UB * 256;
     const float *rrow = Rt + (size_t)k * npad;
+    double accd[REG ? UB : 1][4];
+    if (REG) {
+#pragma unroll
+        for (int u = 0; u < (REG ? UB : 1); u++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) accd[u][v] = 0.0;
+    }
+    float rv[4], xv[4][NB];
+    auto load16 = [&](int64_t i) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            rv[q] = rrow[i + 4 * q + g];
+#pragma unroll
+            for (int fb = 0; fb < NB; fb++) xv[q][fb] = X[(i + 4 * q + g) * DP + 16 * fb + c];
+        }
+    };
+    if (i0 < i1) load16(i0);
     bool first = true;
     for (int64_t s0 = i0; s0 < i1; s0 += FLUSH) {
         const int64_t s1 = (s0 + FLUSH < i1) ? s0 + FLUSH : i1;
         v4f acc[UB];
 #pragma unroll
         for (int u = 0; u < UB; u++) acc[u] = (v4f){0.f, 0.f, 0.f, 0.f};
-        for (int64_t i = s0; i < s1; i += 4) {
-            const float sw = sqrtf(rrow[i + g]) * inv;
-            float t[NB];
+        for (int64_t i = s0; i < s1; i += 16) {
+            float cr[4], cx[4][NB];
 #pragma unroll
-            for (int fb = 0; fb < NB; fb++) t[fb] = ((X[(i + g) * DP + 16 * fb + c] - mh[fb]) - ml[fb]) * sw;
-            int u = 0;
+            for (int q = 0; q < 4; q++) {
+                cr[q] = rv[q];
 #pragma unroll
-            for (int ib = 0; ib < NB; ib++)
+                for (int fb = 0; fb < NB; fb++) cx[q][fb] = xv[q][fb];
+            }
+            if (i + 16 < i1) load16(i + 16);
 #pragma unroll
-                for (int jb = ib; jb < NB; jb++) {
-                    acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(t[ib], t[jb], acc[u], 0, 0, 0);
-                    u++;
+            for (int q = 0; q < 4; q++) {
+                const float sw = sqrtf(cr[q]) * inv;
+                float t[NB];
+#pragma unroll
+                for (int fb = 0; fb < NB; fb++) t[fb] = ((cx[q][fb] - mh[fb]) - ml[fb]) * sw;
+                int u = 0;
+#pragma unroll
+                for (int ib = 0; ib < NB; ib++)
+#pragma unroll
+                    for (int jb = ib; jb < NB; jb++) {
+                        acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(t[ib], t[jb], acc[u], 0, 0, 0);
+                        u++;
+                    }
+            }
+        }
+        if (REG) {
+#pragma unroll
+            for (int u = 0; u < (REG ? UB : 1); u++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) accd[u][v] += (double)acc[u][v];
+        } else {
+            // each wave owns its partial slice: plain read-add-write
+#pragma unroll
+            for (int u = 0; u < UB; u++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    double *o = out + (size_t)u * 256 + (4 * g + v) * 16 + c;
+                    *o = (first ? 0.0 : *o) + (double)acc[u][v];
                 }
         }
-#pragma unroll
-        for (int u = 0; u < UB; u++)
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                double *o = out + (size_t)u * 256 + (4 * g + v) * 16 + c;
-                *o = (first ? 0.0 : *o) + (double)acc[u][v];
-            }
         first = false;
+    }
+    if (REG) {
+#pragma unroll
+        for (int u = 0; u < (REG ? UB : 1); u++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) out[(size_t)u * 256 + (4 * g + v) * 16 + c] = accd[u][v];
     }
 }
 
