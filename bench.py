@@ -84,6 +84,10 @@ def main():
                     help="skip the FP32-form measurement")
     ap.add_argument("--cpu-rows", type=int, default=262144)
     ap.add_argument("--cpu-iters", type=int, default=2)
+    ap.add_argument("--no-ttl", action="store_true",
+                    help="skip the time-to-log-likelihood measurement")
+    ap.add_argument("--ttl-cap", type=int, default=400,
+                    help="sweep cap of the converged FP64 reference run")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -174,6 +178,26 @@ def main():
         "variants": variants,
         "init_seconds": t_init,
     }
+    if not args.no_ttl:
+        from fastem_amd import ttl
+        log("time-to-loglik: plain FP64 reference run")
+        with fem.Context(N, D, K, fem.PREC_FP64, 0) as ctx:
+            ctx.upload(X)
+            plain = ttl.run_plain(ctx, seed=1, tol=1e-6, cap=args.ttl_cap, conv=1e-12)
+            ll_star = plain["ll_final"]
+            log("time-to-loglik: mixed FP32 -> FP64 run")
+            mixed = ttl.run_mixed(ctx, seed=1, ll_star=ll_star, tol=1e-6, switch_eps=1e-7,
+                                  cap=args.ttl_cap)
+        out["time_to_loglik"] = {
+            "tol": 1e-6, "ll_star": ll_star,
+            "ll_star_def": "converged FP64 EM from the same k-means++ init "
+                           f"(|dLL/LL| < 1e-12 or {args.ttl_cap} sweeps; ran {plain['iters']})",
+            "plain_fp64_s": plain["t_reach"], "plain_iters": plain["iters_to_reach"],
+            "mixed_s": mixed["t_reach"], "mixed_fp32_iters": mixed["fp32_iters"],
+            "mixed_fp64_iters": mixed["fp64_iters"], "init_s": plain["t_init"],
+            "speedup": (plain["t_reach"] / mixed["t_reach"]
+                        if plain["t_reach"] and mixed["t_reach"] else None),
+            "init": "kmeans++ + up to 29 Lloyd sweeps on the device, included in both times"}
     if not args.no_cpu_baseline:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(X, w0, mu0, cov0, args.cpu_rows, args.cpu_iters)

@@ -1,0 +1,449 @@
+// fem_tiles.h -- workgroup-level device functions of the FP64 EM kernels
+// (gfx950).  Each function is one unit of work of one workgroup of 256 threads
+// (4 waves) and is called uniformly by all its threads; the single-fit kernels
+// (fem_kernels.hip) run one unit per workgroup, the batched persistent kernels
+// (fem_batch.hip) loop over units of many fits.
+//
+// MFMA 16x16x4 operand layout (wave64, lane l): A[i][k] from lane i + 16k,
+// B[k][j] from lane j + 16k.  Accumulator element v of lane l holds
+// D[4*v + l/16][l%16] for v_mfma_f64_16x16x4f64 (checked on gfx950 with a
+// lane-layout probe).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+namespace fem {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+__device__ __host__ constexpr int pairs_of(int NB) {
+    int p = 0;
+    for (int lb = 0; lb < 4 * NB; lb++) p += NB - lb / 4;
+    return p;
+}
+
+__device__ inline double wave_xor_sum_g(double v) {
+    // sum over the four 16-lane groups (lanes l, l^16, l^32, l^48)
+    v += __shfl_xor(v, 16);
+    v += __shfl_xor(v, 32);
+    return v;
+}
+
+// Fixed-order sum of p[0], p[stride], ..., with 4 interleaved accumulators.
+__device__ inline double chunk_sum4(const double *__restrict__ p, size_t stride, int chunks) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int ch = 0;
+    for (; ch + 4 <= chunks; ch += 4) {
+        s0 += p[(size_t)ch * stride];
+        s1 += p[(size_t)(ch + 1) * stride];
+        s2 += p[(size_t)(ch + 2) * stride];
+        s3 += p[(size_t)(ch + 3) * stride];
+    }
+    for (; ch < chunks; ch++) s0 += p[(size_t)ch * stride];
+    return (s0 + s1) + (s2 + s3);
+}
+
+// ---------------------------------------------------------------------------
+// Stage (d) for component k (EM2/NormalDensity.cpp:43-65; MatrixEM.cpp:83-87).
+// Lower-symmetrise, right-looking Cholesky in LDS A[d*d] (the same operation
+// sequence per element as column Cholesky-Crout), log-det, the columns of
+// L^-1 by forward substitution stored as rows of U = L^-T, then the E-step
+// operands: fragments of U in MFMA A-operand order and m = U^T mu (and the
+// FP32 copies when requested).  fail_s: one LDS int.
+__device__ inline void factor_unit(int k, int d, int dpad, int NB, int npairs, const double *__restrict__ cov,
+                                   const double *__restrict__ w, const double *__restrict__ mu,
+                                   double *__restrict__ logdet, int *__restrict__ info, double *__restrict__ coef,
+                                   double *__restrict__ U, double *__restrict__ ufrag, float *__restrict__ ufrag32,
+                                   double *__restrict__ minit, float *__restrict__ mus32, double *A, int *fail_s) {
+    const int tid = threadIdx.x;
+    const double *C = cov + (size_t)k * d * d;
+    for (int e = tid; e < d * d; e += blockDim.x) {
+        int i = e / d, j = e % d;
+        A[e] = (i >= j) ? C[(size_t)i * d + j] : C[(size_t)j * d + i];
+    }
+    if (tid == 0) *fail_s = 0;
+    __syncthreads();
+    for (int j = 0; j < d; j++) {
+        if (tid == 0) {
+            double p = A[j * d + j];
+            if (!(p > 0.0)) *fail_s = j + 1;
+            else A[j * d + j] = sqrt(p);
+        }
+        __syncthreads();
+        if (*fail_s) break;
+        const double ljj = A[j * d + j];
+        for (int i = j + 1 + tid; i < d; i += blockDim.x) A[i * d + j] = A[i * d + j] / ljj;
+        __syncthreads();
+        const int m = d - j - 1;
+        for (int e = tid; e < m * m; e += blockDim.x) {
+            int i = j + 1 + e / m, c = j + 1 + e % m;
+            if (c <= i) A[i * d + c] = A[i * d + c] - A[i * d + j] * A[c * d + j];
+        }
+        __syncthreads();
+    }
+    const int fail = *fail_s;
+    __syncthreads();  // every thread has read fail before the LDS int is reused
+    if (fail) {
+        if (tid == 0) {
+            info[k] = fail;
+            logdet[k] = NAN;
+            coef[k] = NAN;
+        }
+        return;
+    }
+    if (tid == 0) {
+        double s = 0.0;
+        for (int i = 0; i < d; i++) s += log(A[i * d + i]);
+        info[k] = 0;
+        logdet[k] = 2.0 * s;
+        const double lc = -0.5 * (double)d * log(2.0 * M_PI);
+        coef[k] = log(w[k]) + lc - 0.5 * (2.0 * s);
+    }
+    double *Uk = U + (size_t)k * dpad * dpad;
+    for (int e = tid; e < dpad * dpad; e += blockDim.x) Uk[e] = 0.0;
+    __syncthreads();
+    if (tid < d) {
+        const int c = tid;  // column c of L^-1 = row c of U
+        for (int i = c; i < d; i++) {
+            double t = (i == c) ? 1.0 : 0.0;
+            for (int q = c; q < i; q++) t -= A[i * d + q] * Uk[(size_t)c * dpad + q];
+            Uk[(size_t)c * dpad + i] = t / A[i * d + i];
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < npairs * 64; e += blockDim.x) {
+        int p = e >> 6, lane = e & 63;
+        int lb = 0, rem = p;
+        while (rem >= NB - lb / 4) {
+            rem -= NB - lb / 4;
+            lb++;
+        }
+        int jb = lb / 4 + rem;
+        double v = Uk[(size_t)(4 * lb + (lane >> 4)) * dpad + 16 * jb + (lane & 15)];
+        ufrag[((size_t)k * npairs + p) * 64 + lane] = v;
+        if (ufrag32) ufrag32[((size_t)k * npairs + p) * 64 + lane] = (float)v;
+    }
+    for (int j = tid; j < dpad; j += blockDim.x) {
+        double s = 0.0;
+        if (j < d)
+            for (int f = 0; f <= j; f++) s += Uk[(size_t)f * dpad + j] * mu[(size_t)k * d + f];
+        minit[(size_t)k * dpad + j] = s;
+        if (mus32) {
+            double m = (j < d) ? mu[(size_t)k * d + j] : 0.0;
+            float hi = (float)m;
+            mus32[(size_t)k * 2 * dpad + j] = hi;
+            mus32[(size_t)k * 2 * dpad + dpad + j] = (float)(m - (double)hi);
+        }
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Stage (a), FP64, one tile of 64P points (EM2/MatrixEM.cpp:39-87).
+// Each wave owns P groups of 16 points.  Lane (g = l/16, c = l%16) holds, for
+// point c of each group, features 4*lb + g (lb = 0..4NB-1) as FP32, widened
+// to FP64 per use: the B operand of every MFMA.  For component k the A
+// operands (fragments of U_k^T) and m_k = U_k^T mu_k are double-buffered in
+// LDS (2 x STAGE doubles): the global loads of component k+1 are issued into
+// registers before the MFMAs of component k and stored to the idle buffer after
+// them, one barrier per component.  Accumulators start at -m_k so that
+// D = U^T (x - mu).  log-densities go to the component-major Rt; the row max is
+// tracked as they are produced; a second sweep over the lane's own entries
+// forms the log-sum-exp and the posteriors.  The tile's LL is written to
+// *ll_out by thread 0.  wsum: 4 LDS doubles.
+template <int NB>
+__device__ __host__ constexpr int estep64_stage() { return pairs_of(NB) * 64 + 16 * NB; }
+
+template <int NB, int P>
+__device__ inline void estep64_unit(int64_t tile, const float *__restrict__ X, int64_t n, int64_t npad, int K,
+                                    const double *__restrict__ ufrag, const double *__restrict__ minit,
+                                    const double *__restrict__ coef, double *__restrict__ Rt,
+                                    double *__restrict__ ll_out, double *lds, double *wsum) {
+    constexpr int DP = 16 * NB;
+    constexpr int NPAIR = pairs_of(NB);
+    constexpr int STAGE = NPAIR * 64 + DP;
+    constexpr int PER = (STAGE + 255) / 256;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int g = lane >> 4, c = lane & 15;
+    const int64_t base = tile * (64 * P) + wave * (16 * P);
+
+    float xr[P][4 * NB];
+#pragma unroll
+    for (int p = 0; p < P; p++)
+#pragma unroll
+        for (int lb = 0; lb < 4 * NB; lb++) xr[p][lb] = X[(base + 16 * p + c) * DP + 4 * lb + g];
+
+    auto stage_src = [&](int k, int e) -> double {
+        return (e < NPAIR * 64) ? ufrag[(size_t)k * NPAIR * 64 + e] : minit[(size_t)k * DP + (e - NPAIR * 64)];
+    };
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int e = tid + 256 * q;
+        if (e < STAGE) lds[e] = stage_src(0, e);
+    }
+    __syncthreads();
+
+    double mx[P];
+#pragma unroll
+    for (int p = 0; p < P; p++) mx[p] = -INFINITY;
+
+    for (int k = 0; k < K; k++) {
+        const double *lu = lds + (k & 1) * STAGE;
+        const double *lm = lu + NPAIR * 64;
+        const bool more = k + 1 < K;
+        double pre[PER];
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int e = tid + 256 * q;
+            pre[q] = (more && e < STAGE) ? stage_src(k + 1, e) : 0.0;
+        }
+        const double ck = coef[k];
+        v4d acc[P][NB];
+#pragma unroll
+        for (int jb = 0; jb < NB; jb++) {
+            v4d init;
+            init[0] = -lm[16 * jb + 0 + g];
+            init[1] = -lm[16 * jb + 4 + g];
+            init[2] = -lm[16 * jb + 8 + g];
+            init[3] = -lm[16 * jb + 12 + g];
+#pragma unroll
+            for (int p = 0; p < P; p++) acc[p][jb] = init;
+        }
+        int pr = 0;
+#pragma unroll
+        for (int lb = 0; lb < 4 * NB; lb++) {
+            double b[P];
+#pragma unroll
+            for (int p = 0; p < P; p++) b[p] = (double)xr[p][lb];
+#pragma unroll
+            for (int jb = lb / 4; jb < NB; jb++) {
+                const double a = lu[pr * 64 + lane];
+                pr++;
+#pragma unroll
+                for (int p = 0; p < P; p++)
+                    acc[p][jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[p], acc[p][jb], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            double q = 0.0;
+#pragma unroll
+            for (int jb = 0; jb < NB; jb++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) q += acc[p][jb][v] * acc[p][jb][v];
+            q = wave_xor_sum_g(q);
+            const double s = -0.5 * q + ck;
+            mx[p] = fmax(mx[p], s);
+            if ((k & 3) == g) Rt[(size_t)k * npad + base + 16 * p + c] = s;
+        }
+        if (more) {
+            double *dst = lds + ((k + 1) & 1) * STAGE;
+#pragma unroll
+            for (int q = 0; q < PER; q++) {
+                const int e = tid + 256 * q;
+                if (e < STAGE) dst[e] = pre[q];
+            }
+        }
+        __syncthreads();
+    }
+
+    double llsum = 0.0;
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        const int64_t pt = base + 16 * p + c;
+        double se = 0.0;
+        for (int k = g; k < K; k += 4) se += exp(Rt[(size_t)k * npad + pt] - mx[p]);
+        se = wave_xor_sum_g(se);
+        const double lse = log(se) + mx[p];
+        const bool valid = pt < n;
+        for (int k = g; k < K; k += 4) {
+            double *r = Rt + (size_t)k * npad + pt;
+            *r = valid ? exp(*r - lse) : 0.0;
+        }
+        if (g == 0 && valid) llsum += lse;
+    }
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) llsum += __shfl_xor(llsum, o);
+    if (lane == 0) wsum[wave] = llsum;
+    __syncthreads();
+    if (tid == 0) *ll_out = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Stage (b), FP64, one chunk of 4096 points (EM2/MatrixEM.cpp:319-344).  Wave
+// w owns component blocks kb = w, w+4, ...; per step of 4 points it loads the
+// R fragment once (A[i][k] = R[16kb+i][i0+k]) and one X fragment per feature
+// block (B[k][j] = X[i0+k][16jb+j]), issuing NB MFMAs; the A values also sum to
+// N_k.  Partials: out[KP][DP+1] of this chunk.
+constexpr int kMeansChunk = 4096;
+
+template <int NB>
+__device__ inline void means64_unit(int chunk, const float *__restrict__ X, const double *__restrict__ Rt,
+                                    int64_t npad, int KP, double *__restrict__ out) {
+    constexpr int DP = 16 * NB;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int g = lane >> 4, c = lane & 15;
+    const int64_t i_begin = (int64_t)chunk * kMeansChunk;
+    const int KB = KP / 16;
+    for (int kb = wave; kb < KB; kb += 4) {
+        const double *rrow = Rt + (size_t)(16 * kb + c) * npad + i_begin + g;
+        const float *xrow = X + (i_begin + g) * DP + c;
+        v4d acc[NB];
+#pragma unroll
+        for (int jb = 0; jb < NB; jb++) acc[jb] = (v4d){0.0, 0.0, 0.0, 0.0};
+        double ns = 0.0;
+#pragma unroll 4
+        for (int s = 0; s < kMeansChunk; s += 4) {
+            const double a = rrow[s];
+            ns += a;
+#pragma unroll
+            for (int jb = 0; jb < NB; jb++) {
+                const double bv = (double)xrow[(size_t)s * DP + 16 * jb];
+                acc[jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc[jb], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int jb = 0; jb < NB; jb++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) out[(size_t)(16 * kb + 4 * v + g) * (DP + 1) + 16 * jb + c] = acc[jb][v];
+        ns = wave_xor_sum_g(ns);
+        if (g == 0) out[(size_t)(16 * kb + c) * (DP + 1) + DP] = ns;
+    }
+}
+
+// mu_k (or the raw sums M_k when raw != 0) and N_k from the chunk partials;
+// thread `tid` of column tile `ct` handles column j = 128 ct + tid (128
+// threads per unit).
+__device__ inline void means_finalize_unit(int k, int ct, int d, int dpad, int KP, int chunks,
+                                           const double *__restrict__ mpart, double *__restrict__ mu,
+                                           double *__restrict__ nk, int raw) {
+    const int tid = threadIdx.x;
+    const size_t stride = (size_t)KP * (dpad + 1);
+    const double sn = chunk_sum4(mpart + (size_t)k * (dpad + 1) + dpad, stride, chunks);
+    if (ct == 0 && tid == 0) nk[k] = sn;
+    const int j = ct * 128 + tid;
+    if (tid < 128 && j < d) {
+        const double s = chunk_sum4(mpart + (size_t)k * (dpad + 1) + j, stride, chunks);
+        mu[(size_t)k * d + j] = raw ? s : s / sn;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stage (c), FP64: components 4cg..4cg+3 (one per wave) over one chunk of
+// points (EM2/MatrixEM.cpp:53-67,90-99,346-375).  Per step of 4 points each
+// lane forms T[i0+g][16fb+c] = (x - mu_k) * sqrt(r) / sqrt(N_k) for every
+// feature block fb and issues one MFMA per upper block (ib <= jb):
+// D += T_ib^T T_jb.  The loads of two steps are issued before their arithmetic.
+template <int NB>
+__device__ inline void cov64_unit(int cg, int chunk, const float *__restrict__ X, const double *__restrict__ Rt,
+                                  int64_t npad, int K, int KP, int d, const double *__restrict__ mu,
+                                  const double *__restrict__ nk, int64_t chunk_rows, double *__restrict__ cpart) {
+    constexpr int DP = 16 * NB;
+    constexpr int UB = NB * (NB + 1) / 2;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int g = lane >> 4, c = lane & 15;
+    const int k = 4 * cg + wave;
+    if (k >= K) return;
+    const int64_t i0 = (int64_t)chunk * chunk_rows;
+    const int64_t i1 = (i0 + chunk_rows < npad) ? i0 + chunk_rows : npad;
+    double muv[NB];
+#pragma unroll
+    for (int fb = 0; fb < NB; fb++) muv[fb] = (16 * fb + c < d) ? mu[(size_t)k * d + 16 * fb + c] : 0.0;
+    const double inv = 1.0 / sqrt(nk[k]);
+    v4d acc[UB];
+#pragma unroll
+    for (int u = 0; u < UB; u++) acc[u] = (v4d){0.0, 0.0, 0.0, 0.0};
+    const double *rrow = Rt + (size_t)k * npad;
+    for (int64_t i = i0; i < i1; i += 8) {
+        double rv[2];
+        float xv[2][NB];
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            rv[q] = rrow[i + 4 * q + g];
+#pragma unroll
+            for (int fb = 0; fb < NB; fb++) xv[q][fb] = X[(i + 4 * q + g) * DP + 16 * fb + c];
+        }
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const double sw = sqrt(rv[q]) * inv;
+            double t[NB];
+#pragma unroll
+            for (int fb = 0; fb < NB; fb++) t[fb] = ((double)xv[q][fb] - muv[fb]) * sw;
+            int u = 0;
+#pragma unroll
+            for (int ib = 0; ib < NB; ib++)
+#pragma unroll
+                for (int jb = ib; jb < NB; jb++) {
+                    acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(t[ib], t[jb], acc[u], 0, 0, 0);
+                    u++;
+                }
+        }
+    }
+    double *out = cpart + ((size_t)chunk * KP + k) * UB * 256;
+#pragma unroll
+    for (int u = 0; u < UB; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) out[(size_t)u * 256 + (4 * v + g) * 16 + c] = acc[u][v];
+}
+
+// Sum the chunk partials of upper block u of component k (fixed order) and
+// write it to both triangles of cov.
+__device__ inline void cov_reduce_unit(int k, int u, int d, int NB, int KP, int chunks,
+                                       const double *__restrict__ cpart, double *__restrict__ cov) {
+    const int tid = threadIdx.x;
+    const int UB = NB * (NB + 1) / 2;
+    int ib = 0, rem = u;
+    while (rem >= NB - ib) {
+        rem -= NB - ib;
+        ib++;
+    }
+    const int jb = ib + rem;
+    const int a = 16 * ib + (tid >> 4), bb = 16 * jb + (tid & 15);
+    if (a >= d || bb >= d || a > bb) return;
+    const double s = chunk_sum4(cpart + ((size_t)k * UB + u) * 256 + tid, (size_t)KP * UB * 256, chunks);
+    double *C = cov + (size_t)k * d * d;
+    C[(size_t)a * d + bb] = s;
+    C[(size_t)bb * d + a] = s;
+}
+
+// Regularise component k (EM2/EMAlgorithm.cpp:237-244) and set
+// w_k = N_k / ntotal.  str: one LDS double.
+__device__ inline void cov_finish_unit(int k, int d, const double *__restrict__ nk, double g, double ntotal,
+                                       double *__restrict__ cov, double *__restrict__ w, double *str) {
+    const int tid = threadIdx.x;
+    double *C = cov + (size_t)k * d * d;
+    if (tid == 0) {
+        double tr = 0.0;
+        for (int i = 0; i < d; i++) tr += C[(size_t)i * d + i];
+        *str = tr;
+        w[k] = nk[k] / ntotal;
+    }
+    __syncthreads();
+    const double diag = g * *str / (double)d;
+    for (int e = tid; e < d * d; e += blockDim.x) {
+        const int a = e / d, bb = e % d;
+        C[e] = C[e] * (1.0 - g) + (a == bb ? diag : 0.0);
+    }
+    __syncthreads();
+}
+
+// Deterministic sum of n doubles: fixed strided partials, then a fixed tree
+// over 256 threads.  s: 256 LDS doubles.  Result valid in thread 0.
+__device__ inline double block_sum(const double *__restrict__ parts, int64_t n, double *s) {
+    double a = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 256) a += parts[i];
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
+        __syncthreads();
+    }
+    const double r = s[0];
+    __syncthreads();
+    return r;
+}
+
+}  // namespace fem
