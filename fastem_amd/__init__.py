@@ -52,6 +52,16 @@ _SIGS = {
                                            _dp]),
     "fem_last_error": (ctypes.c_char_p, [_vp]),
     "fem_version": (ctypes.c_char_p, []),
+    "fem_batch_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_int64), _ip, _ip, ctypes.c_int]),
+    "fem_batch_destroy": (ctypes.c_int, [_vp]),
+    "fem_batch_upload": (ctypes.c_int, [_vp, ctypes.c_int, _fp, _i64]),
+    "fem_batch_set_params": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp]),
+    "fem_batch_tune": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int]),
+    "fem_batch_run": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_int]),
+    "fem_batch_get_result": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, _dp, _ip, _ip]),
+    "fem_batch_last_error": (ctypes.c_char_p, [_vp]),
     "fem_gengmm": (ctypes.c_int, [_i64, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                   ctypes.c_double, ctypes.c_double, ctypes.c_uint64, _fp, _ip,
                                   _dp, _dp, _dp]),
@@ -261,3 +271,85 @@ def kmeanspp_init(X, K, seed, lloyd_iters=29):
     if rc != 0:
         raise FemError(rc, "k-means++ init left a cluster with fewer than two points")
     return w, mu, cov
+
+
+class Batch:
+    """Many independent fits of uneven (n, d <= 64, K) advanced together
+    (fem_batch_*): one persistent launch per EM phase, one captured hipGraph
+    per iteration of all fits, device-side convergence per fit."""
+
+    CONVERGED, SWEEP_CAP, FAILED = 1, 2, -1
+
+    def __init__(self, shapes, device=0):
+        self.shapes = [(int(n), int(d), int(K)) for n, d, K in shapes]
+        F = len(self.shapes)
+        n = (ctypes.c_int64 * F)(*[s[0] for s in self.shapes])
+        d = (ctypes.c_int * F)(*[s[1] for s in self.shapes])
+        K = (ctypes.c_int * F)(*[s[2] for s in self.shapes])
+        h = _vp()
+        rc = lib().fem_batch_create(ctypes.byref(h), F, n, d, K, device)
+        if rc != 0:
+            raise FemError(rc, "fem_batch_create failed")
+        self._h = h
+
+    def _check(self, rc):
+        if rc != 0:
+            raise FemError(rc, lib().fem_batch_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().fem_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload(self, fit, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        n, d, _ = self.shapes[fit]
+        assert X.shape == (n, d)
+        self._check(lib().fem_batch_upload(self._h, fit, X.ctypes.data_as(_fp), d))
+
+    def set_params(self, fit, w, mu, cov):
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        mu = np.ascontiguousarray(mu, dtype=np.float64)
+        cov = np.ascontiguousarray(cov, dtype=np.float64)
+        self._check(lib().fem_batch_set_params(self._h, fit, _d(w), _d(mu), _d(cov)))
+
+    def tune(self):
+        buf = ctypes.create_string_buffer(4096)
+        self._check(lib().fem_batch_tune(self._h, buf, 4096))
+        return buf.value.decode()
+
+    def run(self, maxsweeps, eps=1e-5, g=0.0, use_graph=True):
+        self._check(lib().fem_batch_run(self._h, maxsweeps, eps, g, 1 if use_graph else 0))
+
+    def result(self, fit):
+        n, d, K = self.shapes[fit]
+        w = np.empty(K)
+        mu = np.empty((K, d))
+        cov = np.empty((K, d, d))
+        ll = ctypes.c_double()
+        it = ctypes.c_int()
+        st = ctypes.c_int()
+        self._check(lib().fem_batch_get_result(self._h, fit, _d(w), _d(mu), _d(cov),
+                                               ctypes.byref(ll), ctypes.byref(it),
+                                               ctypes.byref(st)))
+        return {"w": w, "mu": mu, "cov": cov, "ll": ll.value, "iter_counter": it.value,
+                "status": st.value}
+
+
+def bic(ll, n, d, K):
+    """Bayesian information criterion of a full-covariance mixture:
+    -2 LL + p ln n with p = (K-1) + K d + K d (d+1)/2."""
+    p = (K - 1) + K * d + K * d * (d + 1) // 2
+    return -2.0 * ll + p * np.log(n)

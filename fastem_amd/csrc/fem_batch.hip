@@ -1,0 +1,667 @@
+// fem_batch.hip -- many independent FP64 GMM fits of uneven (n, d, K) advanced
+// together: one persistent launch per EM phase walks a work list that covers
+// every active fit (E-step tiles, means chunks, covariance chunks x component
+// groups, per-component finalisers, factorisations), a per-fit convergence
+// kernel applies the reference's stopping rule on the device
+// (EM2/EMAlgorithm.cpp:235-267), and one EM iteration of all fits is captured
+// in a hipGraph that the host replays until no fit is active.  Fits that have
+// converged or failed are skipped by every phase.  The per-unit bodies are the
+// ones of the single-fit kernels (fem_tiles.h), so each fit's arithmetic and
+// summation order are those of a single-fit run with the same tiles.
+//
+// The autotuner (fem_batch_tune) times each phase under several persistent
+// grid sizes and E-step point-group counts on the batch itself and keeps the
+// fastest per phase.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fem_internal.h"
+#include "fem_tiles.h"
+
+namespace {
+
+// Device-visible description of one fit: shapes, chunking and the offsets of
+// its arrays in the pooled buffers.
+struct BFit {
+    int64_t n, npad, ntotal;
+    int d, dpad, NB, K, KP, npairs, ub;
+    int m_chunks, c_chunks;
+    int64_t c_chunk_rows;
+    int64_t x;                          // float pool
+    int64_t r;                          // double pool: posteriors KP x npad
+    int64_t w, mu, cov, nk, logdet, coef, U, ufrag, minit, llp, mpart, cpart;  // double pool
+    int64_t info;                       // int pool
+};
+
+// Per-fit iteration state (EM2/EMAlgorithm.cpp:222-267 semantics).
+struct BState {
+    double ll;       // LL of the last E-step
+    double prev;     // previous LL (starts at -DBL_MAX)
+    int active;      // still iterating
+    int sweeps;      // E-steps done; the reference's IterCounter = sweeps + 1 at exit
+    int status;      // 0 running, 1 converged (e < eps), 2 sweep cap, -1 Cholesky failure
+    int pad;
+};
+
+enum { ST_RUN = 0, ST_CONV = 1, ST_CAP = 2, ST_FAIL = -1 };
+
+struct Item {
+    int fit, a, b, pad;
+};
+
+}  // namespace
+
+struct fem_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int F = 0;
+    std::vector<BFit> fits;
+    BFit *dfits = nullptr;
+    BState *dstate = nullptr;
+    float *xpool = nullptr;
+    double *dpool = nullptr;
+    int *ipool = nullptr;
+    int64_t nx = 0, nd = 0, ni = 0;
+    // work lists
+    Item *est2 = nullptr, *est4 = nullptr, *means = nullptr, *cov = nullptr, *comp = nullptr, *covred = nullptr;
+    int n_est2 = 0, n_est4 = 0, n_means = 0, n_cov = 0, n_comp = 0, n_covred = 0;
+    int maxNB = 1;
+    int cus = 256;
+    // tuned launch shape
+    int est_groups = 2;
+    int grid_est = 0, grid_means = 0, grid_cov = 0;
+    // run parameters
+    int maxsweeps = 100;
+    double eps = 1e-5, g = 0.0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::string err;
+    std::string tune_report;
+};
+
+namespace fem {
+
+// ---------------------------------------------------------------------------
+// Phase kernels.  Every loop index is workgroup-uniform.
+
+__global__ __launch_bounds__(256) void kb_factor(const BFit *__restrict__ F, BState *__restrict__ S,
+                                                 double *__restrict__ dp, int *__restrict__ ip,
+                                                 const Item *__restrict__ items, int nitems, int gate) {
+    extern __shared__ double A[];
+    __shared__ int fail;
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const Item itm = items[it];
+        if (gate && !S[itm.fit].active) continue;
+        const BFit &f = F[itm.fit];
+        factor_unit(itm.a, f.d, f.dpad, f.NB, f.npairs, dp + f.cov, dp + f.w, dp + f.mu, dp + f.logdet,
+                    ip + f.info, dp + f.coef, dp + f.U, dp + f.ufrag, nullptr, dp + f.minit, nullptr, A, &fail);
+    }
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void kb_estep(const BFit *__restrict__ F, const BState *__restrict__ S,
+                                                const float *__restrict__ xp, double *__restrict__ dp,
+                                                const Item *__restrict__ items, int nitems) {
+    extern __shared__ double lds[];
+    __shared__ double wsum[4];
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const Item itm = items[it];
+        if (!S[itm.fit].active) continue;
+        const BFit &f = F[itm.fit];
+        const int64_t tile = itm.a;
+        const float *X = xp + f.x;
+        double *Rt = dp + f.r;
+        double *llo = dp + f.llp + tile;
+        switch (f.NB) {
+            case 1: estep64_unit<1, P>(tile, X, f.n, f.npad, f.K, dp + f.ufrag, dp + f.minit, dp + f.coef, Rt, llo, lds, wsum); break;
+            case 2: estep64_unit<2, P>(tile, X, f.n, f.npad, f.K, dp + f.ufrag, dp + f.minit, dp + f.coef, Rt, llo, lds, wsum); break;
+            case 3: estep64_unit<3, P>(tile, X, f.n, f.npad, f.K, dp + f.ufrag, dp + f.minit, dp + f.coef, Rt, llo, lds, wsum); break;
+            case 4: estep64_unit<4, P>(tile, X, f.n, f.npad, f.K, dp + f.ufrag, dp + f.minit, dp + f.coef, Rt, llo, lds, wsum); break;
+        }
+    }
+}
+
+// LL of each active fit: fixed-order sum of its E-step tile partials.
+__global__ __launch_bounds__(256) void kb_ll(const BFit *__restrict__ F, BState *__restrict__ S,
+                                             const double *__restrict__ dp, int est_groups) {
+    __shared__ double s[256];
+    const int fi = blockIdx.x;
+    if (!S[fi].active) return;
+    const BFit &f = F[fi];
+    const double r = block_sum(dp + f.llp, f.npad / (64 * est_groups), s);
+    if (threadIdx.x == 0) S[fi].ll = r;
+}
+
+__global__ __launch_bounds__(256) void kb_means(const BFit *__restrict__ F, const BState *__restrict__ S,
+                                                const float *__restrict__ xp, double *__restrict__ dp,
+                                                const Item *__restrict__ items, int nitems) {
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const Item itm = items[it];
+        if (!S[itm.fit].active) continue;
+        const BFit &f = F[itm.fit];
+        double *out = dp + f.mpart + (size_t)itm.a * f.KP * (f.dpad + 1);
+        switch (f.NB) {
+            case 1: means64_unit<1>(itm.a, xp + f.x, dp + f.r, f.npad, f.KP, out); break;
+            case 2: means64_unit<2>(itm.a, xp + f.x, dp + f.r, f.npad, f.KP, out); break;
+            case 3: means64_unit<3>(itm.a, xp + f.x, dp + f.r, f.npad, f.KP, out); break;
+            case 4: means64_unit<4>(itm.a, xp + f.x, dp + f.r, f.npad, f.KP, out); break;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void kb_means_finalize(const BFit *__restrict__ F, const BState *__restrict__ S,
+                                                         double *__restrict__ dp, const Item *__restrict__ items,
+                                                         int nitems) {
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const Item itm = items[it];
+        if (!S[itm.fit].active) continue;
+        const BFit &f = F[itm.fit];
+        means_finalize_unit(itm.a, 0, f.d, f.dpad, f.KP, f.m_chunks, dp + f.mpart, dp + f.mu, dp + f.nk, 0);
+    }
+}
+
+__global__ __launch_bounds__(256) void kb_cov(const BFit *__restrict__ F, const BState *__restrict__ S,
+                                              const float *__restrict__ xp, double *__restrict__ dp,
+                                              const Item *__restrict__ items, int nitems) {
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const Item itm = items[it];
+        if (!S[itm.fit].active) continue;
+        const BFit &f = F[itm.fit];
+        switch (f.NB) {
+            case 1: cov64_unit<1>(itm.a, itm.b, xp + f.x, dp + f.r, f.npad, f.K, f.KP, f.d, dp + f.mu, dp + f.nk, f.c_chunk_rows, dp + f.cpart); break;
+            case 2: cov64_unit<2>(itm.a, itm.b, xp + f.x, dp + f.r, f.npad, f.K, f.KP, f.d, dp + f.mu, dp + f.nk, f.c_chunk_rows, dp + f.cpart); break;
+            case 3: cov64_unit<3>(itm.a, itm.b, xp + f.x, dp + f.r, f.npad, f.K, f.KP, f.d, dp + f.mu, dp + f.nk, f.c_chunk_rows, dp + f.cpart); break;
+            case 4: cov64_unit<4>(itm.a, itm.b, xp + f.x, dp + f.r, f.npad, f.K, f.KP, f.d, dp + f.mu, dp + f.nk, f.c_chunk_rows, dp + f.cpart); break;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void kb_cov_reduce(const BFit *__restrict__ F, const BState *__restrict__ S,
+                                                     double *__restrict__ dp, const Item *__restrict__ items,
+                                                     int nitems) {
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const Item itm = items[it];
+        if (!S[itm.fit].active) continue;
+        const BFit &f = F[itm.fit];
+        cov_reduce_unit(itm.a, itm.b, f.d, f.NB, f.KP, f.c_chunks, dp + f.cpart, dp + f.cov);
+    }
+}
+
+__global__ __launch_bounds__(256) void kb_cov_finish(const BFit *__restrict__ F, const BState *__restrict__ S,
+                                                     double *__restrict__ dp, const Item *__restrict__ items,
+                                                     int nitems, double g) {
+    __shared__ double str;
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const Item itm = items[it];
+        if (!S[itm.fit].active) continue;
+        const BFit &f = F[itm.fit];
+        cov_finish_unit(itm.a, f.d, dp + f.nk, g, (double)f.ntotal, dp + f.cov, dp + f.w, &str);
+    }
+}
+
+// After the factorisation of the new parameters: Cholesky failure stops the
+// fit (the reference throws), else e = |dLL/LL| (0 on a decrease) against eps
+// and the sweep cap.
+__global__ void kb_converge(const BFit *__restrict__ F, BState *__restrict__ S, const int *__restrict__ ip,
+                            int nfits, int maxsweeps, double eps) {
+    const int fi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (fi >= nfits) return;
+    BState st = S[fi];
+    if (!st.active) return;
+    const BFit &f = F[fi];
+    st.sweeps += 1;
+    bool bad = false;
+    for (int k = 0; k < f.K; k++) bad |= ip[f.info + k] != 0;
+    if (bad) {
+        st.active = 0;
+        st.status = ST_FAIL;
+    } else {
+        double e = fabs((st.ll - st.prev) / st.ll);
+        if (st.ll < st.prev) e = 0.0;
+        st.prev = st.ll;
+        if (e < eps) {
+            st.active = 0;
+            st.status = ST_CONV;
+        } else if (st.sweeps >= maxsweeps) {
+            st.active = 0;
+            st.status = ST_CAP;
+        }
+    }
+    S[fi] = st;
+}
+
+// Initial state: active unless the first factorisation failed.
+__global__ void kb_start(const BFit *__restrict__ F, BState *__restrict__ S, const int *__restrict__ ip,
+                         int nfits, int maxsweeps) {
+    const int fi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (fi >= nfits) return;
+    const BFit &f = F[fi];
+    BState st;
+    st.ll = -1.7976931348623157e308;
+    st.prev = -1.7976931348623157e308;
+    st.sweeps = 0;
+    st.pad = 0;
+    bool bad = false;
+    for (int k = 0; k < f.K; k++) bad |= ip[f.info + k] != 0;
+    st.active = (!bad && maxsweeps > 0) ? 1 : 0;
+    st.status = bad ? ST_FAIL : (maxsweeps > 0 ? ST_RUN : ST_CAP);
+    S[fi] = st;
+}
+
+}  // namespace fem
+
+// ---------------------------------------------------------------------------
+namespace {
+
+int berr(fem_batch *b, int code, const std::string &m) {
+    if (b) b->err = m;
+    return code;
+}
+
+#define BH(b, call)                                                                              \
+    do {                                                                                         \
+        hipError_t _e = (call);                                                                  \
+        if (_e != hipSuccess) return berr((b), FEM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+int grid_of(int mult, int cus, int nitems) { return std::max(1, std::min(nitems, mult * cus)); }
+
+// one EM iteration of all active fits (the captured sequence)
+hipError_t launch_iteration(fem_batch *b) {
+    const size_t lds_e = (size_t)2 * fem::estep64_stage<4>() * sizeof(double);
+    const size_t lds_f = (size_t)64 * 64 * sizeof(double);
+    const Item *est = b->est_groups == 4 ? b->est4 : b->est2;
+    const int nest = b->est_groups == 4 ? b->n_est4 : b->n_est2;
+    if (b->est_groups == 4)
+        hipLaunchKernelGGL(fem::kb_estep<4>, dim3(b->grid_est), dim3(256), lds_e, b->stream, b->dfits, b->dstate,
+                           b->xpool, b->dpool, est, nest);
+    else
+        hipLaunchKernelGGL(fem::kb_estep<2>, dim3(b->grid_est), dim3(256), lds_e, b->stream, b->dfits, b->dstate,
+                           b->xpool, b->dpool, est, nest);
+    hipLaunchKernelGGL(fem::kb_ll, dim3(b->F), dim3(256), 0, b->stream, b->dfits, b->dstate, b->dpool,
+                       b->est_groups);
+    hipLaunchKernelGGL(fem::kb_means, dim3(b->grid_means), dim3(256), 0, b->stream, b->dfits, b->dstate, b->xpool,
+                       b->dpool, b->means, b->n_means);
+    hipLaunchKernelGGL(fem::kb_means_finalize, dim3(grid_of(4, b->cus, b->n_comp)), dim3(256), 0, b->stream,
+                       b->dfits, b->dstate, b->dpool, b->comp, b->n_comp);
+    hipLaunchKernelGGL(fem::kb_cov, dim3(b->grid_cov), dim3(256), 0, b->stream, b->dfits, b->dstate, b->xpool,
+                       b->dpool, b->cov, b->n_cov);
+    hipLaunchKernelGGL(fem::kb_cov_reduce, dim3(grid_of(8, b->cus, b->n_covred)), dim3(256), 0, b->stream,
+                       b->dfits, b->dstate, b->dpool, b->covred, b->n_covred);
+    hipLaunchKernelGGL(fem::kb_cov_finish, dim3(grid_of(4, b->cus, b->n_comp)), dim3(256), 0, b->stream,
+                       b->dfits, b->dstate, b->dpool, b->comp, b->n_comp, b->g);
+    hipLaunchKernelGGL(fem::kb_factor, dim3(grid_of(4, b->cus, b->n_comp)), dim3(256), lds_f, b->stream,
+                       b->dfits, b->dstate, b->dpool, b->ipool, b->comp, b->n_comp, 1);
+    hipLaunchKernelGGL(fem::kb_converge, dim3((b->F + 63) / 64), dim3(64), 0, b->stream, b->dfits, b->dstate,
+                       b->ipool, b->F, b->maxsweeps, b->eps);
+    return hipGetLastError();
+}
+
+void free_batch(fem_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    void *ptrs[] = {b->dfits, b->dstate, b->xpool, b->dpool, b->ipool, b->est2, b->est4,
+                    b->means, b->cov, b->comp, b->covred};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+}
+
+template <typename T>
+hipError_t upload_vec(T **dst, const std::vector<T> &v) {
+    hipError_t e = hipMalloc((void **)dst, std::max<size_t>(1, v.size()) * sizeof(T));
+    if (e != hipSuccess) return e;
+    if (v.empty()) return hipSuccess;
+    return hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+}  // namespace
+
+extern "C" {
+
+int fem_batch_create(fem_batch **out, int nfits, const int64_t *n, const int *d, const int *K, int device) {
+    if (!out || nfits < 1 || !n || !d || !K) return FEM_ERR_ARG;
+    *out = nullptr;
+    fem_batch *b = new fem_batch();
+    b->device = device;
+    b->F = nfits;
+    auto bail = [&](int code, const std::string &m) {
+        free_batch(b);
+        delete b;
+        (void)m;
+        return code;
+    };
+    for (int i = 0; i < nfits; i++)
+        if (n[i] < 1 || d[i] < 1 || d[i] > 64 || K[i] < 1) return bail(FEM_ERR_ARG, "bad fit shape");
+    if (hipSetDevice(device) != hipSuccess) return bail(FEM_ERR_HIP, "hipSetDevice");
+    int cus = 256;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    b->cus = cus;
+    std::vector<Item> est2, est4, means, cov, comp, covred;
+    for (int i = 0; i < nfits; i++) {
+        BFit f{};
+        f.n = n[i];
+        f.ntotal = n[i];
+        f.npad = ((n[i] + fem::kRowAlign - 1) / fem::kRowAlign) * fem::kRowAlign;
+        f.d = d[i];
+        f.NB = (d[i] + 15) / 16;
+        f.dpad = 16 * f.NB;
+        f.K = K[i];
+        f.KP = 16 * ((K[i] + 15) / 16);
+        f.npairs = fem::estep_pairs(f.NB);
+        f.ub = fem::upper_blocks(f.NB);
+        f.m_chunks = (int)(f.npad / 4096);
+        const int64_t units = f.npad / 4096, per = (units + 255) / 256;
+        f.c_chunk_rows = per * 4096;
+        f.c_chunks = (int)((f.npad + f.c_chunk_rows - 1) / f.c_chunk_rows);
+        f.x = b->nx;
+        b->nx += f.npad * f.dpad;
+        auto take = [&](int64_t cnt) {
+            const int64_t o = b->nd;
+            b->nd += (cnt + 31) / 32 * 32;
+            return o;
+        };
+        f.r = take((int64_t)f.KP * f.npad);
+        f.w = take(f.K);
+        f.mu = take((int64_t)f.K * f.d);
+        f.cov = take((int64_t)f.K * f.d * f.d);
+        f.nk = take(f.K);
+        f.logdet = take(f.K);
+        f.coef = take(f.K);
+        f.U = take((int64_t)f.K * f.dpad * f.dpad);
+        f.ufrag = take((int64_t)f.K * f.npairs * 64);
+        f.minit = take((int64_t)f.K * f.dpad);
+        f.llp = take(f.npad / 64);
+        f.mpart = take((int64_t)f.m_chunks * f.KP * (f.dpad + 1));
+        f.cpart = take((int64_t)f.c_chunks * f.KP * f.ub * 256);
+        f.info = b->ni;
+        b->ni += f.K;
+        b->fits.push_back(f);
+        b->maxNB = std::max(b->maxNB, f.NB);
+        for (int64_t t = 0; t < f.npad / 128; t++) est2.push_back({i, (int)t, 0, 0});
+        for (int64_t t = 0; t < f.npad / 256; t++) est4.push_back({i, (int)t, 0, 0});
+        for (int c = 0; c < f.m_chunks; c++) means.push_back({i, c, 0, 0});
+        for (int ch = 0; ch < f.c_chunks; ch++)
+            for (int cg = 0; cg < f.KP / 4; cg++)
+                if (4 * cg < f.K) cov.push_back({i, cg, ch, 0});
+        for (int k = 0; k < f.K; k++) comp.push_back({i, k, 0, 0});
+        for (int k = 0; k < f.K; k++)
+            for (int u = 0; u < f.ub; u++) covred.push_back({i, k, u, 0});
+    }
+    // interleave large work lists across fits so a persistent grid sees a mix
+    auto interleave = [](std::vector<Item> &v) {
+        std::stable_sort(v.begin(), v.end(), [](const Item &x, const Item &y) {
+            if (x.a != y.a) return x.a < y.a;
+            return x.fit < y.fit;
+        });
+    };
+    interleave(est2);
+    interleave(est4);
+    interleave(means);
+    b->n_est2 = (int)est2.size();
+    b->n_est4 = (int)est4.size();
+    b->n_means = (int)means.size();
+    b->n_cov = (int)cov.size();
+    b->n_comp = (int)comp.size();
+    b->n_covred = (int)covred.size();
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = upload_vec(&b->dfits, b->fits)) != hipSuccess ||
+        (e = hipMalloc((void **)&b->dstate, sizeof(BState) * nfits)) != hipSuccess ||
+        (e = hipMalloc((void **)&b->xpool, sizeof(float) * std::max<int64_t>(1, b->nx))) != hipSuccess ||
+        (e = hipMalloc((void **)&b->dpool, sizeof(double) * std::max<int64_t>(1, b->nd))) != hipSuccess ||
+        (e = hipMalloc((void **)&b->ipool, sizeof(int) * std::max<int64_t>(1, b->ni))) != hipSuccess ||
+        (e = upload_vec(&b->est2, est2)) != hipSuccess || (e = upload_vec(&b->est4, est4)) != hipSuccess ||
+        (e = upload_vec(&b->means, means)) != hipSuccess || (e = upload_vec(&b->cov, cov)) != hipSuccess ||
+        (e = upload_vec(&b->comp, comp)) != hipSuccess || (e = upload_vec(&b->covred, covred)) != hipSuccess ||
+        (e = hipMemset(b->xpool, 0, sizeof(float) * std::max<int64_t>(1, b->nx))) != hipSuccess ||
+        (e = hipMemset(b->dpool, 0, sizeof(double) * std::max<int64_t>(1, b->nd))) != hipSuccess ||
+        (e = hipMemset(b->ipool, 0, sizeof(int) * std::max<int64_t>(1, b->ni))) != hipSuccess)
+        return bail(FEM_ERR_HIP, hipGetErrorString(e));
+    const size_t lds_e = (size_t)2 * fem::estep64_stage<4>() * sizeof(double);
+    (void)hipFuncSetAttribute((const void *)fem::kb_estep<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_e);
+    (void)hipFuncSetAttribute((const void *)fem::kb_estep<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_e);
+    b->grid_est = grid_of(2, cus, b->n_est2);
+    b->grid_means = grid_of(2, cus, b->n_means);
+    b->grid_cov = grid_of(2, cus, b->n_cov);
+    *out = b;
+    return FEM_OK;
+}
+
+int fem_batch_destroy(fem_batch *b) {
+    if (!b) return FEM_OK;
+    (void)hipSetDevice(b->device);
+    (void)hipStreamSynchronize(b->stream);
+    free_batch(b);
+    delete b;
+    return FEM_OK;
+}
+
+const char *fem_batch_last_error(const fem_batch *b) { return b ? b->err.c_str() : ""; }
+
+int fem_batch_upload(fem_batch *b, int fit, const float *X, int64_t ld) {
+    if (!b || fit < 0 || fit >= b->F || !X) return berr(b, FEM_ERR_ARG, "bad fit or pointer");
+    const BFit &f = b->fits[fit];
+    if (ld < f.d) return berr(b, FEM_ERR_ARG, "bad row stride");
+    BH(b, hipSetDevice(b->device));
+    BH(b, hipMemcpy2DAsync(b->xpool + f.x, (size_t)f.dpad * sizeof(float), X, (size_t)ld * sizeof(float),
+                           (size_t)f.d * sizeof(float), (size_t)f.n, hipMemcpyHostToDevice, b->stream));
+    BH(b, hipStreamSynchronize(b->stream));
+    return FEM_OK;
+}
+
+int fem_batch_set_params(fem_batch *b, int fit, const double *w, const double *mu, const double *cov) {
+    if (!b || fit < 0 || fit >= b->F || !w || !mu || !cov) return berr(b, FEM_ERR_ARG, "bad fit or pointer");
+    const BFit &f = b->fits[fit];
+    BH(b, hipSetDevice(b->device));
+    BH(b, hipMemcpyAsync(b->dpool + f.w, w, sizeof(double) * f.K, hipMemcpyHostToDevice, b->stream));
+    BH(b, hipMemcpyAsync(b->dpool + f.mu, mu, sizeof(double) * f.K * f.d, hipMemcpyHostToDevice, b->stream));
+    BH(b, hipMemcpyAsync(b->dpool + f.cov, cov, sizeof(double) * f.K * f.d * f.d, hipMemcpyHostToDevice, b->stream));
+    BH(b, hipStreamSynchronize(b->stream));
+    return FEM_OK;
+}
+
+int fem_batch_get_result(fem_batch *b, int fit, double *w, double *mu, double *cov, double *ll, int *iter_counter,
+                         int *status) {
+    if (!b || fit < 0 || fit >= b->F) return berr(b, FEM_ERR_ARG, "bad fit");
+    const BFit &f = b->fits[fit];
+    BH(b, hipSetDevice(b->device));
+    BState st;
+    BH(b, hipMemcpyAsync(&st, b->dstate + fit, sizeof(BState), hipMemcpyDeviceToHost, b->stream));
+    if (w) BH(b, hipMemcpyAsync(w, b->dpool + f.w, sizeof(double) * f.K, hipMemcpyDeviceToHost, b->stream));
+    if (mu) BH(b, hipMemcpyAsync(mu, b->dpool + f.mu, sizeof(double) * f.K * f.d, hipMemcpyDeviceToHost, b->stream));
+    if (cov)
+        BH(b, hipMemcpyAsync(cov, b->dpool + f.cov, sizeof(double) * f.K * f.d * f.d, hipMemcpyDeviceToHost,
+                             b->stream));
+    BH(b, hipStreamSynchronize(b->stream));
+    if (ll) *ll = st.ll;
+    if (iter_counter) *iter_counter = st.sweeps + 1;
+    if (status) *status = st.status;
+    return FEM_OK;
+}
+
+// Factorises every fit's current parameters and marks the fits active.
+static int batch_start(fem_batch *b) {
+    const size_t lds_f = (size_t)64 * 64 * sizeof(double);
+    hipLaunchKernelGGL(fem::kb_factor, dim3(grid_of(4, b->cus, b->n_comp)), dim3(256), lds_f, b->stream, b->dfits,
+                       b->dstate, b->dpool, b->ipool, b->comp, b->n_comp, 0);
+    hipLaunchKernelGGL(fem::kb_start, dim3((b->F + 63) / 64), dim3(64), 0, b->stream, b->dfits, b->dstate, b->ipool,
+                       b->F, b->maxsweeps);
+    BH(b, hipGetLastError());
+    return FEM_OK;
+}
+
+static int count_active(fem_batch *b, int *active) {
+    std::vector<BState> st(b->F);
+    BH(b, hipMemcpyAsync(st.data(), b->dstate, sizeof(BState) * b->F, hipMemcpyDeviceToHost, b->stream));
+    BH(b, hipStreamSynchronize(b->stream));
+    int a = 0;
+    for (auto &s : st) a += s.active;
+    *active = a;
+    return FEM_OK;
+}
+
+// Runs EM on all fits from their current parameters: regularisation
+// coefficient g (applied after every M-step; the initial parameters are used as
+// given), stopping rule e < eps or maxsweeps E-steps.  use_graph != 0 replays a
+// captured iteration, else launches the same kernels eagerly.  Blocks until
+// every fit has stopped.
+int fem_batch_run(fem_batch *b, int maxsweeps, double eps, double g, int use_graph) {
+    if (!b || maxsweeps < 0 || !(eps > 0.0) || !(g >= 0.0 && g <= 1.0)) return berr(b, FEM_ERR_ARG, "bad argument");
+    BH(b, hipSetDevice(b->device));
+    const bool relaunch = b->maxsweeps != maxsweeps || b->eps != eps || b->g != g;
+    b->maxsweeps = maxsweeps;
+    b->eps = eps;
+    b->g = g;
+    if (relaunch && b->exec) {
+        (void)hipGraphExecDestroy(b->exec);
+        (void)hipGraphDestroy(b->graph);
+        b->exec = nullptr;
+        b->graph = nullptr;
+    }
+    if (int r = batch_start(b)) return r;
+    if (use_graph && !b->exec) {
+        BH(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t le = launch_iteration(b);
+        hipGraph_t gr = nullptr;
+        hipError_t ce = hipStreamEndCapture(b->stream, &gr);
+        if (le != hipSuccess) return berr(b, FEM_ERR_HIP, std::string("capture: ") + hipGetErrorString(le));
+        if (ce != hipSuccess) return berr(b, FEM_ERR_HIP, std::string("end capture: ") + hipGetErrorString(ce));
+        b->graph = gr;
+        BH(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    int active = 1;
+    int launched = 0;
+    while (active > 0 && launched < maxsweeps) {
+        const int burst = std::min(4, maxsweeps - launched);
+        for (int i = 0; i < burst; i++) {
+            if (use_graph) BH(b, hipGraphLaunch(b->exec, b->stream));
+            else BH(b, launch_iteration(b));
+        }
+        launched += burst;
+        if (int r = count_active(b, &active)) return r;
+    }
+    BH(b, hipStreamSynchronize(b->stream));
+    return FEM_OK;
+}
+
+// Autotuner: from the current parameters, times one eager iteration of each
+// phase under candidate launch shapes (E-step point groups {2, 4} x grids
+// {1, 2, 4} x CUs; means and covariance grids {1, 2, 4, 8} x CUs) and keeps
+// the fastest.  Parameters are restored afterwards.  The report is written to
+// `report` (may be NULL).
+int fem_batch_tune(fem_batch *b, char *report, int report_len) {
+    if (!b) return FEM_ERR_ARG;
+    BH(b, hipSetDevice(b->device));
+    // the timed phases rewrite the means (and scratch); keep the parameters
+    std::vector<std::vector<double>> saved(b->F);
+    for (int i = 0; i < b->F; i++) {
+        const BFit &f = b->fits[i];
+        const int64_t cnt = f.cov + (int64_t)f.K * f.d * f.d - f.w;
+        saved[i].resize(cnt);
+        BH(b, hipMemcpy(saved[i].data(), b->dpool + f.w, sizeof(double) * cnt, hipMemcpyDeviceToHost));
+    }
+    const int save_max = b->maxsweeps;
+    b->maxsweeps = 1 << 30;
+    if (int r = batch_start(b)) return r;
+    hipEvent_t e0, e1;
+    BH(b, hipEventCreate(&e0));
+    BH(b, hipEventCreate(&e1));
+    auto timeit = [&](auto &&fn) -> float {
+        fn();  // warm
+        (void)hipEventRecord(e0, b->stream);
+        fn();
+        (void)hipEventRecord(e1, b->stream);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        return ms;
+    };
+    const size_t lds_e = (size_t)2 * fem::estep64_stage<4>() * sizeof(double);
+    std::string rep;
+    char line[256];
+    float best = 1e30f;
+    for (int groups : {2, 4}) {
+        const Item *est = groups == 4 ? b->est4 : b->est2;
+        const int nest = groups == 4 ? b->n_est4 : b->n_est2;
+        for (int mult : {1, 2, 4}) {
+            const int grid = grid_of(mult, b->cus, nest);
+            const float ms = timeit([&]() {
+                if (groups == 4)
+                    hipLaunchKernelGGL(fem::kb_estep<4>, dim3(grid), dim3(256), lds_e, b->stream, b->dfits, b->dstate,
+                                       b->xpool, b->dpool, est, nest);
+                else
+                    hipLaunchKernelGGL(fem::kb_estep<2>, dim3(grid), dim3(256), lds_e, b->stream, b->dfits, b->dstate,
+                                       b->xpool, b->dpool, est, nest);
+            });
+            snprintf(line, sizeof line, "estep groups=%d grid=%dxCU: %.3f ms\n", groups, mult, ms);
+            rep += line;
+            if (ms < best) {
+                best = ms;
+                b->est_groups = groups;
+                b->grid_est = grid;
+            }
+        }
+    }
+    hipLaunchKernelGGL(fem::kb_ll, dim3(b->F), dim3(256), 0, b->stream, b->dfits, b->dstate, b->dpool, b->est_groups);
+    best = 1e30f;
+    for (int mult : {1, 2, 4, 8}) {
+        const int grid = grid_of(mult, b->cus, b->n_means);
+        const float ms = timeit([&]() {
+            hipLaunchKernelGGL(fem::kb_means, dim3(grid), dim3(256), 0, b->stream, b->dfits, b->dstate, b->xpool,
+                               b->dpool, b->means, b->n_means);
+        });
+        snprintf(line, sizeof line, "means grid=%dxCU: %.3f ms\n", mult, ms);
+        rep += line;
+        if (ms < best) {
+            best = ms;
+            b->grid_means = grid;
+        }
+    }
+    hipLaunchKernelGGL(fem::kb_means_finalize, dim3(grid_of(4, b->cus, b->n_comp)), dim3(256), 0, b->stream,
+                       b->dfits, b->dstate, b->dpool, b->comp, b->n_comp);
+    best = 1e30f;
+    for (int mult : {1, 2, 4, 8}) {
+        const int grid = grid_of(mult, b->cus, b->n_cov);
+        const float ms = timeit([&]() {
+            hipLaunchKernelGGL(fem::kb_cov, dim3(grid), dim3(256), 0, b->stream, b->dfits, b->dstate, b->xpool,
+                               b->dpool, b->cov, b->n_cov);
+        });
+        snprintf(line, sizeof line, "cov grid=%dxCU: %.3f ms\n", mult, ms);
+        rep += line;
+        if (ms < best) {
+            best = ms;
+            b->grid_cov = grid;
+        }
+    }
+    snprintf(line, sizeof line, "chosen: estep groups=%d grid=%d, means grid=%d, cov grid=%d (CUs=%d)\n",
+             b->est_groups, b->grid_est, b->grid_means, b->grid_cov, b->cus);
+    rep += line;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    BH(b, hipStreamSynchronize(b->stream));
+    for (int i = 0; i < b->F; i++)
+        BH(b, hipMemcpy(b->dpool + b->fits[i].w, saved[i].data(), sizeof(double) * saved[i].size(),
+                        hipMemcpyHostToDevice));
+    b->maxsweeps = save_max;
+    if (b->exec) {
+        (void)hipGraphExecDestroy(b->exec);
+        (void)hipGraphDestroy(b->graph);
+        b->exec = nullptr;
+        b->graph = nullptr;
+    }
+    b->tune_report = rep;
+    if (report && report_len > 0) {
+        std::strncpy(report, rep.c_str(), (size_t)report_len - 1);
+        report[report_len - 1] = 0;
+    }
+    return FEM_OK;
+}
+
+}  // extern "C"

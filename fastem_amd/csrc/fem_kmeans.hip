@@ -17,6 +17,7 @@
 #include <limits>
 #include <random>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "fem_internal.h"
@@ -24,27 +25,52 @@
 namespace fem {
 
 #pragma clang fp contract(off)
-__device__ inline double sqdist_nc(const double *c, const float *x, int d) {
+// Squared distance between a double centroid and a row held in registers
+// (DP floats, zero beyond d); ascending feature order, no contraction.
+template <int DP>
+__device__ inline double sqdist_reg(const double *c, const float (&x)[DP], int d) {
     double s = 0.0;
-    for (int j = 0; j < d; j++) {
-        const double t = c[j] - (double)x[j];
-        s = s + t * t;
+#pragma unroll
+    for (int j = 0; j < DP; j++) {
+        if (j < d) {
+            const double t = c[j] - (double)x[j];
+            s = s + t * t;
+        }
     }
     return s;
 }
 
+template <int DP>
+__device__ inline void load_row(const float *__restrict__ X, int64_t i, int dpad, float (&x)[DP]) {
+#pragma unroll
+    for (int j = 0; j < DP; j += 4) {
+        const float4 v = *reinterpret_cast<const float4 *>(X + i * dpad + j);
+        x[j] = v.x;
+        x[j + 1] = v.y;
+        x[j + 2] = v.z;
+        x[j + 3] = v.w;
+    }
+}
+
+template <int DP>
 __global__ __launch_bounds__(256) void k_kpp_update(const float *__restrict__ X, int64_t n, int dpad, int d,
                                                     const double *__restrict__ cen, double *__restrict__ dmin,
                                                     int reset) {
+    __shared__ double lc[DP];
+    for (int j = threadIdx.x; j < d; j += 256) lc[j] = cen[j];
+    __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    const double s = sqdist_nc(cen, X + i * dpad, d);
+    float x[DP];
+    load_row<DP>(X, i, dpad, x);
+    const double s = sqdist_reg<DP>(lc, x, d);
     if (reset || dmin[i] > s) dmin[i] = s;
 }
 
-// One point per thread; centroids staged in LDS.  Writes the one-hot
-// assignment into the component-major Rt (all KP rows), cls[i], and a
-// fixed-order per-workgroup sum of the minimum squared distances.
+// One point per thread, its row in registers; centroids staged in LDS.
+// Writes the one-hot assignment into the component-major Rt (all KP rows),
+// cls[i], and a fixed-order per-workgroup sum of the minimum squared distances.
+template <int DP>
 __global__ __launch_bounds__(256) void k_assign(const float *__restrict__ X, int64_t n, int64_t npad, int dpad,
                                                 int d, int K, int KP, const double *__restrict__ C,
                                                 double *__restrict__ Rt, int *__restrict__ cls,
@@ -58,10 +84,11 @@ __global__ __launch_bounds__(256) void k_assign(const float *__restrict__ X, int
     if (i < npad) {
         int bj = -1;
         if (i < n) {
+            float x[DP];
+            load_row<DP>(X, i, dpad, x);
             best = std::numeric_limits<double>::max();
-            const float *x = X + i * dpad;
             for (int k = 0; k < K; k++) {
-                const double s = sqdist_nc(lc + (size_t)k * d, x, d);
+                const double s = sqdist_reg<DP>(lc + (size_t)k * d, x, d);
                 if (s < best) {
                     best = s;
                     bj = k;
@@ -80,6 +107,21 @@ __global__ __launch_bounds__(256) void k_assign(const float *__restrict__ X, int
     if (threadIdx.x == 0) fit_part[blockIdx.x] = red[0];
 }
 #pragma clang fp contract(on)
+
+template <typename F>
+hipError_t by_dpad(int dpad, F &&f) {
+    switch (dpad) {
+        case 16: return f(std::integral_constant<int, 16>{});
+        case 32: return f(std::integral_constant<int, 32>{});
+        case 48: return f(std::integral_constant<int, 48>{});
+        case 64: return f(std::integral_constant<int, 64>{});
+        case 80: return f(std::integral_constant<int, 80>{});
+        case 96: return f(std::integral_constant<int, 96>{});
+        case 112: return f(std::integral_constant<int, 112>{});
+        case 128: return f(std::integral_constant<int, 128>{});
+    }
+    return hipErrorInvalidValue;
+}
 
 }  // namespace fem
 
@@ -126,12 +168,17 @@ double assign(fem_ctx *c, const std::vector<double> &C, std::vector<double> *sum
     KM_HIP(hipMemcpyAsync(c->kC, C.data(), sizeof(double) * c->K * c->d, hipMemcpyHostToDevice, c->stream));
     const int blocks = (int)(c->npad / 256);
     const size_t lds = sizeof(double) * c->K * c->d;
-    if (lds > 64 * 1024)
-        KM_HIP(hipFuncSetAttribute((const void *)fem::k_assign, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)lds));
-    hipLaunchKernelGGL(fem::k_assign, dim3(blocks), dim3(256), lds, c->stream, c->X, c->n, c->npad, c->dpad,
-                       c->d, c->K, c->KP, c->kC, c->Rt, c->cls, c->kfit);
-    KM_HIP(hipGetLastError());
+    KM_HIP(fem::by_dpad(c->dpad, [&](auto DPc) -> hipError_t {
+        constexpr int DP = decltype(DPc)::value;
+        if (lds > 64 * 1024) {
+            hipError_t e = hipFuncSetAttribute((const void *)fem::k_assign<DP>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return e;
+        }
+        hipLaunchKernelGGL(fem::k_assign<DP>, dim3(blocks), dim3(256), lds, c->stream, c->X, c->n, c->npad,
+                           c->dpad, c->d, c->K, c->KP, c->kC, c->Rt, c->cls, c->kfit);
+        return hipGetLastError();
+    }));
     std::vector<double> fp(blocks);
     KM_HIP(hipMemcpyAsync(fp.data(), c->kfit, sizeof(double) * blocks, hipMemcpyDeviceToHost, c->stream));
     if (sums) {
@@ -178,15 +225,24 @@ extern "C" int fem_kmeans_init_rng(fem_ctx *c, double (*rnd)(void *), void *rnd_
             // Clust/PlusPlusInitializer/PlusPlusInitializer.cpp:33-84
             const int64_t r0 = (int64_t)(draw() * (double)n);
             fetch_row(c, r0, C.data());
-            std::vector<double> dm(n), agg(n);
+            double *dm = nullptr;
+            KM_HIP(hipHostMalloc((void **)&dm, sizeof(double) * n, hipHostMallocDefault));
+            std::vector<double> agg(n);
+            struct Pinned {
+                double *p;
+                ~Pinned() { (void)hipHostFree(p); }
+            } pin{dm};
             const int blocks = (int)((n + 255) / 256);
             for (int k = 1; k < K; k++) {
                 KM_HIP(hipMemcpyAsync(c->kC, C.data() + (size_t)(k - 1) * d, sizeof(double) * d,
                                       hipMemcpyHostToDevice, c->stream));
-                hipLaunchKernelGGL(fem::k_kpp_update, dim3(blocks), dim3(256), 0, c->stream, c->X, n, c->dpad, d,
-                                   c->kC, c->dmin, k == 1 ? 1 : 0);
-                KM_HIP(hipGetLastError());
-                KM_HIP(hipMemcpyAsync(dm.data(), c->dmin, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+                KM_HIP(fem::by_dpad(c->dpad, [&](auto DPc) -> hipError_t {
+                    constexpr int DP = decltype(DPc)::value;
+                    hipLaunchKernelGGL(fem::k_kpp_update<DP>, dim3(blocks), dim3(256), 0, c->stream, c->X, n,
+                                       c->dpad, d, c->kC, c->dmin, k == 1 ? 1 : 0);
+                    return hipGetLastError();
+                }));
+                KM_HIP(hipMemcpyAsync(dm, c->dmin, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
                 KM_HIP(hipStreamSynchronize(c->stream));
                 double sum = 0.0;
                 for (int64_t i = 0; i < n; i++) {

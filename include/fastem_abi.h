@@ -124,6 +124,33 @@ int fem_kmeans_init(fem_ctx *ctx, uint64_t seed, int seeding, int max_sweeps, do
 int fem_kmeans_init_rng(fem_ctx *ctx, double (*rnd)(void *), void *rnd_state, int seeding,
                         int max_sweeps, double *w, double *mu, double *cov);
 
+/* ---- Many independent fits at once ------------------------------------------
+ * A batch holds nfits datasets of uneven (n, d <= 64, K) in pooled HBM buffers.
+ * Every EM phase is one persistent launch over a work list covering all
+ * active fits; one iteration of all fits is captured in a hipGraph and
+ * replayed until every fit has stopped; each fit follows the stopping rule of
+ * EMAlgorithm::TrainNoCopy (EM2/EMAlgorithm.cpp:235-267) on the device.
+ * Per-fit arithmetic is the FP64 form's.  No reference counterpart: it serves
+ * random restarts and model-order (K) sweeps scored by BIC. */
+typedef struct fem_batch fem_batch;
+int fem_batch_create(fem_batch **batch, int nfits, const int64_t *n, const int *d, const int *K,
+                     int device);
+int fem_batch_destroy(fem_batch *batch);
+int fem_batch_upload(fem_batch *batch, int fit, const float *X, int64_t ld);
+int fem_batch_set_params(fem_batch *batch, int fit, const double *w, const double *mu,
+                         const double *cov);
+/* Times each phase under candidate launch shapes on this batch and keeps the
+ * fastest (report: text, may be NULL). */
+int fem_batch_tune(fem_batch *batch, char *report, int report_len);
+/* EM on all fits until each converges (|dLL/LL| < eps), fails (Cholesky) or
+ * reaches maxsweeps E-steps; g = regularisation after every M-step. */
+int fem_batch_run(fem_batch *batch, int maxsweeps, double eps, double g, int use_graph);
+/* status: 1 converged, 2 sweep cap, -1 Cholesky failure; ll = LL of the last
+ * E-step; iter_counter = E-steps + 1 (the reference's IterCounter at exit). */
+int fem_batch_get_result(fem_batch *batch, int fit, double *w, double *mu, double *cov, double *ll,
+                         int *iter_counter, int *status);
+const char *fem_batch_last_error(const fem_batch *batch);
+
 /* Device time in milliseconds of the last fem_estep / fem_mstep (means and
  * covariance passes) / fem_factor, from HIP events on the context's stream.
  * Any pointer may be NULL. */

@@ -1,0 +1,104 @@
+"""Batched multi-fit engine (GPU): every fit of a batch of uneven (n, d, K)
+matches the same fit run alone through the single-fit context (bit-identical
+trajectory semantics: same LL, same stop, parameters within 1e-12), the graph
+replay equals eager launches bit for bit, and the FP64 oracle agrees to 1e-10."""
+import numpy as np
+import pytest
+
+import fastem_amd as fem
+
+pytestmark = pytest.mark.gpu
+
+
+def make_fits(seed, count):
+    rng = np.random.default_rng(seed)
+    fits = []
+    for i in range(count):
+        n = int(np.exp(rng.uniform(np.log(1e4), np.log(2e5))))
+        d = int(rng.choice([8, 16, 32, 64]))
+        K = int(rng.integers(4, 17))
+        X = fem.gengmm(n, d, K, -10.0, 10.0, 0.1, 1000 + i)[0]
+        w, mu, cov = fem.kmeanspp_init(X, K, seed=i, lloyd_iters=5)
+        fits.append((X, K, (w, mu, cov)))
+    return fits
+
+
+@pytest.fixture(scope="module")
+def fits():
+    return make_fits(7, 6)
+
+
+def run_batch(fits, use_graph, tune=False, maxsweeps=30, eps=1e-8):
+    b = fem.Batch([(X.shape[0], X.shape[1], K) for X, K, _ in fits])
+    for i, (X, K, init) in enumerate(fits):
+        b.upload(i, X)
+        b.set_params(i, *init)
+    report = b.tune() if tune else None
+    b.run(maxsweeps, eps=eps, use_graph=use_graph)
+    res = [b.result(i) for i in range(len(fits))]
+    b.close()
+    return res, report
+
+
+def test_batch_graph_equals_eager_and_single(fits):
+    print("batch graph vs eager vs single", flush=True)
+    rg, report = run_batch(fits, use_graph=True, tune=True)
+    print(report, flush=True)
+    re_, _ = run_batch(fits, use_graph=False)
+    for a, b in zip(rg, re_):
+        assert a["ll"] == b["ll"] and a["iter_counter"] == b["iter_counter"]
+        for key in ("w", "mu", "cov"):
+            assert np.array_equal(a[key], b[key])
+    for i, (X, K, init) in enumerate(fits):
+        with fem.Context(X.shape[0], X.shape[1], K) as ctx:
+            ctx.upload(X)
+            it, ll, (w, mu, cov) = fem.train(ctx, *init, maxsweeps=30, eps=1e-8)
+        r = rg[i]
+        assert r["iter_counter"] == it, (i, r["iter_counter"], it)
+        assert abs(r["ll"] - ll) <= 1e-12 * abs(ll)
+        assert np.max(np.abs(r["cov"] - cov)) <= 1e-12 * np.max(np.abs(cov))
+        assert r["status"] in (fem.Batch.CONVERGED, fem.Batch.SWEEP_CAP)
+
+
+def test_batch_matches_oracle(orc, fits):
+    print("batch vs oracle", flush=True)
+    res, _ = run_batch(fits[:3], use_graph=True, maxsweeps=12, eps=1e-300)
+    for (X, K, init), r in zip(fits[:3], res):
+        ref = orc.train(X, *init, maxsweeps=12, eps=1e-300, svd=False)
+        assert r["iter_counter"] == ref.rc
+        assert abs(r["ll"] - ref.ll) <= 1e-10 * abs(ref.ll)
+        assert np.max(np.abs(r["mu"] - ref.mu)) <= 1e-10 * np.max(np.abs(ref.mu))
+
+
+def test_batch_failure_is_isolated(fits):
+    print("batch failure isolation", flush=True)
+    X, K, (w, mu, cov) = fits[0]
+    bad = cov.copy()
+    bad[0] = -np.eye(X.shape[1])
+    b = fem.Batch([(X.shape[0], X.shape[1], K)] * 2)
+    b.upload(0, X)
+    b.upload(1, X)
+    b.set_params(0, w, mu, bad)
+    b.set_params(1, w, mu, cov)
+    b.run(10, eps=1e-300)
+    assert b.result(0)["status"] == fem.Batch.FAILED
+    r1 = b.result(1)
+    assert r1["status"] == fem.Batch.SWEEP_CAP and r1["iter_counter"] == 11
+    b.close()
+
+
+def test_bic_k_sweep_prefers_true_order():
+    """A K sweep on data drawn from a K=5 mixture scores lowest BIC at K=5."""
+    print("BIC K sweep", flush=True)
+    n, d, Ktrue = 60000, 8, 5
+    X = fem.gengmm(n, d, Ktrue, -12.0, 12.0, 0.2, 99)[0]
+    Ks = [2, 3, 4, 5, 6, 7, 8]
+    b = fem.Batch([(n, d, K) for K in Ks])
+    for i, K in enumerate(Ks):
+        b.upload(i, X)
+        b.set_params(i, *fem.kmeanspp_init(X, K, seed=3, lloyd_iters=29))
+    b.run(300, eps=1e-10)
+    scores = [fem.bic(b.result(i)["ll"], n, d, K) for i, K in enumerate(Ks)]
+    b.close()
+    print("BIC", dict(zip(Ks, scores)), flush=True)
+    assert Ks[int(np.argmin(scores))] == Ktrue
