@@ -12,6 +12,7 @@ import numpy as np
 _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "libfastem.so")
 EM2_PATH = os.path.join(_PKG, "em2")
+EMINIT_PATH = os.path.join(_PKG, "eminit")
 
 PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
 FEM_OK, FEM_ERR_ARG, FEM_ERR_HIP, FEM_ERR_NOT_PD, FEM_ERR_STATE = 0, -1, -2, -3, -4

@@ -14,8 +14,7 @@ HIP_SOURCES = ["fem_kernels.hip", "fem_kernels32.hip", "fem_kmeans.hip", "fem_ba
 HOST_SOURCES = ["fem_tools.cpp", "host/linalg.cpp", "host/rand.cpp", "host/block_manager.cpp",
                 "host/normal_density.cpp", "host/gaussian_mixture.cpp", "host/dataset.cpp",
                 "host/em_algorithm.cpp", "host/em_initializer.cpp"]
-EM2_SOURCE = "host/em2_main.cpp"
-EM2 = os.path.join(PKG, "em2")
+TOOLS = {"em2": "host/em2_main.cpp", "eminit": "host/eminit_main.cpp"}
 
 # Packed FP32 VALU ops are disabled for all device code.
 HIP_FLAGS = [
@@ -52,14 +51,16 @@ def build(verbose=True):
     tmp = LIB + ".tmp"
     _run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-fopenmp", "-o", tmp] + objs)
     os.replace(tmp, LIB)
-    # the em2 driver, linked against the library next to it
-    src = os.path.join(CSRC, EM2_SOURCE)
-    obj = os.path.join(objdir, "em2_main.o")
-    _run([hipcc(), "-O3", "-std=c++17", "-fopenmp", "-I", os.path.join(ROOT, "include"), "-c",
-          src, "-o", obj])
-    _run([hipcc(), "-fopenmp", "-o", EM2 + ".tmp", obj, "-L", PKG, "-lfastem",
-          "-Wl,-rpath,$ORIGIN"])
-    os.replace(EM2 + ".tmp", EM2)
+    # the em2 and eminit drivers, linked against the library next to them
+    for name, rel in TOOLS.items():
+        src = os.path.join(CSRC, rel)
+        obj = os.path.join(objdir, name + "_main.o")
+        exe = os.path.join(PKG, name)
+        _run([hipcc(), "-O3", "-std=c++17", "-fopenmp", "-I", os.path.join(ROOT, "include"), "-c",
+              src, "-o", obj])
+        _run([hipcc(), "-fopenmp", "-o", exe + ".tmp", obj, "-L", PKG, "-lfastem",
+              "-Wl,-rpath,$ORIGIN"])
+        os.replace(exe + ".tmp", exe)
     return LIB
 
 

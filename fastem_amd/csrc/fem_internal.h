@@ -62,6 +62,7 @@ struct fem_ctx {
     // reductions
     int est_blocks = 0;         // capacity of ll_part (one per E-step workgroup)
     int estep_groups = 2;       // 16-point groups per wave in the FP64 E-step (2 or 4)
+    int estep_groups32 = 4;     // same for the FP32 E-step
     double *ll_part = nullptr;  // per E-step workgroup
     double *ll_dev = nullptr;   // 1
     int m_chunks = 0, c_chunks = 0;
@@ -93,7 +94,6 @@ hipError_t launch_means_finalize(fem_ctx *c, double *mu_out, double *nk_out, int
 hipError_t launch_cov64_partials(fem_ctx *c);
 hipError_t launch_cov_finalize(fem_ctx *c, double g);
 // FP32 form (fem_kernels32.hip)
-int estep32_points_per_block(int NB);
 hipError_t launch_estep32(fem_ctx *c);
 hipError_t launch_means32_partials(fem_ctx *c);
 hipError_t launch_cov32_partials(fem_ctx *c);

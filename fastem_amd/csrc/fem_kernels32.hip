@@ -185,14 +185,14 @@ static hipError_t estep32_t(fem_ctx *c) {
     return hipGetLastError();
 }
 
-int estep32_points_per_block(int NB) { return 64 * (NB <= 4 ? 4 : 2); }
-
+// Point groups per wave: 4 by default for d <= 64, 2 when estep_groups32 == 2.
 hipError_t launch_estep32(fem_ctx *c) {
+    const bool p2 = c->estep_groups32 == 2;
     switch (c->NB) {
-        case 1: return estep32_t<1, 4>(c);
-        case 2: return estep32_t<2, 4>(c);
-        case 3: return estep32_t<3, 4>(c);
-        case 4: return estep32_t<4, 4>(c);
+        case 1: return p2 ? estep32_t<1, 2>(c) : estep32_t<1, 4>(c);
+        case 2: return p2 ? estep32_t<2, 2>(c) : estep32_t<2, 4>(c);
+        case 3: return p2 ? estep32_t<3, 2>(c) : estep32_t<3, 4>(c);
+        case 4: return p2 ? estep32_t<4, 2>(c) : estep32_t<4, 4>(c);
         case 5: return estep32_t<5, 2>(c);
         case 6: return estep32_t<6, 2>(c);
         case 7: return estep32_t<7, 2>(c);

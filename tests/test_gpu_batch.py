@@ -28,7 +28,7 @@ def fits():
     return make_fits(7, 6)
 
 
-def run_batch(fits, use_graph, tune=False, maxsweeps=30, eps=1e-8):
+def run_batch(fits, use_graph, tune=False, maxsweeps=30, eps=1e-8, also_eager=False):
     b = fem.Batch([(X.shape[0], X.shape[1], K) for X, K, _ in fits])
     for i, (X, K, init) in enumerate(fits):
         b.upload(i, X)
@@ -36,15 +36,21 @@ def run_batch(fits, use_graph, tune=False, maxsweeps=30, eps=1e-8):
     report = b.tune() if tune else None
     b.run(maxsweeps, eps=eps, use_graph=use_graph)
     res = [b.result(i) for i in range(len(fits))]
+    eager = None
+    if also_eager:
+        # same batch, same tuned launch shape, kernels launched eagerly
+        for i, (X, K, init) in enumerate(fits):
+            b.set_params(i, *init)
+        b.run(maxsweeps, eps=eps, use_graph=False)
+        eager = [b.result(i) for i in range(len(fits))]
     b.close()
-    return res, report
+    return res, report, eager
 
 
 def test_batch_graph_equals_eager_and_single(fits):
     print("batch graph vs eager vs single", flush=True)
-    rg, report = run_batch(fits, use_graph=True, tune=True)
+    rg, report, re_ = run_batch(fits, use_graph=True, tune=True, also_eager=True)
     print(report, flush=True)
-    re_, _ = run_batch(fits, use_graph=False)
     for a, b in zip(rg, re_):
         assert a["ll"] == b["ll"] and a["iter_counter"] == b["iter_counter"]
         for key in ("w", "mu", "cov"):
@@ -62,7 +68,7 @@ def test_batch_graph_equals_eager_and_single(fits):
 
 def test_batch_matches_oracle(orc, fits):
     print("batch vs oracle", flush=True)
-    res, _ = run_batch(fits[:3], use_graph=True, maxsweeps=12, eps=1e-300)
+    res, _, _ = run_batch(fits[:3], use_graph=True, maxsweeps=12, eps=1e-300)
     for (X, K, init), r in zip(fits[:3], res):
         ref = orc.train(X, *init, maxsweeps=12, eps=1e-300, svd=False)
         assert r["iter_counter"] == ref.rc
