@@ -68,6 +68,16 @@ _SIGS = {
                                   _dp, _dp, _dp]),
     "fem_kmeanspp_init": (ctypes.c_int, [_i64, ctypes.c_int, ctypes.c_int, _fp, ctypes.c_uint64,
                                          ctypes.c_int, _dp, _dp, _dp]),
+    "fem_setnet_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_uint64, ctypes.c_int]),
+    "fem_setnet_destroy": (ctypes.c_int, [_vp]),
+    "fem_setnet_last_error": (ctypes.c_char_p, [_vp]),
+    "fem_setnet_num_params": (ctypes.c_int, [_vp]),
+    "fem_setnet_get_params": (ctypes.c_int, [_vp, _fp]),
+    "fem_setnet_set_params": (ctypes.c_int, [_vp, _fp]),
+    "fem_setnet_forward": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _fp, _fp, _fp]),
+    "fem_setnet_step": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _fp, ctypes.c_int, _fp, _fp,
+                                       _fp, ctypes.c_float, ctypes.c_float, _fp, _fp]),
 }
 
 

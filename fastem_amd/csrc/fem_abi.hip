@@ -1,6 +1,7 @@
 // fem_abi.hip -- the C ABI of libfastem (include/fastem_abi.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -8,6 +9,101 @@
 #include <string>
 
 #include "fem_internal.h"
+
+namespace fem {
+
+namespace {
+constexpr size_t kStageBytes = 32u << 20;
+
+struct Staging {
+    char *buf[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    ~Staging() {
+        for (int i = 0; i < 2; i++) {
+            if (ev[i]) (void)hipEventDestroy(ev[i]);
+            if (buf[i]) (void)hipHostFree(buf[i]);
+        }
+    }
+    hipError_t init(size_t bytes) {
+        for (int i = 0; i < 2; i++) {
+            if (hipError_t e = hipHostMalloc((void **)&buf[i], bytes, hipHostMallocDefault)) return e;
+            if (hipError_t e = hipEventCreateWithFlags(&ev[i], hipEventDisableTiming)) return e;
+        }
+        return hipSuccess;
+    }
+};
+}  // namespace
+
+hipError_t copy_rows_h2d(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width, int64_t rows,
+                         hipStream_t s) {
+    if (rows <= 0) return hipSuccess;
+    if (spitch == width && dpitch == width) {
+        if (hipError_t e = hipMemcpyAsync(dst, src, width * (size_t)rows, hipMemcpyHostToDevice, s)) return e;
+        return hipStreamSynchronize(s);
+    }
+    const int64_t per = std::max<int64_t>(1, (int64_t)(kStageBytes / dpitch));
+    Staging st;
+    if (hipError_t e = st.init((size_t)std::min(per, rows) * dpitch)) return e;
+    const char *in = static_cast<const char *>(src);
+    char *out = static_cast<char *>(dst);
+    int it = 0;
+    for (int64_t r0 = 0; r0 < rows; r0 += per, it++) {
+        const int b = it & 1;
+        if (it >= 2)
+            if (hipError_t e = hipEventSynchronize(st.ev[b])) return e;
+        const int64_t nr = std::min(per, rows - r0);
+        for (int64_t r = 0; r < nr; r++) {
+            char *row = st.buf[b] + (size_t)r * dpitch;
+            std::memcpy(row, in + (size_t)(r0 + r) * spitch, width);
+            if (dpitch > width) std::memset(row + width, 0, dpitch - width);
+        }
+        if (hipError_t e = hipMemcpyAsync(out + (size_t)r0 * dpitch, st.buf[b], (size_t)nr * dpitch,
+                                          hipMemcpyHostToDevice, s))
+            return e;
+        if (hipError_t e = hipEventRecord(st.ev[b], s)) return e;
+    }
+    return hipStreamSynchronize(s);
+}
+
+hipError_t copy_rows_d2h(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width, int64_t rows,
+                         hipStream_t s) {
+    if (rows <= 0) return hipSuccess;
+    if (spitch == width && dpitch == width) {
+        if (hipError_t e = hipMemcpyAsync(dst, src, width * (size_t)rows, hipMemcpyDeviceToHost, s)) return e;
+        return hipStreamSynchronize(s);
+    }
+    const int64_t per = std::max<int64_t>(1, (int64_t)(kStageBytes / spitch));
+    Staging st;
+    if (hipError_t e = st.init((size_t)std::min(per, rows) * spitch)) return e;
+    const char *in = static_cast<const char *>(src);
+    char *out = static_cast<char *>(dst);
+    auto unpack = [&](int b, int64_t r0) {
+        const int64_t nr = std::min(per, rows - r0);
+        for (int64_t r = 0; r < nr; r++)
+            std::memcpy(out + (size_t)(r0 + r) * dpitch, st.buf[b] + (size_t)r * spitch, width);
+    };
+    int it = 0;
+    int64_t prev = -1;
+    for (int64_t r0 = 0; r0 < rows; r0 += per, it++) {
+        const int b = it & 1;
+        const int64_t nr = std::min(per, rows - r0);
+        if (hipError_t e = hipMemcpyAsync(st.buf[b], in + (size_t)r0 * spitch, (size_t)nr * spitch,
+                                          hipMemcpyDeviceToHost, s))
+            return e;
+        if (hipError_t e = hipEventRecord(st.ev[b], s)) return e;
+        if (prev >= 0) {
+            if (hipError_t e = hipEventSynchronize(st.ev[b ^ 1])) return e;
+            unpack(b ^ 1, prev);
+        }
+        prev = r0;
+        // the next copy reuses buffer b^1 only after it was unpacked above
+    }
+    if (hipError_t e = hipEventSynchronize(st.ev[(it - 1) & 1])) return e;
+    unpack((it - 1) & 1, prev);
+    return hipStreamSynchronize(s);
+}
+
+}  // namespace fem
 
 namespace {
 
@@ -158,8 +254,12 @@ static int upload(fem_ctx *c, const float *X, int64_t ld, hipMemcpyKind kind) {
     if (int r = check_ctx(c)) return r;
     if (!X || ld < c->d) return fail(c, FEM_ERR_ARG, "bad data pointer or row stride");
     if (int r = set_device(c)) return r;
-    FEM_HIP(c, hipMemcpy2DAsync(c->X, (size_t)c->dpad * sizeof(float), X, (size_t)ld * sizeof(float),
-                                (size_t)c->d * sizeof(float), (size_t)c->n, kind, c->stream));
+    if (kind == hipMemcpyHostToDevice)
+        FEM_HIP(c, fem::copy_rows_h2d(c->X, (size_t)c->dpad * sizeof(float), X, (size_t)ld * sizeof(float),
+                                      (size_t)c->d * sizeof(float), c->n, c->stream));
+    else
+        FEM_HIP(c, hipMemcpy2DAsync(c->X, (size_t)c->dpad * sizeof(float), X, (size_t)ld * sizeof(float),
+                                    (size_t)c->d * sizeof(float), (size_t)c->n, kind, c->stream));
     FEM_HIP(c, hipStreamSynchronize(c->stream));
     return FEM_OK;
 }
@@ -325,9 +425,8 @@ int fem_get_posteriors(fem_ctx *c, double *R, int64_t ld) {
     FEM_HIP(c, hipMalloc((void **)&tmp, (size_t)c->n * c->K * sizeof(double)));
     hipError_t e = fem::launch_transpose_post(c, tmp, c->K);
     if (e == hipSuccess)
-        e = hipMemcpy2DAsync(R, (size_t)ld * sizeof(double), tmp, (size_t)c->K * sizeof(double),
-                             (size_t)c->K * sizeof(double), (size_t)c->n, hipMemcpyDeviceToHost,
-                             c->stream);
+        e = fem::copy_rows_d2h(R, (size_t)ld * sizeof(double), tmp, (size_t)c->K * sizeof(double),
+                               (size_t)c->K * sizeof(double), c->n, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(tmp);
     if (e != hipSuccess)

@@ -450,9 +450,8 @@ int fem_batch_upload(fem_batch *b, int fit, const float *X, int64_t ld) {
     const BFit &f = b->fits[fit];
     if (ld < f.d) return berr(b, FEM_ERR_ARG, "bad row stride");
     BH(b, hipSetDevice(b->device));
-    BH(b, hipMemcpy2DAsync(b->xpool + f.x, (size_t)f.dpad * sizeof(float), X, (size_t)ld * sizeof(float),
-                           (size_t)f.d * sizeof(float), (size_t)f.n, hipMemcpyHostToDevice, b->stream));
-    BH(b, hipStreamSynchronize(b->stream));
+    BH(b, fem::copy_rows_h2d(b->xpool + f.x, (size_t)f.dpad * sizeof(float), X, (size_t)ld * sizeof(float),
+                             (size_t)f.d * sizeof(float), f.n, b->stream));
     return FEM_OK;
 }
 

@@ -89,6 +89,14 @@ hipError_t launch_means(fem_ctx *c);
 hipError_t launch_cov(fem_ctx *c);
 hipError_t launch_zero(double *p, int64_t n, hipStream_t s);
 hipError_t launch_transpose_post(const fem_ctx *c, double *out_dev, int64_t ld);
+// Row copies between pageable host memory and pitched device memory (bytes),
+// staged through two pinned buffers so the DMA engine moves large contiguous
+// chunks while the host packs or unpacks rows; device pad bytes beyond
+// `width` are written as zero.  Synchronous at return.
+hipError_t copy_rows_h2d(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width, int64_t rows,
+                         hipStream_t s);
+hipError_t copy_rows_d2h(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width, int64_t rows,
+                         hipStream_t s);
 hipError_t launch_means64_partials(fem_ctx *c);
 hipError_t launch_means_finalize(fem_ctx *c, double *mu_out, double *nk_out, int raw);
 hipError_t launch_cov64_partials(fem_ctx *c);

@@ -1,0 +1,142 @@
+"""Learned initializer: a permutation-invariant set network that proposes
+starting means and diagonal covariances for EM (fem_setnet_* in
+include/fastem_setnet.h; kernels in csrc/fem_setnet.hip).
+
+The reference has no such initializer: it starts EM from the Jain, k-means,
+k-means++ or file initializers (EM2/EMInitializer.cpp).  Here k-means++ stays
+the fallback and the final EM log-likelihood decides between proposals
+(`best_of`).
+
+Coordinates: a point set X is normalised by its own centroid c and one
+scale s (root mean square deviation over all coordinates), so the network
+sees (X - c) / s and predicts means and log-variances in those units.
+"""
+import ctypes
+
+import numpy as np
+
+from . import FemError, _fp, _vp, lib
+
+
+def normalise(X):
+    X = np.asarray(X, dtype=np.float64)
+    c = X.mean(axis=0)
+    s = float(np.sqrt(np.mean((X - c) ** 2)))
+    if not s > 0:
+        s = 1.0
+    return (X - c) / s, c, s
+
+
+def subsample(X, n, seed):
+    """n rows of X drawn uniformly (with replacement if X has fewer)."""
+    rng = np.random.default_rng(seed)
+    N = X.shape[0]
+    if N <= n:
+        idx = rng.integers(0, N, size=n) if N < n else np.arange(N)
+    else:
+        idx = np.sort(rng.choice(N, size=n, replace=False))
+    return X[idx]
+
+
+def fit_to_targets(w, mu, cov, c, s):
+    """A fitted mixture in the network's normalised units: (M, V, p)."""
+    M = (np.asarray(mu) - c) / s
+    V = np.log(np.maximum(np.diagonal(np.asarray(cov), axis1=1, axis2=2), 1e-300) / (s * s))
+    return M, V, np.asarray(w, dtype=np.float64)
+
+
+class SetNet:
+    """FP32 set network on one HIP device (fem_setnet)."""
+
+    def __init__(self, d, K, seed=0, device=0):
+        self.d, self.K = int(d), int(K)
+        h = _vp()
+        rc = lib().fem_setnet_create(ctypes.byref(h), self.d, self.K, int(seed), device)
+        if rc != 0:
+            raise FemError(rc, "fem_setnet_create failed")
+        self._h = h
+        self.nparams = lib().fem_setnet_num_params(h)
+
+    def _check(self, rc):
+        if rc != 0:
+            raise FemError(rc, lib().fem_setnet_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().fem_setnet_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def params(self):
+        th = np.empty(self.nparams, dtype=np.float32)
+        self._check(lib().fem_setnet_get_params(self._h, th.ctypes.data_as(_fp)))
+        return th
+
+    def set_params(self, theta):
+        th = np.ascontiguousarray(theta, dtype=np.float32)
+        assert th.size == self.nparams
+        self._check(lib().fem_setnet_set_params(self._h, th.ctypes.data_as(_fp)))
+
+    def save(self, path):
+        np.savez(path, d=self.d, K=self.K, theta=self.params())
+
+    @classmethod
+    def load(cls, path, device=0):
+        z = np.load(path, allow_pickle=False)
+        net = cls(int(z["d"]), int(z["K"]), device=device)
+        net.set_params(z["theta"])
+        return net
+
+    def forward(self, Xs):
+        """Xs: (B, n, d) normalised sets -> (mu, lv), each (B, K, d)."""
+        Xs = np.ascontiguousarray(Xs, dtype=np.float32)
+        B, n, d = Xs.shape
+        assert d == self.d
+        mu = np.empty((B, self.K, d), dtype=np.float32)
+        lv = np.empty((B, self.K, d), dtype=np.float32)
+        self._check(lib().fem_setnet_forward(self._h, B, n, Xs.ctypes.data_as(_fp),
+                                             mu.ctypes.data_as(_fp), lv.ctypes.data_as(_fp)))
+        return mu, lv
+
+    def step(self, Xs, M, V, p, lam=0.5, lr=0.0, want_grad=False):
+        """Loss (summed over the batch) and optionally its gradient; with
+        lr > 0 also one Adam update on the batch-mean loss."""
+        Xs = np.ascontiguousarray(Xs, dtype=np.float32)
+        M = np.ascontiguousarray(M, dtype=np.float32)
+        V = np.ascontiguousarray(V, dtype=np.float32)
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        B, n, d = Xs.shape
+        J = M.shape[1]
+        assert M.shape == (B, J, d) and V.shape == (B, J, d) and p.shape == (B, J)
+        grad = np.empty(self.nparams, dtype=np.float32) if want_grad else None
+        loss = ctypes.c_float()
+        self._check(lib().fem_setnet_step(self._h, B, n, Xs.ctypes.data_as(_fp), J,
+                                          M.ctypes.data_as(_fp), V.ctypes.data_as(_fp),
+                                          p.ctypes.data_as(_fp), float(lam), float(lr),
+                                          grad.ctypes.data_as(_fp) if want_grad else None,
+                                          ctypes.byref(loss)))
+        return (loss.value, grad) if want_grad else loss.value
+
+    def propose(self, X, n_sub=1024, seed=0, var_floor=1e-6):
+        """Starting mixture (w, mu, cov) for dataset X: uniform weights,
+        network means, diagonal covariances exp(log-variance)."""
+        Xs = subsample(np.asarray(X), n_sub, seed)
+        Z, c, s = normalise(Xs)
+        mu, lv = self.forward(Z[None].astype(np.float32))
+        mu = c + s * mu[0].astype(np.float64)
+        var = np.maximum(np.exp(np.clip(lv[0].astype(np.float64), -30, 30)) * s * s, var_floor)
+        cov = np.zeros((self.K, self.d, self.d))
+        idx = np.arange(self.d)
+        cov[:, idx, idx] = var
+        return np.full(self.K, 1.0 / self.K), mu, cov

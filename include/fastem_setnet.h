@@ -1,0 +1,58 @@
+/*
+ * fastem_setnet.h -- the learned initializer's set network (libfastem.so).
+ *
+ * No counterpart in the fastem reference: the reference initializes EM with
+ * the Jain / k-means / k-means++ / file initializers of
+ * EM2/EMInitializer.cpp (see fem_kmeans_init in fastem_abi.h for the device
+ * k-means family).  This network maps a set of normalised points to K
+ * starting means and log-variances; its forward, hand-written backward and
+ * Adam update run as HIP kernels in FP32 with fixed-order reductions.
+ *
+ * Parameters are one flat FP32 vector, in this order:
+ *   W1[d][64] b1[64] W2[64][64] b2[64] S[K][64] Wg[64][64]
+ *   Wm[64][d] bm[d] Wv[64][d] bv[d]
+ * Network, per set of n points x_i (width 64):
+ *   h1 = gelu(x W1 + b1), h2 = gelu(h1 W2 + b2), g = mean_i h2_i,
+ *   q_k = S_k + g Wg, a_ik = softmax_i(q_k . h2_i / 8), z_k = sum_i a_ik h2_i,
+ *   mu_k = z_k Wm + bm, lv_k = z_k Wv + bv.
+ * Training loss per set against a fitted mixture (M_j, V_j, weights p_j):
+ *   sum_j p_j |mu_k(j) - M_j|^2 + (1/K) sum_k min_j |mu_k - M_j|^2
+ *   + lam sum_j p_j |lv_k(j) - V_j|^2 / d,   k(j) = argmin_k |mu_k - M_j|^2.
+ * Supported: d in {8, 16, 32, 64}, 1 <= K <= 64, targets J <= 4K.
+ */
+#ifndef FASTEM_SETNET_H
+#define FASTEM_SETNET_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fem_setnet fem_setnet;
+
+/* Creates a network with Gaussian-initialised weights (seeded mt19937_64),
+ * zero biases and zero Adam state on HIP device `device`. */
+int fem_setnet_create(fem_setnet **net, int d, int K, uint64_t seed, int device);
+int fem_setnet_destroy(fem_setnet *net);
+const char *fem_setnet_last_error(const fem_setnet *net);
+int fem_setnet_num_params(const fem_setnet *net);
+int fem_setnet_get_params(fem_setnet *net, float *theta);
+/* Loads parameters and resets the Adam state. */
+int fem_setnet_set_params(fem_setnet *net, const float *theta);
+
+/* Forward pass over B sets of n normalised points (X: B*n*d FP32, host);
+ * writes means and log-variances (B*K*d each, host). */
+int fem_setnet_forward(fem_setnet *net, int B, int n, const float *X, float *mu, float *lv);
+
+/* One training step over B sets: forward, loss against the targets
+ * (M, V: B*J*d; p: B*J), backward.  grad (host, may be NULL) receives the
+ * gradient of the loss summed over the batch; *loss receives that sum.  With
+ * lr > 0 one Adam step (beta1 0.9, beta2 0.999, eps 1e-8) is applied with the
+ * gradient of the batch-mean loss. */
+int fem_setnet_step(fem_setnet *net, int B, int n, const float *X, int J, const float *M, const float *V,
+                    const float *p, float lam, float lr, float *grad, float *loss);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -19,7 +19,7 @@ def declared(header):
     return sorted(set(re.findall(r"\b(fem_[a-z0-9_]+)\s*\(", text)))
 
 
-@pytest.mark.parametrize("header", ["fastem_abi.h", "fastem_tools.h"])
+@pytest.mark.parametrize("header", ["fastem_abi.h", "fastem_tools.h", "fastem_setnet.h"])
 def test_library_exports_every_declared_symbol(header):
     L = fem.lib()
     names = declared(header)

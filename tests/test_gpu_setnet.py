@@ -1,0 +1,113 @@
+"""The set network's HIP forward, backward and Adam step against the FP64
+numpy restatement (oracle/setnet_ref.py, itself pinned by finite
+differences).  FP32 kernels: forward outputs agree to 2e-4 relative to the
+output scale, gradients to 1e-3 relative in norm per parameter block."""
+import numpy as np
+import pytest
+
+from oracle import setnet_ref as sr
+
+pytestmark = pytest.mark.gpu
+
+
+def _net(d, K, seed):
+    from fastem_amd.setnet import SetNet
+    return SetNet(d, K, seed=seed)
+
+
+def _case(d, K, B, n, J, seed):
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((B, n, d)).astype(np.float32)
+    M = (2.0 * rng.standard_normal((B, J, d))).astype(np.float32)
+    V = (0.3 * rng.standard_normal((B, J, d))).astype(np.float32)
+    p = rng.dirichlet(np.ones(J), size=B).astype(np.float32)
+    return X, M, V, p
+
+
+def _assignment_margin(mu, M):
+    """Smallest relative gap between best and second-best squared distance
+    (both directions of the matching); near-ties would make FP32 and FP64
+    pick different matches."""
+    gaps = []
+    for b in range(mu.shape[0]):
+        D = ((mu[b][:, None, :] - M[b][None, :, :]) ** 2).sum(-1)
+        for A in (D, D.T):
+            s = np.sort(A, axis=0)
+            gaps.append(np.min((s[1] - s[0]) / (np.abs(s[1]) + 1e-30)))
+    return min(gaps)
+
+
+@pytest.mark.parametrize("d,K,B,n", [(8, 5, 3, 300), (32, 16, 4, 1024), (64, 64, 2, 700)])
+def test_forward_matches_oracle(d, K, B, n):
+    print(f"setnet forward d={d} K={K} B={B} n={n}", flush=True)
+    with _net(d, K, seed=11) as net:
+        th = net.params().astype(np.float64)
+        X, _, _, _ = _case(d, K, B, n, 3, seed=d + K)
+        mu, lv = net.forward(X)
+        p = sr.unflatten(th, d, K)
+        mu_r, lv_r, _ = sr.forward(p, X.astype(np.float64))
+        scale = max(np.abs(mu_r).max(), np.abs(lv_r).max())
+        assert np.abs(mu - mu_r).max() <= 2e-4 * scale
+        assert np.abs(lv - lv_r).max() <= 2e-4 * scale
+
+
+@pytest.mark.parametrize("d,K,B,n,J", [(8, 5, 3, 300, 4), (32, 16, 4, 1024, 16), (64, 64, 2, 500, 70)])
+def test_gradient_matches_oracle(d, K, B, n, J):
+    print(f"setnet gradient d={d} K={K} B={B} n={n} J={J}", flush=True)
+    with _net(d, K, seed=3) as net:
+        th = net.params().astype(np.float64)
+        p = sr.unflatten(th, d, K)
+        for seed in range(20):
+            X, M, V, pw = _case(d, K, B, n, J, seed=100 + seed)
+            mu_r, lv_r, cache = sr.forward(p, X.astype(np.float64))
+            if _assignment_margin(mu_r, M.astype(np.float64)) > 1e-3:
+                break
+        else:
+            pytest.fail("no case without near-ties")
+        L_r, dmu, dlv = sr.loss_and_grad_outputs(mu_r, lv_r, M.astype(np.float64),
+                                                 V.astype(np.float64), pw.astype(np.float64))
+        g_r = sr.flatten(sr.backward(p, cache, dmu, dlv), d, K)
+        L, g = net.step(X, M, V, pw, lam=0.5, lr=0.0, want_grad=True)
+        assert abs(L - L_r) <= 1e-4 * abs(L_r)
+        gd = sr.unflatten(g.astype(np.float64), d, K)
+        gr = sr.unflatten(g_r, d, K)
+        for name in gr:
+            err = np.linalg.norm(gd[name] - gr[name])
+            ref = np.linalg.norm(gr[name])
+            assert err <= 1e-3 * ref + 1e-7 * np.linalg.norm(g_r), (name, err, ref)
+        # lr = 0 leaves the parameters unchanged
+        assert np.array_equal(net.params(), th.astype(np.float32))
+
+
+def test_adam_step_and_training_reduces_loss():
+    print("setnet adam", flush=True)
+    d, K, B, n, J = 16, 6, 8, 256, 6
+    with _net(d, K, seed=5) as net:
+        X, M, V, pw = _case(d, K, B, n, J, seed=7)
+        th0 = net.params().astype(np.float64)
+        L0, g = net.step(X, M, V, pw, lr=0.0, want_grad=True)
+        lr = 1e-3
+        net.step(X, M, V, pw, lr=lr)
+        th1 = net.params().astype(np.float64)
+        # first Adam step from zero state: theta - lr * ghat / (|ghat| + eps)
+        gh = g.astype(np.float64) / B
+        expect = th0 - lr * gh / (np.abs(gh) + 1e-8)
+        assert np.abs(th1 - expect).max() <= 1e-6 + 1e-4 * lr
+        for _ in range(200):
+            net.step(X, M, V, pw, lr=3e-3)
+        L1 = net.step(X, M, V, pw, lr=0.0)
+        assert L1 < 0.2 * L0
+
+
+def test_loss_is_permutation_invariant_and_deterministic():
+    print("setnet invariance", flush=True)
+    d, K, B, n, J = 32, 16, 2, 512, 16
+    with _net(d, K, seed=9) as net:
+        X, M, V, pw = _case(d, K, B, n, J, seed=1)
+        L1, g1 = net.step(X, M, V, pw, want_grad=True)
+        L2, g2 = net.step(X, M, V, pw, want_grad=True)
+        assert L1 == L2 and np.array_equal(g1, g2)
+        perm = np.random.default_rng(0).permutation(n)
+        mu1, _ = net.forward(X)
+        mu2, _ = net.forward(X[:, perm])
+        assert np.abs(mu1 - mu2).max() <= 1e-4 * np.abs(mu1).max()

@@ -1,0 +1,81 @@
+"""Pins the FP64 numpy set-network reference (oracle/setnet_ref.py) by central
+finite differences on every parameter block and on the loss outputs, and by
+its invariances (permutation of the points, of the target components)."""
+import numpy as np
+import pytest
+
+from oracle import setnet_ref as sr
+
+
+def setup(B=2, n=7, d=3, K=4, J=3, seed=0):
+    rng = np.random.default_rng(seed)
+    p = sr.init_params(d, K, seed, width=8)
+    for k in p:
+        p[k] = p[k] + 0.1 * rng.standard_normal(p[k].shape)
+    X = rng.standard_normal((B, n, d))
+    M = rng.standard_normal((B, J, d)) * 2
+    V = rng.standard_normal((B, J, d)) * 0.3
+    pw = rng.dirichlet(np.ones(J), size=B)
+    return p, X, M, V, pw
+
+
+def total_loss(p, X, M, V, pw):
+    mu, lv, _ = sr.forward(p, X)
+    L, _, _ = sr.loss_and_grad_outputs(mu, lv, M, V, pw)
+    return L
+
+
+def test_backward_matches_finite_differences():
+    p, X, M, V, pw = setup()
+    mu, lv, cache = sr.forward(p, X)
+    L, dmu, dlv = sr.loss_and_grad_outputs(mu, lv, M, V, pw)
+    gr = sr.backward(p, cache, dmu, dlv)
+    h = 1e-6
+    for name in p:
+        flat = p[name].ravel()
+        for idx in range(min(flat.size, 12)):
+            old = flat[idx]
+            flat[idx] = old + h
+            Lp = total_loss(p, X, M, V, pw)
+            flat[idx] = old - h
+            Lm = total_loss(p, X, M, V, pw)
+            flat[idx] = old
+            fd = (Lp - Lm) / (2 * h)
+            an = gr[name].ravel()[idx]
+            assert abs(fd - an) <= 1e-6 * max(1.0, abs(fd)), (name, idx, fd, an)
+
+
+def test_output_gradient_matches_finite_differences():
+    p, X, M, V, pw = setup(seed=3)
+    mu, lv, _ = sr.forward(p, X)
+    L, dmu, dlv = sr.loss_and_grad_outputs(mu, lv, M, V, pw)
+    h = 1e-7
+    for arr, grad in ((mu, dmu), (lv, dlv)):
+        for idx in [(0, 0, 0), (1, 2, 1), (0, 3, 2)]:
+            old = arr[idx]
+            arr[idx] = old + h
+            Lp, _, _ = sr.loss_and_grad_outputs(mu, lv, M, V, pw)
+            arr[idx] = old - h
+            Lm, _, _ = sr.loss_and_grad_outputs(mu, lv, M, V, pw)
+            arr[idx] = old
+            assert abs((Lp - Lm) / (2 * h) - grad[idx]) <= 1e-5
+
+
+def test_invariances():
+    p, X, M, V, pw = setup(seed=5)
+    mu, lv, _ = sr.forward(p, X)
+    perm = np.random.default_rng(1).permutation(X.shape[1])
+    mu2, lv2, _ = sr.forward(p, X[:, perm])
+    np.testing.assert_allclose(mu, mu2, rtol=1e-12, atol=1e-12)
+    L1, _, _ = sr.loss_and_grad_outputs(mu, lv, M, V, pw)
+    jp = [2, 0, 1]
+    L2, _, _ = sr.loss_and_grad_outputs(mu, lv, M[:, jp], V[:, jp], pw[:, jp])
+    assert abs(L1 - L2) <= 1e-12 * abs(L1)
+
+
+def test_flatten_roundtrip():
+    p = sr.init_params(5, 6, 0)
+    theta = sr.flatten(p, 5, 6)
+    q = sr.unflatten(theta, 5, 6)
+    for k in p:
+        assert np.array_equal(p[k], q[k])
