@@ -150,6 +150,13 @@ def main():
             "estep_frac_of_fp32_mfma_peak":
                 algorithmic_flops(N, D, K)["estep"] / (r32["phases_ms"]["estep"] * 1e-3)
                 / PEAK_FP32_MFMA}
+        log("bf16x3 form")
+        rb = run_form(fem.PREC_BF16X3, args.steps, args.warmup)
+        variants["bf16x3"] = {
+            "iters_per_s": rb["iters_per_s"], "phases_ms": rb["phases_ms"],
+            "estep_fp32_equivalent_frac_of_fp32_mfma_peak":
+                algorithmic_flops(N, D, K)["estep"] / (rb["phases_ms"]["estep"] * 1e-3)
+                / PEAK_FP32_MFMA}
 
     fl = algorithmic_flops(N, D, K)
     ph = r64["phases_ms"]

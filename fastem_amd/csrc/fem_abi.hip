@@ -130,7 +130,7 @@ hipError_t dmalloc(T **p, size_t count) {
 
 void free_all(fem_ctx *c) {
     void *ptrs[] = {c->X, c->Rt, c->Rt32, c->w, c->mu, c->cov, c->nk, c->logdet, c->info, c->coef,
-                    c->U, c->ufrag, c->ufrag32, c->minit, c->mus32, c->ll_part, c->ll_dev,
+                    c->U, c->ufrag, c->ufrag32, c->minit, c->mus32, c->ufragb, c->mlob, c->ll_part, c->ll_dev,
                     c->mpart, c->cpart, c->kC, c->dmin, c->cls, c->kfit};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -166,8 +166,8 @@ int fem_create(fem_ctx **out, int64_t n, int d, int K, int precision, int device
         return fail(nullptr, FEM_ERR_ARG, "n, d and K must be positive");
     if (d > fem::kMaxDim)
         return fail(nullptr, FEM_ERR_ARG, "dimension above 128 is not supported");
-    if (precision != FEM_PREC_FP64 && precision != FEM_PREC_FP32)
-        return fail(nullptr, FEM_ERR_ARG, "precision not available in this build");
+    if (precision != FEM_PREC_FP64 && precision != FEM_PREC_FP32 && precision != FEM_PREC_BF16X3)
+        return fail(nullptr, FEM_ERR_ARG, "unknown precision");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return fail(nullptr, FEM_ERR_HIP, "no HIP device");
@@ -188,7 +188,7 @@ int fem_create(fem_ctx **out, int64_t n, int d, int K, int precision, int device
     const int ub = fem::upper_blocks(c->NB);
     c->est_blocks = (int)(c->npad / 64);  // >= workgroups of any E-step variant
     if (const char *e = std::getenv("FASTEM_ESTEP64_GROUPS")) c->estep_groups = std::atoi(e) == 4 ? 4 : 2;
-    if (const char *e = std::getenv("FASTEM_ESTEP32_GROUPS")) c->estep_groups32 = std::atoi(e) == 2 ? 2 : 4;
+    if (const char *e = std::getenv("FASTEM_ESTEP32_GROUPS")) c->estep_groups32 = std::atoi(e) == 4 ? 4 : 2;
     c->m_chunks = (int)(c->npad / 4096);
     // covariance chunks: at most 256 chunks of a multiple of 4096 rows
     int64_t units = c->npad / 4096;
@@ -217,6 +217,10 @@ int fem_create(fem_ctx **out, int64_t n, int d, int K, int precision, int device
     if (!f64 && ((e = dmalloc(&c->ufrag32, (size_t)K * fem::estep_pairs(c->NB) * 64)) != hipSuccess ||
                  (e = dmalloc(&c->mus32, (size_t)K * 2 * c->dpad)) != hipSuccess))
         return bail(e, "hipMalloc(fp32 factors)");
+    if (precision == FEM_PREC_BF16X3 &&
+        ((e = dmalloc(&c->ufragb, (size_t)K * fem::estepb_pairs(c->NB) * 512)) != hipSuccess ||
+         (e = dmalloc(&c->mlob, (size_t)K * c->dpad)) != hipSuccess))
+        return bail(e, "hipMalloc(bf16 factors)");
     if ((e = dmalloc(&c->w, K)) != hipSuccess ||
         (e = dmalloc(&c->mu, (size_t)K * d)) != hipSuccess ||
         (e = dmalloc(&c->cov, (size_t)K * d * d)) != hipSuccess ||
@@ -274,15 +278,15 @@ int fem_upload_data_device(fem_ctx *c, const float *dX, int64_t ld) {
 
 int fem_set_precision(fem_ctx *c, int precision) {
     if (int r = check_ctx(c)) return r;
-    if (precision != FEM_PREC_FP64 && precision != FEM_PREC_FP32)
-        return fail(c, FEM_ERR_ARG, "precision not available in this build");
+    if (precision != FEM_PREC_FP64 && precision != FEM_PREC_FP32 && precision != FEM_PREC_BF16X3)
+        return fail(c, FEM_ERR_ARG, "unknown precision");
     if (int r = set_device(c)) return r;
     const size_t nR = (size_t)c->KP * c->npad;
     if (precision == FEM_PREC_FP64 && !c->Rt) {
         FEM_HIP(c, hipMalloc((void **)&c->Rt, nR * sizeof(double)));
         FEM_HIP(c, hipMemsetAsync(c->Rt, 0, nR * sizeof(double), c->stream));
     }
-    if (precision == FEM_PREC_FP32) {
+    if (precision != FEM_PREC_FP64) {
         if (!c->Rt32) {
             FEM_HIP(c, hipMalloc((void **)&c->Rt32, nR * sizeof(float)));
             FEM_HIP(c, hipMemsetAsync(c->Rt32, 0, nR * sizeof(float), c->stream));
@@ -291,6 +295,12 @@ int fem_set_precision(fem_ctx *c, int precision) {
             FEM_HIP(c, hipMalloc((void **)&c->ufrag32,
                                  sizeof(float) * c->K * fem::estep_pairs(c->NB) * 64));
         if (!c->mus32) FEM_HIP(c, hipMalloc((void **)&c->mus32, sizeof(float) * c->K * 2 * c->dpad));
+    }
+    if (precision == FEM_PREC_BF16X3) {
+        if (!c->ufragb)
+            FEM_HIP(c, hipMalloc((void **)&c->ufragb,
+                                 sizeof(unsigned int) * c->K * fem::estepb_pairs(c->NB) * 512));
+        if (!c->mlob) FEM_HIP(c, hipMalloc((void **)&c->mlob, sizeof(float) * c->K * c->dpad));
     }
     FEM_HIP(c, hipStreamSynchronize(c->stream));
     c->prec = precision;

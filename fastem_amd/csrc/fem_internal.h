@@ -58,11 +58,13 @@ struct fem_ctx {
     float *ufrag32 = nullptr;  // K x pairs x 64   (FP32)
     double *minit = nullptr;   // K x dpad         U_k^T mu_k
     float *mus32 = nullptr;    // K x 2 x dpad     mu split hi/lo (FP32)
+    unsigned int *ufragb = nullptr;  // K x pairsb x 2 x 256  BF16 hi/lo E-step fragments
+    float *mlob = nullptr;     // K x dpad         U_k^T mu_lo (BF16x3 accumulator seed)
 
     // reductions
     int est_blocks = 0;         // capacity of ll_part (one per E-step workgroup)
     int estep_groups = 2;       // 16-point groups per wave in the FP64 E-step (2 or 4)
-    int estep_groups32 = 4;     // same for the FP32 E-step
+    int estep_groups32 = 2;     // same for the FP32 E-step
     double *ll_part = nullptr;  // per E-step workgroup
     double *ll_dev = nullptr;   // 1
     int m_chunks = 0, c_chunks = 0;
@@ -106,4 +108,10 @@ hipError_t launch_estep32(fem_ctx *c);
 hipError_t launch_means32_partials(fem_ctx *c);
 hipError_t launch_cov32_partials(fem_ctx *c);
 hipError_t launch_transpose_post32(const fem_ctx *c, double *out_dev, int64_t ld);
+// BF16x3 form (fem_kernels_bf16.hip); posteriors in Rt32
+int estepb_pairs(int NB);
+hipError_t launch_prep_bf16(fem_ctx *c);
+hipError_t launch_estepb(fem_ctx *c);
+hipError_t launch_meansb_partials(fem_ctx *c);
+hipError_t launch_covb_partials(fem_ctx *c);
 }  // namespace fem

@@ -41,6 +41,11 @@ hipError_t launch_factor(fem_ctx *c) {
     hipLaunchKernelGGL(k_factor, dim3(c->K), dim3(256), lds, c->stream, c->K, c->d, c->dpad,
                        c->NB, npairs, c->cov, c->w, c->mu, c->logdet, c->info, c->coef, c->U,
                        c->ufrag, c->ufrag32, c->minit, c->mus32);
+    if (c->prec == FEM_PREC_BF16X3) {
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        return launch_prep_bf16(c);
+    }
     return hipGetLastError();
 }
 
@@ -85,6 +90,7 @@ static hipError_t estep64_t(fem_ctx *c) {
 
 // Point groups per wave of the FP64 E-step: 2 or 4 (4 only for d <= 64).
 hipError_t launch_estep(fem_ctx *c) {
+    if (c->prec == FEM_PREC_BF16X3) return launch_estepb(c);
     if (c->prec != FEM_PREC_FP64) return launch_estep32(c);
     const bool p4 = c->estep_groups == 4;
     switch (c->NB) {
@@ -144,7 +150,9 @@ hipError_t launch_means_finalize(fem_ctx *c, double *mu_out, double *nk_out, int
 }
 
 hipError_t launch_means(fem_ctx *c) {
-    hipError_t e = (c->prec == FEM_PREC_FP64) ? launch_means64_partials(c) : launch_means32_partials(c);
+    hipError_t e = (c->prec == FEM_PREC_FP64)   ? launch_means64_partials(c)
+                   : (c->prec == FEM_PREC_FP32) ? launch_means32_partials(c)
+                                                : launch_meansb_partials(c);
     if (e != hipSuccess) return e;
     return launch_means_finalize(c, c->mu, c->nk, 0);
 }
@@ -211,7 +219,9 @@ hipError_t launch_cov_finalize(fem_ctx *c, double g) {
 }
 
 hipError_t launch_cov(fem_ctx *c) {
-    hipError_t e = (c->prec == FEM_PREC_FP64) ? launch_cov64_partials(c) : launch_cov32_partials(c);
+    hipError_t e = (c->prec == FEM_PREC_FP64)   ? launch_cov64_partials(c)
+                   : (c->prec == FEM_PREC_FP32) ? launch_cov32_partials(c)
+                                                : launch_covb_partials(c);
     if (e != hipSuccess) return e;
     return launch_cov_finalize(c, c->g);
 }

@@ -185,7 +185,7 @@ static hipError_t estep32_t(fem_ctx *c) {
     return hipGetLastError();
 }
 
-// Point groups per wave: 4 by default for d <= 64, 2 when estep_groups32 == 2.
+// Point groups per wave: 2 by default, 4 for d <= 64 when estep_groups32 == 4.
 hipError_t launch_estep32(fem_ctx *c) {
     const bool p2 = c->estep_groups32 == 2;
     switch (c->NB) {

@@ -1,0 +1,588 @@
+// fem_kernels_bf16.hip -- BF16x3 precision form of the E-step and M-step
+// contractions on v_mfma_f32_16x16x32_bf16 (gfx950).
+//
+// Every FP32 operand value v is split as v = hi + lo with hi = bf16(v) and
+// lo = bf16(v - hi) (both round-to-nearest-even), so |v - hi - lo| <= 2^-17 |v|.
+// A product a*b is formed as hi_a*hi_b + hi_a*lo_b + lo_a*hi_b (three MFMAs
+// accumulating in FP32); the dropped lo_a*lo_b term is below 2^-16 |ab|.
+// Posteriors are FP32 (shared with the FP32 form), the per-chunk partial sums
+// are flushed into FP64 at fixed intervals, and every cross-chunk reduction,
+// the Cholesky and the log-determinant stay FP64.
+//
+// MFMA 16x16x32 operand layout (wave64, lane l, g = l/16, c = l%16):
+// A[i][k] for i = c, k = 8g..8g+7 (eight bf16 in four VGPRs, element e in
+// the low half of word e/2 when e is even); B[k][j] for j = c, the same k;
+// accumulator element v of lane l holds D[4g + v][c].
+#include "fem_internal.h"
+
+#include <cmath>
+
+namespace fem {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+
+// E-step pairs (ls, jb): 32-feature input strip ls, 16-feature output block jb,
+// with jb >= 2 ls (U_k^T is lower triangular).
+__device__ __host__ constexpr int pairs_bf16(int NB) {
+    int p = 0;
+    for (int ls = 0; 2 * ls < NB; ls++) p += NB - 2 * ls;
+    return p;
+}
+
+// bf16 bits of x, round to nearest even (operands here are finite).
+__device__ __host__ inline unsigned int bf16_bits(float x) {
+    const unsigned int u = __builtin_bit_cast(unsigned int, x);
+    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+
+__device__ inline float bf16_value(unsigned int b) { return __builtin_bit_cast(float, b << 16); }
+
+// Split eight floats into packed hi and lo bf16 operands.
+__device__ inline void split8(const float (&t)[8], v8bf &hi, v8bf &lo) {
+    v4u h, l;
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+        const unsigned int h0 = bf16_bits(t[2 * w]), h1 = bf16_bits(t[2 * w + 1]);
+        const unsigned int l0 = bf16_bits(t[2 * w] - bf16_value(h0));
+        const unsigned int l1 = bf16_bits(t[2 * w + 1] - bf16_value(h1));
+        h[w] = h0 | (h1 << 16);
+        l[w] = l0 | (l1 << 16);
+    }
+    hi = __builtin_bit_cast(v8bf, h);
+    lo = __builtin_bit_cast(v8bf, l);
+}
+
+__device__ inline v4f mfma3(v8bf ah, v8bf al, v8bf bh, v8bf bl, v4f acc) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc, 0, 0, 0);
+    return acc;
+}
+
+__device__ inline float xor_sum_gb(float v) {
+    v += __shfl_xor(v, 16);
+    v += __shfl_xor(v, 32);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// E-step operands of component k from the FP64 factor U_k = L_k^-T:
+// fragments A[i][kk] = U_k^T[16 jb + i][32 ls + kk] = U_k[32 ls + kk][16 jb + i]
+// for every pair (ls, jb), split hi/lo, stored [pair][hi|lo][lane][4 words];
+// and mlo = U_k^T mu_lo with mu_lo = mu - float(mu), which seeds the
+// accumulators so that D = U_k^T (x - mu_hi) - U_k^T mu_lo.
+__global__ __launch_bounds__(256) void k_prep_bf16(int d, int dpad, int NB, const double *__restrict__ U,
+                                                   const double *__restrict__ mu, unsigned int *__restrict__ ufragb,
+                                                   float *__restrict__ mlo) {
+    const int k = blockIdx.x;
+    const int np = pairs_bf16(NB);
+    const double *Uk = U + (size_t)k * dpad * dpad;
+    for (int e = threadIdx.x; e < np * 64 * 4; e += blockDim.x) {
+        const int p = e / 256, lane = (e / 4) % 64, w = e % 4;
+        int ls = 0, rem = p;
+        while (rem >= NB - 2 * ls) {
+            rem -= NB - 2 * ls;
+            ls++;
+        }
+        const int jb = 2 * ls + rem;
+        const int g = lane >> 4, c = lane & 15;
+        unsigned int hw = 0, lw = 0;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int f = 32 * ls + 8 * g + 2 * w + h;
+            const double v = (f < dpad) ? Uk[(size_t)f * dpad + 16 * jb + c] : 0.0;
+            const unsigned int hb = bf16_bits((float)v);
+            const unsigned int lb = bf16_bits((float)(v - (double)bf16_value(hb)));
+            hw |= hb << (16 * h);
+            lw |= lb << (16 * h);
+        }
+        unsigned int *dst = ufragb + (((size_t)k * np + p) * 2) * 256;
+        dst[lane * 4 + w] = hw;
+        dst[256 + lane * 4 + w] = lw;
+    }
+    for (int j = threadIdx.x; j < dpad; j += blockDim.x) {
+        double s = 0.0;
+        if (j < d)
+            for (int f = 0; f <= j; f++) {
+                const double m = mu[(size_t)k * d + f];
+                s += Uk[(size_t)f * dpad + j] * (m - (double)(float)m);
+            }
+        mlo[(size_t)k * dpad + j] = (float)s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stage (a), BF16x3.  Workgroup = 4 waves x P groups of 16 points.  Lane
+// (g, c) holds, for point c of each group, features 32 ls + 8g .. +7 of every
+// strip ls as FP32 (the B operand layout).  Per component: centre with mu_hi,
+// split, 3 MFMAs per pair; the component operands (fragments, mu_hi, mlo) are
+// double-buffered in LDS as in the FP32 form, one barrier per component.
+template <int NB>
+__device__ __host__ constexpr int estepb_stage_words() {
+    return pairs_bf16(NB) * 512 + 32 * ((NB + 1) / 2) + 16 * NB;
+}
+
+template <int NB, int P>
+__global__ __launch_bounds__(256) void k_estepb(const float *__restrict__ X, int64_t n, int64_t npad, int K,
+                                                const unsigned int *__restrict__ ufragb,
+                                                const float *__restrict__ mus32, const float *__restrict__ mlo,
+                                                const double *__restrict__ coef, float *__restrict__ Rt,
+                                                double *__restrict__ ll_part) {
+    constexpr int DP = 16 * NB;
+    constexpr int NS = (NB + 1) / 2;
+    constexpr int NPAIR = pairs_bf16(NB);
+    constexpr int FW = NPAIR * 512;  // fragment words per component
+    constexpr int STAGE = estepb_stage_words<NB>();
+    constexpr int PER = (STAGE + 255) / 256;
+    extern __shared__ unsigned int ldsb[];  // 2 * STAGE words
+    __shared__ double wsum[4];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int g = lane >> 4, c = lane & 15;
+    const int64_t base = (int64_t)blockIdx.x * (64 * P) + wave * (16 * P);
+
+    float xr[P][NS][8];
+#pragma unroll
+    for (int p = 0; p < P; p++)
+#pragma unroll
+        for (int ls = 0; ls < NS; ls++) {
+            const int f0 = 32 * ls + 8 * g;
+            if (f0 < DP) {
+                const float4 *src = reinterpret_cast<const float4 *>(X + (base + 16 * p + c) * DP + f0);
+                const float4 u0 = src[0], u1 = src[1];
+                xr[p][ls][0] = u0.x; xr[p][ls][1] = u0.y; xr[p][ls][2] = u0.z; xr[p][ls][3] = u0.w;
+                xr[p][ls][4] = u1.x; xr[p][ls][5] = u1.y; xr[p][ls][6] = u1.z; xr[p][ls][7] = u1.w;
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; e++) xr[p][ls][e] = 0.0f;
+            }
+        }
+
+    // stage word e of component k: fragments, then mu_hi (32 NS, zero beyond
+    // dpad), then mlo (dpad)
+    auto stage_src = [&](int k, int e) -> unsigned int {
+        if (e < FW) return ufragb[(size_t)k * FW + e];
+        e -= FW;
+        if (e < 32 * NS) return (e < DP) ? __builtin_bit_cast(unsigned int, mus32[(size_t)k * 2 * DP + e]) : 0u;
+        e -= 32 * NS;
+        return __builtin_bit_cast(unsigned int, mlo[(size_t)k * DP + e]);
+    };
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int e = tid + 256 * q;
+        if (e < STAGE) ldsb[e] = stage_src(0, e);
+    }
+    __syncthreads();
+
+    float mx[P];
+#pragma unroll
+    for (int p = 0; p < P; p++) mx[p] = -INFINITY;
+
+    for (int k = 0; k < K; k++) {
+        const unsigned int *lf = ldsb + (k & 1) * STAGE;
+        const float *lmh = reinterpret_cast<const float *>(lf + FW);
+        const float *lml = lmh + 32 * NS;
+        unsigned int pre[PER];
+        const bool more = k + 1 < K;
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int e = tid + 256 * q;
+            pre[q] = (more && e < STAGE) ? stage_src(k + 1, e) : 0u;
+        }
+        const float ck = (float)coef[k];
+        v4f acc[P][NB];
+#pragma unroll
+        for (int jb = 0; jb < NB; jb++) {
+            const float4 m = *reinterpret_cast<const float4 *>(lml + 16 * jb + 4 * g);
+#pragma unroll
+            for (int p = 0; p < P; p++) acc[p][jb] = (v4f){-m.x, -m.y, -m.z, -m.w};
+        }
+        int pr = 0;
+#pragma unroll
+        for (int ls = 0; ls < NS; ls++) {
+            const float4 m0 = *reinterpret_cast<const float4 *>(lmh + 32 * ls + 8 * g);
+            const float4 m1 = *reinterpret_cast<const float4 *>(lmh + 32 * ls + 8 * g + 4);
+            const float mh[8] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
+            v8bf bh[P], bl[P];
+#pragma unroll
+            for (int p = 0; p < P; p++) {
+                float t[8];
+#pragma unroll
+                for (int e = 0; e < 8; e++) t[e] = xr[p][ls][e] - mh[e];
+                split8(t, bh[p], bl[p]);
+            }
+#pragma unroll
+            for (int jb = 2 * ls; jb < NB; jb++) {
+                const v4u *fp = reinterpret_cast<const v4u *>(lf + pr * 512);
+                const v8bf ah = __builtin_bit_cast(v8bf, fp[lane]);
+                const v8bf al = __builtin_bit_cast(v8bf, fp[64 + lane]);
+                pr++;
+#pragma unroll
+                for (int p = 0; p < P; p++) acc[p][jb] = mfma3(ah, al, bh[p], bl[p], acc[p][jb]);
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            float q = 0.f;
+#pragma unroll
+            for (int jb = 0; jb < NB; jb++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) q += acc[p][jb][v] * acc[p][jb][v];
+            q = xor_sum_gb(q);
+            const float s = -0.5f * q + ck;
+            mx[p] = fmaxf(mx[p], s);
+            if ((k & 3) == g) Rt[(size_t)k * npad + base + 16 * p + c] = s;
+        }
+        if (more) {
+            unsigned int *dst = ldsb + ((k + 1) & 1) * STAGE;
+#pragma unroll
+            for (int q = 0; q < PER; q++) {
+                const int e = tid + 256 * q;
+                if (e < STAGE) dst[e] = pre[q];
+            }
+        }
+        __syncthreads();
+    }
+
+    double llsum = 0.0;
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        const int64_t pt = base + 16 * p + c;
+        float se = 0.f;
+        for (int k = g; k < K; k += 4) se += expf(Rt[(size_t)k * npad + pt] - mx[p]);
+        se = xor_sum_gb(se);
+        const float lse = logf(se) + mx[p];
+        const bool valid = pt < n;
+        for (int k = g; k < K; k += 4) {
+            float *r = Rt + (size_t)k * npad + pt;
+            *r = valid ? expf(*r - lse) : 0.0f;
+        }
+        if (g == 0 && valid) llsum += (double)lse;
+    }
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) llsum += __shfl_xor(llsum, o);
+    if (lane == 0) wsum[wave] = llsum;
+    __syncthreads();
+    if (tid == 0) ll_part[blockIdx.x] = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
+}
+
+__global__ __launch_bounds__(256) void k_reduce_sumb(const double *__restrict__ parts, int64_t n,
+                                                     double *__restrict__ out) {
+    __shared__ double s[256];
+    double a = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 256) a += parts[i];
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = s[0];
+}
+
+hipError_t launch_prep_bf16(fem_ctx *c) {
+    hipLaunchKernelGGL(k_prep_bf16, dim3(c->K), dim3(256), 0, c->stream, c->d, c->dpad, c->NB, c->U, c->mu,
+                       c->ufragb, c->mlob);
+    return hipGetLastError();
+}
+
+template <int NB, int P>
+static hipError_t estepb_t(fem_ctx *c) {
+    const size_t lds = (size_t)2 * estepb_stage_words<NB>() * sizeof(unsigned int);
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_estepb<NB, P>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    const int blocks = (int)(c->npad / (64 * P));
+    hipLaunchKernelGGL((k_estepb<NB, P>), dim3(blocks), dim3(256), lds, c->stream, c->X, c->n, c->npad, c->K,
+                       c->ufragb, c->mus32, c->mlob, c->coef, c->Rt32, c->ll_part);
+    hipLaunchKernelGGL(k_reduce_sumb, dim3(1), dim3(256), 0, c->stream, c->ll_part, (int64_t)blocks, c->ll_dev);
+    return hipGetLastError();
+}
+
+// Point groups per wave: 2 by default, 4 for d <= 64 when estep_groups32 == 4.
+hipError_t launch_estepb(fem_ctx *c) {
+    const bool p4 = c->estep_groups32 == 4;
+    switch (c->NB) {
+        case 1: return p4 ? estepb_t<1, 4>(c) : estepb_t<1, 2>(c);
+        case 2: return p4 ? estepb_t<2, 4>(c) : estepb_t<2, 2>(c);
+        case 3: return p4 ? estepb_t<3, 4>(c) : estepb_t<3, 2>(c);
+        case 4: return p4 ? estepb_t<4, 4>(c) : estepb_t<4, 2>(c);
+        case 5: return estepb_t<5, 2>(c);
+        case 6: return estepb_t<6, 2>(c);
+        case 7: return estepb_t<7, 2>(c);
+        case 8: return estepb_t<8, 2>(c);
+    }
+    return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------
+// Stage (b), BF16x3: M[k][j] = sum_i R[i][k] X[i][j] per 4096-point chunk.
+// A = R^T fragment (components 16 kb + c, points 8g..8g+7 of the step), B = X
+// fragment (points 8g.., features 16 jb + c); FP32 accumulators flushed into
+// FP64 every 1024 points.  Output layout as the FP32 form.
+template <int NB>
+__global__ __launch_bounds__(256) void k_meansb(const float *__restrict__ X, const float *__restrict__ Rt,
+                                                int64_t npad, int KP, double *__restrict__ mpart) {
+    constexpr int DP = 16 * NB;
+    constexpr int CH = 4096, FLUSH = 1024;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int g = lane >> 4, c = lane & 15;
+    const int chunk = blockIdx.x;
+    const int64_t i_begin = (int64_t)chunk * CH;
+    const int KB = KP / 16;
+    double *out = mpart + (size_t)chunk * KP * (DP + 1);
+    for (int kb = wave; kb < KB; kb += 4) {
+        const float *rrow = Rt + (size_t)(16 * kb + c) * npad + i_begin + 8 * g;
+        const float *xrow = X + (i_begin + 8 * g) * DP + c;
+        double accd[NB][4];
+#pragma unroll
+        for (int jb = 0; jb < NB; jb++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) accd[jb][v] = 0.0;
+        double nsd = 0.0;
+        for (int s0 = 0; s0 < CH; s0 += FLUSH) {
+            v4f acc[NB];
+#pragma unroll
+            for (int jb = 0; jb < NB; jb++) acc[jb] = (v4f){0.f, 0.f, 0.f, 0.f};
+            float ns = 0.f;
+            for (int s = s0; s < s0 + FLUSH; s += 32) {
+                const float4 r0 = *reinterpret_cast<const float4 *>(rrow + s);
+                const float4 r1 = *reinterpret_cast<const float4 *>(rrow + s + 4);
+                const float rv[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
+                ns += ((rv[0] + rv[1]) + (rv[2] + rv[3])) + ((rv[4] + rv[5]) + (rv[6] + rv[7]));
+                v8bf rh, rl;
+                split8(rv, rh, rl);
+#pragma unroll
+                for (int jb = 0; jb < NB; jb++) {
+                    float xv[8];
+#pragma unroll
+                    for (int e = 0; e < 8; e++) xv[e] = xrow[(size_t)(s + e) * DP + 16 * jb];
+                    v8bf xh, xl;
+                    split8(xv, xh, xl);
+                    acc[jb] = mfma3(rh, rl, xh, xl, acc[jb]);
+                }
+            }
+#pragma unroll
+            for (int jb = 0; jb < NB; jb++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) accd[jb][v] += (double)acc[jb][v];
+            nsd += (double)ns;
+        }
+#pragma unroll
+        for (int jb = 0; jb < NB; jb++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) out[(size_t)(16 * kb + 4 * g + v) * (DP + 1) + 16 * jb + c] = accd[jb][v];
+        nsd += __shfl_xor(nsd, 16);
+        nsd += __shfl_xor(nsd, 32);
+        if (g == 0) out[(size_t)(16 * kb + c) * (DP + 1) + DP] = nsd;
+    }
+}
+
+template <int NB>
+static hipError_t meansb_t(fem_ctx *c) {
+    hipLaunchKernelGGL(k_meansb<NB>, dim3(c->m_chunks), dim3(256), 0, c->stream, c->X, c->Rt32, c->npad, c->KP,
+                       c->mpart);
+    return hipGetLastError();
+}
+
+hipError_t launch_meansb_partials(fem_ctx *c) {
+    switch (c->NB) {
+        case 1: return meansb_t<1>(c);
+        case 2: return meansb_t<2>(c);
+        case 3: return meansb_t<3>(c);
+        case 4: return meansb_t<4>(c);
+        case 5: return meansb_t<5>(c);
+        case 6: return meansb_t<6>(c);
+        case 7: return meansb_t<7>(c);
+        case 8: return meansb_t<8>(c);
+    }
+    return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------
+// Stage (c), BF16x3: T = (x - mu_hi) sqrt(r / N_k) in FP32, split, and
+// D += T_ib^T T_jb (upper blocks) with three MFMAs per block per 32 points.
+// The same register fragment is the A operand (T_ib^T) and the B operand
+// (T_jb).  Centring with mu_hi alone changes D by N_k mu_lo mu_lo^T plus a
+// term proportional to sum_i r_ik (x_i - mu_k), both below 1e-9 of D here.
+//
+// For d > 64 the upper blocks of a component are split into NPART = 2 parts
+// (contiguous in row-major block order, equal counts), each its own wave, so
+// a wave's accumulators stay in registers; a part forms T only for the
+// feature blocks its upper blocks touch.  FP32 accumulators are flushed into
+// FP64 every 2048 points (registers for d <= 48, else the wave's own blocks of
+// the partial slice).  Grid and XCD remapping as the FP32 form, with the part
+// as the fastest-varying index; the loads of the next 32 points are issued
+// once the current 32 are split, before their MFMAs.
+__device__ __host__ constexpr int cov_parts(int NB) { return NB > 4 ? 2 : 1; }
+
+// Part of upper block u (row-major over ib <= jb) among NPART parts.
+__device__ __host__ constexpr int cov_part_of(int u, int UB, int NPART) { return (u * NPART) / UB; }
+
+__device__ __host__ constexpr bool cov_needs_fb(int NB, int PART, int fb) {
+    const int UB = NB * (NB + 1) / 2, NPART = cov_parts(NB);
+    int u = 0;
+    for (int ib = 0; ib < NB; ib++)
+        for (int jb = ib; jb < NB; jb++, u++)
+            if (cov_part_of(u, UB, NPART) == PART && (ib == fb || jb == fb)) return true;
+    return false;
+}
+
+__device__ __host__ constexpr int cov_part_begin(int NB, int PART) {
+    const int UB = NB * (NB + 1) / 2, NPART = cov_parts(NB);
+    for (int u = 0; u < UB; u++)
+        if (cov_part_of(u, UB, NPART) == PART) return u;
+    return UB;
+}
+
+__device__ __host__ constexpr int cov_part_size(int NB, int PART) {
+    const int UB = NB * (NB + 1) / 2, NPART = cov_parts(NB);
+    int s = 0;
+    for (int u = 0; u < UB; u++) s += cov_part_of(u, UB, NPART) == PART;
+    return s;
+}
+
+template <int NB, int PART>
+__device__ inline void covb_wave(const float *__restrict__ X, const float *__restrict__ rrow, int64_t i0, int64_t i1,
+                                 const float (&mh)[NB], float inv, int g, int c, double *__restrict__ out) {
+    constexpr int DP = 16 * NB;
+    constexpr int UB = NB * (NB + 1) / 2;
+    constexpr int NPART = cov_parts(NB);
+    constexpr int U0 = cov_part_begin(NB, PART);
+    constexpr int NU = cov_part_size(NB, PART);
+    constexpr int FLUSH = 2048;
+    constexpr bool REG = NB <= 3;
+    double accd[REG ? NU : 1][4];
+    if (REG) {
+#pragma unroll
+        for (int u = 0; u < (REG ? NU : 1); u++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) accd[u][v] = 0.0;
+    }
+    float rv[8], xv[NB][8];
+    auto load32 = [&](int64_t i) {
+        const float *rp = rrow + i + 8 * g;
+        const float4 r0 = *reinterpret_cast<const float4 *>(rp);
+        const float4 r1 = *reinterpret_cast<const float4 *>(rp + 4);
+        rv[0] = r0.x; rv[1] = r0.y; rv[2] = r0.z; rv[3] = r0.w;
+        rv[4] = r1.x; rv[5] = r1.y; rv[6] = r1.z; rv[7] = r1.w;
+        const float *xp = X + (i + 8 * g) * DP + c;
+#pragma unroll
+        for (int fb = 0; fb < NB; fb++)
+            if (cov_needs_fb(NB, PART, fb))
+#pragma unroll
+                for (int e = 0; e < 8; e++) xv[fb][e] = xp[e * DP + 16 * fb];
+    };
+    if (i0 < i1) load32(i0);
+    bool first = true;
+    for (int64_t s0 = i0; s0 < i1; s0 += FLUSH) {
+        const int64_t s1 = (s0 + FLUSH < i1) ? s0 + FLUSH : i1;
+        v4f acc[NU];
+#pragma unroll
+        for (int u = 0; u < NU; u++) acc[u] = (v4f){0.f, 0.f, 0.f, 0.f};
+        for (int64_t i = s0; i < s1; i += 32) {
+            float sw[8];
+#pragma unroll
+            for (int e = 0; e < 8; e++) sw[e] = sqrtf(rv[e]) * inv;
+            v8bf th[NB], tl[NB];
+#pragma unroll
+            for (int fb = 0; fb < NB; fb++) {
+                if (!cov_needs_fb(NB, PART, fb)) continue;
+                float t[8];
+#pragma unroll
+                for (int e = 0; e < 8; e++) t[e] = (xv[fb][e] - mh[fb]) * sw[e];
+                split8(t, th[fb], tl[fb]);
+            }
+            if (i + 32 < i1) load32(i + 32);
+            int u = 0;
+#pragma unroll
+            for (int ib = 0; ib < NB; ib++)
+#pragma unroll
+                for (int jb = ib; jb < NB; jb++, u++) {
+                    if (cov_part_of(u, UB, NPART) != PART) continue;
+                    acc[u - U0] = mfma3(th[ib], tl[ib], th[jb], tl[jb], acc[u - U0]);
+                }
+        }
+        if (REG) {
+#pragma unroll
+            for (int u = 0; u < (REG ? NU : 1); u++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) accd[u][v] += (double)acc[u][v];
+        } else {
+#pragma unroll
+            for (int u = 0; u < NU; u++) {
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    double *o = out + (size_t)(U0 + u) * 256 + (4 * g + v) * 16 + c;
+                    *o = (first ? 0.0 : *o) + (double)acc[u][v];
+                }
+                __asm__ volatile("" ::: "memory");  // one block's read-add-writes at a time
+            }
+        }
+        first = false;
+    }
+    if (REG) {
+#pragma unroll
+        for (int u = 0; u < (REG ? NU : 1); u++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) out[(size_t)(U0 + u) * 256 + (4 * g + v) * 16 + c] = accd[u][v];
+    }
+}
+
+template <int NB>
+__global__ __launch_bounds__(256) void k_covb(const float *__restrict__ X, const float *__restrict__ Rt,
+                                              int64_t npad, int K, int KP, int d, const double *__restrict__ mu,
+                                              const double *__restrict__ nk, int64_t chunk_rows,
+                                              double *__restrict__ cpart) {
+    constexpr int UB = NB * (NB + 1) / 2;
+    constexpr int NPART = cov_parts(NB);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int g = lane >> 4, c = lane & 15;
+    const int KG = KP / 4;
+    const int nblk = gridDim.x;
+    const int b = blockIdx.x;
+    const int logical = (nblk % 8 == 0) ? (b % 8) * (nblk / 8) + b / 8 : b;
+    const int part = logical % NPART, rest = logical / NPART;
+    const int cg = rest % KG, chunk = rest / KG;
+    const int k = 4 * cg + wave;
+    if (k >= K) return;
+    const int64_t i0 = (int64_t)chunk * chunk_rows;
+    const int64_t i1 = (i0 + chunk_rows < npad) ? i0 + chunk_rows : npad;
+    float mh[NB];
+#pragma unroll
+    for (int fb = 0; fb < NB; fb++) mh[fb] = (16 * fb + c < d) ? (float)mu[(size_t)k * d + 16 * fb + c] : 0.0f;
+    const float inv = (float)(1.0 / sqrt(nk[k]));
+    double *out = cpart + ((size_t)chunk * KP + k) * UB * 256;
+    const float *rrow = Rt + (size_t)k * npad;
+    if (part == 0) covb_wave<NB, 0>(X, rrow, i0, i1, mh, inv, g, c, out);
+    else if (NPART > 1) covb_wave<NB, (NPART > 1 ? 1 : 0)>(X, rrow, i0, i1, mh, inv, g, c, out);
+}
+
+template <int NB>
+static hipError_t covb_t(fem_ctx *c) {
+    const int blocks = (c->KP / 4) * c->c_chunks * cov_parts(NB);
+    hipLaunchKernelGGL(k_covb<NB>, dim3(blocks), dim3(256), 0, c->stream, c->X, c->Rt32, c->npad, c->K, c->KP,
+                       c->d, c->mu, c->nk, c->c_chunk_rows, c->cpart);
+    return hipGetLastError();
+}
+
+hipError_t launch_covb_partials(fem_ctx *c) {
+    switch (c->NB) {
+        case 1: return covb_t<1>(c);
+        case 2: return covb_t<2>(c);
+        case 3: return covb_t<3>(c);
+        case 4: return covb_t<4>(c);
+        case 5: return covb_t<5>(c);
+        case 6: return covb_t<6>(c);
+        case 7: return covb_t<7>(c);
+        case 8: return covb_t<8>(c);
+    }
+    return hipErrorInvalidValue;
+}
+
+int estepb_pairs(int NB) { return pairs_bf16(NB); }
+
+}  // namespace fem
