@@ -51,6 +51,8 @@ _SIGS = {
                                        _dp]),
     "fem_kmeans_init_rng": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _dp, _dp,
                                            _dp]),
+    "fem_kmeans_init_seeded": (ctypes.c_int, [_vp, _dp, ctypes.c_uint64, ctypes.c_int, _dp, _dp,
+                                              _dp]),
     "fem_last_error": (ctypes.c_char_p, [_vp]),
     "fem_version": (ctypes.c_char_p, []),
     "fem_batch_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int,
@@ -206,6 +208,18 @@ class Context:
         cov = np.empty((self.K, self.d, self.d))
         self._check(lib().fem_kmeans_init(self._h, seed, seeding, max_sweeps, _d(w), _d(mu),
                                           _d(cov)))
+        return w, mu, cov
+
+    def kmeans_init_seeded(self, C0, seed=0, max_sweeps=29):
+        """Lloyd sweeps and the cluster mixture from given centroids C0
+        (K x d); becomes the parameters."""
+        C0 = np.ascontiguousarray(C0, dtype=np.float64)
+        assert C0.shape == (self.K, self.d)
+        w = np.empty(self.K)
+        mu = np.empty((self.K, self.d))
+        cov = np.empty((self.K, self.d, self.d))
+        self._check(lib().fem_kmeans_init_seeded(self._h, _d(C0), seed, max_sweeps, _d(w), _d(mu),
+                                                 _d(cov)))
         return w, mu, cov
 
     def times(self):

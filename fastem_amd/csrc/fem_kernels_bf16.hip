@@ -22,6 +22,8 @@ namespace fem {
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+typedef float v2f __attribute__((ext_vector_type(2)));
 
 // E-step pairs (ls, jb): 32-feature input strip ls, 16-feature output block jb,
 // with jb >= 2 ls (U_k^T is lower triangular).
@@ -39,16 +41,19 @@ __device__ __host__ inline unsigned int bf16_bits(float x) {
 
 __device__ inline float bf16_value(unsigned int b) { return __builtin_bit_cast(float, b << 16); }
 
-// Split eight floats into packed hi and lo bf16 operands.
+// Split eight floats into packed hi and lo bf16 operands (v_cvt_pk_bf16_f32,
+// round to nearest even; the hi halves are widened back by a shift or mask).
 __device__ inline void split8(const float (&t)[8], v8bf &hi, v8bf &lo) {
     v4u h, l;
 #pragma unroll
     for (int w = 0; w < 4; w++) {
-        const unsigned int h0 = bf16_bits(t[2 * w]), h1 = bf16_bits(t[2 * w + 1]);
-        const unsigned int l0 = bf16_bits(t[2 * w] - bf16_value(h0));
-        const unsigned int l1 = bf16_bits(t[2 * w + 1] - bf16_value(h1));
-        h[w] = h0 | (h1 << 16);
-        l[w] = l0 | (l1 << 16);
+        const v2bf hb = __builtin_convertvector((v2f){t[2 * w], t[2 * w + 1]}, v2bf);
+        const unsigned int hw = __builtin_bit_cast(unsigned int, hb);
+        const float h0 = __builtin_bit_cast(float, hw << 16);
+        const float h1 = __builtin_bit_cast(float, hw & 0xFFFF0000u);
+        const v2bf lb = __builtin_convertvector((v2f){t[2 * w] - h0, t[2 * w + 1] - h1}, v2bf);
+        h[w] = hw;
+        l[w] = __builtin_bit_cast(unsigned int, lb);
     }
     hi = __builtin_bit_cast(v8bf, h);
     lo = __builtin_bit_cast(v8bf, l);

@@ -194,6 +194,59 @@ double assign(fem_ctx *c, const std::vector<double> &C, std::vector<double> *sum
     return fit / (double)c->n;
 }
 
+// Lloyd sweeps from centroids C (NaiveKMA::RunKMeans, MinRel 1e-4) and the
+// mixture of the final clustering (EM2/EMInitializer.cpp:103-134): counts,
+// covariances about the centroids divided by (n_k - 1), weights n_k / n.
+// Empty clusters are re-seeded with a row drawn by `draw`.  Throws KmErr.
+template <typename Draw>
+int lloyd_and_mixture(fem_ctx *c, std::vector<double> &C, Draw &&draw, int max_sweeps, double *w, double *mu,
+                      double *cov) {
+    const int K = c->K, d = c->d;
+    const int64_t n = c->n;
+    double best = std::numeric_limits<double>::max();
+    std::vector<double> sums((size_t)K * d), counts(K);
+    for (int it = 1; it <= max_sweeps; it++) {
+        const double fit = assign(c, C, &sums, &counts);
+        for (int k = 0; k < K; k++) {
+            if (counts[k] > 0) {
+                const double f = 1.0 / counts[k];
+                for (int j = 0; j < d; j++) C[(size_t)k * d + j] = sums[(size_t)k * d + j] * f;
+            } else {
+                const int64_t src = (int64_t)(draw() * (double)n);
+                fetch_row(c, src, C.data() + (size_t)k * d);
+            }
+        }
+        const double rel = (best - fit) / best;
+        if (fit < std::nextafter(fit, best)) best = fit;
+        else break;
+        if (rel < 1e-4) break;
+        if (it == max_sweeps) break;
+    }
+    // mixture of the final clustering: counts, then covariances about the
+    // centroids divided by (n_k - 1), weights n_k / n
+    assign(c, C, &sums, &counts);
+    int rc = FEM_OK;
+    std::vector<double> nk1(K);
+    for (int k = 0; k < K; k++) {
+        nk1[k] = counts[k] - 1.0;
+        if (counts[k] < 2) rc = FEM_ERR_ARG;
+    }
+    KM_HIP(hipMemcpyAsync(c->mu, C.data(), sizeof(double) * K * d, hipMemcpyHostToDevice, c->stream));
+    KM_HIP(hipMemcpyAsync(c->nk, nk1.data(), sizeof(double) * K, hipMemcpyHostToDevice, c->stream));
+    KM_HIP(fem::launch_cov64_partials(c));
+    KM_HIP(fem::launch_cov_finalize(c, 0.0));
+    for (int k = 0; k < K; k++) w[k] = counts[k] / (double)n;
+    KM_HIP(hipMemcpyAsync(c->w, w, sizeof(double) * K, hipMemcpyHostToDevice, c->stream));
+    KM_HIP(hipMemcpyAsync(cov, c->cov, sizeof(double) * K * d * d, hipMemcpyDeviceToHost, c->stream));
+    KM_HIP(hipStreamSynchronize(c->stream));
+    std::memcpy(mu, C.data(), sizeof(double) * K * d);
+    c->have_params = true;
+    c->factored = false;
+    c->have_post = false;
+    if (rc != FEM_OK) c->err = "k-means left a cluster with fewer than two points";
+    return rc;
+}
+
 }  // namespace
 
 extern "C" int fem_kmeans_init_rng(fem_ctx *c, double (*rnd)(void *), void *rnd_state, int seeding,
@@ -280,49 +333,30 @@ extern "C" int fem_kmeans_init_rng(fem_ctx *c, double (*rnd)(void *), void *rnd_
                 for (int j = 0; j < d; j++) C[(size_t)k * d + j] = sums[(size_t)k * d + j] * f;
             }
         }
-        // Lloyd sweeps (NaiveKMA::RunKMeans, MinRel 1e-4)
-        double best = std::numeric_limits<double>::max();
-        std::vector<double> sums((size_t)K * d), counts(K);
-        for (int it = 1; it <= max_sweeps; it++) {
-            const double fit = assign(c, C, &sums, &counts);
-            for (int k = 0; k < K; k++) {
-                if (counts[k] > 0) {
-                    const double f = 1.0 / counts[k];
-                    for (int j = 0; j < d; j++) C[(size_t)k * d + j] = sums[(size_t)k * d + j] * f;
-                } else {
-                    const int64_t src = (int64_t)(draw() * (double)n);
-                    fetch_row(c, src, C.data() + (size_t)k * d);
-                }
-            }
-            const double rel = (best - fit) / best;
-            if (fit < std::nextafter(fit, best)) best = fit;
-            else break;
-            if (rel < 1e-4) break;
-            if (it == max_sweeps) break;
-        }
-        // mixture of the final clustering: counts, then covariances about the
-        // centroids divided by (n_k - 1), weights n_k / n
-        assign(c, C, &sums, &counts);
-        int rc = FEM_OK;
-        std::vector<double> nk1(K);
-        for (int k = 0; k < K; k++) {
-            nk1[k] = counts[k] - 1.0;
-            if (counts[k] < 2) rc = FEM_ERR_ARG;
-        }
-        KM_HIP(hipMemcpyAsync(c->mu, C.data(), sizeof(double) * K * d, hipMemcpyHostToDevice, c->stream));
-        KM_HIP(hipMemcpyAsync(c->nk, nk1.data(), sizeof(double) * K, hipMemcpyHostToDevice, c->stream));
-        KM_HIP(fem::launch_cov64_partials(c));
-        KM_HIP(fem::launch_cov_finalize(c, 0.0));
-        for (int k = 0; k < K; k++) w[k] = counts[k] / (double)n;
-        KM_HIP(hipMemcpyAsync(c->w, w, sizeof(double) * K, hipMemcpyHostToDevice, c->stream));
-        KM_HIP(hipMemcpyAsync(cov, c->cov, sizeof(double) * K * d * d, hipMemcpyDeviceToHost, c->stream));
-        KM_HIP(hipStreamSynchronize(c->stream));
-        std::memcpy(mu, C.data(), sizeof(double) * K * d);
-        c->have_params = true;
-        c->factored = false;
-        c->have_post = false;
-        if (rc != FEM_OK) c->err = "k-means left a cluster with fewer than two points";
-        return rc;
+        return lloyd_and_mixture(c, C, draw, max_sweeps, w, mu, cov);
+    } catch (KmErr &e) {
+        c->err = e.msg;
+        return e.code;
+    }
+}
+
+// Lloyd sweeps and the cluster mixture from caller-supplied centroids
+// (K x d, e.g. the learned initializer's proposals); empty clusters are
+// re-seeded from a mt19937_64(seed) row draw.
+extern "C" int fem_kmeans_init_seeded(fem_ctx *c, const double *C0, uint64_t seed, int max_sweeps, double *w,
+                                      double *mu, double *cov) {
+    if (!c) return FEM_ERR_ARG;
+    if (!C0 || !w || !mu || !cov || max_sweeps < 0) {
+        c->err = "fem_kmeans_init_seeded: bad argument";
+        return FEM_ERR_ARG;
+    }
+    try {
+        KM_HIP(hipSetDevice(c->device));
+        ensure_scratch(c);
+        std::vector<double> C(C0, C0 + (size_t)c->K * c->d);
+        std::mt19937_64 rng(seed);
+        auto draw = [&]() { return canonical(rng); };
+        return lloyd_and_mixture(c, C, draw, max_sweeps, w, mu, cov);
     } catch (KmErr &e) {
         c->err = e.msg;
         return e.code;

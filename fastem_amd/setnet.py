@@ -3,9 +3,10 @@ starting means and diagonal covariances for EM (fem_setnet_* in
 include/fastem_setnet.h; kernels in csrc/fem_setnet.hip).
 
 The reference has no such initializer: it starts EM from the Jain, k-means,
-k-means++ or file initializers (EM2/EMInitializer.cpp).  Here k-means++ stays
-the fallback and the final EM log-likelihood decides between proposals
-(`best_of`).
+k-means++ or file initializers (EM2/EMInitializer.cpp).  Here the network's
+means replace the k-means++ seeding pass (`init_context`: Lloyd sweeps and the
+cluster mixture follow on the device as for k-means++), k-means++ stays the
+fallback, and the final EM log-likelihood judges each proposal.
 
 Coordinates: a point set X is normalised by its own centroid c and one
 scale s (root mean square deviation over all coordinates), so the network
@@ -127,6 +128,26 @@ class SetNet:
                                           grad.ctypes.data_as(_fp) if want_grad else None,
                                           ctypes.byref(loss)))
         return (loss.value, grad) if want_grad else loss.value
+
+    def propose_means(self, X, n_sub=1024, seed=0):
+        """The network's K starting means for dataset X (K x d, FP64)."""
+        Xs = subsample(np.asarray(X), n_sub, seed)
+        Z, c, s = normalise(Xs)
+        mu, _ = self.forward(Z[None].astype(np.float32))
+        return c + s * mu[0].astype(np.float64)
+
+    def init_context(self, ctx, X, n_sub=1024, seed=0, lloyd_sweeps=29, fallback_seed=None):
+        """Learned initialisation of a device context holding X: the network's
+        means seed Lloyd sweeps and the cluster mixture on the device
+        (fem_kmeans_init_seeded) in place of the k-means++ seeding pass.  If
+        that leaves a cluster with fewer than two points, device k-means++
+        (fem_kmeans_init) is the fallback.  Returns (w, mu, cov, used)."""
+        C0 = self.propose_means(X, n_sub=n_sub, seed=seed)
+        try:
+            return ctx.kmeans_init_seeded(C0, seed=seed, max_sweeps=lloyd_sweeps) + ("learned",)
+        except FemError:
+            fs = seed if fallback_seed is None else fallback_seed
+            return ctx.kmeans_init(fs, seeding=1, max_sweeps=lloyd_sweeps) + ("kmeans++",)
 
     def propose(self, X, n_sub=1024, seed=0, var_floor=1e-6):
         """Starting mixture (w, mu, cov) for dataset X: uniform weights,

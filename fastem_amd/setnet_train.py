@@ -9,10 +9,11 @@
    data (normalised) and the converged mixture in the same units.
 2. Training: Adam on the set network's HIP forward/backward kernels
    (fem_setnet_step), random signed coordinate permutations as augmentation.
-3. Evaluation on `--eval` fresh seeds: EM from the network's proposal and EM
-   from device k-means++ (+ Lloyd) run in one batch; reported per seed are the
-   iterations to converge, the final log-likelihood of both, and the time of
-   each initializer on resident data.
+3. Evaluation on `--eval` fresh seeds: EM from the learned initialisation
+   (network means -> Lloyd -> cluster mixture) and EM from device k-means++
+   (seeding -> Lloyd -> cluster mixture) run in one batch to the reference's
+   stopping rule; reported per seed are the iterations to converge, the final
+   log-likelihood of both, and the time of each initializer on resident data.
 """
 import argparse
 import json
@@ -112,9 +113,14 @@ def train(net, data, steps, batch=64, lr=2e-3, seed=0, val_frac=0.02, log=print,
     return {"val_loss": hist, "seconds": time.perf_counter() - t0}
 
 
-def evaluate(net, seeds, n, d, K, gen, n_sub=1024, eps=1e-8, maxsweeps=1000, lloyd_iters=29, chunk=128,
+def evaluate(net, seeds, n, d, K, gen, n_sub=1024, eps=1e-5, maxsweeps=1000, lloyd_iters=29, chunk=128,
              log=print):
-    """EM from the network's proposal vs EM from k-means++ on fresh seeds."""
+    """EM from the learned initialisation vs EM from k-means++ on fresh seeds.
+
+    Both start with Lloyd sweeps and the cluster mixture on the device; they
+    differ in the seeds: the network's means (fem_kmeans_init_seeded) or the
+    k-means++ seeding pass (fem_kmeans_init).  EM then runs to the reference's
+    stopping rule |dLL/LL| < eps for both in one batch."""
     lo, hi, diag = gen
     recs = []
     ctx = fem.Context(n, d, K)
@@ -131,24 +137,35 @@ def evaluate(net, seeds, n, d, K, gen, n_sub=1024, eps=1e-8, maxsweeps=1000, llo
                 kpp = None
             t_kpp = time.perf_counter() - t
             t = time.perf_counter()
-            lrn = net.propose(X, n_sub=n_sub, seed=int(s))
+            try:
+                w, mu, cov, used = net.init_context(ctx, X, n_sub=n_sub, seed=int(s),
+                                                    lloyd_sweeps=lloyd_iters)
+                lrn = (w, mu, cov)
+            except fem.FemError:
+                lrn, used = None, "failed"
             t_lrn = time.perf_counter() - t
-            inits.append((X, kpp, lrn, t_kpp, t_lrn))
+            inits.append((X, kpp, lrn, used, t_kpp, t_lrn))
         b = fem.Batch([(n, d, K)] * (2 * len(cs)))
-        for i, (X, kpp, lrn, _, _) in enumerate(inits):
+        for i, (X, kpp, lrn, _, _, _) in enumerate(inits):
+            a0 = kpp if kpp is not None else lrn
+            a1 = lrn if lrn is not None else kpp
+            if a0 is None:
+                continue
             b.upload(2 * i, X)
             b.upload(2 * i + 1, X)
-            b.set_params(2 * i, *(kpp if kpp is not None else lrn))
-            b.set_params(2 * i + 1, *lrn)
+            b.set_params(2 * i, *a0)
+            b.set_params(2 * i + 1, *a1)
         b.run(maxsweeps, eps=eps)
         for i, s in enumerate(cs):
             rk, rl = b.result(2 * i), b.result(2 * i + 1)
             kpp_ok = inits[i][1] is not None and rk["status"] != fem.Batch.FAILED
+            lrn_ok = inits[i][2] is not None and rl["status"] != fem.Batch.FAILED
             recs.append({"seed": int(s), "kpp_ok": bool(kpp_ok), "kpp_ll": rk["ll"] if kpp_ok else None,
                          "kpp_iters": rk["iter_counter"] if kpp_ok else None,
-                         "learned_ok": rl["status"] != fem.Batch.FAILED, "learned_ll": rl["ll"],
-                         "learned_iters": rl["iter_counter"], "kpp_init_s": inits[i][3],
-                         "learned_init_s": inits[i][4]})
+                         "learned_ok": bool(lrn_ok), "learned_ll": rl["ll"] if lrn_ok else None,
+                         "learned_iters": rl["iter_counter"] if lrn_ok else None,
+                         "learned_used": inits[i][3],
+                         "kpp_init_s": inits[i][4], "learned_init_s": inits[i][5]})
         b.close()
         log(f"eval {min(c0 + chunk, len(seeds))}/{len(seeds)}")
     ctx.close()
@@ -177,6 +194,7 @@ def summarise(recs, tol=1e-6):
                                            if np.isfinite(k))),
         "mean_init_s_kpp": float(np.mean([r["kpp_init_s"] for r in recs])),
         "mean_init_s_learned": float(np.mean([r["learned_init_s"] for r in recs])),
+        "learned_fell_back_to_kpp": sum(r["learned_used"] != "learned" for r in recs),
     }
 
 
@@ -192,6 +210,7 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--eval", type=int, default=1000)
     ap.add_argument("--eval-seed0", type=int, default=10_000_000)
+    ap.add_argument("--eval-eps", type=float, default=1e-5)
     ap.add_argument("--out", default=None)
     ap.add_argument("--report", default=None)
     ap.add_argument("--net-seed", type=int, default=0)
@@ -211,7 +230,7 @@ def main(argv=None):
         net.save(a.out)
         log(f"saved {a.out}")
     recs = evaluate(net, np.arange(a.eval_seed0, a.eval_seed0 + a.eval), a.n, a.d, a.K, gen,
-                    n_sub=a.keep, log=log) if a.eval > 0 else []
+                    n_sub=a.keep, eps=a.eval_eps, log=log) if a.eval > 0 else []
     summ = summarise(recs) if recs else {}
     log(json.dumps(summ))
     if a.report:

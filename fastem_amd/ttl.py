@@ -8,9 +8,10 @@ LL* - tol*|LL*|.  Times are wall-clock from the start of the initialisation
 
 Forms:
 * plain: k-means++ init, then FP64 EM (the parity form).
-* mixed: the same init, FP32-form EM until the relative LL change falls below
-  `switch_eps`, then FP64 iterations on the same context until the target is
-  met; the LL that decides the criterion is always an FP64 E-step.
+* mixed: the same init, reduced-precision EM stages (BF16x3, FP32) each until
+  the relative LL change falls below its switch threshold, then FP64
+  iterations on the same context until the target is met; the LL that decides
+  the criterion is always an FP64 E-step.
 """
 import time
 
@@ -52,8 +53,14 @@ def run_plain(ctx, seed, tol=1e-6, cap=500, conv=1e-12, ll_star=None, init=None)
             "iters_to_reach": None if hit is None else hit + 1, "iters": len(trace)}
 
 
-def run_mixed(ctx, seed, ll_star, tol=1e-6, switch_eps=1e-7, cap=500, init=None):
-    """FP32-form iterations, then FP64 polish on the same context."""
+def run_schedule(ctx, seed, ll_star, stages, tol=1e-6, cap=500, init=None):
+    """Reduced-precision stages, then FP64 until the target is met.
+
+    stages: [(precision, switch_eps), ...]; each stage iterates in its form
+    until the relative LL change falls below switch_eps or the LL decreases
+    (the reference's convergence rule treats a decrease as converged), then
+    the next stage continues from the same parameters on the same context.
+    The LL that decides the criterion is always an FP64 E-step."""
     target = ll_star - tol * abs(ll_star)
     trace = []
     t0 = time.perf_counter()
@@ -63,30 +70,40 @@ def run_mixed(ctx, seed, ll_star, tol=1e-6, switch_eps=1e-7, cap=500, init=None)
     else:
         ctx.set_params(*init)
     t_init = time.perf_counter() - t0
-    ctx.set_precision(fem.PREC_FP32)
-    ctx.factor()
-    prev = -np.finfo(np.float64).max
-    n32 = 0
-    for _ in range(cap):
-        ll = ctx.estep()
-        trace.append((time.perf_counter() - t0, ll, "fp32"))
-        ctx.mstep(copy=False)
+    counts = []
+    for prec, switch_eps in stages:
+        ctx.set_precision(prec)
         ctx.factor()
-        n32 += 1
-        e = abs((ll - prev) / ll)
-        prev = ll
-        if e < switch_eps:
-            break
+        prev = None
+        n = 0
+        for _ in range(cap):
+            ll = ctx.estep()
+            trace.append((time.perf_counter() - t0, ll, prec))
+            ctx.mstep(copy=False)
+            ctx.factor()
+            n += 1
+            if prev is not None and (ll < prev or abs((ll - prev) / ll) < switch_eps):
+                break
+            prev = ll
+        counts.append(n)
     ctx.set_precision(fem.PREC_FP64)
     ctx.factor()
+    res = {"trace": trace, "t_reach": None, "t_init": t_init, "stage_iters": counts, "fp64_iters": 0}
     for _ in range(cap):
         ll = ctx.estep()
         t = time.perf_counter() - t0
-        trace.append((t, ll, "fp64"))
+        trace.append((t, ll, fem.PREC_FP64))
+        res["fp64_iters"] += 1
         if ll >= target:
-            return {"trace": trace, "t_reach": t, "t_init": t_init, "fp32_iters": n32,
-                    "fp64_iters": len(trace) - n32}
+            res["t_reach"] = t
+            return res
         ctx.mstep(copy=False)
         ctx.factor()
-    return {"trace": trace, "t_reach": None, "t_init": t_init, "fp32_iters": n32,
-            "fp64_iters": len(trace) - n32}
+    return res
+
+
+def run_mixed(ctx, seed, ll_star, tol=1e-6, switch_eps=1e-7, cap=500, init=None):
+    """FP32-form iterations, then FP64 polish on the same context."""
+    r = run_schedule(ctx, seed, ll_star, [(fem.PREC_FP32, switch_eps)], tol=tol, cap=cap, init=init)
+    r["fp32_iters"] = r["stage_iters"][0]
+    return r

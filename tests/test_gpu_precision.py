@@ -82,9 +82,11 @@ def test_trajectory(orc, prec):
 def test_bf16x3_d128_k128_per_iteration_errors(orc):
     """The d = 128, K = 128 precision case at a size the oracle finishes in
     seconds: both forms step from the same start (the generating mixture with
-    perturbed means and inflated covariances); the relative errors of LL,
-    means and covariances are reported for every iteration."""
-    n, d, K, iters = 16384, 128, 128, 4
+    perturbed means and inflated covariances), with covariance regularisation
+    g = 0.01 on both sides since the smallest Dirichlet components hold fewer
+    than d points at this n; the relative errors of LL, means and covariances
+    are reported for every iteration."""
+    n, d, K, iters, g = 32768, 128, 128, 4, 0.01
     print("bf16x3 d=128 K=128 trajectory", flush=True)
     X, w, mu, cov = fem.gengmm(n, d, K, -20.0, 20.0, 0.05, 31)
     mu = mu + 0.3 * np.random.default_rng(3).standard_normal(mu.shape)
@@ -93,12 +95,14 @@ def test_bf16x3_d128_k128_per_iteration_errors(orc):
     rows = []
     with fem.Context(n, d, K, fem.PREC_BF16X3) as ctx:
         ctx.upload(X)
+        ctx.set_regularization(g)
         wb, mub, covb = w, mu, cov
         wr, mur, covr = w, mu, cov
         for it in range(iters):
             U, ld = orc.factor_all(covr)
             ll_ref, R_ref = orc.estep(X, wr, mur, U, ld)
             wr, mur, covr = orc.mstep(X, R_ref)
+            covr = orc.regularize(covr, g)
             ctx.set_params(wb, mub, covb)
             ctx.factor()
             ll = ctx.estep()
