@@ -62,6 +62,87 @@ def cpu_baseline(X, w0, mu0, cov0, rows, iters):
                       f"by {N}/{rows}; {t:.1f} s measured"}
 
 
+def batch_workload(nfits=256, seed=7, maxsweeps=100, eps=1e-6, compare=False, log=print):
+    """Many independent fits advanced together (fem_batch_*): nfits fits in
+    groups of 8 sharing a dataset (N log-uniform in [1e4, 1e6], d in
+    {8, 16, 32, 64}, true K uniform in [4, 64]); each group is a K sweep over
+    four orders around the true K with two random restarts each, scored by
+    BIC.  Init per fit: K distinct random rows as means, the data's diagonal
+    variance, equal weights.  One persistent launch per EM phase over all
+    fits, iterations replayed from a captured hipGraph, launch shapes from the
+    autotuner.  Reports fit-iterations/s and algorithmic GFLOP/s."""
+    import fastem_amd as fem
+    rng = np.random.default_rng(seed)
+    groups = nfits // 8
+    fits, data = [], []
+    for gi in range(groups):
+        n = int(round(np.exp(rng.uniform(np.log(1e4), np.log(1e6)))))
+        d = int(rng.choice([8, 16, 32, 64]))
+        kt = int(rng.integers(4, 65))
+        X = fem.gengmm(n, d, kt, -10.0, 10.0, 0.1, 1000 + gi)[0]
+        data.append(X)
+        var = X.var(axis=0).astype(np.float64)
+        ks = [max(4, kt - 2), kt, min(64, kt + 2), min(64, kt + 4)]
+        for K in ks:
+            for r in range(2):
+                idx = np.random.default_rng(seed * 1000003 + gi * 16 + K * 2 + r).choice(n, K, replace=False)
+                cov = np.zeros((K, d, d))
+                cov[:, np.arange(d), np.arange(d)] = var
+                fits.append((gi, n, d, K, kt, (np.full(K, 1.0 / K), X[idx].astype(np.float64), cov)))
+    log(f"batch: {len(fits)} fits, {sum(f[1] for f in fits)} points in total")
+    b = fem.Batch([(f[1], f[2], f[3]) for f in fits])
+    for i, f in enumerate(fits):
+        b.upload(i, data[f[0]])
+        b.set_params(i, *f[5])
+    t = time.perf_counter()
+    report = b.tune()
+    t_tune = time.perf_counter() - t
+    for i, f in enumerate(fits):
+        b.set_params(i, *f[5])
+    t = time.perf_counter()
+    b.run(maxsweeps, eps=eps, use_graph=True)
+    t_run = time.perf_counter() - t
+    res = [b.result(i) for i in range(len(fits))]
+    b.close()
+    iters = np.array([r["iter_counter"] for r in res])
+    flop = sum(it * f[1] * f[3] * (2 * f[2] * f[2] + 3 * f[2]) for it, f in zip(iters, fits))
+    bic_pick, bic_ok = [], 0
+    for gi in range(groups):
+        cand = [(fem.bic(r["ll"], f[1], f[2], f[3]), f[3]) for r, f in zip(res, fits)
+                if f[0] == gi and r["status"] != fem.Batch.FAILED]
+        if cand:
+            kb = min(cand)[1]
+            kt = next(f[4] for f in fits if f[0] == gi)
+            bic_pick.append((kt, kb))
+            bic_ok += int(kb == kt)
+    out = {"fits": len(fits), "datasets": groups, "points_total": int(sum(f[1] for f in fits)),
+           "seconds": t_run, "tune_seconds": t_tune, "fit_iterations": int(iters.sum()),
+           "fit_iterations_per_s": float(iters.sum() / t_run), "gflops_algorithmic": flop / t_run / 1e9,
+           "converged": int(sum(r["status"] == fem.Batch.CONVERGED for r in res)),
+           "failed": int(sum(r["status"] == fem.Batch.FAILED for r in res)),
+           "bic_true_k_recovered": f"{bic_ok}/{len(bic_pick)}",
+           "autotuner": report.strip().splitlines()[-1] if report else None}
+    if compare:
+        # the same fits one at a time on single-fit contexts, same iteration counts
+        t = time.perf_counter()
+        for f, it in zip(fits, iters):
+            with fem.Context(f[1], f[2], f[3]) as ctx:
+                ctx.upload(data[f[0]])
+                ctx.set_params(*f[5])
+                try:
+                    ctx.factor()
+                    for _ in range(int(it)):
+                        ctx.estep()
+                        ctx.mstep(copy=False)
+                        ctx.factor()
+                except fem.FemError:
+                    pass
+        out["sequential_single_fit_seconds"] = time.perf_counter() - t
+        out["sequential_note"] = ("same fits and iteration counts on single-fit contexts, "
+                                  "including per-fit context creation and upload")
+    return out
+
+
 def calib_copy():
     """A plain device-to-device copy of exactly 1 GiB, three times: the
     reference point that scales the FETCH_SIZE / WRITE_SIZE counters."""
@@ -88,6 +169,11 @@ def main():
                     help="skip the time-to-log-likelihood measurement")
     ap.add_argument("--ttl-cap", type=int, default=400,
                     help="sweep cap of the converged FP64 reference run")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the many-fit batched workload")
+    ap.add_argument("--batch-fits", type=int, default=256)
+    ap.add_argument("--batch-compare", action="store_true",
+                    help="also time the batched fits one at a time")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -205,6 +291,9 @@ def main():
             "speedup": (plain["t_reach"] / mixed["t_reach"]
                         if plain["t_reach"] and mixed["t_reach"] else None),
             "init": "kmeans++ + up to 29 Lloyd sweeps on the device, included in both times"}
+    if not args.no_batch:
+        log("batched fits")
+        out["batch"] = batch_workload(args.batch_fits, compare=args.batch_compare, log=log)
     if not args.no_cpu_baseline:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(X, w0, mu0, cov0, args.cpu_rows, args.cpu_iters)

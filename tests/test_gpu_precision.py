@@ -9,9 +9,15 @@ over a 20-iteration trajectory the per-iteration LL stays within 1e-5.
 BF16x3 form: every contraction operand split into two bf16 (hi + lo), three
 bf16 MFMAs per product with FP32 accumulation, FP32 posteriors, FP64 Cholesky
 / log-det / cross-chunk reductions.  Stated tolerances on one step:
-  log-likelihood <= 2e-5, posteriors <= 2e-2 absolute, weights <= 1e-4,
-  means <= 1e-4, covariances <= 1e-3;
-per-iteration LL over a trajectory within 1e-4.  The errors are printed.
+  log-likelihood <= 2e-6, posteriors <= 5e-3 absolute, weights <= 1e-5,
+  means <= 1e-5, covariances <= 2e-5;
+per-iteration LL over a 20-iteration trajectory within 2e-6; at d = 128,
+K = 128 the per-iteration errors are reported and bounded by 1e-4 (LL),
+1e-4 (means) and 1e-4 (covariances).  The errors are printed.
+
+The overlapping case (means in [-1.5, 1.5]^d, 0.5 I added to the
+covariances) has fractional posteriors everywhere; in the separated cases
+most posteriors are 0 or 1 in both forms.
 """
 import numpy as np
 import pytest
@@ -26,11 +32,12 @@ def rel(a, b):
 
 
 TOL = {fem.PREC_FP32: {"ll": 2e-6, "R": 1e-3, "w": 1e-5, "mu": 1e-5, "cov": 1e-4, "traj": 1e-5},
-       fem.PREC_BF16X3: {"ll": 2e-5, "R": 2e-2, "w": 1e-4, "mu": 1e-4, "cov": 1e-3, "traj": 1e-4}}
+       fem.PREC_BF16X3: {"ll": 2e-6, "R": 5e-3, "w": 1e-5, "mu": 1e-5, "cov": 2e-5, "traj": 2e-6}}
 NAME = {fem.PREC_FP32: "fp32", fem.PREC_BF16X3: "bf16x3"}
 
 SHAPES = [(100000, 16, 8, -10.0, 10.0, 0.1), (50000, 64, 16, -20.0, 20.0, 0.05),
-          (20000, 17, 5, -10.0, 10.0, 0.1), (30000, 128, 4, -20.0, 20.0, 0.05)]
+          (20000, 17, 5, -10.0, 10.0, 0.1), (30000, 128, 4, -20.0, 20.0, 0.05),
+          (50000, 32, 8, -1.5, 1.5, 0.5)]
 BF16_SHAPES = [(40000, 48, 6, -10.0, 10.0, 0.1), (20000, 80, 5, -20.0, 20.0, 0.05),
                (12000, 128, 16, -20.0, 20.0, 0.05)]
 
@@ -110,4 +117,4 @@ def test_bf16x3_d128_k128_per_iteration_errors(orc):
             rows.append((abs(ll - ll_ref) / abs(ll_ref), rel(mub, mur), rel(covb, covr)))
             print(f"iter {it}: LL {rows[-1][0]:.2e} mu {rows[-1][1]:.2e} cov {rows[-1][2]:.2e}", flush=True)
     for ll_e, mu_e, cov_e in rows:
-        assert ll_e <= 1e-4 and mu_e <= 1e-3 and cov_e <= 1e-2
+        assert ll_e <= 1e-4 and mu_e <= 1e-4 and cov_e <= 1e-4
