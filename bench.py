@@ -143,6 +143,20 @@ def batch_workload(nfits=256, seed=7, maxsweeps=100, eps=1e-6, compare=False, lo
     return out
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated against a 1 GiB copy); None
+    if the summary does not list it."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_n{N}_d{D}_k{K}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.rstrip("\n").split("\t")
+        if not line.startswith("#") and len(parts) == 4 and parts[0] == kernel:
+            return float(parts[3])
+    return None
+
+
 def calib_copy():
     """A plain device-to-device copy of exactly 1 GiB, three times: the
     reference point that scales the FETCH_SIZE / WRITE_SIZE counters."""
@@ -264,7 +278,11 @@ def main():
                    "precision": "fp64", "init": "kmeans++ + 29 Lloyd sweeps (device)"},
         "roofline": {"bound": "mfma", "kernel": "k_estep64<4,2>", "achieved": achieved / 1e12,
                      "peak": PEAK_FP64_MFMA / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_MFMA, "traffic": None},
+                     "frac": achieved / PEAK_FP64_MFMA,
+                     "traffic": measured_traffic("fem::k_estep64<4, 2>"),
+                     "traffic_note": "HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE "
+                                     "(profiles/hbm_traffic_n4194304_d64_k64.txt); algorithmic "
+                                     "bytes: X read 1.07e9 + FP64 posteriors written 2.15e9"},
         "phases_ms": ph,
         "phase_tflops": {k2: fl[k2] / ph[k2] / 1e9 for k2 in ("estep", "means", "cov")},
         "loglik_first_last": r64["ll"],
@@ -278,19 +296,37 @@ def main():
             ctx.upload(X)
             plain = ttl.run_plain(ctx, seed=1, tol=1e-6, cap=args.ttl_cap, conv=1e-12)
             ll_star = plain["ll_final"]
-            log("time-to-loglik: mixed FP32 -> FP64 run")
-            mixed = ttl.run_mixed(ctx, seed=1, ll_star=ll_star, tol=1e-6, switch_eps=1e-7,
-                                  cap=args.ttl_cap)
+            runs = {}
+            variants = [("fp32", [(fem.PREC_FP32, 1e-9)], None),
+                        ("bf16x3", [(fem.PREC_BF16X3, 1e-9)], None)]
+            net_path = os.path.join(ROOT, "fastem_amd", "data", f"setnet_d{D}_k{K}.npz")
+            if os.path.exists(net_path):
+                from fastem_amd.setnet import SetNet
+                net = SetNet.load(net_path)
+                used = []
+                variants.append(("learned+bf16x3", [(fem.PREC_BF16X3, 1e-9)],
+                                 lambda c: used.append(net.init_context(c, X, n_sub=2048, seed=1)[3])))
+            for name, stages, init in variants:
+                log(f"time-to-loglik: {name} -> FP64 run")
+                r = ttl.run_schedule(ctx, seed=1, ll_star=ll_star, stages=stages, tol=1e-6,
+                                     cap=args.ttl_cap, init=init)
+                runs[name] = {"seconds": r["t_reach"], "init_s": r["t_init"],
+                              "reduced_iters": r["stage_iters"][0], "fp64_iters": r["fp64_iters"],
+                              "ll_final": r["trace"][-1][1],
+                              "speedup": (plain["t_reach"] / r["t_reach"]
+                                          if plain["t_reach"] and r["t_reach"] else None)}
         out["time_to_loglik"] = {
             "tol": 1e-6, "ll_star": ll_star,
             "ll_star_def": "converged FP64 EM from the same k-means++ init "
                            f"(|dLL/LL| < 1e-12 or {args.ttl_cap} sweeps; ran {plain['iters']})",
             "plain_fp64_s": plain["t_reach"], "plain_iters": plain["iters_to_reach"],
-            "mixed_s": mixed["t_reach"], "mixed_fp32_iters": mixed["fp32_iters"],
-            "mixed_fp64_iters": mixed["fp64_iters"], "init_s": plain["t_init"],
-            "speedup": (plain["t_reach"] / mixed["t_reach"]
-                        if plain["t_reach"] and mixed["t_reach"] else None),
-            "init": "kmeans++ + up to 29 Lloyd sweeps on the device, included in both times"}
+            "init_s": plain["t_init"],
+            "mixed": runs,
+            "schedule": "reduced-precision EM until |dLL/LL| < 1e-9 (or a decrease), then FP64 "
+                        "until the FP64 E-step LL is within tol of ll_star",
+            "init": "kmeans++ + up to 29 Lloyd sweeps on the device (learned: the set network's "
+                    "proposal on a 2048-point subsample seeds the same Lloyd sweeps), included in "
+                    "every time"}
     if not args.no_batch:
         log("batched fits")
         out["batch"] = batch_workload(args.batch_fits, compare=args.batch_compare, log=log)

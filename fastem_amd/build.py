@@ -16,6 +16,9 @@ HOST_SOURCES = ["fem_tools.cpp", "host/linalg.cpp", "host/rand.cpp", "host/block
                 "host/normal_density.cpp", "host/gaussian_mixture.cpp", "host/dataset.cpp",
                 "host/em_algorithm.cpp", "host/em_initializer.cpp"]
 TOOLS = {"em2": "host/em2_main.cpp", "eminit": "host/eminit_main.cpp"}
+# Per-file device flags: the BF16x3 kernels keep MFMA accumulators in VGPRs,
+# which saves the accumulator-read moves when they are squared or flushed.
+FILE_FLAGS = {"fem_kernels_bf16.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 # Packed FP32 VALU ops are disabled for all device code.
 HIP_FLAGS = [
@@ -44,7 +47,7 @@ def build(verbose=True):
     for s in HIP_SOURCES + HOST_SOURCES:
         src = os.path.join(CSRC, s)
         obj = os.path.join(objdir, os.path.splitext(s.replace("/", "_"))[0] + ".o")
-        flags = list(HIP_FLAGS)
+        flags = list(HIP_FLAGS) + FILE_FLAGS.get(s, [])
         if s.endswith(".cpp"):
             flags += ["-fopenmp"]
         _run([hipcc()] + flags + ["-I", os.path.join(ROOT, "include"), "-c", src, "-o", obj])

@@ -130,7 +130,7 @@ hipError_t dmalloc(T **p, size_t count) {
 
 void free_all(fem_ctx *c) {
     void *ptrs[] = {c->X, c->Rt, c->Rt32, c->w, c->mu, c->cov, c->nk, c->logdet, c->info, c->coef,
-                    c->U, c->ufrag, c->ufrag32, c->minit, c->mus32, c->ufragb, c->mlob, c->ll_part, c->ll_dev,
+                    c->U, c->ufrag, c->ufrag32, c->minit, c->mus32, c->stageb, c->ll_part, c->ll_dev,
                     c->mpart, c->cpart, c->kC, c->dmin, c->cls, c->kfit};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -218,8 +218,7 @@ int fem_create(fem_ctx **out, int64_t n, int d, int K, int precision, int device
                  (e = dmalloc(&c->mus32, (size_t)K * 2 * c->dpad)) != hipSuccess))
         return bail(e, "hipMalloc(fp32 factors)");
     if (precision == FEM_PREC_BF16X3 &&
-        ((e = dmalloc(&c->ufragb, (size_t)K * fem::estepb_pairs(c->NB) * 512)) != hipSuccess ||
-         (e = dmalloc(&c->mlob, (size_t)K * c->dpad)) != hipSuccess))
+        (e = dmalloc(&c->stageb, (size_t)K * fem::estepb_stage(c->NB))) != hipSuccess)
         return bail(e, "hipMalloc(bf16 factors)");
     if ((e = dmalloc(&c->w, K)) != hipSuccess ||
         (e = dmalloc(&c->mu, (size_t)K * d)) != hipSuccess ||
@@ -297,10 +296,8 @@ int fem_set_precision(fem_ctx *c, int precision) {
         if (!c->mus32) FEM_HIP(c, hipMalloc((void **)&c->mus32, sizeof(float) * c->K * 2 * c->dpad));
     }
     if (precision == FEM_PREC_BF16X3) {
-        if (!c->ufragb)
-            FEM_HIP(c, hipMalloc((void **)&c->ufragb,
-                                 sizeof(unsigned int) * c->K * fem::estepb_pairs(c->NB) * 512));
-        if (!c->mlob) FEM_HIP(c, hipMalloc((void **)&c->mlob, sizeof(float) * c->K * c->dpad));
+        if (!c->stageb)
+            FEM_HIP(c, hipMalloc((void **)&c->stageb, sizeof(unsigned int) * c->K * fem::estepb_stage(c->NB)));
     }
     FEM_HIP(c, hipStreamSynchronize(c->stream));
     c->prec = precision;

@@ -58,8 +58,7 @@ struct fem_ctx {
     float *ufrag32 = nullptr;  // K x pairs x 64   (FP32)
     double *minit = nullptr;   // K x dpad         U_k^T mu_k
     float *mus32 = nullptr;    // K x 2 x dpad     mu split hi/lo (FP32)
-    unsigned int *ufragb = nullptr;  // K x pairsb x 2 x 256  BF16 hi/lo E-step fragments
-    float *mlob = nullptr;     // K x dpad         U_k^T mu_lo (BF16x3 accumulator seed)
+    unsigned int *stageb = nullptr;  // K x stage words: BF16 hi/lo E-step fragments, mu_hi, U_k^T mu_lo
 
     // reductions
     int est_blocks = 0;         // capacity of ll_part (one per E-step workgroup)
@@ -109,7 +108,7 @@ hipError_t launch_means32_partials(fem_ctx *c);
 hipError_t launch_cov32_partials(fem_ctx *c);
 hipError_t launch_transpose_post32(const fem_ctx *c, double *out_dev, int64_t ld);
 // BF16x3 form (fem_kernels_bf16.hip); posteriors in Rt32
-int estepb_pairs(int NB);
+int estepb_stage(int NB);
 hipError_t launch_prep_bf16(fem_ctx *c);
 hipError_t launch_estepb(fem_ctx *c);
 hipError_t launch_meansb_partials(fem_ctx *c);

@@ -73,17 +73,31 @@ __device__ inline float xor_sum_gb(float v) {
 }
 
 // ---------------------------------------------------------------------------
-// E-step operands of component k from the FP64 factor U_k = L_k^-T:
-// fragments A[i][kk] = U_k^T[16 jb + i][32 ls + kk] = U_k[32 ls + kk][16 jb + i]
-// for every pair (ls, jb), split hi/lo, stored [pair][hi|lo][lane][4 words];
-// and mlo = U_k^T mu_lo with mu_lo = mu - float(mu), which seeds the
-// accumulators so that D = U_k^T (x - mu_hi) - U_k^T mu_lo.
+// E-step operands of component k, packed as one contiguous stage block of
+// estepb_stage_words(NB) words so the E-step copies it with plain strided
+// loads:
+//   [0, FW)            fragments A[i][kk] = U_k^T[16 jb + i][32 ls + kk]
+//                      = U_k[32 ls + kk][16 jb + i] for every pair (ls, jb),
+//                      split hi/lo, stored [pair][hi|lo][lane][4 words];
+//   [FW, FW + 32 NS)   mu_hi = float(mu_k), zero beyond d;
+//   [.., + 16 NB)      mlo = U_k^T mu_lo with mu_lo = mu_k - mu_hi: the
+//                      E-step forms D = U_k^T (x - mu_hi) - mlo.
+template <int NB>
+__device__ __host__ constexpr int estepb_stage_words() {
+    return pairs_bf16(NB) * 512 + 32 * ((NB + 1) / 2) + 16 * NB;
+}
+
+__device__ __host__ inline int stage_words_of(int NB) {
+    return pairs_bf16(NB) * 512 + 32 * ((NB + 1) / 2) + 16 * NB;
+}
+
 __global__ __launch_bounds__(256) void k_prep_bf16(int d, int dpad, int NB, const double *__restrict__ U,
-                                                   const double *__restrict__ mu, unsigned int *__restrict__ ufragb,
-                                                   float *__restrict__ mlo) {
+                                                   const double *__restrict__ mu, unsigned int *__restrict__ stage) {
     const int k = blockIdx.x;
-    const int np = pairs_bf16(NB);
+    const int np = pairs_bf16(NB), NS = (NB + 1) / 2;
+    const int FW = np * 512;
     const double *Uk = U + (size_t)k * dpad * dpad;
+    unsigned int *st = stage + (size_t)k * stage_words_of(NB);
     for (int e = threadIdx.x; e < np * 64 * 4; e += blockDim.x) {
         const int p = e / 256, lane = (e / 4) % 64, w = e % 4;
         int ls = 0, rem = p;
@@ -103,10 +117,12 @@ __global__ __launch_bounds__(256) void k_prep_bf16(int d, int dpad, int NB, cons
             hw |= hb << (16 * h);
             lw |= lb << (16 * h);
         }
-        unsigned int *dst = ufragb + (((size_t)k * np + p) * 2) * 256;
+        unsigned int *dst = st + p * 512;
         dst[lane * 4 + w] = hw;
         dst[256 + lane * 4 + w] = lw;
     }
+    for (int j = threadIdx.x; j < 32 * NS; j += blockDim.x)
+        st[FW + j] = __builtin_bit_cast(unsigned int, (j < d) ? (float)mu[(size_t)k * d + j] : 0.0f);
     for (int j = threadIdx.x; j < dpad; j += blockDim.x) {
         double s = 0.0;
         if (j < d)
@@ -114,7 +130,7 @@ __global__ __launch_bounds__(256) void k_prep_bf16(int d, int dpad, int NB, cons
                 const double m = mu[(size_t)k * d + f];
                 s += Uk[(size_t)f * dpad + j] * (m - (double)(float)m);
             }
-        mlo[(size_t)k * dpad + j] = (float)s;
+        st[FW + 32 * NS + j] = __builtin_bit_cast(unsigned int, (float)s);
     }
 }
 
@@ -122,17 +138,12 @@ __global__ __launch_bounds__(256) void k_prep_bf16(int d, int dpad, int NB, cons
 // Stage (a), BF16x3.  Workgroup = 4 waves x P groups of 16 points.  Lane
 // (g, c) holds, for point c of each group, features 32 ls + 8g .. +7 of every
 // strip ls as FP32 (the B operand layout).  Per component: centre with mu_hi,
-// split, 3 MFMAs per pair; the component operands (fragments, mu_hi, mlo) are
-// double-buffered in LDS as in the FP32 form, one barrier per component.
-template <int NB>
-__device__ __host__ constexpr int estepb_stage_words() {
-    return pairs_bf16(NB) * 512 + 32 * ((NB + 1) / 2) + 16 * NB;
-}
-
+// split, 3 MFMAs per pair, subtract mlo from the accumulators while squaring;
+// the component's stage block is double-buffered in LDS as in the FP32 form,
+// one barrier per component.
 template <int NB, int P>
 __global__ __launch_bounds__(256) void k_estepb(const float *__restrict__ X, int64_t n, int64_t npad, int K,
-                                                const unsigned int *__restrict__ ufragb,
-                                                const float *__restrict__ mus32, const float *__restrict__ mlo,
+                                                const unsigned int *__restrict__ stage,
                                                 const double *__restrict__ coef, float *__restrict__ Rt,
                                                 double *__restrict__ ll_part) {
     constexpr int DP = 16 * NB;
@@ -164,19 +175,10 @@ __global__ __launch_bounds__(256) void k_estepb(const float *__restrict__ X, int
             }
         }
 
-    // stage word e of component k: fragments, then mu_hi (32 NS, zero beyond
-    // dpad), then mlo (dpad)
-    auto stage_src = [&](int k, int e) -> unsigned int {
-        if (e < FW) return ufragb[(size_t)k * FW + e];
-        e -= FW;
-        if (e < 32 * NS) return (e < DP) ? __builtin_bit_cast(unsigned int, mus32[(size_t)k * 2 * DP + e]) : 0u;
-        e -= 32 * NS;
-        return __builtin_bit_cast(unsigned int, mlo[(size_t)k * DP + e]);
-    };
 #pragma unroll
     for (int q = 0; q < PER; q++) {
         const int e = tid + 256 * q;
-        if (e < STAGE) ldsb[e] = stage_src(0, e);
+        if (e < STAGE) ldsb[e] = stage[e];
     }
     __syncthreads();
 
@@ -193,16 +195,14 @@ __global__ __launch_bounds__(256) void k_estepb(const float *__restrict__ X, int
 #pragma unroll
         for (int q = 0; q < PER; q++) {
             const int e = tid + 256 * q;
-            pre[q] = (more && e < STAGE) ? stage_src(k + 1, e) : 0u;
+            pre[q] = (more && e < STAGE) ? stage[(size_t)(k + 1) * STAGE + e] : 0u;
         }
         const float ck = (float)coef[k];
         v4f acc[P][NB];
 #pragma unroll
-        for (int jb = 0; jb < NB; jb++) {
-            const float4 m = *reinterpret_cast<const float4 *>(lml + 16 * jb + 4 * g);
+        for (int p = 0; p < P; p++)
 #pragma unroll
-            for (int p = 0; p < P; p++) acc[p][jb] = (v4f){-m.x, -m.y, -m.z, -m.w};
-        }
+            for (int jb = 0; jb < NB; jb++) acc[p][jb] = (v4f){0.f, 0.f, 0.f, 0.f};
         int pr = 0;
 #pragma unroll
         for (int ls = 0; ls < NS; ls++) {
@@ -227,13 +227,25 @@ __global__ __launch_bounds__(256) void k_estepb(const float *__restrict__ X, int
                 for (int p = 0; p < P; p++) acc[p][jb] = mfma3(ah, al, bh[p], bl[p], acc[p][jb]);
             }
         }
+        float ml[NB][4];
+#pragma unroll
+        for (int jb = 0; jb < NB; jb++) {
+            const float4 m = *reinterpret_cast<const float4 *>(lml + 16 * jb + 4 * g);
+            ml[jb][0] = m.x;
+            ml[jb][1] = m.y;
+            ml[jb][2] = m.z;
+            ml[jb][3] = m.w;
+        }
 #pragma unroll
         for (int p = 0; p < P; p++) {
             float q = 0.f;
 #pragma unroll
             for (int jb = 0; jb < NB; jb++)
 #pragma unroll
-                for (int v = 0; v < 4; v++) q += acc[p][jb][v] * acc[p][jb][v];
+                for (int v = 0; v < 4; v++) {
+                    const float y = acc[p][jb][v] - ml[jb][v];
+                    q += y * y;
+                }
             q = xor_sum_gb(q);
             const float s = -0.5f * q + ck;
             mx[p] = fmaxf(mx[p], s);
@@ -288,7 +300,7 @@ __global__ __launch_bounds__(256) void k_reduce_sumb(const double *__restrict__ 
 
 hipError_t launch_prep_bf16(fem_ctx *c) {
     hipLaunchKernelGGL(k_prep_bf16, dim3(c->K), dim3(256), 0, c->stream, c->d, c->dpad, c->NB, c->U, c->mu,
-                       c->ufragb, c->mlob);
+                       c->stageb);
     return hipGetLastError();
 }
 
@@ -302,7 +314,7 @@ static hipError_t estepb_t(fem_ctx *c) {
     }
     const int blocks = (int)(c->npad / (64 * P));
     hipLaunchKernelGGL((k_estepb<NB, P>), dim3(blocks), dim3(256), lds, c->stream, c->X, c->n, c->npad, c->K,
-                       c->ufragb, c->mus32, c->mlob, c->coef, c->Rt32, c->ll_part);
+                       c->stageb, c->coef, c->Rt32, c->ll_part);
     hipLaunchKernelGGL(k_reduce_sumb, dim3(1), dim3(256), 0, c->stream, c->ll_part, (int64_t)blocks, c->ll_dev);
     return hipGetLastError();
 }
@@ -418,8 +430,8 @@ hipError_t launch_meansb_partials(fem_ctx *c) {
 // (contiguous in row-major block order, equal counts), each its own wave, so
 // a wave's accumulators stay in registers; a part forms T only for the
 // feature blocks its upper blocks touch.  FP32 accumulators are flushed into
-// FP64 every 2048 points (registers for d <= 48, else the wave's own blocks of
-// the partial slice).  Grid and XCD remapping as the FP32 form, with the part
+// FP64 every 2048 points: into registers for d <= 48, else into the wave's
+// own slice of LDS (written to the partial slice once at the end).  Grid and XCD remapping as the FP32 form, with the part
 // as the fastest-varying index; the loads of the next 32 points are issued
 // once the current 32 are split, before their MFMAs.
 __device__ __host__ constexpr int cov_parts(int NB) { return NB > 4 ? 2 : 1; }
@@ -450,9 +462,17 @@ __device__ __host__ constexpr int cov_part_size(int NB, int PART) {
     return s;
 }
 
+// LDS doubles per wave for the FP64 accumulators (0 when they are registers).
+__device__ __host__ constexpr int cov_lds_doubles(int NB) {
+    int m = 0;
+    for (int p = 0; p < cov_parts(NB); p++) m = cov_part_size(NB, p) > m ? cov_part_size(NB, p) : m;
+    return NB <= 3 ? 0 : m * 256;
+}
+
 template <int NB, int PART>
 __device__ inline void covb_wave(const float *__restrict__ X, const float *__restrict__ rrow, int64_t i0, int64_t i1,
-                                 const float (&mh)[NB], float inv, int g, int c, double *__restrict__ out) {
+                                 const float (&mh)[NB], float inv, int g, int c, double *__restrict__ out,
+                                 double *__restrict__ lacc) {
     constexpr int DP = 16 * NB;
     constexpr int UB = NB * (NB + 1) / 2;
     constexpr int NPART = cov_parts(NB);
@@ -461,11 +481,15 @@ __device__ inline void covb_wave(const float *__restrict__ X, const float *__res
     constexpr int FLUSH = 2048;
     constexpr bool REG = NB <= 3;
     double accd[REG ? NU : 1][4];
+    const int lane = 16 * g + c;
     if (REG) {
 #pragma unroll
         for (int u = 0; u < (REG ? NU : 1); u++)
 #pragma unroll
             for (int v = 0; v < 4; v++) accd[u][v] = 0.0;
+    } else {
+        // the wave's FP64 accumulators live in its own LDS slice [NU][4][64]
+        for (int e = lane; e < NU * 256; e += 64) lacc[e] = 0.0;
     }
     float rv[8], xv[NB][8];
     auto load32 = [&](int64_t i) {
@@ -482,7 +506,6 @@ __device__ inline void covb_wave(const float *__restrict__ X, const float *__res
                 for (int e = 0; e < 8; e++) xv[fb][e] = xp[e * DP + 16 * fb];
     };
     if (i0 < i1) load32(i0);
-    bool first = true;
     for (int64_t s0 = i0; s0 < i1; s0 += FLUSH) {
         const int64_t s1 = (s0 + FLUSH < i1) ? s0 + FLUSH : i1;
         v4f acc[NU];
@@ -518,23 +541,16 @@ __device__ inline void covb_wave(const float *__restrict__ X, const float *__res
                 for (int v = 0; v < 4; v++) accd[u][v] += (double)acc[u][v];
         } else {
 #pragma unroll
-            for (int u = 0; u < NU; u++) {
+            for (int u = 0; u < NU; u++)
 #pragma unroll
-                for (int v = 0; v < 4; v++) {
-                    double *o = out + (size_t)(U0 + u) * 256 + (4 * g + v) * 16 + c;
-                    *o = (first ? 0.0 : *o) + (double)acc[u][v];
-                }
-                __asm__ volatile("" ::: "memory");  // one block's read-add-writes at a time
-            }
+                for (int v = 0; v < 4; v++) lacc[(u * 4 + v) * 64 + lane] += (double)acc[u][v];
         }
-        first = false;
     }
-    if (REG) {
 #pragma unroll
-        for (int u = 0; u < (REG ? NU : 1); u++)
+    for (int u = 0; u < NU; u++)
 #pragma unroll
-            for (int v = 0; v < 4; v++) out[(size_t)(U0 + u) * 256 + (4 * g + v) * 16 + c] = accd[u][v];
-    }
+        for (int v = 0; v < 4; v++)
+            out[(size_t)(U0 + u) * 256 + (4 * g + v) * 16 + c] = REG ? accd[REG ? u : 0][v] : lacc[(u * 4 + v) * 64 + lane];
 }
 
 template <int NB>
@@ -562,14 +578,22 @@ __global__ __launch_bounds__(256) void k_covb(const float *__restrict__ X, const
     const float inv = (float)(1.0 / sqrt(nk[k]));
     double *out = cpart + ((size_t)chunk * KP + k) * UB * 256;
     const float *rrow = Rt + (size_t)k * npad;
-    if (part == 0) covb_wave<NB, 0>(X, rrow, i0, i1, mh, inv, g, c, out);
-    else if (NPART > 1) covb_wave<NB, (NPART > 1 ? 1 : 0)>(X, rrow, i0, i1, mh, inv, g, c, out);
+    extern __shared__ double lacc_all[];  // 4 waves x cov_lds_doubles(NB)
+    double *lacc = lacc_all + (size_t)wave * cov_lds_doubles(NB);
+    if (part == 0) covb_wave<NB, 0>(X, rrow, i0, i1, mh, inv, g, c, out, lacc);
+    else if (NPART > 1) covb_wave<NB, (NPART > 1 ? 1 : 0)>(X, rrow, i0, i1, mh, inv, g, c, out, lacc);
 }
 
 template <int NB>
 static hipError_t covb_t(fem_ctx *c) {
     const int blocks = (c->KP / 4) * c->c_chunks * cov_parts(NB);
-    hipLaunchKernelGGL(k_covb<NB>, dim3(blocks), dim3(256), 0, c->stream, c->X, c->Rt32, c->npad, c->K, c->KP,
+    const size_t lds = (size_t)4 * cov_lds_doubles(NB) * sizeof(double);
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_covb<NB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_covb<NB>, dim3(blocks), dim3(256), lds, c->stream, c->X, c->Rt32, c->npad, c->K, c->KP,
                        c->d, c->mu, c->nk, c->c_chunk_rows, c->cpart);
     return hipGetLastError();
 }
@@ -588,6 +612,6 @@ hipError_t launch_covb_partials(fem_ctx *c) {
     return hipErrorInvalidValue;
 }
 
-int estepb_pairs(int NB) { return pairs_bf16(NB); }
+int estepb_stage(int NB) { return stage_words_of(NB); }
 
 }  // namespace fem

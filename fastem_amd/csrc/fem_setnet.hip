@@ -6,12 +6,14 @@
 //   g = mean_i h2_i;  q_k = S_k + g Wg                 K seed queries
 //   a_ik = softmax_i(q_k . h2_i / 8);  z_k = sum_i a_ik h2_i
 //   mu_k = z_k Wm + bm;  lv_k = z_k Wv + bv            means, log-variances
+//   m_i = sum_j softmax_j((h2_i Wp) . (h2_j Wq) / 8) x_j   point head
 // Loss (per set, against a fitted mixture M_j, V_j, p_j):
 //   sum_j p_j |mu_k(j) - M_j|^2 + (1/K) sum_k min_j |mu_k - M_j|^2
-//   + lam sum_j p_j |lv_k(j) - V_j|^2 / d,  k(j) = argmin_k |mu_k - M_j|^2.
+//   + lam sum_j p_j |lv_k(j) - V_j|^2 / d,  k(j) = argmin_k |mu_k - M_j|^2
+//   + gam (1/n) sum_i |m_i - T_i|^2        (T_i: mean of point i's cluster).
 // Training arithmetic is FP32; every reduction has a fixed order.  Parameters
 // are one flat FP32 vector: W1[d][64] b1[64] W2[64][64] b2[64] S[K][64]
-// Wg[64][64] Wm[64][d] bm[d] Wv[64][d] bv[d].
+// Wg[64][64] Wm[64][d] bm[d] Wv[64][d] bv[d] Wp[64][64] Wq[64][64].
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -28,7 +30,7 @@ namespace {
 constexpr int W = 64;
 
 struct Offsets {
-    int W1, b1, W2, b2, S, Wg, Wm, bm, Wv, bv, total;
+    int W1, b1, W2, b2, S, Wg, Wm, bm, Wv, bv, Wp, Wq, total;
 };
 
 Offsets offsets(int d, int K) {
@@ -44,6 +46,8 @@ Offsets offsets(int d, int K) {
     o.bm = p; p += d;
     o.Wv = p; p += W * d;
     o.bv = p; p += d;
+    o.Wp = p; p += W * W;
+    o.Wq = p; p += W * W;
     o.total = p;
     return o;
 }
@@ -340,14 +344,15 @@ __global__ void k_query_bwd_weight(const float *__restrict__ dq, const float *__
     }
 }
 
-// dZ2 row (b,i): (sum_k alpha dz_k + ds q_k + dg/n) * gelu'(Z2).
+// dZ2 row (b,i): (sum_k alpha dz_k + ds q_k + dg/n + dh2x) * gelu'(Z2), dh2x
+// being the point head's contribution.
 __global__ void k_dh2(const float *__restrict__ alpha, const float *__restrict__ dz, const float *__restrict__ ds,
-                      const float *__restrict__ q, const float *__restrict__ dg, const float *__restrict__ Z2, int n,
-                      int K, int rows, float *__restrict__ dZ2) {
+                      const float *__restrict__ q, const float *__restrict__ dg, const float *__restrict__ dh2x,
+                      const float *__restrict__ Z2, int n, int K, int rows, float *__restrict__ dZ2) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= rows * W) return;
     const int r = e / W, w = e % W, b = r / n, i = r % n;
-    float s = dg[b * W + w] / (float)n;
+    float s = dg[b * W + w] / (float)n + dh2x[e];
     for (int k = 0; k < K; k++) {
         const size_t bk = (size_t)b * K + k;
         s += alpha[bk * n + i] * dz[bk * W + w] + ds[bk * n + i] * q[bk * W + w];
@@ -407,6 +412,219 @@ __global__ void k_adam(float *__restrict__ th, const float *__restrict__ gr, flo
     th[e] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
 }
 
+// ---------------------------------------------------------------------------
+// Point head: P = H2 Wp, Q = H2 Wq, E_ij = P_i . Q_j / 8, beta = softmax_j(E),
+// m_i = sum_j beta_ij x_j.
+
+// A = X Wa, Bo = X Wb (no bias, no activation); X: M x 64.
+__global__ __launch_bounds__(256) void k_linear2(const float *__restrict__ X, int M, const float *__restrict__ Wa,
+                                                 const float *__restrict__ Wb, float *__restrict__ A,
+                                                 float *__restrict__ Bo) {
+    __shared__ float xs[64][W + 1];
+    __shared__ float wa[W][W], wb[W][W];
+    const int t = threadIdx.x;
+    const int r0 = blockIdx.x * 64;
+    for (int e = t; e < 64 * W; e += 256) {
+        const int r = e / W, i = e % W;
+        xs[r][i] = (r0 + r < M) ? X[(size_t)(r0 + r) * W + i] : 0.f;
+        wa[e / W][e % W] = Wa[e];
+        wb[e / W][e % W] = Wb[e];
+    }
+    __syncthreads();
+    const int r = t >> 2, c0 = (t & 3) * 16;
+    if (r0 + r >= M) return;
+    for (int c = c0; c < c0 + 16; c++) {
+        float sa = 0.f, sb = 0.f;
+        for (int i = 0; i < W; i++) {
+            sa += xs[r][i] * wa[i][c];
+            sb += xs[r][i] * wb[i][c];
+        }
+        A[(size_t)(r0 + r) * W + c] = sa;
+        Bo[(size_t)(r0 + r) * W + c] = sb;
+    }
+}
+
+// E[b][i][j] = P_i . Q_j / 8 for i, j < n; one 64 x 64 tile per workgroup.
+__global__ __launch_bounds__(256) void k_pscores(const float *__restrict__ P, const float *__restrict__ Q, int n,
+                                                 float *__restrict__ E) {
+    __shared__ float ps[64][W + 1], qs[64][W + 1];
+    const int t = threadIdx.x;
+    const int b = blockIdx.z, i0 = blockIdx.y * 64, j0 = blockIdx.x * 64;
+    for (int e = t; e < 64 * W; e += 256) {
+        const int r = e / W, w = e % W;
+        ps[r][w] = (i0 + r < n) ? P[((size_t)b * n + i0 + r) * W + w] : 0.f;
+        qs[r][w] = (j0 + r < n) ? Q[((size_t)b * n + j0 + r) * W + w] : 0.f;
+    }
+    __syncthreads();
+    const int ti = t >> 4, tj = t & 15;
+    float acc[4][4] = {};
+    for (int w = 0; w < W; w++) {
+        float qv[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) qv[c] = qs[tj + 16 * c][w];
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+            const float pv = ps[ti + 16 * a][w];
+#pragma unroll
+            for (int c = 0; c < 4; c++) acc[a][c] += pv * qv[c];
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+        const int i = i0 + ti + 16 * a;
+        if (i >= n) continue;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int j = j0 + tj + 16 * c;
+            if (j < n) E[((size_t)b * n + i) * n + j] = acc[a][c] * 0.125f;
+        }
+    }
+}
+
+// Row (b, i): softmax over j of E in place, then m_i = sum_j beta_ij x_j.
+__global__ __launch_bounds__(256) void k_psoftmax(float *__restrict__ E, const float *__restrict__ X, int n, int d,
+                                                  float *__restrict__ m) {
+    __shared__ float red[256];
+    __shared__ float mp[4][64];
+    const int r = blockIdx.x, b = r / n, t = threadIdx.x;
+    float *e = E + (size_t)r * n;
+    float mx = -INFINITY;
+    for (int j = t; j < n; j += 256) mx = fmaxf(mx, e[j]);
+    red[t] = mx;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (t < st) red[t] = fmaxf(red[t], red[t + st]);
+        __syncthreads();
+    }
+    mx = red[0];
+    __syncthreads();
+    float se = 0.f;
+    for (int j = t; j < n; j += 256) {
+        const float v = expf(e[j] - mx);
+        e[j] = v;
+        se += v;
+    }
+    red[t] = se;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (t < st) red[t] += red[t + st];
+        __syncthreads();
+    }
+    const float inv = 1.0f / red[0];
+    __syncthreads();
+    for (int j = t; j < n; j += 256) e[j] *= inv;
+    __syncthreads();
+    const int grp = t >> 6, a = t & 63;
+    float acc = 0.f;
+    if (a < d)
+        for (int j = grp; j < n; j += 4) acc += e[j] * X[((size_t)b * n + j) * d + a];
+    mp[grp][a] = acc;
+    __syncthreads();
+    if (t < d) m[(size_t)r * d + t] = (mp[0][t] + mp[1][t]) + (mp[2][t] + mp[3][t]);
+}
+
+// Point loss gam/n |m_i - T_i|^2 per row and its gradient.
+__global__ void k_ploss(const float *__restrict__ m, const float *__restrict__ T, int rows, int n, int d,
+                        float gam, float *__restrict__ dm, float *__restrict__ lrow) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    float L = 0.f;
+    for (int a = 0; a < d; a++) {
+        const float diff = m[(size_t)r * d + a] - T[(size_t)r * d + a];
+        dm[(size_t)r * d + a] = 2.f * gam * diff / (float)n;
+        L += diff * diff;
+    }
+    lrow[r] = gam * L / (float)n;
+}
+
+// loss[b] += sum of the set's row losses (fixed order).
+__global__ void k_ploss_sum(const float *__restrict__ lrow, int B, int n, float *__restrict__ loss) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    float s = 0.f;
+    for (int i = 0; i < n; i++) s += lrow[(size_t)b * n + i];
+    loss[b] += s;
+}
+
+__global__ void k_fillf(float *__restrict__ p, size_t cnt, float v) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < cnt) p[e] = v;
+}
+
+// Row (b, i): with db_j = dm_i . x_j, ds_ij = beta_ij (db_j - sum_j beta_ij db_j) / 8
+// written over beta.
+__global__ __launch_bounds__(256) void k_pattn_bwd_rows(float *__restrict__ E, const float *__restrict__ X,
+                                                        const float *__restrict__ dm, int n, int d) {
+    __shared__ float dms[64];
+    __shared__ float red[256];
+    const int r = blockIdx.x, b = r / n, t = threadIdx.x;
+    if (t < 64) dms[t] = (t < d) ? dm[(size_t)r * d + t] : 0.f;
+    __syncthreads();
+    float *e = E + (size_t)r * n;
+    auto dot = [&](int j) {
+        const float *x = X + ((size_t)b * n + j) * d;
+        float s = 0.f;
+        for (int a = 0; a < d; a++) s += dms[a] * x[a];
+        return s;
+    };
+    float acc = 0.f;
+    for (int j = t; j < n; j += 256) acc += e[j] * dot(j);
+    red[t] = acc;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (t < st) red[t] += red[t + st];
+        __syncthreads();
+    }
+    const float tot = red[0];
+    __syncthreads();
+    for (int j = t; j < n; j += 256) e[j] = e[j] * (dot(j) - tot) * 0.125f;
+}
+
+// TRANS = 0: out[b][i][:] = sum_j DS[b][i][j] Y[b][j][:]   (dP with Y = Q)
+// TRANS = 1: out[b][j][:] = sum_i DS[b][i][j] Y[b][i][:]   (dQ with Y = P)
+template <int TRANS>
+__global__ __launch_bounds__(256) void k_pgrad(const float *__restrict__ DS, const float *__restrict__ Y, int n,
+                                               float *__restrict__ out) {
+    __shared__ float dsl[64][65];
+    __shared__ float yl[64][W + 1];
+    const int t = threadIdx.x;
+    const int b = blockIdx.y, r0 = blockIdx.x * 64;
+    const int orow = t >> 2, wcol = (t & 3) * 16;
+    float acc[16] = {};
+    for (int k0 = 0; k0 < n; k0 += 64) {
+        for (int e = t; e < 64 * 64; e += 256) {
+            const int kk = e / 64, o = e % 64;
+            float v = 0.f;
+            if (r0 + o < n && k0 + kk < n)
+                v = TRANS ? DS[((size_t)b * n + k0 + kk) * n + r0 + o] : DS[((size_t)b * n + r0 + o) * n + k0 + kk];
+            if (TRANS) dsl[o][kk] = v;
+            else dsl[o][kk] = v;
+            yl[kk][o] = (k0 + kk < n) ? Y[((size_t)b * n + k0 + kk) * W + o] : 0.f;
+        }
+        __syncthreads();
+        for (int kk = 0; kk < 64; kk++) {
+            const float dv = dsl[orow][kk];
+#pragma unroll
+            for (int c = 0; c < 16; c++) acc[c] += dv * yl[kk][wcol + c];
+        }
+        __syncthreads();
+    }
+    if (r0 + orow < n)
+#pragma unroll
+        for (int c = 0; c < 16; c++) out[((size_t)b * n + r0 + orow) * W + wcol + c] = acc[c];
+}
+
+// dH2x = dP Wp^T + dQ Wq^T (rows x 64).
+__global__ void k_pback_h2(const float *__restrict__ dP, const float *__restrict__ dQ, int rows,
+                           const float *__restrict__ Wp, const float *__restrict__ Wq, float *__restrict__ out) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= rows * W) return;
+    const int r = e / W, w = e % W;
+    float s = 0.f;
+    for (int o = 0; o < W; o++) s += dP[(size_t)r * W + o] * Wp[w * W + o] + dQ[(size_t)r * W + o] * Wq[w * W + o];
+    out[e] = s;
+}
+
 }  // namespace
 
 struct fem_setnet {
@@ -420,6 +638,10 @@ struct fem_setnet {
     float *M = nullptr, *V = nullptr, *pw = nullptr, *dmu = nullptr, *dlv = nullptr, *loss = nullptr;
     float *dz = nullptr, *ds = nullptr, *dq = nullptr, *dqs = nullptr, *dg = nullptr, *dZ2 = nullptr, *dZ1 = nullptr;
     float *part = nullptr;
+    // point head
+    size_t cap_e = 0;  // capacity of E in floats (sum over sets of n^2)
+    float *P = nullptr, *Q = nullptr, *E = nullptr, *pm = nullptr, *T = nullptr, *dm = nullptr, *lrow = nullptr;
+    float *dP = nullptr, *dQ = nullptr, *dh2x = nullptr, *dummy = nullptr;
     int step = 0;
     std::string err;
 };
@@ -438,20 +660,23 @@ int serr(fem_setnet *s, int code, const std::string &m) {
     } while (0)
 
 void free_ws(fem_setnet *s) {
-    float *ptrs[] = {s->X, s->Z1, s->H1, s->Z2, s->H2, s->g, s->q, s->alpha, s->z, s->mu, s->lv, s->M, s->V,
-                     s->pw, s->dmu, s->dlv, s->loss, s->dz, s->ds, s->dq, s->dqs, s->dg, s->dZ2, s->dZ1, s->part};
-    for (float *p : ptrs)
-        if (p) (void)hipFree(p);
-    s->X = s->Z1 = s->H1 = s->Z2 = s->H2 = s->g = s->q = s->alpha = s->z = s->mu = s->lv = nullptr;
-    s->M = s->V = s->pw = s->dmu = s->dlv = s->loss = s->dz = s->ds = s->dq = s->dqs = s->dg = nullptr;
-    s->dZ2 = s->dZ1 = s->part = nullptr;
+    float **ptrs[] = {&s->X,  &s->Z1,  &s->H1, &s->Z2, &s->H2,  &s->g,    &s->q,  &s->alpha, &s->z,
+                      &s->mu, &s->lv,  &s->M,  &s->V,  &s->pw,  &s->dmu,  &s->dlv, &s->loss, &s->dz,
+                      &s->ds, &s->dq,  &s->dqs, &s->dg, &s->dZ2, &s->dZ1, &s->part, &s->P,   &s->Q,
+                      &s->E,  &s->pm,  &s->T,  &s->dm, &s->lrow, &s->dP,  &s->dQ,  &s->dh2x, &s->dummy};
+    for (float **p : ptrs) {
+        if (*p) (void)hipFree(*p);
+        *p = nullptr;
+    }
     s->cap_rows = s->cap_sets = 0;
+    s->cap_e = 0;
 }
 
 // Workspace for B sets of n points; target arrays are sized for J <= 4K.
 int ensure_ws(fem_setnet *s, int B, int n) {
     const int rows = B * n;
-    if (rows <= s->cap_rows && B <= s->cap_sets) return FEM_OK;
+    const size_t ecount = (size_t)rows * n;
+    if (rows <= s->cap_rows && B <= s->cap_sets && ecount <= s->cap_e) return FEM_OK;
     free_ws(s);
     const int d = s->d, K = s->K;
     const int Jc = s->K * 4;  // target components capacity
@@ -465,10 +690,14 @@ int ensure_ws(fem_setnet *s, int B, int n) {
         A(&s->dlv, (size_t)B * K * d) || A(&s->loss, (size_t)B) || A(&s->dz, (size_t)B * K * W) ||
         A(&s->ds, (size_t)B * K * n) || A(&s->dq, (size_t)B * K * W) || A(&s->dqs, (size_t)B * W) ||
         A(&s->dg, (size_t)B * W) || A(&s->dZ2, (size_t)rows * W) || A(&s->dZ1, (size_t)rows * W) ||
-        A(&s->part, (size_t)chunks * 65 * W))
+        A(&s->part, (size_t)chunks * 65 * W) || A(&s->P, (size_t)rows * W) || A(&s->Q, (size_t)rows * W) ||
+        A(&s->E, ecount) || A(&s->pm, (size_t)rows * d) || A(&s->T, (size_t)rows * d) ||
+        A(&s->dm, (size_t)rows * d) || A(&s->lrow, (size_t)rows) || A(&s->dP, (size_t)rows * W) ||
+        A(&s->dQ, (size_t)rows * W) || A(&s->dh2x, (size_t)rows * W) || A(&s->dummy, (size_t)W))
         return serr(s, FEM_ERR_HIP, "setnet workspace allocation failed");
     s->cap_rows = rows;
     s->cap_sets = B;
+    s->cap_e = ecount;
     return FEM_OK;
 }
 
@@ -496,6 +725,12 @@ void forward(fem_setnet *s, int B, int n) {
     hipLaunchKernelGGL(k_attn_fwd, dim3(B * K), dim3(256), 0, s->stream, s->H2, s->q, n, K, s->alpha, s->z);
     hipLaunchKernelGGL(k_heads, dim3((B * K * d + 255) / 256), dim3(256), 0, s->stream, s->z, B * K, d,
                        th + s->off.Wm, th + s->off.bm, th + s->off.Wv, th + s->off.bv, s->mu, s->lv);
+    // point head
+    hipLaunchKernelGGL(k_linear2, dim3((rows + 63) / 64), dim3(256), 0, s->stream, s->H2, rows, th + s->off.Wp,
+                       th + s->off.Wq, s->P, s->Q);
+    const int nt = (n + 63) / 64;
+    hipLaunchKernelGGL(k_pscores, dim3(nt, nt, B), dim3(256), 0, s->stream, s->P, s->Q, n, s->E);
+    hipLaunchKernelGGL(k_psoftmax, dim3(rows), dim3(256), 0, s->stream, s->E, s->X, n, d, s->pm);
 }
 
 // backward from s->dmu, s->dlv into s->grad
@@ -511,9 +746,24 @@ void backward(fem_setnet *s, int B, int n) {
     hipLaunchKernelGGL(k_query_bwd_set, dim3(B), dim3(W), 0, s->stream, s->dq, K, th + s->off.Wg, s->dqs, s->dg);
     hipLaunchKernelGGL(k_query_bwd_weight, dim3((K * W + W * W + 255) / 256), dim3(256), 0, s->stream, s->dq, s->g,
                        s->dqs, B, K, gr + s->off.S, gr + s->off.Wg);
-    hipLaunchKernelGGL(k_dh2, dim3((rows * W + 255) / 256), dim3(256), 0, s->stream, s->alpha, s->dz, s->ds, s->q,
-                       s->dg, s->Z2, n, K, rows, s->dZ2);
+    // point head: ds over E, dP, dQ, their weight gradients and dH2 share
     const int chunks = (rows + 255) / 256;
+    hipLaunchKernelGGL(k_pattn_bwd_rows, dim3(rows), dim3(256), 0, s->stream, s->E, s->X, s->dm, n, d);
+    const int nt = (n + 63) / 64;
+    hipLaunchKernelGGL(k_pgrad<0>, dim3(nt, B), dim3(256), 0, s->stream, s->E, s->Q, n, s->dP);
+    hipLaunchKernelGGL(k_pgrad<1>, dim3(nt, B), dim3(256), 0, s->stream, s->E, s->P, n, s->dQ);
+    hipLaunchKernelGGL(k_dense_bwd_weight_part<W>, dim3(chunks), dim3(256), 0, s->stream, s->H2, s->dP, rows,
+                       s->part);
+    hipLaunchKernelGGL(k_sum_parts, dim3((65 * W + 255) / 256), dim3(256), 0, s->stream, s->part, chunks, W,
+                       gr + s->off.Wp, s->dummy);
+    hipLaunchKernelGGL(k_dense_bwd_weight_part<W>, dim3(chunks), dim3(256), 0, s->stream, s->H2, s->dQ, rows,
+                       s->part);
+    hipLaunchKernelGGL(k_sum_parts, dim3((65 * W + 255) / 256), dim3(256), 0, s->stream, s->part, chunks, W,
+                       gr + s->off.Wq, s->dummy);
+    hipLaunchKernelGGL(k_pback_h2, dim3((rows * W + 255) / 256), dim3(256), 0, s->stream, s->dP, s->dQ, rows,
+                       th + s->off.Wp, th + s->off.Wq, s->dh2x);
+    hipLaunchKernelGGL(k_dh2, dim3((rows * W + 255) / 256), dim3(256), 0, s->stream, s->alpha, s->dz, s->ds, s->q,
+                       s->dg, s->dh2x, s->Z2, n, K, rows, s->dZ2);
     hipLaunchKernelGGL(k_dense_bwd_weight_part<W>, dim3(chunks), dim3(256), 0, s->stream, s->H1, s->dZ2, rows,
                        s->part);
     hipLaunchKernelGGL(k_sum_parts, dim3((65 * W + 255) / 256), dim3(256), 0, s->stream, s->part, chunks, W,
@@ -554,6 +804,8 @@ int fem_setnet_create(fem_setnet **out, int d, int K, uint64_t seed, int device)
     fill(s->off.Wg, W, W, 1.0 / std::sqrt((double)W));
     fill(s->off.Wm, W, d, 1.0 / std::sqrt((double)W));
     fill(s->off.Wv, W, d, 1.0 / std::sqrt((double)W));
+    fill(s->off.Wp, W, W, 1.0 / std::sqrt((double)W));
+    fill(s->off.Wq, W, W, 1.0 / std::sqrt((double)W));
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc((void **)&s->theta, P * sizeof(float)) != hipSuccess ||
         hipMalloc((void **)&s->grad, P * sizeof(float)) != hipSuccess ||
@@ -602,8 +854,8 @@ int fem_setnet_set_params(fem_setnet *s, const float *theta) {
 }
 
 // Forward for B normalised point sets of n points (X: B*n*d, host): means and
-// log-variances (B*K*d, host).
-int fem_setnet_forward(fem_setnet *s, int B, int n, const float *X, float *mu, float *lv) {
+// log-variances (B*K*d, host) and, if m != NULL, the point head (B*n*d).
+int fem_setnet_forward(fem_setnet *s, int B, int n, const float *X, float *mu, float *lv, float *m) {
     if (!s || B < 1 || n < 1 || !X || !mu || !lv) return FEM_ERR_ARG;
     SH(s, hipSetDevice(s->device));
     if (int r = ensure_ws(s, B, n)) return r;
@@ -612,27 +864,39 @@ int fem_setnet_forward(fem_setnet *s, int B, int n, const float *X, float *mu, f
     SH(s, hipGetLastError());
     SH(s, hipMemcpyAsync(mu, s->mu, sizeof(float) * B * s->K * s->d, hipMemcpyDeviceToHost, s->stream));
     SH(s, hipMemcpyAsync(lv, s->lv, sizeof(float) * B * s->K * s->d, hipMemcpyDeviceToHost, s->stream));
+    if (m) SH(s, hipMemcpyAsync(m, s->pm, sizeof(float) * (size_t)B * n * s->d, hipMemcpyDeviceToHost, s->stream));
     SH(s, hipStreamSynchronize(s->stream));
     return FEM_OK;
 }
 
-// Loss and gradient for a batch (targets M, V: B*J*d, weights pw: B*J).
-// grad (host, may be NULL) receives the parameter gradient of the summed
-// loss; if lr > 0 an Adam step (beta 0.9/0.999, eps 1e-8) is applied with the
-// gradient of the mean loss over the batch.  *loss_out = summed loss.
+// Loss and gradient for a batch (targets M, V: B*J*d, weights pw: B*J, point
+// targets T: B*n*d or NULL for no point loss).  grad (host, may be NULL)
+// receives the parameter gradient of the summed loss; if lr > 0 an Adam step
+// (beta 0.9/0.999, eps 1e-8) is applied with the gradient of the mean loss
+// over the batch.  *loss_out = summed loss.
 int fem_setnet_step(fem_setnet *s, int B, int n, const float *X, int J, const float *M, const float *V,
-                    const float *pw, float lam, float lr, float *grad, float *loss_out) {
+                    const float *pw, const float *T, float lam, float gam, float lr, float *grad, float *loss_out) {
     if (!s || B < 1 || n < 1 || J < 1 || J > 4 * s->K || !X || !M || !V || !pw) return FEM_ERR_ARG;
     SH(s, hipSetDevice(s->device));
     if (int r = ensure_ws(s, B, n)) return r;
     const int d = s->d, K = s->K;
-    SH(s, hipMemcpyAsync(s->X, X, sizeof(float) * (size_t)B * n * d, hipMemcpyHostToDevice, s->stream));
+    const int rows = B * n;
+    SH(s, hipMemcpyAsync(s->X, X, sizeof(float) * (size_t)rows * d, hipMemcpyHostToDevice, s->stream));
     SH(s, hipMemcpyAsync(s->M, M, sizeof(float) * B * J * d, hipMemcpyHostToDevice, s->stream));
     SH(s, hipMemcpyAsync(s->V, V, sizeof(float) * B * J * d, hipMemcpyHostToDevice, s->stream));
     SH(s, hipMemcpyAsync(s->pw, pw, sizeof(float) * B * J, hipMemcpyHostToDevice, s->stream));
+    if (T) SH(s, hipMemcpyAsync(s->T, T, sizeof(float) * (size_t)rows * d, hipMemcpyHostToDevice, s->stream));
     forward(s, B, n);
     hipLaunchKernelGGL(k_loss, dim3((B + 63) / 64), dim3(64), 0, s->stream, s->mu, s->lv, s->M, s->V, s->pw, B, K, J,
                        d, lam, s->dmu, s->dlv, s->loss);
+    if (T) {
+        hipLaunchKernelGGL(k_ploss, dim3((rows + 255) / 256), dim3(256), 0, s->stream, s->pm, s->T, rows, n, d, gam,
+                           s->dm, s->lrow);
+        hipLaunchKernelGGL(k_ploss_sum, dim3((B + 63) / 64), dim3(64), 0, s->stream, s->lrow, B, n, s->loss);
+    } else {
+        const size_t cnt = (size_t)rows * d;
+        hipLaunchKernelGGL(k_fillf, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s->stream, s->dm, cnt, 0.0f);
+    }
     backward(s, B, n);
     SH(s, hipGetLastError());
     std::vector<float> lb(B);

@@ -3,10 +3,17 @@ starting means and diagonal covariances for EM (fem_setnet_* in
 include/fastem_setnet.h; kernels in csrc/fem_setnet.hip).
 
 The reference has no such initializer: it starts EM from the Jain, k-means,
-k-means++ or file initializers (EM2/EMInitializer.cpp).  Here the network's
-means replace the k-means++ seeding pass (`init_context`: Lloyd sweeps and the
-cluster mixture follow on the device as for k-means++), k-means++ stays the
-fallback, and the final EM log-likelihood judges each proposal.
+k-means++ or file initializers (EM2/EMInitializer.cpp).  Here the network
+proposes the starting means and they replace the k-means++ seeding pass
+(`init_context`: Lloyd sweeps and the cluster mixture follow on the device as
+for k-means++); k-means++ stays the fallback, and the final EM
+log-likelihood judges each proposal.
+
+Two heads propose means: the seed-query head (K attention-pooled queries ->
+means and log-variances) and the point head (a learned mean-shift step that
+moves every point of the subsample towards its cluster's mean; k-means++
+seeding over the moved points then picks one seed per cluster).  The point
+head's proposal is the one `init_context` uses.
 
 Coordinates: a point set X is normalised by its own centroid c and one
 scale s (root mean square deviation over all coordinates), so the network
@@ -37,6 +44,29 @@ def subsample(X, n, seed):
     else:
         idx = np.sort(rng.choice(N, size=n, replace=False))
     return X[idx]
+
+
+def kmeanspp_points(P, K, seed):
+    """k-means++ seeding (first point uniform, then D^2 sampling) over the
+    rows of P with numpy generator `seed`; K x d."""
+    rng = np.random.default_rng(seed)
+    n = P.shape[0]
+    C = np.empty((K, P.shape[1]))
+    C[0] = P[rng.integers(n)]
+    d2 = ((P - C[0]) ** 2).sum(axis=1)
+    for k in range(1, K):
+        tot = d2.sum()
+        i = rng.choice(n, p=d2 / tot) if tot > 0 else rng.integers(n)
+        C[k] = P[i]
+        d2 = np.minimum(d2, ((P - C[k]) ** 2).sum(axis=1))
+    return C
+
+
+def point_targets(Z, M):
+    """Point-head targets: the nearest of the fitted means M (K x d) to every
+    row of Z (same normalised units)."""
+    D = (Z * Z).sum(1)[:, None] - 2.0 * Z @ M.T + (M * M).sum(1)[None, :]
+    return D.argmin(axis=1)
 
 
 def fit_to_targets(w, mu, cov, c, s):
@@ -99,20 +129,24 @@ class SetNet:
         net.set_params(z["theta"])
         return net
 
-    def forward(self, Xs):
-        """Xs: (B, n, d) normalised sets -> (mu, lv), each (B, K, d)."""
+    def forward(self, Xs, points=False):
+        """Xs: (B, n, d) normalised sets -> (mu, lv), each (B, K, d), and with
+        points=True also the point head's outputs m (B, n, d)."""
         Xs = np.ascontiguousarray(Xs, dtype=np.float32)
         B, n, d = Xs.shape
         assert d == self.d
         mu = np.empty((B, self.K, d), dtype=np.float32)
         lv = np.empty((B, self.K, d), dtype=np.float32)
+        m = np.empty((B, n, d), dtype=np.float32) if points else None
         self._check(lib().fem_setnet_forward(self._h, B, n, Xs.ctypes.data_as(_fp),
-                                             mu.ctypes.data_as(_fp), lv.ctypes.data_as(_fp)))
-        return mu, lv
+                                             mu.ctypes.data_as(_fp), lv.ctypes.data_as(_fp),
+                                             m.ctypes.data_as(_fp) if points else None))
+        return (mu, lv, m) if points else (mu, lv)
 
-    def step(self, Xs, M, V, p, lam=0.5, lr=0.0, want_grad=False):
+    def step(self, Xs, M, V, p, T=None, lam=0.5, gam=1.0, lr=0.0, want_grad=False):
         """Loss (summed over the batch) and optionally its gradient; with
-        lr > 0 also one Adam update on the batch-mean loss."""
+        lr > 0 also one Adam update on the batch-mean loss.  T: point targets
+        (B, n, d) or None (point head left out of the loss)."""
         Xs = np.ascontiguousarray(Xs, dtype=np.float32)
         M = np.ascontiguousarray(M, dtype=np.float32)
         V = np.ascontiguousarray(V, dtype=np.float32)
@@ -120,28 +154,44 @@ class SetNet:
         B, n, d = Xs.shape
         J = M.shape[1]
         assert M.shape == (B, J, d) and V.shape == (B, J, d) and p.shape == (B, J)
+        if T is not None:
+            T = np.ascontiguousarray(T, dtype=np.float32)
+            assert T.shape == (B, n, d)
         grad = np.empty(self.nparams, dtype=np.float32) if want_grad else None
         loss = ctypes.c_float()
         self._check(lib().fem_setnet_step(self._h, B, n, Xs.ctypes.data_as(_fp), J,
                                           M.ctypes.data_as(_fp), V.ctypes.data_as(_fp),
-                                          p.ctypes.data_as(_fp), float(lam), float(lr),
+                                          p.ctypes.data_as(_fp),
+                                          T.ctypes.data_as(_fp) if T is not None else None,
+                                          float(lam), float(gam), float(lr),
                                           grad.ctypes.data_as(_fp) if want_grad else None,
                                           ctypes.byref(loss)))
         return (loss.value, grad) if want_grad else loss.value
 
     def propose_means(self, X, n_sub=1024, seed=0):
-        """The network's K starting means for dataset X (K x d, FP64)."""
+        """K starting means for dataset X (K x d, FP64): the point head moves
+        each point of an n_sub subsample towards its cluster's mean, and
+        k-means++ seeding (D^2 sampling, numpy generator `seed`) over the moved
+        points picks one per cluster."""
+        Xs = subsample(np.asarray(X), n_sub, seed)
+        Z, c, s = normalise(Xs)
+        _, _, m = self.forward(Z[None].astype(np.float32), points=True)
+        return c + s * kmeanspp_points(m[0].astype(np.float64), self.K, seed)
+
+    def propose_query_means(self, X, n_sub=1024, seed=0):
+        """The seed-query head's K means for dataset X (K x d, FP64)."""
         Xs = subsample(np.asarray(X), n_sub, seed)
         Z, c, s = normalise(Xs)
         mu, _ = self.forward(Z[None].astype(np.float32))
         return c + s * mu[0].astype(np.float64)
 
     def init_context(self, ctx, X, n_sub=1024, seed=0, lloyd_sweeps=29, fallback_seed=None):
-        """Learned initialisation of a device context holding X: the network's
+        """Learned initialisation of a device context holding X: the proposed
         means seed Lloyd sweeps and the cluster mixture on the device
-        (fem_kmeans_init_seeded) in place of the k-means++ seeding pass.  If
-        that leaves a cluster with fewer than two points, device k-means++
-        (fem_kmeans_init) is the fallback.  Returns (w, mu, cov, used)."""
+        (fem_kmeans_init_seeded) in place of the k-means++ seeding pass over
+        all n points.  If that leaves a cluster with fewer than two points,
+        device k-means++ (fem_kmeans_init) is the fallback.  Returns
+        (w, mu, cov, used)."""
         C0 = self.propose_means(X, n_sub=n_sub, seed=seed)
         try:
             return ctx.kmeans_init_seeded(C0, seed=seed, max_sweeps=lloyd_sweeps) + ("learned",)

@@ -24,7 +24,7 @@ from concurrent.futures import ThreadPoolExecutor
 import numpy as np
 
 import fastem_amd as fem
-from fastem_amd.setnet import SetNet, fit_to_targets, normalise, subsample
+from fastem_amd.setnet import SetNet, fit_to_targets, normalise, point_targets, subsample
 
 GEN = {(32, 16): (-10.0, 10.0, 0.1), (64, 64): (-20.0, 20.0, 0.05)}
 
@@ -42,6 +42,7 @@ def make_targets(seeds, n, d, K, gen, keep, eps=1e-8, maxsweeps=200, chunk=256, 
     M = np.empty((F, K, d), dtype=np.float32)
     V = np.empty((F, K, d), dtype=np.float32)
     P = np.empty((F, K), dtype=np.float32)
+    TI = np.zeros((F, keep), dtype=np.uint8)
     ok = np.zeros(F, dtype=bool)
     iters = np.zeros(F, dtype=np.int32)
     t0 = time.perf_counter()
@@ -69,11 +70,12 @@ def make_targets(seeds, n, d, K, gen, keep, eps=1e-8, maxsweeps=200, chunk=256, 
                     continue
                 m, v, p = fit_to_targets(r["w"], r["mu"], r["cov"], c, s)
                 M[f], V[f], P[f] = m, v, p
+                TI[f] = point_targets(Z, m)
                 ok[f] = True
             log(f"targets {min(c0 + chunk, F)}/{F}  {time.perf_counter() - t0:.1f} s")
     if batch is not None:
         batch.close()
-    return {"sub": sub[ok], "M": M[ok], "V": V[ok], "P": P[ok], "iters": iters[ok],
+    return {"sub": sub[ok], "M": M[ok], "V": V[ok], "P": P[ok], "TI": TI[ok], "iters": iters[ok],
             "failed": int((~ok).sum()), "seconds": time.perf_counter() - t0}
 
 
@@ -90,7 +92,12 @@ def augment(rng, Xs, M, V):
     return Xa, Ma, Va
 
 
-def train(net, data, steps, batch=64, lr=2e-3, seed=0, val_frac=0.02, log=print, log_every=200):
+def gather_targets(M, TI):
+    """Point targets T[b, i] = M[b, TI[b, i]]."""
+    return np.take_along_axis(M, TI[:, :, None].astype(np.int64), axis=1)
+
+
+def train(net, data, steps, batch=64, lr=2e-3, gam=1.0, seed=0, val_frac=0.02, log=print, log_every=200):
     rng = np.random.default_rng(seed)
     F = data["sub"].shape[0]
     nval = max(1, int(F * val_frac))
@@ -98,7 +105,8 @@ def train(net, data, steps, batch=64, lr=2e-3, seed=0, val_frac=0.02, log=print,
     vb = np.arange(min(nval, 256))
 
     def val_loss():
-        return net.step(data["sub"][vb], data["M"][vb], data["V"][vb], data["P"][vb]) / len(vb)
+        return net.step(data["sub"][vb], data["M"][vb], data["V"][vb], data["P"][vb],
+                        T=gather_targets(data["M"][vb], data["TI"][vb]), gam=gam) / len(vb)
 
     hist = [(0, val_loss())]
     t0 = time.perf_counter()
@@ -106,7 +114,7 @@ def train(net, data, steps, batch=64, lr=2e-3, seed=0, val_frac=0.02, log=print,
         idx = rng.choice(tr, size=batch, replace=False)
         Xs, M, V = augment(rng, data["sub"][idx], data["M"][idx], data["V"][idx])
         lr_t = lr * 0.5 * (1.0 + np.cos(np.pi * (it - 1) / steps))
-        net.step(Xs, M, V, data["P"][idx], lr=lr_t)
+        net.step(Xs, M, V, data["P"][idx], T=gather_targets(M, data["TI"][idx]), gam=gam, lr=lr_t)
         if it % log_every == 0 or it == steps:
             hist.append((it, val_loss()))
             log(f"step {it}/{steps} val loss {hist[-1][1]:.4f}  {time.perf_counter() - t0:.1f} s")
@@ -144,27 +152,34 @@ def evaluate(net, seeds, n, d, K, gen, n_sub=1024, eps=1e-5, maxsweeps=1000, llo
             except fem.FemError:
                 lrn, used = None, "failed"
             t_lrn = time.perf_counter() - t
-            inits.append((X, kpp, lrn, used, t_kpp, t_lrn))
-        b = fem.Batch([(n, d, K)] * (2 * len(cs)))
-        for i, (X, kpp, lrn, _, _, _) in enumerate(inits):
-            a0 = kpp if kpp is not None else lrn
-            a1 = lrn if lrn is not None else kpp
-            if a0 is None:
+            try:
+                qry = ctx.kmeans_init_seeded(net.propose_query_means(X, n_sub=n_sub, seed=int(s)),
+                                             seed=int(s), max_sweeps=lloyd_iters)
+            except fem.FemError:
+                qry = None
+            inits.append((X, kpp, lrn, used, t_kpp, t_lrn, qry))
+        b = fem.Batch([(n, d, K)] * (3 * len(cs)))
+        for i, (X, kpp, lrn, _, _, _, qry) in enumerate(inits):
+            cands = [kpp, lrn, qry]
+            fallback = next((c for c in cands if c is not None), None)
+            if fallback is None:
                 continue
-            b.upload(2 * i, X)
-            b.upload(2 * i + 1, X)
-            b.set_params(2 * i, *a0)
-            b.set_params(2 * i + 1, *a1)
+            for j, c in enumerate(cands):
+                b.upload(3 * i + j, X)
+                b.set_params(3 * i + j, *(c if c is not None else fallback))
         b.run(maxsweeps, eps=eps)
         for i, s in enumerate(cs):
-            rk, rl = b.result(2 * i), b.result(2 * i + 1)
+            rk, rl, rq = b.result(3 * i), b.result(3 * i + 1), b.result(3 * i + 2)
             kpp_ok = inits[i][1] is not None and rk["status"] != fem.Batch.FAILED
             lrn_ok = inits[i][2] is not None and rl["status"] != fem.Batch.FAILED
+            qry_ok = inits[i][6] is not None and rq["status"] != fem.Batch.FAILED
             recs.append({"seed": int(s), "kpp_ok": bool(kpp_ok), "kpp_ll": rk["ll"] if kpp_ok else None,
                          "kpp_iters": rk["iter_counter"] if kpp_ok else None,
                          "learned_ok": bool(lrn_ok), "learned_ll": rl["ll"] if lrn_ok else None,
                          "learned_iters": rl["iter_counter"] if lrn_ok else None,
                          "learned_used": inits[i][3],
+                         "query_ok": bool(qry_ok), "query_ll": rq["ll"] if qry_ok else None,
+                         "query_iters": rq["iter_counter"] if qry_ok else None,
                          "kpp_init_s": inits[i][4], "learned_init_s": inits[i][5]})
         b.close()
         log(f"eval {min(c0 + chunk, len(seeds))}/{len(seeds)}")
@@ -195,7 +210,16 @@ def summarise(recs, tol=1e-6):
         "mean_init_s_kpp": float(np.mean([r["kpp_init_s"] for r in recs])),
         "mean_init_s_learned": float(np.mean([r["learned_init_s"] for r in recs])),
         "learned_fell_back_to_kpp": sum(r["learned_used"] != "learned" for r in recs),
+        "query_head": _vs_kpp(recs, "query", tol),
     }
+
+
+def _vs_kpp(recs, key, tol):
+    both = [r for r in recs if r["kpp_ok"] and r[key + "_ok"]]
+    rel = np.array([(r[key + "_ll"] - r["kpp_ll"]) / abs(r["kpp_ll"]) for r in both])
+    return {"better": int((rel > tol).sum()), "same": int((np.abs(rel) <= tol).sum()),
+            "worse": int((rel < -tol).sum()),
+            "mean_iters": float(np.mean([r[key + "_iters"] for r in both])) if both else None}
 
 
 def main(argv=None):
@@ -208,6 +232,7 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--gam", type=float, default=1.0, help="weight of the point-head loss")
     ap.add_argument("--eval", type=int, default=1000)
     ap.add_argument("--eval-seed0", type=int, default=10_000_000)
     ap.add_argument("--eval-eps", type=float, default=1e-5)
@@ -225,7 +250,7 @@ def main(argv=None):
     log(f"targets done: {data['sub'].shape[0]} fits, {data['failed']} failed, "
         f"{data['seconds']:.1f} s, mean EM iters {data['iters'].mean():.1f}")
     net = SetNet(a.d, a.K, seed=a.net_seed)
-    tr = train(net, data, a.steps, batch=a.batch, lr=a.lr, log=log)
+    tr = train(net, data, a.steps, batch=a.batch, lr=a.lr, gam=a.gam, log=log)
     if a.out:
         net.save(a.out)
         log(f"saved {a.out}")

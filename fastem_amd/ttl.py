@@ -56,6 +56,8 @@ def run_plain(ctx, seed, tol=1e-6, cap=500, conv=1e-12, ll_star=None, init=None)
 def run_schedule(ctx, seed, ll_star, stages, tol=1e-6, cap=500, init=None):
     """Reduced-precision stages, then FP64 until the target is met.
 
+    init: None (device k-means++), (w, mu, cov), or a callable that
+    initialises the context itself (timed as part of the run).
     stages: [(precision, switch_eps), ...]; each stage iterates in its form
     until the relative LL change falls below switch_eps or the LL decreases
     (the reference's convergence rule treats a decrease as converged), then
@@ -67,6 +69,8 @@ def run_schedule(ctx, seed, ll_star, stages, tol=1e-6, cap=500, init=None):
     ctx.set_precision(fem.PREC_FP64)
     if init is None:
         kmeanspp(ctx, seed)
+    elif callable(init):
+        init(ctx)
     else:
         ctx.set_params(*init)
     t_init = time.perf_counter() - t0

@@ -10,14 +10,17 @@
  *
  * Parameters are one flat FP32 vector, in this order:
  *   W1[d][64] b1[64] W2[64][64] b2[64] S[K][64] Wg[64][64]
- *   Wm[64][d] bm[d] Wv[64][d] bv[d]
+ *   Wm[64][d] bm[d] Wv[64][d] bv[d] Wp[64][64] Wq[64][64]
  * Network, per set of n points x_i (width 64):
  *   h1 = gelu(x W1 + b1), h2 = gelu(h1 W2 + b2), g = mean_i h2_i,
  *   q_k = S_k + g Wg, a_ik = softmax_i(q_k . h2_i / 8), z_k = sum_i a_ik h2_i,
- *   mu_k = z_k Wm + bm, lv_k = z_k Wv + bv.
+ *   mu_k = z_k Wm + bm, lv_k = z_k Wv + bv,
+ * and the point head, a learned mean-shift step over the set:
+ *   m_i = sum_j softmax_j((h2_i Wp) . (h2_j Wq) / 8) x_j.
  * Training loss per set against a fitted mixture (M_j, V_j, weights p_j):
  *   sum_j p_j |mu_k(j) - M_j|^2 + (1/K) sum_k min_j |mu_k - M_j|^2
- *   + lam sum_j p_j |lv_k(j) - V_j|^2 / d,   k(j) = argmin_k |mu_k - M_j|^2.
+ *   + lam sum_j p_j |lv_k(j) - V_j|^2 / d,   k(j) = argmin_k |mu_k - M_j|^2
+ *   + gam (1/n) sum_i |m_i - T_i|^2,          T_i the mean of point i's cluster.
  * Supported: d in {8, 16, 32, 64}, 1 <= K <= 64, targets J <= 4K.
  */
 #ifndef FASTEM_SETNET_H
@@ -41,16 +44,18 @@ int fem_setnet_get_params(fem_setnet *net, float *theta);
 int fem_setnet_set_params(fem_setnet *net, const float *theta);
 
 /* Forward pass over B sets of n normalised points (X: B*n*d FP32, host);
- * writes means and log-variances (B*K*d each, host). */
-int fem_setnet_forward(fem_setnet *net, int B, int n, const float *X, float *mu, float *lv);
+ * writes means and log-variances (B*K*d each, host) and, if m is not NULL,
+ * the point head's outputs (B*n*d). */
+int fem_setnet_forward(fem_setnet *net, int B, int n, const float *X, float *mu, float *lv, float *m);
 
 /* One training step over B sets: forward, loss against the targets
- * (M, V: B*J*d; p: B*J), backward.  grad (host, may be NULL) receives the
+ * (M, V: B*J*d; p: B*J; point targets T: B*n*d, or NULL to leave the point
+ * head out of the loss), backward.  grad (host, may be NULL) receives the
  * gradient of the loss summed over the batch; *loss receives that sum.  With
  * lr > 0 one Adam step (beta1 0.9, beta2 0.999, eps 1e-8) is applied with the
  * gradient of the batch-mean loss. */
 int fem_setnet_step(fem_setnet *net, int B, int n, const float *X, int J, const float *M, const float *V,
-                    const float *p, float lam, float lr, float *grad, float *loss);
+                    const float *p, const float *T, float lam, float gam, float lr, float *grad, float *loss);
 
 #ifdef __cplusplus
 }

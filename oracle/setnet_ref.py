@@ -14,9 +14,12 @@ outputs):
   a_ik = softmax_i(q_k . h2_i / sqrt(W))                        attention over points
   z_k  = sum_i a_ik h2_i
   mu_k = z_k Wm + bm,  lv_k = z_k Wv + bv                       means, log-variances
+Point head (a learned mean-shift step over the set):
+  b_ij = softmax_j((h2_i Wp) . (h2_j Wq) / sqrt(W)),  m_i = sum_j b_ij x_j
 Loss against a fitted mixture (means M_j, log-variances V_j, weights p_j):
   L = sum_j p_j ||mu_{k(j)} - M_j||^2  +  (1/K) sum_k min_j ||mu_k - M_j||^2
       + lam * sum_j p_j ||lv_{k(j)} - V_j||^2 / d,   k(j) = argmin_k ||mu_k - M_j||^2
+      + gam * (1/n) sum_i ||m_i - T_i||^2,            T_i = M_{c(i)} (point i's cluster mean)
 """
 import numpy as np
 
@@ -26,7 +29,7 @@ W = 64
 def param_shapes(d, K, width=W):
     return [("W1", (d, width)), ("b1", (width,)), ("W2", (width, width)), ("b2", (width,)),
             ("S", (K, width)), ("Wg", (width, width)), ("Wm", (width, d)), ("bm", (d,)),
-            ("Wv", (width, d)), ("bv", (d,))]
+            ("Wv", (width, d)), ("bv", (d,)), ("Wp", (width, width)), ("Wq", (width, width))]
 
 
 def unflatten(theta, d, K, width=W):
@@ -71,8 +74,22 @@ def forward(p, X):
     z = np.einsum("bki,biw->bkw", a, h2)
     mu = z @ p["Wm"] + p["bm"]
     lv = z @ p["Wv"] + p["bv"]
-    cache = dict(X=X, z1=z1, h1=h1, z2=z2, h2=h2, g=g, q=q, a=a, z=z)
+    P = h2 @ p["Wp"]
+    Q = h2 @ p["Wq"]
+    e = np.einsum("biw,bjw->bij", P, Q) / np.sqrt(width)
+    e = e - e.max(axis=2, keepdims=True)
+    bm_ = np.exp(e)
+    bm_ = bm_ / bm_.sum(axis=2, keepdims=True)            # softmax over j
+    m = np.einsum("bij,bjd->bid", bm_, X)
+    cache = dict(X=X, z1=z1, h1=h1, z2=z2, h2=h2, g=g, q=q, a=a, z=z, P=P, Q=Q, beta=bm_, m=m)
     return mu, lv, cache
+
+
+def point_loss_and_grad(m, T, gam=1.0):
+    """gam/n * sum_i ||m_i - T_i||^2 summed over the batch, and d/dm."""
+    n = m.shape[1]
+    diff = m - T
+    return gam * float((diff * diff).sum()) / n, 2.0 * gam * diff / n
 
 
 def loss_and_grad_outputs(mu, lv, M, V, pw, lam=0.5):
@@ -101,12 +118,14 @@ def loss_and_grad_outputs(mu, lv, M, V, pw, lam=0.5):
     return L, dmu, dlv
 
 
-def backward(p, cache, dmu, dlv):
-    """Gradients of sum(dmu*mu) + sum(dlv*lv) w.r.t. every parameter."""
+def backward(p, cache, dmu, dlv, dm=None):
+    """Gradients of sum(dmu*mu) + sum(dlv*lv) + sum(dm*m) w.r.t. every parameter."""
     width = p["W2"].shape[0]
     X, z1, h1, z2, h2 = cache["X"], cache["z1"], cache["h1"], cache["z2"], cache["h2"]
     g, q, a, z = cache["g"], cache["q"], cache["a"], cache["z"]
     n = X.shape[1]
+    if dm is None:
+        dm = np.zeros_like(cache["m"])
     gr = {}
     gr["Wm"] = np.einsum("bkw,bkd->wd", z, dmu)
     gr["bm"] = dmu.sum(axis=(0, 1))
@@ -124,6 +143,15 @@ def backward(p, cache, dmu, dlv):
     gr["Wg"] = g.T @ dqs
     dg = dqs @ p["Wg"].T
     dh2 += dg[:, None, :] / n
+    # point head
+    P, Q, beta = cache["P"], cache["Q"], cache["beta"]
+    db = np.einsum("bid,bjd->bij", dm, X)
+    de = beta * (db - (beta * db).sum(axis=2, keepdims=True)) / np.sqrt(width)
+    dP = np.einsum("bij,bjw->biw", de, Q)
+    dQ = np.einsum("bij,biw->bjw", de, P)
+    gr["Wp"] = np.einsum("biw,biv->wv", h2, dP)
+    gr["Wq"] = np.einsum("biw,biv->wv", h2, dQ)
+    dh2 += dP @ p["Wp"].T + dQ @ p["Wq"].T
     dz2 = dh2 * gelu_grad(z2)
     gr["W2"] = np.einsum("biw,biv->wv", h1, dz2)
     gr["b2"] = dz2.sum(axis=(0, 1))

@@ -19,17 +19,26 @@ def setup(B=2, n=7, d=3, K=4, J=3, seed=0):
     return p, X, M, V, pw
 
 
+def point_targets(X, M):
+    """Nearest target mean of every point (the point head's target)."""
+    D = ((X[:, :, None, :] - M[:, None, :, :]) ** 2).sum(-1)
+    idx = D.argmin(axis=2)
+    return np.take_along_axis(M, idx[:, :, None], axis=1)
+
+
 def total_loss(p, X, M, V, pw):
-    mu, lv, _ = sr.forward(p, X)
+    mu, lv, cache = sr.forward(p, X)
     L, _, _ = sr.loss_and_grad_outputs(mu, lv, M, V, pw)
-    return L
+    Lp, _ = sr.point_loss_and_grad(cache["m"], point_targets(X, M), gam=0.7)
+    return L + Lp
 
 
 def test_backward_matches_finite_differences():
     p, X, M, V, pw = setup()
     mu, lv, cache = sr.forward(p, X)
     L, dmu, dlv = sr.loss_and_grad_outputs(mu, lv, M, V, pw)
-    gr = sr.backward(p, cache, dmu, dlv)
+    _, dm = sr.point_loss_and_grad(cache["m"], point_targets(X, M), gam=0.7)
+    gr = sr.backward(p, cache, dmu, dlv, dm)
     h = 1e-6
     for name in p:
         flat = p[name].ravel()
@@ -59,6 +68,26 @@ def test_output_gradient_matches_finite_differences():
             Lm, _, _ = sr.loss_and_grad_outputs(mu, lv, M, V, pw)
             arr[idx] = old
             assert abs((Lp - Lm) / (2 * h) - grad[idx]) <= 1e-5
+
+
+def test_point_head_is_a_convex_combination_and_equivariant():
+    p, X, M, V, pw = setup(seed=4)
+    _, _, cache = sr.forward(p, X)
+    beta, m = cache["beta"], cache["m"]
+    np.testing.assert_allclose(beta.sum(axis=2), 1.0, rtol=0, atol=1e-12)
+    assert (beta >= 0).all()
+    # m_i lies in the convex hull: bounded by the per-coordinate min/max of the set
+    assert (m <= X.max(axis=1, keepdims=True) + 1e-12).all()
+    assert (m >= X.min(axis=1, keepdims=True) - 1e-12).all()
+    perm = np.random.default_rng(2).permutation(X.shape[1])
+    _, _, c2 = sr.forward(p, X[:, perm])
+    np.testing.assert_allclose(c2["m"], m[:, perm], rtol=1e-12, atol=1e-12)
+    # with Wp = 0 every score is equal: m_i is the set mean
+    p0 = dict(p)
+    p0["Wp"] = np.zeros_like(p["Wp"])
+    _, _, c0 = sr.forward(p0, X)
+    np.testing.assert_allclose(c0["m"], np.broadcast_to(X.mean(axis=1, keepdims=True), m.shape),
+                               rtol=1e-12, atol=1e-12)
 
 
 def test_invariances():
