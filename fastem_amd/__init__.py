@@ -242,7 +242,7 @@ def regularize(cov, g):
 
 
 def train(ctx, w, mu, cov, maxsweeps, eps=1e-5, g=0.0, ll_trace=None):
-    """EMAlgorithm::TrainNoCopy (EM2/EMAlgorithm.cpp:222-274) over the C ABI,
+    """EMAlgorithm::TrainNoCopy (EM2/EMAlgorithm.cpp:139-191) over the C ABI,
     without the condition-number check.  Returns (iter_counter, LL, (w, mu,
     cov)) with the reference's IterCounter semantics."""
     ctx.set_regularization(g)

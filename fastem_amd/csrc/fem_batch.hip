@@ -3,7 +3,7 @@
 // every active fit (E-step tiles, means chunks, covariance chunks x component
 // groups, per-component finalisers, factorisations), a per-fit convergence
 // kernel applies the reference's stopping rule on the device
-// (EM2/EMAlgorithm.cpp:235-267), and one EM iteration of all fits is captured
+// (EM2/EMAlgorithm.cpp:166-183), and one EM iteration of all fits is captured
 // in a hipGraph that the host replays until no fit is active.  Fits that have
 // converged or failed are skipped by every phase.  The per-unit bodies are the
 // ones of the single-fit kernels (fem_tiles.h), so each fit's arithmetic and
@@ -39,7 +39,7 @@ struct BFit {
     int64_t info;                       // int pool
 };
 
-// Per-fit iteration state (EM2/EMAlgorithm.cpp:222-267 semantics).
+// Per-fit iteration state (EM2/EMAlgorithm.cpp:152-183 semantics).
 struct BState {
     double ll;       // LL of the last E-step
     double prev;     // previous LL (starts at -DBL_MAX)

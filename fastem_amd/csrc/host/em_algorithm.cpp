@@ -1,7 +1,7 @@
 // em_algorithm.cpp -- EMAlgorithm / MatrixEM / SimpleMatrixEM / CachedEM over
 // the device engine.  The loop, convergence rule, IterCounter bookkeeping,
 // regularisation before the loop, condition-number abort and exception
-// messages follow EM2/EMAlgorithm.cpp:216-274; the heavy hooks call the C ABI.
+// messages follow EM2/EMAlgorithm.cpp:139-191; the heavy hooks call the C ABI.
 #include <omp.h>
 
 #include <cmath>

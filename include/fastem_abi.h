@@ -137,7 +137,7 @@ int fem_kmeans_init_seeded(fem_ctx *ctx, const double *C0, uint64_t seed, int ma
  * Every EM phase is one persistent launch over a work list covering all
  * active fits; one iteration of all fits is captured in a hipGraph and
  * replayed until every fit has stopped; each fit follows the stopping rule of
- * EMAlgorithm::TrainNoCopy (EM2/EMAlgorithm.cpp:235-267) on the device.
+ * EMAlgorithm::TrainNoCopy (EM2/EMAlgorithm.cpp:152-183) on the device.
  * Per-fit arithmetic is the FP64 form's.  No reference counterpart: it serves
  * random restarts and model-order (K) sweeps scored by BIC. */
 typedef struct fem_batch fem_batch;

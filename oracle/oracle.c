@@ -414,7 +414,7 @@ static double max_cond(int K, int d, const double *cov) {
     return mc;
 }
 
-/* EM2/EMAlgorithm.cpp:216-274 (Train -> TrainNoCopy). */
+/* EM2/EMAlgorithm.cpp:133-191 (Train -> TrainNoCopy). */
 int orc_train(int64_t n, int d, int K, const float *X, double *w, double *mu,
               double *cov, int maxsweeps, double eps, double athr, double g,
               int svd, int nvt, int nblk, int log2red, double *ll_trace,

@@ -54,7 +54,7 @@ void orc_mstep(int64_t n, int d, int K, const float *X, const double *R,
                int64_t ntotal, double *w, double *mu, double *cov,
                int nvt, int nblk, int log2red);
 
-/* ---- EM2/EMAlgorithm.cpp:216-274 TrainNoCopy loop.  Parameters are in/out.
+/* ---- EM2/EMAlgorithm.cpp:139-191 TrainNoCopy loop.  Parameters are in/out.
  * ll_trace (may be NULL) receives the LL of every sweep (up to maxsweeps).
  * Returns the final IterCounter (reference semantics) or a negative code:
  * -1 Cholesky failure, -2 condition-number abort. *out_ll gets the returned LL. */

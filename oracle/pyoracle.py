@@ -201,7 +201,7 @@ class TrainResult:
 
 def train(X, w, mu, cov, maxsweeps, eps=1e-5, athr=1e5, g=0.0, svd=True, nvt=16, nblk=1,
           log2red=False):
-    """EM2/EMAlgorithm.cpp:216-274.  rc >= 0 is IterCounter; -1 Cholesky
+    """EM2/EMAlgorithm.cpp:133-191.  rc >= 0 is IterCounter; -1 Cholesky
     failure; -2 condition-number abort."""
     X = _c32(X)
     n, d = X.shape

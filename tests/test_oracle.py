@@ -191,7 +191,7 @@ def test_mstep_ntotal_scales_weights(orc, golden):
     np.testing.assert_allclose(w2 * 2, w1, rtol=1e-15)
 
 
-# ---- Train loop: EM2/EMAlgorithm.cpp:216-274 vs scikit-learn ----------------------
+# ---- Train loop: EM2/EMAlgorithm.cpp:133-191 vs scikit-learn ----------------------
 
 @pytest.mark.parametrize("tag", ["t1", "t2"])
 def test_trajectory_vs_sklearn(orc, golden, tag):
