@@ -16,14 +16,16 @@ HOST_SOURCES = ["fem_tools.cpp", "host/linalg.cpp", "host/rand.cpp", "host/block
                 "host/normal_density.cpp", "host/gaussian_mixture.cpp", "host/dataset.cpp",
                 "host/em_algorithm.cpp", "host/em_initializer.cpp"]
 TOOLS = {"em2": "host/em2_main.cpp", "eminit": "host/eminit_main.cpp"}
-# Per-file device flags: the BF16x3 kernels keep MFMA accumulators in VGPRs,
-# which saves the accumulator-read moves when they are squared or flushed.
-FILE_FLAGS = {"fem_kernels_bf16.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+# Per-file device flags (none at present).
+FILE_FLAGS = {}
 
 # Packed FP32 VALU ops are disabled for all device code.
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # MFMA accumulators in VGPRs: no accumulator-read moves when the results
+    # are squared, reduced or flushed, and fewer registers in the M-step kernels
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-Wno-unused-command-line-argument",
 ]
 

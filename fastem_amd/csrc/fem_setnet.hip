@@ -444,27 +444,30 @@ __global__ __launch_bounds__(256) void k_linear2(const float *__restrict__ X, in
     }
 }
 
-// E[b][i][j] = P_i . Q_j / 8 for i, j < n; one 64 x 64 tile per workgroup.
-__global__ __launch_bounds__(256) void k_pscores(const float *__restrict__ P, const float *__restrict__ Q, int n,
-                                                 float *__restrict__ E) {
-    __shared__ float ps[64][W + 1], qs[64][W + 1];
+// out[b][i][j] = scale * (A_i . B_j) over I features, for i, j < n (A, B:
+// B x n x I); one 64 x 64 tile per workgroup.  Scores E = P Q^T / 8 (I = 64)
+// and DB = dM X^T (I = d).
+template <int I>
+__global__ __launch_bounds__(256) void k_gemm_nt(const float *__restrict__ A, const float *__restrict__ Bm, int n,
+                                                 float scale, float *__restrict__ out) {
+    __shared__ float as[64][I + 1], bs[64][I + 1];
     const int t = threadIdx.x;
     const int b = blockIdx.z, i0 = blockIdx.y * 64, j0 = blockIdx.x * 64;
-    for (int e = t; e < 64 * W; e += 256) {
-        const int r = e / W, w = e % W;
-        ps[r][w] = (i0 + r < n) ? P[((size_t)b * n + i0 + r) * W + w] : 0.f;
-        qs[r][w] = (j0 + r < n) ? Q[((size_t)b * n + j0 + r) * W + w] : 0.f;
+    for (int e = t; e < 64 * I; e += 256) {
+        const int r = e / I, w = e % I;
+        as[r][w] = (i0 + r < n) ? A[((size_t)b * n + i0 + r) * I + w] : 0.f;
+        bs[r][w] = (j0 + r < n) ? Bm[((size_t)b * n + j0 + r) * I + w] : 0.f;
     }
     __syncthreads();
     const int ti = t >> 4, tj = t & 15;
     float acc[4][4] = {};
-    for (int w = 0; w < W; w++) {
+    for (int w = 0; w < I; w++) {
         float qv[4];
 #pragma unroll
-        for (int c = 0; c < 4; c++) qv[c] = qs[tj + 16 * c][w];
+        for (int c = 0; c < 4; c++) qv[c] = bs[tj + 16 * c][w];
 #pragma unroll
         for (int a = 0; a < 4; a++) {
-            const float pv = ps[ti + 16 * a][w];
+            const float pv = as[ti + 16 * a][w];
 #pragma unroll
             for (int c = 0; c < 4; c++) acc[a][c] += pv * qv[c];
         }
@@ -476,17 +479,15 @@ __global__ __launch_bounds__(256) void k_pscores(const float *__restrict__ P, co
 #pragma unroll
         for (int c = 0; c < 4; c++) {
             const int j = j0 + tj + 16 * c;
-            if (j < n) E[((size_t)b * n + i) * n + j] = acc[a][c] * 0.125f;
+            if (j < n) out[((size_t)b * n + i) * n + j] = acc[a][c] * scale;
         }
     }
 }
 
-// Row (b, i): softmax over j of E in place, then m_i = sum_j beta_ij x_j.
-__global__ __launch_bounds__(256) void k_psoftmax(float *__restrict__ E, const float *__restrict__ X, int n, int d,
-                                                  float *__restrict__ m) {
+// Row (b, i): softmax over j of E in place.
+__global__ __launch_bounds__(256) void k_psoftmax(float *__restrict__ E, int n) {
     __shared__ float red[256];
-    __shared__ float mp[4][64];
-    const int r = blockIdx.x, b = r / n, t = threadIdx.x;
+    const int r = blockIdx.x, t = threadIdx.x;
     float *e = E + (size_t)r * n;
     float mx = -INFINITY;
     for (int j = t; j < n; j += 256) mx = fmaxf(mx, e[j]);
@@ -511,16 +512,7 @@ __global__ __launch_bounds__(256) void k_psoftmax(float *__restrict__ E, const f
         __syncthreads();
     }
     const float inv = 1.0f / red[0];
-    __syncthreads();
     for (int j = t; j < n; j += 256) e[j] *= inv;
-    __syncthreads();
-    const int grp = t >> 6, a = t & 63;
-    float acc = 0.f;
-    if (a < d)
-        for (int j = grp; j < n; j += 4) acc += e[j] * X[((size_t)b * n + j) * d + a];
-    mp[grp][a] = acc;
-    __syncthreads();
-    if (t < d) m[(size_t)r * d + t] = (mp[0][t] + mp[1][t]) + (mp[2][t] + mp[3][t]);
 }
 
 // Point loss gam/n |m_i - T_i|^2 per row and its gradient.
@@ -551,24 +543,15 @@ __global__ void k_fillf(float *__restrict__ p, size_t cnt, float v) {
     if (e < cnt) p[e] = v;
 }
 
-// Row (b, i): with db_j = dm_i . x_j, ds_ij = beta_ij (db_j - sum_j beta_ij db_j) / 8
-// written over beta.
-__global__ __launch_bounds__(256) void k_pattn_bwd_rows(float *__restrict__ E, const float *__restrict__ X,
-                                                        const float *__restrict__ dm, int n, int d) {
-    __shared__ float dms[64];
+// Row (b, i): ds_ij = beta_ij (db_ij - sum_j beta_ij db_ij) / 8 written over
+// beta, with db = dM X^T precomputed.
+__global__ __launch_bounds__(256) void k_pattn_bwd_rows(float *__restrict__ E, const float *__restrict__ DB, int n) {
     __shared__ float red[256];
-    const int r = blockIdx.x, b = r / n, t = threadIdx.x;
-    if (t < 64) dms[t] = (t < d) ? dm[(size_t)r * d + t] : 0.f;
-    __syncthreads();
+    const int r = blockIdx.x, t = threadIdx.x;
     float *e = E + (size_t)r * n;
-    auto dot = [&](int j) {
-        const float *x = X + ((size_t)b * n + j) * d;
-        float s = 0.f;
-        for (int a = 0; a < d; a++) s += dms[a] * x[a];
-        return s;
-    };
+    const float *db = DB + (size_t)r * n;
     float acc = 0.f;
-    for (int j = t; j < n; j += 256) acc += e[j] * dot(j);
+    for (int j = t; j < n; j += 256) acc += e[j] * db[j];
     red[t] = acc;
     __syncthreads();
     for (int st = 128; st > 0; st >>= 1) {
@@ -576,42 +559,45 @@ __global__ __launch_bounds__(256) void k_pattn_bwd_rows(float *__restrict__ E, c
         __syncthreads();
     }
     const float tot = red[0];
-    __syncthreads();
-    for (int j = t; j < n; j += 256) e[j] = e[j] * (dot(j) - tot) * 0.125f;
+    for (int j = t; j < n; j += 256) e[j] = e[j] * (db[j] - tot) * 0.125f;
 }
 
-// TRANS = 0: out[b][i][:] = sum_j DS[b][i][j] Y[b][j][:]   (dP with Y = Q)
-// TRANS = 1: out[b][j][:] = sum_i DS[b][i][j] Y[b][i][:]   (dQ with Y = P)
-template <int TRANS>
-__global__ __launch_bounds__(256) void k_pgrad(const float *__restrict__ DS, const float *__restrict__ Y, int n,
-                                               float *__restrict__ out) {
-    __shared__ float dsl[64][65];
-    __shared__ float yl[64][W + 1];
+// TRANS = 0: out[b][i][:] = sum_j S[b][i][j] Y[b][j][:]   (m = beta X, dP = DS Q)
+// TRANS = 1: out[b][j][:] = sum_i S[b][i][j] Y[b][i][:]   (dQ = DS^T P)
+// Y and out: B x n x C.  64 output rows per workgroup, C/4 columns per thread.
+template <int TRANS, int C>
+__global__ __launch_bounds__(256) void k_gemm_nn(const float *__restrict__ S, const float *__restrict__ Y, int n,
+                                                 float *__restrict__ out) {
+    constexpr int CPT = C / 4;
+    __shared__ float sl[64][65];
+    __shared__ float yl[64][C + 1];
     const int t = threadIdx.x;
     const int b = blockIdx.y, r0 = blockIdx.x * 64;
-    const int orow = t >> 2, wcol = (t & 3) * 16;
-    float acc[16] = {};
+    const int orow = t >> 2, c0 = (t & 3) * CPT;
+    float acc[CPT] = {};
     for (int k0 = 0; k0 < n; k0 += 64) {
         for (int e = t; e < 64 * 64; e += 256) {
             const int kk = e / 64, o = e % 64;
             float v = 0.f;
             if (r0 + o < n && k0 + kk < n)
-                v = TRANS ? DS[((size_t)b * n + k0 + kk) * n + r0 + o] : DS[((size_t)b * n + r0 + o) * n + k0 + kk];
-            if (TRANS) dsl[o][kk] = v;
-            else dsl[o][kk] = v;
-            yl[kk][o] = (k0 + kk < n) ? Y[((size_t)b * n + k0 + kk) * W + o] : 0.f;
+                v = TRANS ? S[((size_t)b * n + k0 + kk) * n + r0 + o] : S[((size_t)b * n + r0 + o) * n + k0 + kk];
+            sl[o][kk] = v;
+        }
+        for (int e = t; e < 64 * C; e += 256) {
+            const int kk = e / C, o = e % C;
+            yl[kk][o] = (k0 + kk < n) ? Y[((size_t)b * n + k0 + kk) * C + o] : 0.f;
         }
         __syncthreads();
         for (int kk = 0; kk < 64; kk++) {
-            const float dv = dsl[orow][kk];
+            const float dv = sl[orow][kk];
 #pragma unroll
-            for (int c = 0; c < 16; c++) acc[c] += dv * yl[kk][wcol + c];
+            for (int c = 0; c < CPT; c++) acc[c] += dv * yl[kk][c0 + c];
         }
         __syncthreads();
     }
     if (r0 + orow < n)
 #pragma unroll
-        for (int c = 0; c < 16; c++) out[((size_t)b * n + r0 + orow) * W + wcol + c] = acc[c];
+        for (int c = 0; c < CPT; c++) out[((size_t)b * n + r0 + orow) * C + c0 + c] = acc[c];
 }
 
 // dH2x = dP Wp^T + dQ Wq^T (rows x 64).
@@ -641,7 +627,7 @@ struct fem_setnet {
     // point head
     size_t cap_e = 0;  // capacity of E in floats (sum over sets of n^2)
     float *P = nullptr, *Q = nullptr, *E = nullptr, *pm = nullptr, *T = nullptr, *dm = nullptr, *lrow = nullptr;
-    float *dP = nullptr, *dQ = nullptr, *dh2x = nullptr, *dummy = nullptr;
+    float *dP = nullptr, *dQ = nullptr, *dh2x = nullptr, *dummy = nullptr, *DB = nullptr;
     int step = 0;
     std::string err;
 };
@@ -663,7 +649,7 @@ void free_ws(fem_setnet *s) {
     float **ptrs[] = {&s->X,  &s->Z1,  &s->H1, &s->Z2, &s->H2,  &s->g,    &s->q,  &s->alpha, &s->z,
                       &s->mu, &s->lv,  &s->M,  &s->V,  &s->pw,  &s->dmu,  &s->dlv, &s->loss, &s->dz,
                       &s->ds, &s->dq,  &s->dqs, &s->dg, &s->dZ2, &s->dZ1, &s->part, &s->P,   &s->Q,
-                      &s->E,  &s->pm,  &s->T,  &s->dm, &s->lrow, &s->dP,  &s->dQ,  &s->dh2x, &s->dummy};
+                      &s->E,  &s->pm,  &s->T,  &s->dm, &s->lrow, &s->dP,  &s->dQ,  &s->dh2x, &s->dummy, &s->DB};
     for (float **p : ptrs) {
         if (*p) (void)hipFree(*p);
         *p = nullptr;
@@ -693,7 +679,8 @@ int ensure_ws(fem_setnet *s, int B, int n) {
         A(&s->part, (size_t)chunks * 65 * W) || A(&s->P, (size_t)rows * W) || A(&s->Q, (size_t)rows * W) ||
         A(&s->E, ecount) || A(&s->pm, (size_t)rows * d) || A(&s->T, (size_t)rows * d) ||
         A(&s->dm, (size_t)rows * d) || A(&s->lrow, (size_t)rows) || A(&s->dP, (size_t)rows * W) ||
-        A(&s->dQ, (size_t)rows * W) || A(&s->dh2x, (size_t)rows * W) || A(&s->dummy, (size_t)W))
+        A(&s->dQ, (size_t)rows * W) || A(&s->dh2x, (size_t)rows * W) || A(&s->dummy, (size_t)W) ||
+        A(&s->DB, ecount))
         return serr(s, FEM_ERR_HIP, "setnet workspace allocation failed");
     s->cap_rows = rows;
     s->cap_sets = B;
@@ -729,8 +716,12 @@ void forward(fem_setnet *s, int B, int n) {
     hipLaunchKernelGGL(k_linear2, dim3((rows + 63) / 64), dim3(256), 0, s->stream, s->H2, rows, th + s->off.Wp,
                        th + s->off.Wq, s->P, s->Q);
     const int nt = (n + 63) / 64;
-    hipLaunchKernelGGL(k_pscores, dim3(nt, nt, B), dim3(256), 0, s->stream, s->P, s->Q, n, s->E);
-    hipLaunchKernelGGL(k_psoftmax, dim3(rows), dim3(256), 0, s->stream, s->E, s->X, n, d, s->pm);
+    hipLaunchKernelGGL(k_gemm_nt<W>, dim3(nt, nt, B), dim3(256), 0, s->stream, s->P, s->Q, n, 0.125f, s->E);
+    hipLaunchKernelGGL(k_psoftmax, dim3(rows), dim3(256), 0, s->stream, s->E, n);
+    by_d(d, [&](auto Ic) {
+        constexpr int I = decltype(Ic)::value;
+        hipLaunchKernelGGL((k_gemm_nn<0, I>), dim3(nt, B), dim3(256), 0, s->stream, s->E, s->X, n, s->pm);
+    });
 }
 
 // backward from s->dmu, s->dlv into s->grad
@@ -748,10 +739,14 @@ void backward(fem_setnet *s, int B, int n) {
                        s->dqs, B, K, gr + s->off.S, gr + s->off.Wg);
     // point head: ds over E, dP, dQ, their weight gradients and dH2 share
     const int chunks = (rows + 255) / 256;
-    hipLaunchKernelGGL(k_pattn_bwd_rows, dim3(rows), dim3(256), 0, s->stream, s->E, s->X, s->dm, n, d);
     const int nt = (n + 63) / 64;
-    hipLaunchKernelGGL(k_pgrad<0>, dim3(nt, B), dim3(256), 0, s->stream, s->E, s->Q, n, s->dP);
-    hipLaunchKernelGGL(k_pgrad<1>, dim3(nt, B), dim3(256), 0, s->stream, s->E, s->P, n, s->dQ);
+    by_d(d, [&](auto Ic) {
+        constexpr int I = decltype(Ic)::value;
+        hipLaunchKernelGGL(k_gemm_nt<I>, dim3(nt, nt, B), dim3(256), 0, s->stream, s->dm, s->X, n, 1.0f, s->DB);
+    });
+    hipLaunchKernelGGL(k_pattn_bwd_rows, dim3(rows), dim3(256), 0, s->stream, s->E, s->DB, n);
+    hipLaunchKernelGGL((k_gemm_nn<0, W>), dim3(nt, B), dim3(256), 0, s->stream, s->E, s->Q, n, s->dP);
+    hipLaunchKernelGGL((k_gemm_nn<1, W>), dim3(nt, B), dim3(256), 0, s->stream, s->E, s->P, n, s->dQ);
     hipLaunchKernelGGL(k_dense_bwd_weight_part<W>, dim3(chunks), dim3(256), 0, s->stream, s->H2, s->dP, rows,
                        s->part);
     hipLaunchKernelGGL(k_sum_parts, dim3((65 * W + 255) / 256), dim3(256), 0, s->stream, s->part, chunks, W,

@@ -34,7 +34,7 @@ def _gen(args):
     return fem.gengmm(n, d, K, lo, hi, diag, seed)
 
 
-def make_targets(seeds, n, d, K, gen, keep, eps=1e-8, maxsweeps=200, chunk=256, log=print):
+def make_targets(seeds, n, d, K, gen, keep, eps=1e-8, maxsweeps=200, chunk=256, g=0.0, log=print):
     """Converged engine fits of the mixtures `seeds` (see module docstring)."""
     lo, hi, diag = gen
     F = len(seeds)
@@ -59,7 +59,7 @@ def make_targets(seeds, n, d, K, gen, keep, eps=1e-8, maxsweeps=200, chunk=256, 
                 batch.upload(i, X)
                 batch.set_params(i, w, mu, cov)
                 subs.append(subsample(X, keep, int(cs[i])))
-            batch.run(maxsweeps, eps=eps)
+            batch.run(maxsweeps, eps=eps, g=g)
             for i in range(len(cs)):
                 r = batch.result(i)
                 f = c0 + i
@@ -122,7 +122,7 @@ def train(net, data, steps, batch=64, lr=2e-3, gam=1.0, seed=0, val_frac=0.02, l
 
 
 def evaluate(net, seeds, n, d, K, gen, n_sub=1024, eps=1e-5, maxsweeps=1000, lloyd_iters=29, chunk=128,
-             log=print):
+             g=0.0, log=print):
     """EM from the learned initialisation vs EM from k-means++ on fresh seeds.
 
     Both start with Lloyd sweeps and the cluster mixture on the device; they
@@ -167,7 +167,7 @@ def evaluate(net, seeds, n, d, K, gen, n_sub=1024, eps=1e-5, maxsweeps=1000, llo
             for j, c in enumerate(cands):
                 b.upload(3 * i + j, X)
                 b.set_params(3 * i + j, *(c if c is not None else fallback))
-        b.run(maxsweeps, eps=eps)
+        b.run(maxsweeps, eps=eps, g=g)
         for i, s in enumerate(cs):
             rk, rl, rq = b.result(3 * i), b.result(3 * i + 1), b.result(3 * i + 2)
             kpp_ok = inits[i][1] is not None and rk["status"] != fem.Batch.FAILED
@@ -236,6 +236,9 @@ def main(argv=None):
     ap.add_argument("--eval", type=int, default=1000)
     ap.add_argument("--eval-seed0", type=int, default=10_000_000)
     ap.add_argument("--eval-eps", type=float, default=1e-5)
+    ap.add_argument("--eval-n", type=int, default=None, help="points per evaluation dataset (default --n)")
+    ap.add_argument("--g", type=float, default=0.0,
+                    help="covariance regularisation of the target fits and of the evaluation EM")
     ap.add_argument("--out", default=None)
     ap.add_argument("--report", default=None)
     ap.add_argument("--net-seed", type=int, default=0)
@@ -246,7 +249,7 @@ def main(argv=None):
 
     gen = GEN.get((a.d, a.K), (-10.0, 10.0, 0.1))
     log(f"targets: {a.fits} fits n={a.n} d={a.d} K={a.K} gen={gen}")
-    data = make_targets(np.arange(1, a.fits + 1), a.n, a.d, a.K, gen, a.keep, log=log)
+    data = make_targets(np.arange(1, a.fits + 1), a.n, a.d, a.K, gen, a.keep, g=a.g, log=log)
     log(f"targets done: {data['sub'].shape[0]} fits, {data['failed']} failed, "
         f"{data['seconds']:.1f} s, mean EM iters {data['iters'].mean():.1f}")
     net = SetNet(a.d, a.K, seed=a.net_seed)
@@ -254,8 +257,8 @@ def main(argv=None):
     if a.out:
         net.save(a.out)
         log(f"saved {a.out}")
-    recs = evaluate(net, np.arange(a.eval_seed0, a.eval_seed0 + a.eval), a.n, a.d, a.K, gen,
-                    n_sub=a.keep, eps=a.eval_eps, log=log) if a.eval > 0 else []
+    recs = evaluate(net, np.arange(a.eval_seed0, a.eval_seed0 + a.eval), a.eval_n or a.n, a.d, a.K, gen,
+                    n_sub=a.keep, eps=a.eval_eps, g=a.g, log=log) if a.eval > 0 else []
     summ = summarise(recs) if recs else {}
     log(json.dumps(summ))
     if a.report:
