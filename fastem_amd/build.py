@@ -16,16 +16,17 @@ HOST_SOURCES = ["fem_tools.cpp", "host/linalg.cpp", "host/rand.cpp", "host/block
                 "host/normal_density.cpp", "host/gaussian_mixture.cpp", "host/dataset.cpp",
                 "host/em_algorithm.cpp", "host/em_initializer.cpp"]
 TOOLS = {"em2": "host/em2_main.cpp", "eminit": "host/eminit_main.cpp"}
-# Per-file device flags (none at present).
-FILE_FLAGS = {}
+# MFMA accumulators in VGPRs (no accumulator-read moves when results are
+# squared, reduced or flushed) for the FP32, BF16x3, k-means and set-network
+# kernels; the FP64 kernels measured faster with the default AGPR form.
+VGPR_FORM = ["-mllvm", "-amdgpu-mfma-vgpr-form"]
+FILE_FLAGS = {"fem_kernels32.hip": VGPR_FORM, "fem_kernels_bf16.hip": VGPR_FORM,
+              "fem_kmeans.hip": VGPR_FORM, "fem_setnet.hip": VGPR_FORM}
 
 # Packed FP32 VALU ops are disabled for all device code.
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    # MFMA accumulators in VGPRs: no accumulator-read moves when the results
-    # are squared, reduced or flushed, and fewer registers in the M-step kernels
-    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-Wno-unused-command-line-argument",
 ]
 

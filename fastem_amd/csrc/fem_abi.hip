@@ -130,7 +130,7 @@ hipError_t dmalloc(T **p, size_t count) {
 
 void free_all(fem_ctx *c) {
     void *ptrs[] = {c->X, c->Rt, c->Rt32, c->w, c->mu, c->cov, c->nk, c->logdet, c->info, c->coef,
-                    c->U, c->ufrag, c->ufrag32, c->minit, c->mus32, c->stageb, c->ll_part, c->ll_dev,
+                    c->U, c->stage64, c->stage32, c->stageb, c->ll_part, c->ll_dev,
                     c->mpart, c->cpart, c->kC, c->dmin, c->cls, c->kfit};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -214,8 +214,7 @@ int fem_create(fem_ctx **out, int64_t n, int d, int K, int precision, int device
     const bool f64 = precision == FEM_PREC_FP64;
     if (f64 && (e = dmalloc(&c->Rt, nR)) != hipSuccess) return bail(e, "hipMalloc(R)");
     if (!f64 && (e = dmalloc(&c->Rt32, nR)) != hipSuccess) return bail(e, "hipMalloc(R32)");
-    if (!f64 && ((e = dmalloc(&c->ufrag32, (size_t)K * fem::estep_pairs(c->NB) * 64)) != hipSuccess ||
-                 (e = dmalloc(&c->mus32, (size_t)K * 2 * c->dpad)) != hipSuccess))
+    if (!f64 && (e = dmalloc(&c->stage32, (size_t)K * fem::stage32_words(c->NB))) != hipSuccess)
         return bail(e, "hipMalloc(fp32 factors)");
     if (precision == FEM_PREC_BF16X3 &&
         (e = dmalloc(&c->stageb, (size_t)K * fem::estepb_stage(c->NB))) != hipSuccess)
@@ -227,8 +226,7 @@ int fem_create(fem_ctx **out, int64_t n, int d, int K, int precision, int device
         (e = dmalloc(&c->info, K)) != hipSuccess || (e = dmalloc(&c->coef, K)) != hipSuccess)
         return bail(e, "hipMalloc(params)");
     if ((e = dmalloc(&c->U, (size_t)K * c->dpad * c->dpad)) != hipSuccess ||
-        (e = dmalloc(&c->ufrag, (size_t)K * npairs * 64)) != hipSuccess ||
-        (e = dmalloc(&c->minit, (size_t)K * c->dpad)) != hipSuccess)
+        (e = dmalloc(&c->stage64, (size_t)K * (npairs * 64 + c->dpad))) != hipSuccess)
         return bail(e, "hipMalloc(factors)");
     if ((e = dmalloc(&c->ll_part, c->est_blocks)) != hipSuccess ||
         (e = dmalloc(&c->ll_dev, 1)) != hipSuccess ||
@@ -290,10 +288,8 @@ int fem_set_precision(fem_ctx *c, int precision) {
             FEM_HIP(c, hipMalloc((void **)&c->Rt32, nR * sizeof(float)));
             FEM_HIP(c, hipMemsetAsync(c->Rt32, 0, nR * sizeof(float), c->stream));
         }
-        if (!c->ufrag32)
-            FEM_HIP(c, hipMalloc((void **)&c->ufrag32,
-                                 sizeof(float) * c->K * fem::estep_pairs(c->NB) * 64));
-        if (!c->mus32) FEM_HIP(c, hipMalloc((void **)&c->mus32, sizeof(float) * c->K * 2 * c->dpad));
+        if (!c->stage32)
+            FEM_HIP(c, hipMalloc((void **)&c->stage32, sizeof(float) * c->K * fem::stage32_words(c->NB)));
     }
     if (precision == FEM_PREC_BF16X3) {
         if (!c->stageb)

@@ -35,7 +35,7 @@ struct BFit {
     int64_t c_chunk_rows;
     int64_t x;                          // float pool
     int64_t r;                          // double pool: posteriors KP x npad
-    int64_t w, mu, cov, nk, logdet, coef, U, ufrag, minit, llp, mpart, cpart;  // double pool
+    int64_t w, mu, cov, nk, logdet, coef, U, stage64, llp, mpart, cpart;  // double pool
     int64_t info;                       // int pool
 };
 
@@ -100,7 +100,7 @@ __global__ __launch_bounds__(256) void kb_factor(const BFit *__restrict__ F, BSt
         if (gate && !S[itm.fit].active) continue;
         const BFit &f = F[itm.fit];
         factor_unit(itm.a, f.d, f.dpad, f.NB, f.npairs, dp + f.cov, dp + f.w, dp + f.mu, dp + f.logdet,
-                    ip + f.info, dp + f.coef, dp + f.U, dp + f.ufrag, nullptr, dp + f.minit, nullptr, A, &fail);
+                    ip + f.info, dp + f.coef, dp + f.U, dp + f.stage64, nullptr, A, &fail);
     }
 }
 
@@ -119,10 +119,10 @@ __global__ __launch_bounds__(256) void kb_estep(const BFit *__restrict__ F, cons
         double *Rt = dp + f.r;
         double *llo = dp + f.llp + tile;
         switch (f.NB) {
-            case 1: estep64_unit<1, P>(tile, X, f.n, f.npad, f.K, dp + f.ufrag, dp + f.minit, dp + f.coef, Rt, llo, lds, wsum); break;
-            case 2: estep64_unit<2, P>(tile, X, f.n, f.npad, f.K, dp + f.ufrag, dp + f.minit, dp + f.coef, Rt, llo, lds, wsum); break;
-            case 3: estep64_unit<3, P>(tile, X, f.n, f.npad, f.K, dp + f.ufrag, dp + f.minit, dp + f.coef, Rt, llo, lds, wsum); break;
-            case 4: estep64_unit<4, P>(tile, X, f.n, f.npad, f.K, dp + f.ufrag, dp + f.minit, dp + f.coef, Rt, llo, lds, wsum); break;
+            case 1: estep64_unit<1, P>(tile, X, f.n, f.npad, f.K, dp + f.stage64, dp + f.coef, Rt, llo, lds, wsum); break;
+            case 2: estep64_unit<2, P>(tile, X, f.n, f.npad, f.K, dp + f.stage64, dp + f.coef, Rt, llo, lds, wsum); break;
+            case 3: estep64_unit<3, P>(tile, X, f.n, f.npad, f.K, dp + f.stage64, dp + f.coef, Rt, llo, lds, wsum); break;
+            case 4: estep64_unit<4, P>(tile, X, f.n, f.npad, f.K, dp + f.stage64, dp + f.coef, Rt, llo, lds, wsum); break;
         }
     }
 }
@@ -375,8 +375,7 @@ int fem_batch_create(fem_batch **out, int nfits, const int64_t *n, const int *d,
         f.logdet = take(f.K);
         f.coef = take(f.K);
         f.U = take((int64_t)f.K * f.dpad * f.dpad);
-        f.ufrag = take((int64_t)f.K * f.npairs * 64);
-        f.minit = take((int64_t)f.K * f.dpad);
+        f.stage64 = take((int64_t)f.K * (f.npairs * 64 + f.dpad));
         f.llp = take(f.npad / 64);
         f.mpart = take((int64_t)f.m_chunks * f.KP * (f.dpad + 1));
         f.cpart = take((int64_t)f.c_chunks * f.KP * f.ub * 256);

@@ -23,6 +23,8 @@ inline int estep_pairs(int NB) {
     return p;
 }
 inline int upper_blocks(int NB) { return NB * (NB + 1) / 2; }
+// FP32 E-step stage block per component: fragments, mu_hi, -U^T mu_lo.
+inline int stage32_words(int NB) { return estep_pairs(NB) * 64 + 2 * 16 * NB; }
 
 struct Launch {
     hipStream_t stream;
@@ -54,10 +56,8 @@ struct fem_ctx {
     int *info = nullptr;       // K
     double *coef = nullptr;    // K   log w_k - d/2 log 2pi - logdet_k/2
     double *U = nullptr;       // K x dpad x dpad  U_k = L_k^-T (zero padded)
-    double *ufrag = nullptr;   // K x pairs x 64   E-step A fragments (FP64)
-    float *ufrag32 = nullptr;  // K x pairs x 64   (FP32)
-    double *minit = nullptr;   // K x dpad         U_k^T mu_k
-    float *mus32 = nullptr;    // K x 2 x dpad     mu split hi/lo (FP32)
+    double *stage64 = nullptr; // K x (pairs x 64 + dpad): FP64 E-step fragments, -U_k^T mu_k
+    float *stage32 = nullptr;  // K x stage32_words: FP32 fragments, mu_hi, -U_k^T mu_lo
     unsigned int *stageb = nullptr;  // K x stage words: BF16 hi/lo E-step fragments, mu_hi, U_k^T mu_lo
 
     // reductions

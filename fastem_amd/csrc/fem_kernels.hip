@@ -22,12 +22,10 @@ __global__ __launch_bounds__(256) void k_factor(int K, int d, int dpad, int NB, 
                                                 const double *__restrict__ mu,
                                                 double *__restrict__ logdet, int *__restrict__ info,
                                                 double *__restrict__ coef, double *__restrict__ U,
-                                                double *__restrict__ ufrag, float *__restrict__ ufrag32,
-                                                double *__restrict__ minit, float *__restrict__ mus32) {
+                                                double *__restrict__ stage64, float *__restrict__ stage32) {
     extern __shared__ double A[];  // d x d
     __shared__ int fail;
-    factor_unit(blockIdx.x, d, dpad, NB, npairs, cov, w, mu, logdet, info, coef, U, ufrag, ufrag32, minit,
-                mus32, A, &fail);
+    factor_unit(blockIdx.x, d, dpad, NB, npairs, cov, w, mu, logdet, info, coef, U, stage64, stage32, A, &fail);
 }
 
 hipError_t launch_factor(fem_ctx *c) {
@@ -40,7 +38,7 @@ hipError_t launch_factor(fem_ctx *c) {
     }
     hipLaunchKernelGGL(k_factor, dim3(c->K), dim3(256), lds, c->stream, c->K, c->d, c->dpad,
                        c->NB, npairs, c->cov, c->w, c->mu, c->logdet, c->info, c->coef, c->U,
-                       c->ufrag, c->ufrag32, c->minit, c->mus32);
+                       c->stage64, c->stage32);
     if (c->prec == FEM_PREC_BF16X3) {
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
@@ -53,14 +51,13 @@ hipError_t launch_factor(fem_ctx *c) {
 template <int NB, int P>
 __global__ __launch_bounds__(256) void k_estep64(const float *__restrict__ X, int64_t n,
                                                  int64_t npad, int K,
-                                                 const double *__restrict__ ufrag,
-                                                 const double *__restrict__ minit,
+                                                 const double *__restrict__ stage,
                                                  const double *__restrict__ coef,
                                                  double *__restrict__ Rt,
                                                  double *__restrict__ ll_part) {
     extern __shared__ double lds[];  // 2 * estep64_stage<NB>()
     __shared__ double wsum[4];
-    estep64_unit<NB, P>(blockIdx.x, X, n, npad, K, ufrag, minit, coef, Rt, ll_part + blockIdx.x, lds, wsum);
+    estep64_unit<NB, P>(blockIdx.x, X, n, npad, K, stage, coef, Rt, ll_part + blockIdx.x, lds, wsum);
 }
 
 // Deterministic sum of n doubles into *out: fixed strided partials, then a
@@ -82,7 +79,7 @@ static hipError_t estep64_t(fem_ctx *c) {
     }
     const int blocks = (int)(c->npad / (64 * P));
     hipLaunchKernelGGL((k_estep64<NB, P>), dim3(blocks), dim3(256), lds, c->stream, c->X, c->n,
-                       c->npad, c->K, c->ufrag, c->minit, c->coef, c->Rt, c->ll_part);
+                       c->npad, c->K, c->stage64, c->coef, c->Rt, c->ll_part);
     hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(256), 0, c->stream, c->ll_part,
                        (int64_t)blocks, c->ll_dev);
     return hipGetLastError();

@@ -8,10 +8,11 @@
 // D[4*(l/16) + v][l%16]; A[i][k] comes from lane i + 16k, B[k][j] from lane
 // j + 16k.
 //
-// Centring: the E-step and the covariance pass form x - mu as
-// (x - mu_hi) - mu_lo with mu = mu_hi + mu_lo split into two floats, so the
-// centred value is accurate to FP32 rounding of the centred value rather than
-// of |mu| (means reach |mu| ~ 20 sigma).
+// Centring: with mu = mu_hi + mu_lo split into two floats, the E-step forms
+// U^T (x - mu_hi) - U^T mu_lo (the second term precomputed in FP64 and used as
+// the accumulator seed) and the covariance pass forms (x - mu_hi) - mu_lo, so
+// the centred value is accurate to FP32 rounding of the centred value rather
+// than of |mu| (means reach |mu| ~ 20 sigma).
 #include "fem_internal.h"
 
 #include <cmath>
@@ -33,14 +34,15 @@ __device__ inline float xor_sum_g(float v) {
 }
 
 // ---------------------------------------------------------------------------
-// Stage (a), FP32.  Workgroup = 4 waves x P groups of 16 points.  Component
-// operands (U_k fragments, mu_k hi/lo) are double-buffered in LDS: the loads of
-// component k+1 are issued into registers before the MFMAs of component k and
-// written to the idle buffer after them, one barrier per component.
+// Stage (a), FP32.  Workgroup = 4 waves x P groups of 16 points.  The
+// component's stage block (U_k fragments, mu_hi, -U_k^T mu_lo; written by the
+// factorisation) is double-buffered in LDS: the loads of component k+1 are
+// issued into registers before the MFMAs of component k and written to the
+// idle buffer after them, one barrier per component.  The B operand is
+// x - mu_hi and the accumulators start at -U_k^T mu_lo, so D = U_k^T (x - mu).
 template <int NB, int P>
 __global__ __launch_bounds__(256) void k_estep32(const float *__restrict__ X, int64_t n, int64_t npad,
-                                                 int K, const float *__restrict__ ufrag,
-                                                 const float *__restrict__ mus,
+                                                 int K, const float *__restrict__ stage,
                                                  const double *__restrict__ coef,
                                                  float *__restrict__ Rt, double *__restrict__ ll_part) {
     constexpr int DP = 16 * NB;
@@ -59,16 +61,10 @@ __global__ __launch_bounds__(256) void k_estep32(const float *__restrict__ X, in
 #pragma unroll
         for (int lb = 0; lb < 4 * NB; lb++) xr[p][lb] = X[(base + 16 * p + c) * DP + 4 * lb + g];
 
-    auto stage_src = [&](int k, int e) -> float {
-        return (e < NPAIR * 64) ? ufrag[(size_t)k * NPAIR * 64 + e]
-                                : mus[(size_t)k * 2 * DP + (e - NPAIR * 64)];
-    };
-    {
 #pragma unroll
-        for (int q = 0; q < PER; q++) {
-            const int e = tid + 256 * q;
-            if (e < STAGE) lds32[e] = stage_src(0, e);
-        }
+    for (int q = 0; q < PER; q++) {
+        const int e = tid + 256 * q;
+        if (e < STAGE) lds32[e] = stage[e];
     }
     __syncthreads();
 
@@ -85,21 +81,23 @@ __global__ __launch_bounds__(256) void k_estep32(const float *__restrict__ X, in
 #pragma unroll
         for (int q = 0; q < PER; q++) {
             const int e = tid + 256 * q;
-            pre[q] = (more && e < STAGE) ? stage_src(k + 1, e) : 0.0f;
+            pre[q] = (more && e < STAGE) ? stage[(size_t)(k + 1) * STAGE + e] : 0.0f;
         }
         const float ck = (float)coef[k];
         v4f acc[P][NB];
 #pragma unroll
-        for (int p = 0; p < P; p++)
+        for (int jb = 0; jb < NB; jb++) {
+            const float4 m = *reinterpret_cast<const float4 *>(lml + 16 * jb + 4 * g);
 #pragma unroll
-            for (int jb = 0; jb < NB; jb++) acc[p][jb] = (v4f){0.f, 0.f, 0.f, 0.f};
+            for (int p = 0; p < P; p++) acc[p][jb] = (v4f){m.x, m.y, m.z, m.w};
+        }
         int pr = 0;
 #pragma unroll
         for (int lb = 0; lb < 4 * NB; lb++) {
-            const float mh = lmh[4 * lb + g], ml = lml[4 * lb + g];
+            const float mh = lmh[4 * lb + g];
             float b[P];
 #pragma unroll
-            for (int p = 0; p < P; p++) b[p] = (xr[p][lb] - mh) - ml;
+            for (int p = 0; p < P; p++) b[p] = xr[p][lb] - mh;
 #pragma unroll
             for (int jb = lb / 4; jb < NB; jb++) {
                 const float a = lu[pr * 64 + lane];
@@ -179,7 +177,7 @@ static hipError_t estep32_t(fem_ctx *c) {
     }
     const int blocks = (int)(c->npad / (64 * P));
     hipLaunchKernelGGL((k_estep32<NB, P>), dim3(blocks), dim3(256), lds, c->stream, c->X, c->n, c->npad,
-                       c->K, c->ufrag32, c->mus32, c->coef, c->Rt32, c->ll_part);
+                       c->K, c->stage32, c->coef, c->Rt32, c->ll_part);
     hipLaunchKernelGGL(k_reduce_sum32, dim3(1), dim3(256), 0, c->stream, c->ll_part, (int64_t)blocks,
                        c->ll_dev);
     return hipGetLastError();
@@ -276,11 +274,14 @@ hipError_t launch_means32_partials(fem_ctx *c) {
 }
 
 // ---------------------------------------------------------------------------
-// Stage (c), FP32 products: T = ((x - mu_hi) - mu_lo) * sqrt(r) / sqrt(N_k) in
-// FP32, D += T_ib^T T_jb on FP32 MFMA, accumulators flushed into FP64 every
-// 2048 points (into registers for d <= 64, else into the wave's own partial
-// slice in global memory).  The loads of the next 16 points are issued before
-// the MFMAs of the current 16.  Same grid and XCD remapping as k_cov64.
+// Stage (c), FP32 products: T = (x - mu_hi) * sqrt(r) / sqrt(N_k) in FP32
+// (v_sqrt_f32), D += T_ib^T T_jb on FP32 MFMA, accumulators flushed into FP64
+// every 2048 points (into registers for d <= 64, else into the wave's own
+// partial slice in global memory).  Centring with mu_hi = float(mu) alone
+// changes D by N_k mu_lo mu_lo^T plus a term proportional to
+// sum_i r_ik (x_i - mu_k), both below 1e-9 of D.  The loads of the next 16
+// points are issued once the current 16 are formed, before their MFMAs.  Same
+// grid and XCD remapping as k_cov64.
 template <int NB>
 __global__ __launch_bounds__(256) void k_cov32(const float *__restrict__ X, const float *__restrict__ Rt,
                                                int64_t npad, int K, int KP, int d,
@@ -302,13 +303,9 @@ __global__ __launch_bounds__(256) void k_cov32(const float *__restrict__ X, cons
     if (k >= K) return;
     const int64_t i0 = (int64_t)chunk * chunk_rows;
     const int64_t i1 = (i0 + chunk_rows < npad) ? i0 + chunk_rows : npad;
-    float mh[NB], ml[NB];
+    float mh[NB];
 #pragma unroll
-    for (int fb = 0; fb < NB; fb++) {
-        const double m = (16 * fb + c < d) ? mu[(size_t)k * d + 16 * fb + c] : 0.0;
-        mh[fb] = (float)m;
-        ml[fb] = (float)(m - (double)mh[fb]);
-    }
+    for (int fb = 0; fb < NB; fb++) mh[fb] = (16 * fb + c < d) ? (float)mu[(size_t)k * d + 16 * fb + c] : 0.0f;
     const float inv = (float)(1.0 / sqrt(nk[k]));
     double *out = cpart + ((size_t)chunk * KP + k) * UB * 256;
     const float *rrow = Rt + (size_t)k * npad;
@@ -336,26 +333,22 @@ __global__ __launch_bounds__(256) void k_cov32(const float *__restrict__ X, cons
 #pragma unroll
         for (int u = 0; u < UB; u++) acc[u] = (v4f){0.f, 0.f, 0.f, 0.f};
         for (int64_t i = s0; i < s1; i += 16) {
-            float cr[4], cx[4][NB];
+            float t[4][NB];
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                cr[q] = rv[q];
+                const float sw = __builtin_amdgcn_sqrtf(rv[q]) * inv;
 #pragma unroll
-                for (int fb = 0; fb < NB; fb++) cx[q][fb] = xv[q][fb];
+                for (int fb = 0; fb < NB; fb++) t[q][fb] = (xv[q][fb] - mh[fb]) * sw;
             }
             if (i + 16 < i1) load16(i + 16);
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const float sw = sqrtf(cr[q]) * inv;
-                float t[NB];
-#pragma unroll
-                for (int fb = 0; fb < NB; fb++) t[fb] = ((cx[q][fb] - mh[fb]) - ml[fb]) * sw;
                 int u = 0;
 #pragma unroll
                 for (int ib = 0; ib < NB; ib++)
 #pragma unroll
                     for (int jb = ib; jb < NB; jb++) {
-                        acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(t[ib], t[jb], acc[u], 0, 0, 0);
+                        acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(t[q][ib], t[q][jb], acc[u], 0, 0, 0);
                         u++;
                     }
             }

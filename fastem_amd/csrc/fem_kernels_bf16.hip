@@ -80,8 +80,8 @@ __device__ inline float xor_sum_gb(float v) {
 //                      = U_k[32 ls + kk][16 jb + i] for every pair (ls, jb),
 //                      split hi/lo, stored [pair][hi|lo][lane][4 words];
 //   [FW, FW + 32 NS)   mu_hi = float(mu_k), zero beyond d;
-//   [.., + 16 NB)      mlo = U_k^T mu_lo with mu_lo = mu_k - mu_hi: the
-//                      E-step forms D = U_k^T (x - mu_hi) - mlo.
+//   [.., + 16 NB)      -U_k^T mu_lo with mu_lo = mu_k - mu_hi, the seed of
+//                      the accumulators: D = U_k^T (x - mu_hi) - U_k^T mu_lo.
 template <int NB>
 __device__ __host__ constexpr int estepb_stage_words() {
     return pairs_bf16(NB) * 512 + 32 * ((NB + 1) / 2) + 16 * NB;
@@ -130,17 +130,17 @@ __global__ __launch_bounds__(256) void k_prep_bf16(int d, int dpad, int NB, cons
                 const double m = mu[(size_t)k * d + f];
                 s += Uk[(size_t)f * dpad + j] * (m - (double)(float)m);
             }
-        st[FW + 32 * NS + j] = __builtin_bit_cast(unsigned int, (float)s);
+        st[FW + 32 * NS + j] = __builtin_bit_cast(unsigned int, (float)(-s));
     }
 }
 
 // ---------------------------------------------------------------------------
 // Stage (a), BF16x3.  Workgroup = 4 waves x P groups of 16 points.  Lane
 // (g, c) holds, for point c of each group, features 32 ls + 8g .. +7 of every
-// strip ls as FP32 (the B operand layout).  Per component: centre with mu_hi,
-// split, 3 MFMAs per pair, subtract mlo from the accumulators while squaring;
-// the component's stage block is double-buffered in LDS as in the FP32 form,
-// one barrier per component.
+// strip ls as FP32 (the B operand layout).  Per component: accumulators
+// seeded with -U_k^T mu_lo, centre with mu_hi, split, 3 MFMAs per pair; the
+// component's stage block is double-buffered in LDS as in the FP32 form, one
+// barrier per component.
 template <int NB, int P>
 __global__ __launch_bounds__(256) void k_estepb(const float *__restrict__ X, int64_t n, int64_t npad, int K,
                                                 const unsigned int *__restrict__ stage,
@@ -200,9 +200,11 @@ __global__ __launch_bounds__(256) void k_estepb(const float *__restrict__ X, int
         const float ck = (float)coef[k];
         v4f acc[P][NB];
 #pragma unroll
-        for (int p = 0; p < P; p++)
+        for (int jb = 0; jb < NB; jb++) {
+            const float4 m = *reinterpret_cast<const float4 *>(lml + 16 * jb + 4 * g);
 #pragma unroll
-            for (int jb = 0; jb < NB; jb++) acc[p][jb] = (v4f){0.f, 0.f, 0.f, 0.f};
+            for (int p = 0; p < P; p++) acc[p][jb] = (v4f){m.x, m.y, m.z, m.w};
+        }
         int pr = 0;
 #pragma unroll
         for (int ls = 0; ls < NS; ls++) {
@@ -227,25 +229,13 @@ __global__ __launch_bounds__(256) void k_estepb(const float *__restrict__ X, int
                 for (int p = 0; p < P; p++) acc[p][jb] = mfma3(ah, al, bh[p], bl[p], acc[p][jb]);
             }
         }
-        float ml[NB][4];
-#pragma unroll
-        for (int jb = 0; jb < NB; jb++) {
-            const float4 m = *reinterpret_cast<const float4 *>(lml + 16 * jb + 4 * g);
-            ml[jb][0] = m.x;
-            ml[jb][1] = m.y;
-            ml[jb][2] = m.z;
-            ml[jb][3] = m.w;
-        }
 #pragma unroll
         for (int p = 0; p < P; p++) {
             float q = 0.f;
 #pragma unroll
             for (int jb = 0; jb < NB; jb++)
 #pragma unroll
-                for (int v = 0; v < 4; v++) {
-                    const float y = acc[p][jb][v] - ml[jb][v];
-                    q += y * y;
-                }
+                for (int v = 0; v < 4; v++) q += acc[p][jb][v] * acc[p][jb][v];
             q = xor_sum_gb(q);
             const float s = -0.5f * q + ck;
             mx[p] = fmaxf(mx[p], s);
@@ -514,7 +504,7 @@ __device__ inline void covb_wave(const float *__restrict__ X, const float *__res
         for (int64_t i = s0; i < s1; i += 32) {
             float sw[8];
 #pragma unroll
-            for (int e = 0; e < 8; e++) sw[e] = sqrtf(rv[e]) * inv;
+            for (int e = 0; e < 8; e++) sw[e] = __builtin_amdgcn_sqrtf(rv[e]) * inv;
             v8bf th[NB], tl[NB];
 #pragma unroll
             for (int fb = 0; fb < NB; fb++) {

@@ -50,13 +50,16 @@ __device__ inline double chunk_sum4(const double *__restrict__ p, size_t stride,
 // Lower-symmetrise, right-looking Cholesky in LDS A[d*d] (the same operation
 // sequence per element as column Cholesky-Crout), log-det, the columns of
 // L^-1 by forward substitution stored as rows of U = L^-T, then the E-step
-// operands: fragments of U in MFMA A-operand order and m = U^T mu (and the
-// FP32 copies when requested).  fail_s: one LDS int.
+// operands as one stage block per component, [fragments of U in MFMA
+// A-operand order (npairs x 64) | -U^T mu (dpad)], and, when stage32 is
+// given, the FP32 form's per-component stage block
+// [fragments (npairs x 64) | mu_hi (dpad) | -U^T mu_lo (dpad)] with
+// mu_hi = float(mu), mu_lo = mu - mu_hi.  fail_s: one LDS int.
 __device__ inline void factor_unit(int k, int d, int dpad, int NB, int npairs, const double *__restrict__ cov,
                                    const double *__restrict__ w, const double *__restrict__ mu,
                                    double *__restrict__ logdet, int *__restrict__ info, double *__restrict__ coef,
-                                   double *__restrict__ U, double *__restrict__ ufrag, float *__restrict__ ufrag32,
-                                   double *__restrict__ minit, float *__restrict__ mus32, double *A, int *fail_s) {
+                                   double *__restrict__ U, double *__restrict__ stage64, float *__restrict__ stage32,
+                                   double *A, int *fail_s) {
     const int tid = threadIdx.x;
     const double *C = cov + (size_t)k * d * d;
     for (int e = tid; e < d * d; e += blockDim.x) {
@@ -122,19 +125,22 @@ __device__ inline void factor_unit(int k, int d, int dpad, int NB, int npairs, c
         }
         int jb = lb / 4 + rem;
         double v = Uk[(size_t)(4 * lb + (lane >> 4)) * dpad + 16 * jb + (lane & 15)];
-        ufrag[((size_t)k * npairs + p) * 64 + lane] = v;
-        if (ufrag32) ufrag32[((size_t)k * npairs + p) * 64 + lane] = (float)v;
+        stage64[(size_t)k * (npairs * 64 + dpad) + p * 64 + lane] = v;
+        if (stage32) stage32[(size_t)k * (npairs * 64 + 2 * dpad) + p * 64 + lane] = (float)v;
     }
     for (int j = tid; j < dpad; j += blockDim.x) {
-        double s = 0.0;
+        double s = 0.0, slo = 0.0;
         if (j < d)
-            for (int f = 0; f <= j; f++) s += Uk[(size_t)f * dpad + j] * mu[(size_t)k * d + f];
-        minit[(size_t)k * dpad + j] = s;
-        if (mus32) {
-            double m = (j < d) ? mu[(size_t)k * d + j] : 0.0;
-            float hi = (float)m;
-            mus32[(size_t)k * 2 * dpad + j] = hi;
-            mus32[(size_t)k * 2 * dpad + dpad + j] = (float)(m - (double)hi);
+            for (int f = 0; f <= j; f++) {
+                const double m = mu[(size_t)k * d + f];
+                s += Uk[(size_t)f * dpad + j] * m;
+                slo += Uk[(size_t)f * dpad + j] * (m - (double)(float)m);
+            }
+        stage64[(size_t)k * (npairs * 64 + dpad) + npairs * 64 + j] = -s;
+        if (stage32) {
+            float *st = stage32 + (size_t)k * (npairs * 64 + 2 * dpad) + npairs * 64;
+            st[j] = (j < d) ? (float)mu[(size_t)k * d + j] : 0.0f;
+            st[dpad + j] = (float)(-slo);
         }
     }
     __syncthreads();
@@ -144,9 +150,9 @@ __device__ inline void factor_unit(int k, int d, int dpad, int NB, int npairs, c
 // Stage (a), FP64, one tile of 64P points (EM2/MatrixEM.cpp:39-87).
 // Each wave owns P groups of 16 points.  Lane (g = l/16, c = l%16) holds, for
 // point c of each group, features 4*lb + g (lb = 0..4NB-1) as FP32, widened
-// to FP64 per use: the B operand of every MFMA.  For component k the A
-// operands (fragments of U_k^T) and m_k = U_k^T mu_k are double-buffered in
-// LDS (2 x STAGE doubles): the global loads of component k+1 are issued into
+// to FP64 per use: the B operand of every MFMA.  Component k's stage block
+// (fragments of U_k^T, then -m_k = -U_k^T mu_k) is double-buffered in LDS
+// (2 x STAGE doubles): the global loads of component k+1 are issued into
 // registers before the MFMAs of component k and stored to the idle buffer after
 // them, one barrier per component.  Accumulators start at -m_k so that
 // D = U^T (x - mu).  log-densities go to the component-major Rt; the row max is
@@ -158,8 +164,8 @@ __device__ __host__ constexpr int estep64_stage() { return pairs_of(NB) * 64 + 1
 
 template <int NB, int P>
 __device__ inline void estep64_unit(int64_t tile, const float *__restrict__ X, int64_t n, int64_t npad, int K,
-                                    const double *__restrict__ ufrag, const double *__restrict__ minit,
-                                    const double *__restrict__ coef, double *__restrict__ Rt,
+                                    const double *__restrict__ stage, const double *__restrict__ coef,
+                                    double *__restrict__ Rt,
                                     double *__restrict__ ll_out, double *lds, double *wsum) {
     constexpr int DP = 16 * NB;
     constexpr int NPAIR = pairs_of(NB);
@@ -175,13 +181,10 @@ __device__ inline void estep64_unit(int64_t tile, const float *__restrict__ X, i
 #pragma unroll
         for (int lb = 0; lb < 4 * NB; lb++) xr[p][lb] = X[(base + 16 * p + c) * DP + 4 * lb + g];
 
-    auto stage_src = [&](int k, int e) -> double {
-        return (e < NPAIR * 64) ? ufrag[(size_t)k * NPAIR * 64 + e] : minit[(size_t)k * DP + (e - NPAIR * 64)];
-    };
 #pragma unroll
     for (int q = 0; q < PER; q++) {
         const int e = tid + 256 * q;
-        if (e < STAGE) lds[e] = stage_src(0, e);
+        if (e < STAGE) lds[e] = stage[e];
     }
     __syncthreads();
 
@@ -197,17 +200,17 @@ __device__ inline void estep64_unit(int64_t tile, const float *__restrict__ X, i
 #pragma unroll
         for (int q = 0; q < PER; q++) {
             const int e = tid + 256 * q;
-            pre[q] = (more && e < STAGE) ? stage_src(k + 1, e) : 0.0;
+            pre[q] = (more && e < STAGE) ? stage[(size_t)(k + 1) * STAGE + e] : 0.0;
         }
         const double ck = coef[k];
         v4d acc[P][NB];
 #pragma unroll
         for (int jb = 0; jb < NB; jb++) {
             v4d init;
-            init[0] = -lm[16 * jb + 0 + g];
-            init[1] = -lm[16 * jb + 4 + g];
-            init[2] = -lm[16 * jb + 8 + g];
-            init[3] = -lm[16 * jb + 12 + g];
+            init[0] = lm[16 * jb + 0 + g];
+            init[1] = lm[16 * jb + 4 + g];
+            init[2] = lm[16 * jb + 8 + g];
+            init[3] = lm[16 * jb + 12 + g];
 #pragma unroll
             for (int p = 0; p < P; p++) acc[p][jb] = init;
         }
@@ -334,9 +337,10 @@ __device__ inline void means_finalize_unit(int k, int ct, int d, int dpad, int K
 // ---------------------------------------------------------------------------
 // Stage (c), FP64: components 4cg..4cg+3 (one per wave) over one chunk of
 // points (EM2/MatrixEM.cpp:53-67,90-99,346-375).  Per step of 4 points each
-// lane forms T[i0+g][16fb+c] = (x - mu_k) * sqrt(r) / sqrt(N_k) for every
-// feature block fb and issues one MFMA per upper block (ib <= jb):
-// D += T_ib^T T_jb.  The loads of two steps are issued before their arithmetic.
+// lane forms t = x - mu_k and w t with w = r / N_k for every feature block fb
+// and issues one MFMA per upper block (ib <= jb): D += (w t)_ib^T t_jb, the
+// same sum as the reference's T^T T with T = sqrt(w) t, without the square
+// root.  The loads of two steps are issued before their arithmetic.
 template <int NB>
 __device__ inline void cov64_unit(int cg, int chunk, const float *__restrict__ X, const double *__restrict__ Rt,
                                   int64_t npad, int K, int KP, int d, const double *__restrict__ mu,
@@ -352,7 +356,7 @@ __device__ inline void cov64_unit(int cg, int chunk, const float *__restrict__ X
     double muv[NB];
 #pragma unroll
     for (int fb = 0; fb < NB; fb++) muv[fb] = (16 * fb + c < d) ? mu[(size_t)k * d + 16 * fb + c] : 0.0;
-    const double inv = 1.0 / sqrt(nk[k]);
+    const double invn = 1.0 / nk[k];
     v4d acc[UB];
 #pragma unroll
     for (int u = 0; u < UB; u++) acc[u] = (v4d){0.0, 0.0, 0.0, 0.0};
@@ -368,16 +372,19 @@ __device__ inline void cov64_unit(int cg, int chunk, const float *__restrict__ X
         }
 #pragma unroll
         for (int q = 0; q < 2; q++) {
-            const double sw = sqrt(rv[q]) * inv;
-            double t[NB];
+            const double wq = rv[q] * invn;
+            double t[NB], wt[NB];
 #pragma unroll
-            for (int fb = 0; fb < NB; fb++) t[fb] = ((double)xv[q][fb] - muv[fb]) * sw;
+            for (int fb = 0; fb < NB; fb++) {
+                t[fb] = (double)xv[q][fb] - muv[fb];
+                wt[fb] = t[fb] * wq;
+            }
             int u = 0;
 #pragma unroll
             for (int ib = 0; ib < NB; ib++)
 #pragma unroll
                 for (int jb = ib; jb < NB; jb++) {
-                    acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(t[ib], t[jb], acc[u], 0, 0, 0);
+                    acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(wt[ib], t[jb], acc[u], 0, 0, 0);
                     u++;
                 }
         }
