@@ -14,6 +14,7 @@
 // the centred value is accurate to FP32 rounding of the centred value rather
 // than of |mu| (means reach |mu| ~ 20 sigma).
 #include "fem_internal.h"
+#include "fem_tiles.h"
 
 #include <cmath>
 
@@ -27,11 +28,7 @@ __device__ __host__ constexpr int pairs32(int NB) {
     return p;
 }
 
-__device__ inline float xor_sum_g(float v) {
-    v += __shfl_xor(v, 16);
-    v += __shfl_xor(v, 32);
-    return v;
-}
+__device__ inline float xor_sum_g(float v) { return group_sum4(v); }
 
 // ---------------------------------------------------------------------------
 // Stage (a), FP32.  Workgroup = 4 waves x P groups of 16 points.  The
@@ -92,6 +89,7 @@ __global__ __launch_bounds__(256) void k_estep32(const float *__restrict__ X, in
             for (int p = 0; p < P; p++) acc[p][jb] = (v4f){m.x, m.y, m.z, m.w};
         }
         int pr = 0;
+        float4 a4 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int lb = 0; lb < 4 * NB; lb++) {
             const float mh = lmh[4 * lb + g];
@@ -100,7 +98,8 @@ __global__ __launch_bounds__(256) void k_estep32(const float *__restrict__ X, in
             for (int p = 0; p < P; p++) b[p] = xr[p][lb] - mh;
 #pragma unroll
             for (int jb = lb / 4; jb < NB; jb++) {
-                const float a = lu[pr * 64 + lane];
+                if ((pr & 3) == 0) a4 = *reinterpret_cast<const float4 *>(lu + (pr >> 2) * 256 + lane * 4);
+                const float a = (pr & 3) == 0 ? a4.x : (pr & 3) == 1 ? a4.y : (pr & 3) == 2 ? a4.z : a4.w;
                 pr++;
 #pragma unroll
                 for (int p = 0; p < P; p++)

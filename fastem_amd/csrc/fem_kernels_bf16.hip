@@ -14,6 +14,7 @@
 // the low half of word e/2 when e is even); B[k][j] for j = c, the same k;
 // accumulator element v of lane l holds D[4g + v][c].
 #include "fem_internal.h"
+#include "fem_tiles.h"
 
 #include <cmath>
 
@@ -66,11 +67,7 @@ __device__ inline v4f mfma3(v8bf ah, v8bf al, v8bf bh, v8bf bl, v4f acc) {
     return acc;
 }
 
-__device__ inline float xor_sum_gb(float v) {
-    v += __shfl_xor(v, 16);
-    v += __shfl_xor(v, 32);
-    return v;
-}
+__device__ inline float xor_sum_gb(float v) { return group_sum4(v); }
 
 // ---------------------------------------------------------------------------
 // E-step operands of component k, packed as one contiguous stage block of

@@ -24,12 +24,36 @@ __device__ __host__ constexpr int pairs_of(int NB) {
     return p;
 }
 
-__device__ inline double wave_xor_sum_g(double v) {
-    // sum over the four 16-lane groups (lanes l, l^16, l^32, l^48)
-    v += __shfl_xor(v, 16);
-    v += __shfl_xor(v, 32);
-    return v;
+// Sum over the four 16-lane groups (lanes l, l^16, l^32, l^48) as
+// (v_0 + v_1) + (v_2 + v_3) for groups 0..3, in every lane, with the gfx950
+// lane-swap instructions instead of an LDS permute: permlane16_swap(v, v)
+// returns (v of the even group, v of the odd group) of each 32-lane half,
+// permlane32_swap(v, v) returns (v of the lower half, v of the upper half).
+__device__ inline float group_sum4(float v) {
+    const unsigned u = __builtin_bit_cast(unsigned, v);
+    const auto a = __builtin_amdgcn_permlane16_swap(u, u, false, false);
+    const float s = __builtin_bit_cast(float, a[0]) + __builtin_bit_cast(float, a[1]);
+    const unsigned us = __builtin_bit_cast(unsigned, s);
+    const auto b = __builtin_amdgcn_permlane32_swap(us, us, false, false);
+    return __builtin_bit_cast(float, b[0]) + __builtin_bit_cast(float, b[1]);
 }
+
+__device__ inline double group_sum4(double v) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)u, hi = (unsigned)(u >> 32);
+    const auto al = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    const auto ah = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    const double s = __builtin_bit_cast(double, ((unsigned long long)ah[0] << 32) | al[0]) +
+                     __builtin_bit_cast(double, ((unsigned long long)ah[1] << 32) | al[1]);
+    const unsigned long long us = __builtin_bit_cast(unsigned long long, s);
+    const unsigned slo = (unsigned)us, shi = (unsigned)(us >> 32);
+    const auto bl = __builtin_amdgcn_permlane32_swap(slo, slo, false, false);
+    const auto bh = __builtin_amdgcn_permlane32_swap(shi, shi, false, false);
+    return __builtin_bit_cast(double, ((unsigned long long)bh[0] << 32) | bl[0]) +
+           __builtin_bit_cast(double, ((unsigned long long)bh[1] << 32) | bl[1]);
+}
+
+__device__ inline double wave_xor_sum_g(double v) { return group_sum4(v); }
 
 // Fixed-order sum of p[0], p[stride], ..., with 4 interleaved accumulators.
 __device__ inline double chunk_sum4(const double *__restrict__ p, size_t stride, int chunks) {
@@ -125,8 +149,12 @@ __device__ inline void factor_unit(int k, int d, int dpad, int NB, int npairs, c
         }
         int jb = lb / 4 + rem;
         double v = Uk[(size_t)(4 * lb + (lane >> 4)) * dpad + 16 * jb + (lane & 15)];
-        stage64[(size_t)k * (npairs * 64 + dpad) + p * 64 + lane] = v;
-        if (stage32) stage32[(size_t)k * (npairs * 64 + 2 * dpad) + p * 64 + lane] = (float)v;
+        // pairs interleaved per lane (2 per 16 bytes FP64, 4 FP32) so the
+        // E-step reads several consecutive pairs with one 16-byte LDS load;
+        // npairs = 2 NB (NB + 1) is a multiple of 4
+        stage64[(size_t)k * (npairs * 64 + dpad) + (p >> 1) * 128 + lane * 2 + (p & 1)] = v;
+        if (stage32)
+            stage32[(size_t)k * (npairs * 64 + 2 * dpad) + (p >> 2) * 256 + lane * 4 + (p & 3)] = (float)v;
     }
     for (int j = tid; j < dpad; j += blockDim.x) {
         double s = 0.0, slo = 0.0;
@@ -215,6 +243,7 @@ __device__ inline void estep64_unit(int64_t tile, const float *__restrict__ X, i
             for (int p = 0; p < P; p++) acc[p][jb] = init;
         }
         int pr = 0;
+        double2 a2 = {0.0, 0.0};
 #pragma unroll
         for (int lb = 0; lb < 4 * NB; lb++) {
             double b[P];
@@ -222,7 +251,8 @@ __device__ inline void estep64_unit(int64_t tile, const float *__restrict__ X, i
             for (int p = 0; p < P; p++) b[p] = (double)xr[p][lb];
 #pragma unroll
             for (int jb = lb / 4; jb < NB; jb++) {
-                const double a = lu[pr * 64 + lane];
+                if ((pr & 1) == 0) a2 = *reinterpret_cast<const double2 *>(lu + (pr >> 1) * 128 + lane * 2);
+                const double a = (pr & 1) ? a2.y : a2.x;
                 pr++;
 #pragma unroll
                 for (int p = 0; p < P; p++)
