@@ -25,17 +25,17 @@ __device__ __host__ constexpr int pairs_of(int NB) {
 }
 
 // Sum over the four 16-lane groups (lanes l, l^16, l^32, l^48) as
-// (v_0 + v_1) + (v_2 + v_3) for groups 0..3, in every lane, with the gfx950
-// lane-swap instructions instead of an LDS permute: permlane16_swap(v, v)
-// returns (v of the even group, v of the odd group) of each 32-lane half,
-// permlane32_swap(v, v) returns (v of the lower half, v of the upper half).
+// (v_0 + v_1) + (v_2 + v_3) for groups 0..3, in every lane.  The FP32 form
+// uses LDS permutes; the FP64 form uses the gfx950 lane-swap instructions:
+// permlane16_swap(v, v) returns (v of the even group, v of the odd group) of
+// each 32-lane half, permlane32_swap(v, v) returns (v of the lower half, v of
+// the upper half), so each step is one swap of each 32-bit half and one add.
+// (The 32-bit swap builtin is not used for FP32 sums: this compiler combined
+// its two results into one for a single-register operand.)
 __device__ inline float group_sum4(float v) {
-    const unsigned u = __builtin_bit_cast(unsigned, v);
-    const auto a = __builtin_amdgcn_permlane16_swap(u, u, false, false);
-    const float s = __builtin_bit_cast(float, a[0]) + __builtin_bit_cast(float, a[1]);
-    const unsigned us = __builtin_bit_cast(unsigned, s);
-    const auto b = __builtin_amdgcn_permlane32_swap(us, us, false, false);
-    return __builtin_bit_cast(float, b[0]) + __builtin_bit_cast(float, b[1]);
+    v += __shfl_xor(v, 16);
+    v += __shfl_xor(v, 32);
+    return v;
 }
 
 __device__ inline double group_sum4(double v) {
