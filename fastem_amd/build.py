@@ -10,7 +10,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libfastem.so")
 
-HIP_SOURCES = ["fem_kernels.hip", "fem_kernels32.hip", "fem_kernels_bf16.hip", "fem_kmeans.hip",
+HIP_SOURCES = ["fem_kernels.hip", "fem_cov64.hip", "fem_kernels32.hip", "fem_kernels_bf16.hip", "fem_kmeans.hip",
                "fem_batch.hip", "fem_setnet.hip", "fem_abi.hip"]
 HOST_SOURCES = ["fem_tools.cpp", "host/linalg.cpp", "host/rand.cpp", "host/block_manager.cpp",
                 "host/normal_density.cpp", "host/gaussian_mixture.cpp", "host/dataset.cpp",
@@ -20,7 +20,7 @@ TOOLS = {"em2": "host/em2_main.cpp", "eminit": "host/eminit_main.cpp"}
 # squared, reduced or flushed) for the FP32, BF16x3, k-means and set-network
 # kernels; the FP64 kernels measured faster with the default AGPR form.
 VGPR_FORM = ["-mllvm", "-amdgpu-mfma-vgpr-form"]
-FILE_FLAGS = {"fem_kernels32.hip": VGPR_FORM, "fem_kernels_bf16.hip": VGPR_FORM,
+FILE_FLAGS = {"fem_kernels.hip": VGPR_FORM, "fem_kernels32.hip": VGPR_FORM, "fem_kernels_bf16.hip": VGPR_FORM,
               "fem_kmeans.hip": VGPR_FORM, "fem_setnet.hip": VGPR_FORM}
 
 # Packed FP32 VALU ops are disabled for all device code.

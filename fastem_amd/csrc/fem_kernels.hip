@@ -8,8 +8,8 @@
 //                        and posteriors (EM2/MatrixEM.cpp:39-87).
 // Stage (b) k_means64  : R^T X and N_k per chunk of points on MFMA
 //                        (EM2/MatrixEM.cpp:319-344).
-// Stage (c) k_cov64    : weighted SYRK per component and chunk on MFMA
-//                        (EM2/MatrixEM.cpp:53-67,90-99,346-375).
+// Stage (c) k_cov64    : in fem_cov64.hip (built with AGPR accumulators, which
+//                        suit it better than the VGPR form this file uses).
 // Finalisers reduce the per-chunk partials in a fixed order in FP64.
 #include "fem_internal.h"
 #include "fem_tiles.h"
@@ -154,23 +154,6 @@ hipError_t launch_means(fem_ctx *c) {
     return launch_means_finalize(c, c->mu, c->nk, 0);
 }
 
-// ---------------------------------------------------------------------------
-// Workgroups are remapped so the ones that share a chunk of X run on the same
-// XCD (dispatch is round-robin over 8 XCDs).
-template <int NB>
-__global__ __launch_bounds__(256) void k_cov64(const float *__restrict__ X,
-                                               const double *__restrict__ Rt, int64_t npad,
-                                               int K, int KP, int d,
-                                               const double *__restrict__ mu,
-                                               const double *__restrict__ nk, int64_t chunk_rows,
-                                               double *__restrict__ cpart) {
-    const int KG = KP / 4;
-    const int nblk = gridDim.x;
-    const int b = blockIdx.x;
-    const int logical = (nblk % 8 == 0) ? (b % 8) * (nblk / 8) + b / 8 : b;
-    cov64_unit<NB>(logical % KG, logical / KG, X, Rt, npad, K, KP, d, mu, nk, chunk_rows, cpart);
-}
-
 __global__ __launch_bounds__(256) void k_cov_reduce(int K, int d, int NB, int KP, int chunks,
                                                     const double *__restrict__ cpart,
                                                     double *__restrict__ cov) {
@@ -182,28 +165,6 @@ __global__ __launch_bounds__(256) void k_cov_finish(int K, int d, const double *
                                                     double *__restrict__ w) {
     __shared__ double str;
     cov_finish_unit(blockIdx.x, d, nk, g, ntotal, cov, w, &str);
-}
-
-template <int NB>
-static hipError_t cov_t(fem_ctx *c) {
-    const int blocks = (c->KP / 4) * c->c_chunks;
-    hipLaunchKernelGGL(k_cov64<NB>, dim3(blocks), dim3(256), 0, c->stream, c->X, c->Rt, c->npad,
-                       c->K, c->KP, c->d, c->mu, c->nk, c->c_chunk_rows, c->cpart);
-    return hipGetLastError();
-}
-
-hipError_t launch_cov64_partials(fem_ctx *c) {
-    switch (c->NB) {
-        case 1: return cov_t<1>(c);
-        case 2: return cov_t<2>(c);
-        case 3: return cov_t<3>(c);
-        case 4: return cov_t<4>(c);
-        case 5: return cov_t<5>(c);
-        case 6: return cov_t<6>(c);
-        case 7: return cov_t<7>(c);
-        case 8: return cov_t<8>(c);
-    }
-    return hipErrorInvalidValue;
 }
 
 hipError_t launch_cov_finalize(fem_ctx *c, double g) {

@@ -44,9 +44,10 @@ __global__ __launch_bounds__(256) void k_estep32(const float *__restrict__ X, in
                                                  float *__restrict__ Rt, double *__restrict__ ll_part) {
     constexpr int DP = 16 * NB;
     constexpr int NPAIR = pairs32(NB);
-    constexpr int STAGE = NPAIR * 64 + 2 * DP;
-    constexpr int PER = (STAGE + 255) / 256;
-    extern __shared__ float lds32[];  // 2 * STAGE
+    constexpr int STAGE = NPAIR * 64 + 2 * DP;  // a multiple of 4
+    constexpr int STAGE4 = STAGE / 4;
+    constexpr int PER = (STAGE4 + 255) / 256;
+    extern __shared__ __align__(16) float lds32[];  // 2 * STAGE
     __shared__ double wsum[4];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int g = lane >> 4, c = lane & 15;
@@ -58,10 +59,14 @@ __global__ __launch_bounds__(256) void k_estep32(const float *__restrict__ X, in
 #pragma unroll
         for (int lb = 0; lb < 4 * NB; lb++) xr[p][lb] = X[(base + 16 * p + c) * DP + 4 * lb + g];
 
+    // stage blocks move in 16-byte pieces; piece indices past the block are
+    // clamped to its last piece (same value to the same address)
+    const float4 *stage4 = reinterpret_cast<const float4 *>(stage);
+    float4 *lds4 = reinterpret_cast<float4 *>(lds32);
 #pragma unroll
     for (int q = 0; q < PER; q++) {
-        const int e = tid + 256 * q;
-        if (e < STAGE) lds32[e] = stage[e];
+        const int e = min(tid + 256 * q, STAGE4 - 1);
+        lds4[e] = stage4[e];
     }
     __syncthreads();
 
@@ -73,13 +78,11 @@ __global__ __launch_bounds__(256) void k_estep32(const float *__restrict__ X, in
         const float *lu = lds32 + (k & 1) * STAGE;
         const float *lmh = lu + NPAIR * 64;
         const float *lml = lmh + DP;
-        float pre[PER];
         const bool more = k + 1 < K;
+        const float4 *next4 = stage4 + (size_t)(more ? k + 1 : k) * STAGE4;
+        float4 pre[PER];
 #pragma unroll
-        for (int q = 0; q < PER; q++) {
-            const int e = tid + 256 * q;
-            pre[q] = (more && e < STAGE) ? stage[(size_t)(k + 1) * STAGE + e] : 0.0f;
-        }
+        for (int q = 0; q < PER; q++) pre[q] = next4[min(tid + 256 * q, STAGE4 - 1)];
         const float ck = (float)coef[k];
         v4f acc[P][NB];
 #pragma unroll
@@ -88,14 +91,19 @@ __global__ __launch_bounds__(256) void k_estep32(const float *__restrict__ X, in
 #pragma unroll
             for (int p = 0; p < P; p++) acc[p][jb] = (v4f){m.x, m.y, m.z, m.w};
         }
+        float mh[4 * NB];
+#pragma unroll
+        for (int q = 0; q < NB; q++) {
+            const float4 m = *reinterpret_cast<const float4 *>(lmh + 4 * NB * g + 4 * q);
+            mh[4 * q] = m.x, mh[4 * q + 1] = m.y, mh[4 * q + 2] = m.z, mh[4 * q + 3] = m.w;
+        }
         int pr = 0;
         float4 a4 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int lb = 0; lb < 4 * NB; lb++) {
-            const float mh = lmh[4 * lb + g];
             float b[P];
 #pragma unroll
-            for (int p = 0; p < P; p++) b[p] = xr[p][lb] - mh;
+            for (int p = 0; p < P; p++) b[p] = xr[p][lb] - mh[lb];
 #pragma unroll
             for (int jb = lb / 4; jb < NB; jb++) {
                 if ((pr & 3) == 0) a4 = *reinterpret_cast<const float4 *>(lu + (pr >> 2) * 256 + lane * 4);
@@ -106,6 +114,7 @@ __global__ __launch_bounds__(256) void k_estep32(const float *__restrict__ X, in
                     acc[p][jb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b[p], acc[p][jb], 0, 0, 0);
             }
         }
+        float sv[P];
 #pragma unroll
         for (int p = 0; p < P; p++) {
             float q = 0.f;
@@ -114,17 +123,20 @@ __global__ __launch_bounds__(256) void k_estep32(const float *__restrict__ X, in
 #pragma unroll
                 for (int v = 0; v < 4; v++) q += acc[p][jb][v] * acc[p][jb][v];
             q = xor_sum_g(q);
-            const float s = -0.5f * q + ck;
-            mx[p] = fmaxf(mx[p], s);
-            if ((k & 3) == g) Rt[(size_t)k * npad + base + 16 * p + c] = s;
+            sv[p] = -0.5f * q + ck;
+            mx[p] = fmaxf(mx[p], sv[p]);
         }
-        if (more) {
-            float *dst = lds32 + ((k + 1) & 1) * STAGE;
+        {
+            // unconditional (after the last component the idle buffer is
+            // dead) and ahead of the branches below, so that the loads of the
+            // next block stay in this basic block, ahead of the MFMAs
+            float4 *dst = lds4 + ((k + 1) & 1) * STAGE4;
 #pragma unroll
-            for (int q = 0; q < PER; q++) {
-                const int e = tid + 256 * q;
-                if (e < STAGE) dst[e] = pre[q];
-            }
+            for (int q = 0; q < PER; q++) dst[min(tid + 256 * q, STAGE4 - 1)] = pre[q];
+        }
+        if ((k & 3) == g) {
+#pragma unroll
+            for (int p = 0; p < P; p++) Rt[(size_t)k * npad + base + 16 * p + c] = sv[p];
         }
         __syncthreads();
     }
@@ -133,15 +145,8 @@ __global__ __launch_bounds__(256) void k_estep32(const float *__restrict__ X, in
 #pragma unroll
     for (int p = 0; p < P; p++) {
         const int64_t pt = base + 16 * p + c;
-        float se = 0.f;
-        for (int k = g; k < K; k += 4) se += expf(Rt[(size_t)k * npad + pt] - mx[p]);
-        se = xor_sum_g(se);
-        const float lse = logf(se) + mx[p];
         const bool valid = pt < n;
-        for (int k = g; k < K; k += 4) {
-            float *r = Rt + (size_t)k * npad + pt;
-            *r = valid ? expf(*r - lse) : 0.0f;
-        }
+        const float lse = lse_posteriors(Rt, npad, pt, K, g, mx[p], valid);
         if (g == 0 && valid) llsum += (double)lse;
     }
 #pragma unroll
@@ -199,8 +204,10 @@ hipError_t launch_estep32(fem_ctx *c) {
 }
 
 // ---------------------------------------------------------------------------
-// Stage (b), FP32 products: as k_means64 (one R fragment per step, NB
-// MFMAs), FP32 accumulators flushed into FP64 every 1024 points.
+// Stage (b), FP32 products: as k_means64 (four R values per lane read at once
+// per 16 points, a lane's NB X values of a point in one load, feature NB j + jb
+// in column j of block jb), FP32 accumulators flushed into FP64 every 1024
+// points.
 template <int NB>
 __global__ __launch_bounds__(256) void k_means32(const float *__restrict__ X, const float *__restrict__ Rt,
                                                  int64_t npad, int KP, double *__restrict__ mpart) {
@@ -213,8 +220,8 @@ __global__ __launch_bounds__(256) void k_means32(const float *__restrict__ X, co
     const int KB = KP / 16;
     double *out = mpart + (size_t)chunk * KP * (DP + 1);
     for (int kb = wave; kb < KB; kb += 4) {
-        const float *rrow = Rt + (size_t)(16 * kb + c) * npad + i_begin + g;
-        const float *xrow = X + (i_begin + g) * DP + c;
+        const float *rrow = Rt + (size_t)(16 * kb + c) * npad + i_begin + 4 * g;
+        const float *xrow = X + (i_begin + 4 * g) * DP + NB * c;
         double accd[NB][4];
 #pragma unroll
         for (int jb = 0; jb < NB; jb++)
@@ -226,14 +233,19 @@ __global__ __launch_bounds__(256) void k_means32(const float *__restrict__ X, co
 #pragma unroll
             for (int jb = 0; jb < NB; jb++) acc[jb] = (v4f){0.f, 0.f, 0.f, 0.f};
             float ns = 0.f;
-#pragma unroll 4
-            for (int s = s0; s < s0 + FLUSH; s += 4) {
-                const float a = rrow[s];
-                ns += a;
+#pragma unroll 2
+            for (int s = s0; s < s0 + FLUSH; s += 16) {
+                const float4 r4 = *reinterpret_cast<const float4 *>(rrow + s);
+                const float rv[4] = {r4.x, r4.y, r4.z, r4.w};
+                ns += (rv[0] + rv[1]) + (rv[2] + rv[3]);
 #pragma unroll
-                for (int jb = 0; jb < NB; jb++)
-                    acc[jb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, xrow[(size_t)s * DP + 16 * jb], acc[jb], 0,
-                                                                 0, 0);
+                for (int m = 0; m < 4; m++) {
+                    float xv[NB];
+                    load_consecutive<NB>(xrow + (size_t)(s + m) * DP, xv);
+#pragma unroll
+                    for (int jb = 0; jb < NB; jb++)
+                        acc[jb] = __builtin_amdgcn_mfma_f32_16x16x4f32(rv[m], xv[jb], acc[jb], 0, 0, 0);
+                }
             }
 #pragma unroll
             for (int jb = 0; jb < NB; jb++)
@@ -244,7 +256,7 @@ __global__ __launch_bounds__(256) void k_means32(const float *__restrict__ X, co
 #pragma unroll
         for (int jb = 0; jb < NB; jb++)
 #pragma unroll
-            for (int v = 0; v < 4; v++) out[(size_t)(16 * kb + 4 * g + v) * (DP + 1) + 16 * jb + c] = accd[jb][v];
+            for (int v = 0; v < 4; v++) out[(size_t)(16 * kb + 4 * g + v) * (DP + 1) + NB * c + jb] = accd[jb][v];
         nsd += __shfl_xor(nsd, 16);
         nsd += __shfl_xor(nsd, 32);
         if (g == 0) out[(size_t)(16 * kb + c) * (DP + 1) + DP] = nsd;

@@ -253,15 +253,8 @@ __global__ __launch_bounds__(256) void k_estepb(const float *__restrict__ X, int
 #pragma unroll
     for (int p = 0; p < P; p++) {
         const int64_t pt = base + 16 * p + c;
-        float se = 0.f;
-        for (int k = g; k < K; k += 4) se += expf(Rt[(size_t)k * npad + pt] - mx[p]);
-        se = xor_sum_gb(se);
-        const float lse = logf(se) + mx[p];
         const bool valid = pt < n;
-        for (int k = g; k < K; k += 4) {
-            float *r = Rt + (size_t)k * npad + pt;
-            *r = valid ? expf(*r - lse) : 0.0f;
-        }
+        const float lse = lse_posteriors(Rt, npad, pt, K, g, mx[p], valid);
         if (g == 0 && valid) llsum += (double)lse;
     }
 #pragma unroll
@@ -325,8 +318,9 @@ hipError_t launch_estepb(fem_ctx *c) {
 // ---------------------------------------------------------------------------
 // Stage (b), BF16x3: M[k][j] = sum_i R[i][k] X[i][j] per 4096-point chunk.
 // A = R^T fragment (components 16 kb + c, points 8g..8g+7 of the step), B = X
-// fragment (points 8g.., features 16 jb + c); FP32 accumulators flushed into
-// FP64 every 1024 points.  Output layout as the FP32 form.
+// fragment (points 8g.., feature NB c + jb in column c of block jb, so a lane
+// reads its NB values of a point in wide loads); FP32 accumulators flushed
+// into FP64 every 1024 points.  Output layout as the FP32 form.
 template <int NB>
 __global__ __launch_bounds__(256) void k_meansb(const float *__restrict__ X, const float *__restrict__ Rt,
                                                 int64_t npad, int KP, double *__restrict__ mpart) {
@@ -340,7 +334,7 @@ __global__ __launch_bounds__(256) void k_meansb(const float *__restrict__ X, con
     double *out = mpart + (size_t)chunk * KP * (DP + 1);
     for (int kb = wave; kb < KB; kb += 4) {
         const float *rrow = Rt + (size_t)(16 * kb + c) * npad + i_begin + 8 * g;
-        const float *xrow = X + (i_begin + 8 * g) * DP + c;
+        const float *xrow = X + (i_begin + 8 * g) * DP + NB * c;
         double accd[NB][4];
 #pragma unroll
         for (int jb = 0; jb < NB; jb++)
@@ -359,11 +353,14 @@ __global__ __launch_bounds__(256) void k_meansb(const float *__restrict__ X, con
                 ns += ((rv[0] + rv[1]) + (rv[2] + rv[3])) + ((rv[4] + rv[5]) + (rv[6] + rv[7]));
                 v8bf rh, rl;
                 split8(rv, rh, rl);
+                float xp[8][NB];
+#pragma unroll
+                for (int e = 0; e < 8; e++) load_consecutive<NB>(xrow + (size_t)(s + e) * DP, xp[e]);
 #pragma unroll
                 for (int jb = 0; jb < NB; jb++) {
                     float xv[8];
 #pragma unroll
-                    for (int e = 0; e < 8; e++) xv[e] = xrow[(size_t)(s + e) * DP + 16 * jb];
+                    for (int e = 0; e < 8; e++) xv[e] = xp[e][jb];
                     v8bf xh, xl;
                     split8(xv, xh, xl);
                     acc[jb] = mfma3(rh, rl, xh, xl, acc[jb]);
@@ -378,7 +375,7 @@ __global__ __launch_bounds__(256) void k_meansb(const float *__restrict__ X, con
 #pragma unroll
         for (int jb = 0; jb < NB; jb++)
 #pragma unroll
-            for (int v = 0; v < 4; v++) out[(size_t)(16 * kb + 4 * g + v) * (DP + 1) + 16 * jb + c] = accd[jb][v];
+            for (int v = 0; v < 4; v++) out[(size_t)(16 * kb + 4 * g + v) * (DP + 1) + NB * c + jb] = accd[jb][v];
         nsd += __shfl_xor(nsd, 16);
         nsd += __shfl_xor(nsd, 32);
         if (g == 0) out[(size_t)(16 * kb + c) * (DP + 1) + DP] = nsd;

@@ -78,7 +78,9 @@ __device__ inline double chunk_sum4(const double *__restrict__ p, size_t stride,
 // A-operand order (npairs x 64) | -U^T mu (dpad)], and, when stage32 is
 // given, the FP32 form's per-component stage block
 // [fragments (npairs x 64) | mu_hi (dpad) | -U^T mu_lo (dpad)] with
-// mu_hi = float(mu), mu_lo = mu - mu_hi.  fail_s: one LDS int.
+// mu_hi = float(mu), mu_lo = mu - mu_hi; mu_hi is stored lane-group major
+// (feature 4 lb + g at g dpad/4 + lb) so that a lane reads its features in
+// 16-byte loads.  fail_s: one LDS int.
 __device__ inline void factor_unit(int k, int d, int dpad, int NB, int npairs, const double *__restrict__ cov,
                                    const double *__restrict__ w, const double *__restrict__ mu,
                                    double *__restrict__ logdet, int *__restrict__ info, double *__restrict__ coef,
@@ -167,11 +169,61 @@ __device__ inline void factor_unit(int k, int d, int dpad, int NB, int npairs, c
         stage64[(size_t)k * (npairs * 64 + dpad) + npairs * 64 + j] = -s;
         if (stage32) {
             float *st = stage32 + (size_t)k * (npairs * 64 + 2 * dpad) + npairs * 64;
-            st[j] = (j < d) ? (float)mu[(size_t)k * d + j] : 0.0f;
+            st[(j & 3) * (dpad >> 2) + (j >> 2)] = (j < d) ? (float)mu[(size_t)k * d + j] : 0.0f;
             st[dpad + j] = (float)(-slo);
         }
     }
     __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Log-sum-exp and posteriors of one point (EM2/MatrixEM.cpp:39-51) whose K
+// log-densities sit in the component-major Rt; the lane of group g owns
+// components g, g+4, ... (the four groups of a 16-lane column share the
+// point).  mx: the row max.  With K <= 64 the lane's 16 entries are read in
+// one batch of independent loads, e_k = exp(s_k - mx) is formed once and the
+// posteriors are e_k / sum; larger K streams the entries twice,
+// r_k = exp(s_k - lse).  Invalid (padding) points get zero posteriors.
+// Returns lse = log(sum) + mx for every lane of the column.
+__device__ inline float exp_of(float v) { return expf(v); }
+__device__ inline double exp_of(double v) { return exp(v); }
+__device__ inline float log_of(float v) { return logf(v); }
+__device__ inline double log_of(double v) { return log(v); }
+
+template <typename T>
+__device__ __attribute__((always_inline)) inline T lse_posteriors(T *__restrict__ Rt, int64_t npad, int64_t pt, int K, int g, T mx, bool valid) {
+    constexpr int KQ = 16;
+    if (K <= 4 * KQ) {
+        T e[KQ];
+#pragma unroll
+        for (int i = 0; i < KQ; i++) {
+            const int k = g + 4 * i;
+            e[i] = k < K ? Rt[(size_t)k * npad + pt] : T(0);
+        }
+        T se = T(0);
+#pragma unroll
+        for (int i = 0; i < KQ; i++) {
+            e[i] = g + 4 * i < K ? exp_of(e[i] - mx) : T(0);
+            se += e[i];
+        }
+        se = group_sum4(se);
+        const T inv = T(1) / se;
+#pragma unroll
+        for (int i = 0; i < KQ; i++) {
+            const int k = g + 4 * i;
+            if (k < K) Rt[(size_t)k * npad + pt] = valid ? e[i] * inv : T(0);
+        }
+        return log_of(se) + mx;
+    }
+    T se = T(0);
+    for (int k = g; k < K; k += 4) se += exp_of(Rt[(size_t)k * npad + pt] - mx);
+    se = group_sum4(se);
+    const T lse = log_of(se) + mx;
+    for (int k = g; k < K; k += 4) {
+        T *r = Rt + (size_t)k * npad + pt;
+        *r = valid ? exp_of(*r - lse) : T(0);
+    }
+    return lse;
 }
 
 // ---------------------------------------------------------------------------
@@ -197,8 +249,9 @@ __device__ inline void estep64_unit(int64_t tile, const float *__restrict__ X, i
                                     double *__restrict__ ll_out, double *lds, double *wsum) {
     constexpr int DP = 16 * NB;
     constexpr int NPAIR = pairs_of(NB);
-    constexpr int STAGE = NPAIR * 64 + DP;
-    constexpr int PER = (STAGE + 255) / 256;
+    constexpr int STAGE = NPAIR * 64 + DP;  // even
+    constexpr int STAGE2 = STAGE / 2;
+    constexpr int PER = (STAGE2 + 255) / 256;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int g = lane >> 4, c = lane & 15;
     const int64_t base = tile * (64 * P) + wave * (16 * P);
@@ -209,10 +262,14 @@ __device__ inline void estep64_unit(int64_t tile, const float *__restrict__ X, i
 #pragma unroll
         for (int lb = 0; lb < 4 * NB; lb++) xr[p][lb] = X[(base + 16 * p + c) * DP + 4 * lb + g];
 
+    // stage blocks move in 16-byte pieces; piece indices past the block are
+    // clamped to its last piece (same value to the same address)
+    const double2 *stage2 = reinterpret_cast<const double2 *>(stage);
+    double2 *lds2 = reinterpret_cast<double2 *>(lds);
 #pragma unroll
     for (int q = 0; q < PER; q++) {
-        const int e = tid + 256 * q;
-        if (e < STAGE) lds[e] = stage[e];
+        const int e = min(tid + 256 * q, STAGE2 - 1);
+        lds2[e] = stage2[e];
     }
     __syncthreads();
 
@@ -223,13 +280,10 @@ __device__ inline void estep64_unit(int64_t tile, const float *__restrict__ X, i
     for (int k = 0; k < K; k++) {
         const double *lu = lds + (k & 1) * STAGE;
         const double *lm = lu + NPAIR * 64;
-        const bool more = k + 1 < K;
-        double pre[PER];
+        const double2 *next2 = stage2 + (size_t)(k + 1 < K ? k + 1 : k) * STAGE2;
+        double2 pre[PER];
 #pragma unroll
-        for (int q = 0; q < PER; q++) {
-            const int e = tid + 256 * q;
-            pre[q] = (more && e < STAGE) ? stage[(size_t)(k + 1) * STAGE + e] : 0.0;
-        }
+        for (int q = 0; q < PER; q++) pre[q] = next2[min(tid + 256 * q, STAGE2 - 1)];
         const double ck = coef[k];
         v4d acc[P][NB];
 #pragma unroll
@@ -259,6 +313,7 @@ __device__ inline void estep64_unit(int64_t tile, const float *__restrict__ X, i
                     acc[p][jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[p], acc[p][jb], 0, 0, 0);
             }
         }
+        double sv[P];
 #pragma unroll
         for (int p = 0; p < P; p++) {
             double q = 0.0;
@@ -267,17 +322,20 @@ __device__ inline void estep64_unit(int64_t tile, const float *__restrict__ X, i
 #pragma unroll
                 for (int v = 0; v < 4; v++) q += acc[p][jb][v] * acc[p][jb][v];
             q = wave_xor_sum_g(q);
-            const double s = -0.5 * q + ck;
-            mx[p] = fmax(mx[p], s);
-            if ((k & 3) == g) Rt[(size_t)k * npad + base + 16 * p + c] = s;
+            sv[p] = -0.5 * q + ck;
+            mx[p] = fmax(mx[p], sv[p]);
         }
-        if (more) {
-            double *dst = lds + ((k + 1) & 1) * STAGE;
+        {
+            // unconditional (after the last component the idle buffer is
+            // dead) and ahead of the branch below, so that the loads of the
+            // next block stay in this basic block, ahead of the MFMAs
+            double2 *dst = lds2 + ((k + 1) & 1) * STAGE2;
 #pragma unroll
-            for (int q = 0; q < PER; q++) {
-                const int e = tid + 256 * q;
-                if (e < STAGE) dst[e] = pre[q];
-            }
+            for (int q = 0; q < PER; q++) dst[min(tid + 256 * q, STAGE2 - 1)] = pre[q];
+        }
+        if ((k & 3) == g) {
+#pragma unroll
+            for (int p = 0; p < P; p++) Rt[(size_t)k * npad + base + 16 * p + c] = sv[p];
         }
         __syncthreads();
     }
@@ -286,15 +344,8 @@ __device__ inline void estep64_unit(int64_t tile, const float *__restrict__ X, i
 #pragma unroll
     for (int p = 0; p < P; p++) {
         const int64_t pt = base + 16 * p + c;
-        double se = 0.0;
-        for (int k = g; k < K; k += 4) se += exp(Rt[(size_t)k * npad + pt] - mx[p]);
-        se = wave_xor_sum_g(se);
-        const double lse = log(se) + mx[p];
         const bool valid = pt < n;
-        for (int k = g; k < K; k += 4) {
-            double *r = Rt + (size_t)k * npad + pt;
-            *r = valid ? exp(*r - lse) : 0.0;
-        }
+        const double lse = lse_posteriors(Rt, npad, pt, K, g, mx[p], valid);
         if (g == 0 && valid) llsum += lse;
     }
 #pragma unroll
@@ -305,12 +356,36 @@ __device__ inline void estep64_unit(int64_t tile, const float *__restrict__ X, i
     __syncthreads();
 }
 
+// NB consecutive floats p[0..NB-1] in loads of W floats, W the largest power
+// of two (at most 4) that divides NB; p must be aligned to W floats.
+template <int NB>
+__device__ inline void load_consecutive(const float *__restrict__ p, float (&v)[NB]) {
+    if constexpr (NB % 4 == 0) {
+#pragma unroll
+        for (int j = 0; j < NB; j += 4) {
+            const float4 q = *reinterpret_cast<const float4 *>(p + j);
+            v[j] = q.x, v[j + 1] = q.y, v[j + 2] = q.z, v[j + 3] = q.w;
+        }
+    } else if constexpr (NB % 2 == 0) {
+#pragma unroll
+        for (int j = 0; j < NB; j += 2) {
+            const float2 q = *reinterpret_cast<const float2 *>(p + j);
+            v[j] = q.x, v[j + 1] = q.y;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < NB; j++) v[j] = p[j];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Stage (b), FP64, one chunk of 4096 points (EM2/MatrixEM.cpp:319-344).  Wave
-// w owns component blocks kb = w, w+4, ...; per step of 4 points it loads the
-// R fragment once (A[i][k] = R[16kb+i][i0+k]) and one X fragment per feature
-// block (B[k][j] = X[i0+k][16jb+j]), issuing NB MFMAs; the A values also sum to
-// N_k.  Partials: out[KP][DP+1] of this chunk.
+// w owns component blocks kb = w, w+4, ...; per 16 points each lane reads its
+// four R values at once (A[i][k] = R[16kb+i][pt] with pt = 4k + m at MFMA step
+// m).  Column j of feature block jb is feature NB j + jb, so lane (j, k) reads
+// its NB X values of a point in one load (B[k][j] = X[pt][NB j + jb]); 4 NB
+// MFMAs per 16 points, and the A values also sum to N_k.  Partials:
+// out[KP][DP+1] of this chunk, in feature order.
 constexpr int kMeansChunk = 4096;
 
 template <int NB>
@@ -322,26 +397,32 @@ __device__ inline void means64_unit(int chunk, const float *__restrict__ X, cons
     const int64_t i_begin = (int64_t)chunk * kMeansChunk;
     const int KB = KP / 16;
     for (int kb = wave; kb < KB; kb += 4) {
-        const double *rrow = Rt + (size_t)(16 * kb + c) * npad + i_begin + g;
-        const float *xrow = X + (i_begin + g) * DP + c;
+        // per 16 points, lane group g owns points 4g..4g+3: its R values are
+        // one 32-byte read, and MFMA step m takes point 4g + m of each group
+        const double *rrow = Rt + (size_t)(16 * kb + c) * npad + i_begin + 4 * g;
+        const float *xrow = X + (i_begin + 4 * g) * DP + NB * c;
         v4d acc[NB];
 #pragma unroll
         for (int jb = 0; jb < NB; jb++) acc[jb] = (v4d){0.0, 0.0, 0.0, 0.0};
         double ns = 0.0;
-#pragma unroll 4
-        for (int s = 0; s < kMeansChunk; s += 4) {
-            const double a = rrow[s];
-            ns += a;
+#pragma unroll 2
+        for (int s = 0; s < kMeansChunk; s += 16) {
+            const double4 r4 = *reinterpret_cast<const double4 *>(rrow + s);
+            const double rv[4] = {r4.x, r4.y, r4.z, r4.w};
+            ns += (rv[0] + rv[1]) + (rv[2] + rv[3]);
 #pragma unroll
-            for (int jb = 0; jb < NB; jb++) {
-                const double bv = (double)xrow[(size_t)s * DP + 16 * jb];
-                acc[jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc[jb], 0, 0, 0);
+            for (int m = 0; m < 4; m++) {
+                float xv[NB];
+                load_consecutive<NB>(xrow + (size_t)(s + m) * DP, xv);
+#pragma unroll
+                for (int jb = 0; jb < NB; jb++)
+                    acc[jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(rv[m], (double)xv[jb], acc[jb], 0, 0, 0);
             }
         }
 #pragma unroll
         for (int jb = 0; jb < NB; jb++)
 #pragma unroll
-            for (int v = 0; v < 4; v++) out[(size_t)(16 * kb + 4 * v + g) * (DP + 1) + 16 * jb + c] = acc[jb][v];
+            for (int v = 0; v < 4; v++) out[(size_t)(16 * kb + 4 * v + g) * (DP + 1) + NB * c + jb] = acc[jb][v];
         ns = wave_xor_sum_g(ns);
         if (g == 0) out[(size_t)(16 * kb + c) * (DP + 1) + DP] = ns;
     }

@@ -25,12 +25,15 @@ def kmeanspp(ctx, seed, sweeps=29):
 
 
 def run_plain(ctx, seed, tol=1e-6, cap=500, conv=1e-12, ll_star=None, init=None):
-    """FP64 EM; returns dict(trace=[(t, LL)], ll_final, t_reach, iters)."""
+    """FP64 EM; returns dict(trace=[(t, LL)], ll_final, t_reach, iters).
+    init as for run_schedule."""
     ctx.set_precision(fem.PREC_FP64)
     trace = []
     t0 = time.perf_counter()
     if init is None:
         kmeanspp(ctx, seed)
+    elif callable(init):
+        init(ctx)
     else:
         ctx.set_params(*init)
     t_init = time.perf_counter() - t0
