@@ -152,6 +152,72 @@ def batch_workload(nfits=256, seed=7, maxsweeps=100, eps=1e-6, g=1e-3, compare=F
     return out
 
 
+def precision_variant(iters=8, seed=2025, log=print):
+    """The precision workload: N = 4,194,304 FP32 points, d = 128, K = 128, the
+    benchmark's generator (means U[-20, 20], covariances A A^T/d + 0.05 I).
+    One device k-means++ init, then `iters` EM iterations in each form from
+    it on the same context; per iteration the FP32 and BF16x3 forms' LL
+    (E-step) and weights, means and covariances (M-step) against the FP64
+    form's, as max-norm relative errors, and each form's GPU time per
+    iteration (HIP events of the four phases).  The FP64 form matches the
+    FP64 oracle to 1e-10 in the parity tests, so it stands in for the oracle
+    at this size."""
+    import fastem_amd as fem
+    n, d, k = 4194304, 128, 128
+    X = fem.gengmm(n, d, k, -20.0, 20.0, 0.05, seed)[0]
+    out = {"n": n, "d": d, "k": k, "iters": iters, "reference": "fp64 form"}
+    with fem.Context(n, d, k, fem.PREC_FP64, 0) as ctx:
+        ctx.upload(X)
+        del X
+        try:
+            init = ctx.kmeans_init(seed=1, seeding=1, max_sweeps=29)
+        except fem.FemError:
+            init = ctx.get_params()   # the mixture written before the error
+        traj = {}
+        for name, prec in (("fp64", fem.PREC_FP64), ("fp32", fem.PREC_FP32), ("bf16x3", fem.PREC_BF16X3)):
+            log(f"precision workload: {name}")
+            ctx.set_precision(prec)
+            ctx.set_params(*init)
+            ctx.factor()
+            lls, params, ms, ems = [], [], [], []
+            for _ in range(iters):
+                lls.append(ctx.estep())
+                t = ctx.times()
+                params.append(ctx.mstep(copy=True))
+                t2 = ctx.times()
+                ctx.factor()
+                t3 = ctx.times()
+                ms.append(t["estep_ms"] + t2["means_ms"] + t2["cov_ms"] + t3["factor_ms"])
+                ems.append(t["estep_ms"])
+            traj[name] = (lls, params)
+            out[f"{name}_gpu_ms_per_iter"] = float(np.median(ms))
+            out[f"{name}_estep_tflops"] = algorithmic_flops(n, d, k)["estep"] / (np.median(ems) * 1e-3) / 1e12
+        ref_ll, ref_p = traj["fp64"]
+
+        def rel(a, b):
+            return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / np.max(np.abs(np.asarray(b))))
+
+        def rel_per_component(a, b):
+            # worst component of max|a_k - b_k| / max|b_k|
+            a, b = np.asarray(a), np.asarray(b)
+            axes = tuple(range(1, a.ndim))
+            return float(np.max(np.max(np.abs(a - b), axis=axes) / np.max(np.abs(b), axis=axes)))
+
+        for name in ("fp32", "bf16x3"):
+            lls, params = traj[name]
+            out[f"{name}_rel_err_per_iter"] = {
+                "ll": [abs(a - b) / abs(b) for a, b in zip(lls, ref_ll)],
+                "w": [rel(p[0], q[0]) for p, q in zip(params, ref_p)],
+                "mu": [rel(p[1], q[1]) for p, q in zip(params, ref_p)],
+                "cov": [rel(p[2], q[2]) for p, q in zip(params, ref_p)],
+                "mu_worst_component": [rel_per_component(p[1], q[1]) for p, q in zip(params, ref_p)],
+                "cov_worst_component": [rel_per_component(p[2], q[2]) for p, q in zip(params, ref_p)]}
+        out["fp64_cov_max_abs_per_iter"] = [float(np.max(np.abs(q[2]))) for q in ref_p]
+        out["fp64_cov_change_per_iter"] = [rel(b[2], a[2]) for a, b in zip(ref_p[:-1], ref_p[1:])]
+        out["fp64_ll_per_iter"] = ref_ll
+    return out
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
     summary (FETCH_SIZE + WRITE_SIZE, calibrated against a 1 GiB copy); None
@@ -197,6 +263,8 @@ def main():
     ap.add_argument("--batch-fits", type=int, default=256)
     ap.add_argument("--batch-compare", action="store_true",
                     help="also time the batched fits one at a time")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the d=128, K=128 precision workload")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -365,6 +433,9 @@ def main():
             "learned_own_target_def": "ll_star of converged FP64 EM from the learned init itself "
                                       "(|dLL/LL| < 1e-12 or the sweep cap); the learned runs timed "
                                       "to within tol of it"}
+    if not args.no_precision:
+        log("precision workload, N=4194304 d=128 K=128")
+        out["precision_d128_k128"] = precision_variant(log=log)
     if not args.no_batch:
         log("batched fits")
         out["batch"] = batch_workload(args.batch_fits, compare=args.batch_compare, log=log)

@@ -17,11 +17,11 @@ HOST_SOURCES = ["fem_tools.cpp", "host/linalg.cpp", "host/rand.cpp", "host/block
                 "host/em_algorithm.cpp", "host/em_initializer.cpp"]
 TOOLS = {"em2": "host/em2_main.cpp", "eminit": "host/eminit_main.cpp"}
 # MFMA accumulators in VGPRs (no accumulator-read moves when results are
-# squared, reduced or flushed) for the FP32, BF16x3, k-means and set-network
-# kernels; the FP64 kernels measured faster with the default AGPR form.
+# squared, reduced or flushed) for every file but fem_cov64.hip, whose FP64
+# covariance kernel measured faster with the default AGPR form.
 VGPR_FORM = ["-mllvm", "-amdgpu-mfma-vgpr-form"]
 FILE_FLAGS = {"fem_kernels.hip": VGPR_FORM, "fem_kernels32.hip": VGPR_FORM, "fem_kernels_bf16.hip": VGPR_FORM,
-              "fem_kmeans.hip": VGPR_FORM, "fem_setnet.hip": VGPR_FORM}
+              "fem_kmeans.hip": VGPR_FORM, "fem_batch.hip": VGPR_FORM, "fem_setnet.hip": VGPR_FORM}
 
 # Packed FP32 VALU ops are disabled for all device code.
 HIP_FLAGS = [

@@ -290,9 +290,13 @@ hipError_t launch_means32_partials(fem_ctx *c) {
 // every 2048 points (into registers for d <= 64, else into the wave's own
 // partial slice in global memory).  Centring with mu_hi = float(mu) alone
 // changes D by N_k mu_lo mu_lo^T plus a term proportional to
-// sum_i r_ik (x_i - mu_k), both below 1e-9 of D.  The loads of the next 16
-// points are issued once the current 16 are formed, before their MFMAs.  Same
-// grid and XCD remapping as k_cov64.
+// sum_i r_ik (x_i - mu_k), both below 1e-9 of D.  Strided feature blocks as
+// k_cov64 (a lane's NB features of a point in one load); lane group g takes
+// points 4g..4g+3 of each 16, so its four R values are one 16-byte load.  For
+// d <= 64 two register buffers of 16 points: the loads for 32 points ahead are
+// issued once the current 16 are formed, before their MFMAs (one buffer and
+// 16 points ahead for larger d).  Same grid and XCD remapping
+// as k_cov64.
 template <int NB>
 __global__ __launch_bounds__(256) void k_cov32(const float *__restrict__ X, const float *__restrict__ Rt,
                                                int64_t npad, int K, int KP, int d,
@@ -316,10 +320,11 @@ __global__ __launch_bounds__(256) void k_cov32(const float *__restrict__ X, cons
     const int64_t i1 = (i0 + chunk_rows < npad) ? i0 + chunk_rows : npad;
     float mh[NB];
 #pragma unroll
-    for (int fb = 0; fb < NB; fb++) mh[fb] = (16 * fb + c < d) ? (float)mu[(size_t)k * d + 16 * fb + c] : 0.0f;
+    for (int fb = 0; fb < NB; fb++) mh[fb] = (NB * c + fb < d) ? (float)mu[(size_t)k * d + NB * c + fb] : 0.0f;
     const float inv = (float)(1.0 / sqrt(nk[k]));
     double *out = cpart + ((size_t)chunk * KP + k) * UB * 256;
-    const float *rrow = Rt + (size_t)k * npad;
+    const float *rrow = Rt + (size_t)k * npad + 4 * g;
+    const float *xrow = X + 4 * g * DP + NB * c;
     double accd[REG ? UB : 1][4];
     if (REG) {
 #pragma unroll
@@ -327,41 +332,49 @@ __global__ __launch_bounds__(256) void k_cov32(const float *__restrict__ X, cons
 #pragma unroll
             for (int v = 0; v < 4; v++) accd[u][v] = 0.0;
     }
-    float rv[4], xv[4][NB];
-    auto load16 = [&](int64_t i) {
+    // H register buffers of 16 points each (two for d <= 64): the loads run H
+    // steps ahead
+    constexpr int H = NB <= 4 ? 2 : 1;
+    float rv[H][4], xv[H][4][NB];
+    auto load16 = [&](int h, int64_t i) {
+        const float4 r4 = *reinterpret_cast<const float4 *>(rrow + i);
+        rv[h][0] = r4.x, rv[h][1] = r4.y, rv[h][2] = r4.z, rv[h][3] = r4.w;
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            rv[q] = rrow[i + 4 * q + g];
-#pragma unroll
-            for (int fb = 0; fb < NB; fb++) xv[q][fb] = X[(i + 4 * q + g) * DP + 16 * fb + c];
-        }
+        for (int q = 0; q < 4; q++) load_consecutive<NB>(xrow + (size_t)(i + q) * DP, xv[h][q]);
     };
-    if (i0 < i1) load16(i0);
+    // chunk bounds are multiples of 4096 points, so every step has a partner
+    if (i0 < i1) {
+#pragma unroll
+        for (int h = 0; h < H; h++) load16(h, i0 + 16 * h);
+    }
     bool first = true;
     for (int64_t s0 = i0; s0 < i1; s0 += FLUSH) {
         const int64_t s1 = (s0 + FLUSH < i1) ? s0 + FLUSH : i1;
         v4f acc[UB];
 #pragma unroll
         for (int u = 0; u < UB; u++) acc[u] = (v4f){0.f, 0.f, 0.f, 0.f};
-        for (int64_t i = s0; i < s1; i += 16) {
-            float t[4][NB];
+        for (int64_t i = s0; i < s1; i += 16 * H) {
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const float sw = __builtin_amdgcn_sqrtf(rv[q]) * inv;
+            for (int h = 0; h < H; h++) {
+                float t[4][NB];
 #pragma unroll
-                for (int fb = 0; fb < NB; fb++) t[q][fb] = (xv[q][fb] - mh[fb]) * sw;
-            }
-            if (i + 16 < i1) load16(i + 16);
+                for (int q = 0; q < 4; q++) {
+                    const float sw = __builtin_amdgcn_sqrtf(rv[h][q]) * inv;
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                int u = 0;
+                    for (int fb = 0; fb < NB; fb++) t[q][fb] = (xv[h][q][fb] - mh[fb]) * sw;
+                }
+                if (i + 16 * (H + h) < i1) load16(h, i + 16 * (H + h));
 #pragma unroll
-                for (int ib = 0; ib < NB; ib++)
+                for (int q = 0; q < 4; q++) {
+                    int u = 0;
 #pragma unroll
-                    for (int jb = ib; jb < NB; jb++) {
-                        acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(t[q][ib], t[q][jb], acc[u], 0, 0, 0);
-                        u++;
-                    }
+                    for (int ib = 0; ib < NB; ib++)
+#pragma unroll
+                        for (int jb = ib; jb < NB; jb++) {
+                            acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(t[q][ib], t[q][jb], acc[u], 0, 0, 0);
+                            u++;
+                        }
+                }
             }
         }
         if (REG) {

@@ -415,9 +415,11 @@ hipError_t launch_meansb_partials(fem_ctx *c) {
 // a wave's accumulators stay in registers; a part forms T only for the
 // feature blocks its upper blocks touch.  FP32 accumulators are flushed into
 // FP64 every 2048 points: into registers for d <= 48, else into the wave's
-// own slice of LDS (written to the partial slice once at the end).  Grid and XCD remapping as the FP32 form, with the part
-// as the fastest-varying index; the loads of the next 32 points are issued
-// once the current 32 are split, before their MFMAs.
+// own slice of LDS (written to the partial slice once at the end).  Grid and
+// XCD remapping as the FP32 form, with the part as the fastest-varying index;
+// strided feature blocks as the FP32 form (a lane's NB features of a point in
+// wide loads); the loads of the next 32 points are issued once the current 32
+// are split, before their MFMAs.
 __device__ __host__ constexpr int cov_parts(int NB) { return NB > 4 ? 2 : 1; }
 
 // Part of upper block u (row-major over ib <= jb) among NPART parts.
@@ -475,48 +477,55 @@ __device__ inline void covb_wave(const float *__restrict__ X, const float *__res
         // the wave's FP64 accumulators live in its own LDS slice [NU][4][64]
         for (int e = lane; e < NU * 256; e += 64) lacc[e] = 0.0;
     }
-    float rv[8], xv[NB][8];
-    auto load32 = [&](int64_t i) {
+    // H register buffers of 32 points used in turn (two for d <= 64, so the
+    // prefetched values need no loop-carried copies)
+    constexpr int H = NB <= 4 ? 2 : 1;
+    float rv[H][8], xv[H][NB][8];
+    auto load32 = [&](int h, int64_t i) {
         const float *rp = rrow + i + 8 * g;
         const float4 r0 = *reinterpret_cast<const float4 *>(rp);
         const float4 r1 = *reinterpret_cast<const float4 *>(rp + 4);
-        rv[0] = r0.x; rv[1] = r0.y; rv[2] = r0.z; rv[3] = r0.w;
-        rv[4] = r1.x; rv[5] = r1.y; rv[6] = r1.z; rv[7] = r1.w;
-        const float *xp = X + (i + 8 * g) * DP + c;
+        rv[h][0] = r0.x; rv[h][1] = r0.y; rv[h][2] = r0.z; rv[h][3] = r0.w;
+        rv[h][4] = r1.x; rv[h][5] = r1.y; rv[h][6] = r1.z; rv[h][7] = r1.w;
+        const float *xp = X + (i + 8 * g) * DP + NB * c;
 #pragma unroll
         for (int fb = 0; fb < NB; fb++)
             if (cov_needs_fb(NB, PART, fb))
 #pragma unroll
-                for (int e = 0; e < 8; e++) xv[fb][e] = xp[e * DP + 16 * fb];
+                for (int e = 0; e < 8; e++) xv[h][fb][e] = xp[e * DP + fb];
     };
-    if (i0 < i1) load32(i0);
+    // chunk bounds are multiples of 4096 points: every step has its partners
+    if (i0 < i1) load32(0, i0);
     for (int64_t s0 = i0; s0 < i1; s0 += FLUSH) {
         const int64_t s1 = (s0 + FLUSH < i1) ? s0 + FLUSH : i1;
         v4f acc[NU];
 #pragma unroll
         for (int u = 0; u < NU; u++) acc[u] = (v4f){0.f, 0.f, 0.f, 0.f};
-        for (int64_t i = s0; i < s1; i += 32) {
-            float sw[8];
+        for (int64_t i = s0; i < s1; i += 32 * H) {
 #pragma unroll
-            for (int e = 0; e < 8; e++) sw[e] = __builtin_amdgcn_sqrtf(rv[e]) * inv;
-            v8bf th[NB], tl[NB];
+            for (int h = 0; h < H; h++) {
+                float sw[8];
 #pragma unroll
-            for (int fb = 0; fb < NB; fb++) {
-                if (!cov_needs_fb(NB, PART, fb)) continue;
-                float t[8];
+                for (int e = 0; e < 8; e++) sw[e] = __builtin_amdgcn_sqrtf(rv[h][e]) * inv;
+                v8bf th[NB], tl[NB];
 #pragma unroll
-                for (int e = 0; e < 8; e++) t[e] = (xv[fb][e] - mh[fb]) * sw[e];
-                split8(t, th[fb], tl[fb]);
-            }
-            if (i + 32 < i1) load32(i + 32);
-            int u = 0;
+                for (int fb = 0; fb < NB; fb++) {
+                    if (!cov_needs_fb(NB, PART, fb)) continue;
+                    float t[8];
 #pragma unroll
-            for (int ib = 0; ib < NB; ib++)
-#pragma unroll
-                for (int jb = ib; jb < NB; jb++, u++) {
-                    if (cov_part_of(u, UB, NPART) != PART) continue;
-                    acc[u - U0] = mfma3(th[ib], tl[ib], th[jb], tl[jb], acc[u - U0]);
+                    for (int e = 0; e < 8; e++) t[e] = (xv[h][fb][e] - mh[fb]) * sw[e];
+                    split8(t, th[fb], tl[fb]);
                 }
+                if (i + 32 * (h + 1) < i1) load32((h + 1) % H, i + 32 * (h + 1));
+                int u = 0;
+#pragma unroll
+                for (int ib = 0; ib < NB; ib++)
+#pragma unroll
+                    for (int jb = ib; jb < NB; jb++, u++) {
+                        if (cov_part_of(u, UB, NPART) != PART) continue;
+                        acc[u - U0] = mfma3(th[ib], tl[ib], th[jb], tl[jb], acc[u - U0]);
+                    }
+            }
         }
         if (REG) {
 #pragma unroll
@@ -538,10 +547,9 @@ __device__ inline void covb_wave(const float *__restrict__ X, const float *__res
 }
 
 template <int NB>
-__global__ __launch_bounds__(256) void k_covb(const float *__restrict__ X, const float *__restrict__ Rt,
-                                              int64_t npad, int K, int KP, int d, const double *__restrict__ mu,
-                                              const double *__restrict__ nk, int64_t chunk_rows,
-                                              double *__restrict__ cpart) {
+__device__ __attribute__((always_inline)) inline void covb_body(const float *__restrict__ X, const float *__restrict__ Rt, int64_t npad, int K,
+                                 int KP, int d, const double *__restrict__ mu, const double *__restrict__ nk,
+                                 int64_t chunk_rows, double *__restrict__ cpart) {
     constexpr int UB = NB * (NB + 1) / 2;
     constexpr int NPART = cov_parts(NB);
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -558,7 +566,7 @@ __global__ __launch_bounds__(256) void k_covb(const float *__restrict__ X, const
     const int64_t i1 = (i0 + chunk_rows < npad) ? i0 + chunk_rows : npad;
     float mh[NB];
 #pragma unroll
-    for (int fb = 0; fb < NB; fb++) mh[fb] = (16 * fb + c < d) ? (float)mu[(size_t)k * d + 16 * fb + c] : 0.0f;
+    for (int fb = 0; fb < NB; fb++) mh[fb] = (NB * c + fb < d) ? (float)mu[(size_t)k * d + NB * c + fb] : 0.0f;
     const float inv = (float)(1.0 / sqrt(nk[k]));
     double *out = cpart + ((size_t)chunk * KP + k) * UB * 256;
     const float *rrow = Rt + (size_t)k * npad;
@@ -568,17 +576,34 @@ __global__ __launch_bounds__(256) void k_covb(const float *__restrict__ X, const
     else if (NPART > 1) covb_wave<NB, (NPART > 1 ? 1 : 0)>(X, rrow, i0, i1, mh, inv, g, c, out, lacc);
 }
 
+// d <= 64: registers capped for two waves per SIMD (two 32-point buffers fit);
+// wider d keeps the compiler's allocation (one wave per SIMD).
+template <int NB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_covb(
+    const float *__restrict__ X, const float *__restrict__ Rt, int64_t npad, int K, int KP, int d,
+    const double *__restrict__ mu, const double *__restrict__ nk, int64_t chunk_rows, double *__restrict__ cpart) {
+    covb_body<NB>(X, Rt, npad, K, KP, d, mu, nk, chunk_rows, cpart);
+}
+
+template <int NB>
+__global__ __launch_bounds__(256) void k_covb_wide(const float *__restrict__ X, const float *__restrict__ Rt,
+                                                   int64_t npad, int K, int KP, int d,
+                                                   const double *__restrict__ mu, const double *__restrict__ nk,
+                                                   int64_t chunk_rows, double *__restrict__ cpart) {
+    covb_body<NB>(X, Rt, npad, K, KP, d, mu, nk, chunk_rows, cpart);
+}
+
 template <int NB>
 static hipError_t covb_t(fem_ctx *c) {
     const int blocks = (c->KP / 4) * c->c_chunks * cov_parts(NB);
     const size_t lds = (size_t)4 * cov_lds_doubles(NB) * sizeof(double);
+    auto kern = NB <= 4 ? k_covb<NB> : k_covb_wide<NB>;
     if (lds > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_covb<NB>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)lds);
+        hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_covb<NB>, dim3(blocks), dim3(256), lds, c->stream, c->X, c->Rt32, c->npad, c->K, c->KP,
-                       c->d, c->mu, c->nk, c->c_chunk_rows, c->cpart);
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), lds, c->stream, c->X, c->Rt32, c->npad, c->K, c->KP, c->d,
+                       c->mu, c->nk, c->c_chunk_rows, c->cpart);
     return hipGetLastError();
 }
 

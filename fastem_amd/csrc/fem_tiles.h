@@ -449,9 +449,13 @@ __device__ inline void means_finalize_unit(int k, int ct, int d, int dpad, int K
 // Stage (c), FP64: components 4cg..4cg+3 (one per wave) over one chunk of
 // points (EM2/MatrixEM.cpp:53-67,90-99,346-375).  Per step of 4 points each
 // lane forms t = x - mu_k and w t with w = r / N_k for every feature block fb
-// and issues one MFMA per upper block (ib <= jb): D += (w t)_ib^T t_jb, the
+// and issues one MFMA per block pair ib <= jb: D += (w t)_ib^T t_jb, the
 // same sum as the reference's T^T T with T = sqrt(w) t, without the square
-// root.  The loads of two steps are issued before their arithmetic.
+// root.  Feature blocks are strided: column c of block fb is feature
+// NB c + fb, so a lane reads its NB features of a point in one load, and the
+// pairs ib <= jb of strided blocks still hold every feature pair of the
+// symmetric matrix once (cov_reduce_unit maps them back).  The loads of two
+// steps are issued before their arithmetic.
 template <int NB>
 __device__ inline void cov64_unit(int cg, int chunk, const float *__restrict__ X, const double *__restrict__ Rt,
                                   int64_t npad, int K, int KP, int d, const double *__restrict__ mu,
@@ -466,7 +470,7 @@ __device__ inline void cov64_unit(int cg, int chunk, const float *__restrict__ X
     const int64_t i1 = (i0 + chunk_rows < npad) ? i0 + chunk_rows : npad;
     double muv[NB];
 #pragma unroll
-    for (int fb = 0; fb < NB; fb++) muv[fb] = (16 * fb + c < d) ? mu[(size_t)k * d + 16 * fb + c] : 0.0;
+    for (int fb = 0; fb < NB; fb++) muv[fb] = (NB * c + fb < d) ? mu[(size_t)k * d + NB * c + fb] : 0.0;
     const double invn = 1.0 / nk[k];
     v4d acc[UB];
 #pragma unroll
@@ -478,8 +482,7 @@ __device__ inline void cov64_unit(int cg, int chunk, const float *__restrict__ X
 #pragma unroll
         for (int q = 0; q < 2; q++) {
             rv[q] = rrow[i + 4 * q + g];
-#pragma unroll
-            for (int fb = 0; fb < NB; fb++) xv[q][fb] = X[(i + 4 * q + g) * DP + 16 * fb + c];
+            load_consecutive<NB>(X + (i + 4 * q + g) * DP + NB * c, xv[q]);
         }
 #pragma unroll
         for (int q = 0; q < 2; q++) {
@@ -507,8 +510,11 @@ __device__ inline void cov64_unit(int cg, int chunk, const float *__restrict__ X
         for (int v = 0; v < 4; v++) out[(size_t)u * 256 + (4 * v + g) * 16 + c] = acc[u][v];
 }
 
-// Sum the chunk partials of upper block u of component k (fixed order) and
-// write it to both triangles of cov.
+// Sum the chunk partials of block pair u = (ib <= jb) of component k (fixed
+// order) and write them to both triangles of cov.  The blocks are strided
+// (row r of block ib is feature NB r + ib): an off-diagonal pair holds each of
+// its feature pairs once, a diagonal pair holds (a, b) and (b, a), of which
+// the entry with row <= column is taken.
 __device__ inline void cov_reduce_unit(int k, int u, int d, int NB, int KP, int chunks,
                                        const double *__restrict__ cpart, double *__restrict__ cov) {
     const int tid = threadIdx.x;
@@ -519,8 +525,9 @@ __device__ inline void cov_reduce_unit(int k, int u, int d, int NB, int KP, int 
         ib++;
     }
     const int jb = ib + rem;
-    const int a = 16 * ib + (tid >> 4), bb = 16 * jb + (tid & 15);
-    if (a >= d || bb >= d || a > bb) return;
+    const int r = tid >> 4, col = tid & 15;
+    const int a = NB * r + ib, bb = NB * col + jb;
+    if (a >= d || bb >= d || (ib == jb && r > col)) return;
     const double s = chunk_sum4(cpart + ((size_t)k * UB + u) * 256 + tid, (size_t)KP * UB * 256, chunks);
     double *C = cov + (size_t)k * d * d;
     C[(size_t)a * d + bb] = s;

@@ -20,7 +20,8 @@ def test_kmeanspp_seeding_only(orc):
     np.testing.assert_array_equal(w, np.bincount(cls, minlength=4) / 5000)
 
 
-@pytest.mark.parametrize("n,d,K,seed", [(30000, 6, 5, 9), (200000, 64, 64, 3), (50001, 17, 12, 4)])
+@pytest.mark.parametrize("n,d,K,seed", [(30000, 6, 5, 9), (200000, 64, 64, 3), (50001, 17, 12, 4),
+                                        (20000, 128, 32, 5)])
 def test_kmeanspp_lloyd_mixture(orc, n, d, K, seed):
     print(f"kmeans++ + Lloyd n={n} d={d} K={K}", flush=True)
     X = fem.gengmm(n, d, K, -20.0, 20.0, 0.05, seed)[0]
