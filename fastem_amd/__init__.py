@@ -78,6 +78,7 @@ _SIGS = {
     "fem_setnet_get_params": (ctypes.c_int, [_vp, _fp]),
     "fem_setnet_set_params": (ctypes.c_int, [_vp, _fp]),
     "fem_setnet_forward": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _fp, _fp, _fp, _fp]),
+    "fem_setnet_forward_bf16": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _fp, _fp, _fp, _fp]),
     "fem_setnet_step": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _fp, ctypes.c_int, _fp, _fp,
                                        _fp, _fp, ctypes.c_float, ctypes.c_float, ctypes.c_float, _fp,
                                        _fp]),

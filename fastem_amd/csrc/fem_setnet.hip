@@ -1,5 +1,7 @@
-// fem_setnet.hip -- the learned initializer's set network on the GPU: forward,
-// hand-written backward, Adam, and BF16-MFMA inference.
+// fem_setnet.hip -- the learned initializer's set network on the GPU: FP32
+// forward, hand-written backward and Adam for training, and an inference
+// forward whose products run on BF16 MFMA (v_mfma_f32_16x16x32_bf16, FP32
+// accumulation).
 //
 // Per point set b of n normalised points x_i in R^d (width W = 64, K outputs):
 //   h1 = gelu(x W1 + b1), h2 = gelu(h1 W2 + b2)        per point
@@ -613,6 +615,93 @@ __global__ void k_pback_h2(const float *__restrict__ dP, const float *__restrict
 
 }  // namespace
 
+namespace {
+
+typedef float v4f_s __attribute__((ext_vector_type(4)));
+typedef float v8f_s __attribute__((ext_vector_type(8)));
+typedef __bf16 v8bf_s __attribute__((ext_vector_type(8)));
+
+// Eight floats p[k0 .. k0+7] of a row of `cols` (zero past the end), in two
+// 16-byte loads when the row segment is whole and aligned.
+__device__ inline v8f_s load8_row(const float *__restrict__ row, int k0, int cols) {
+    v8f_s v;
+    const float *p = row + k0;
+    if (k0 + 8 <= cols && ((reinterpret_cast<uintptr_t>(p) & 15) == 0)) {
+        const float4 a = *reinterpret_cast<const float4 *>(p);
+        const float4 b = *reinterpret_cast<const float4 *>(p + 4);
+        v = (v8f_s){a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    } else {
+#pragma unroll
+        for (int e = 0; e < 8; e++) v[e] = (k0 + e < cols) ? p[e] : 0.f;
+    }
+    return v;
+}
+
+// C = act(alpha A Bt^T + bias) per batch entry z (blockIdx.z), with A: M x Kd
+// (row stride lda), Bt: N x Kd (row stride ldb), C: M x N (row stride ldc).
+// Products on v_mfma_f32_16x16x32_bf16 with both operands rounded to bf16
+// (round to nearest even), FP32 accumulation; ACT = 1 applies GELU.  Each wave
+// computes 16 rows x 64 columns (four MFMA tiles sharing the A operand); a
+// workgroup of four waves covers 64 x 64.  Lane (c, g) supplies row c of the
+// A tile and row c of each Bt tile at k = 8g .. 8g+7 of every 32-wide step.
+template <int ACT>
+__global__ __launch_bounds__(256) void k_bgemm_nt(int M, int N, int Kd, const float *__restrict__ A, int lda,
+                                                  int64_t sA, const float *__restrict__ Bt, int ldb, int64_t sB,
+                                                  const float *__restrict__ bias, float alpha, float *__restrict__ C,
+                                                  int ldc, int64_t sC) {
+    const int z = blockIdx.z;
+    A += z * sA;
+    Bt += z * sB;
+    C += z * sC;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int g = lane >> 4, c = lane & 15;
+    const int i0 = blockIdx.y * 64 + wave * 16, j0 = blockIdx.x * 64;
+    if (i0 >= M) return;
+    v4f_s acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) acc[t] = (v4f_s){0.f, 0.f, 0.f, 0.f};
+    const int ia = i0 + c;
+    const float *arow = A + (size_t)(ia < M ? ia : M - 1) * lda;
+    for (int k0 = 0; k0 < Kd; k0 += 32) {
+        const int kk = k0 + 8 * g;
+        v8f_s av = load8_row(arow, kk, Kd);
+        if (ia >= M) av = (v8f_s){0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        const v8bf_s ab = __builtin_convertvector(av, v8bf_s);
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int jb = j0 + 16 * t + c;
+            v8f_s bv = load8_row(Bt + (size_t)(jb < N ? jb : N - 1) * ldb, kk, Kd);
+            if (jb >= N) bv = (v8f_s){0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ab, __builtin_convertvector(bv, v8bf_s), acc[t], 0,
+                                                             0, 0);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const int j = j0 + 16 * t + c;
+        if (j >= N) continue;
+        const float bj = bias ? bias[j] : 0.f;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int i = i0 + 4 * g + v;
+            if (i >= M) continue;
+            const float x = alpha * acc[t][v] + bj;
+            C[(size_t)i * ldc + j] = ACT ? gelu_f(x) : x;
+        }
+    }
+}
+
+// Out[z][j][i] = In[z][i][j] for In: rows x cols per batch entry.
+__global__ void k_transpose(const float *__restrict__ In, int rows, int cols, float *__restrict__ Out) {
+    const int z = blockIdx.y;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)rows * cols) return;
+    const int i = (int)(e / cols), j = (int)(e % cols);
+    Out[(size_t)z * rows * cols + (size_t)j * rows + i] = In[(size_t)z * rows * cols + e];
+}
+
+}  // namespace
+
 struct fem_setnet {
     int d = 0, K = 0, device = 0;
     Offsets off{};
@@ -628,6 +717,8 @@ struct fem_setnet {
     size_t cap_e = 0;  // capacity of E in floats (sum over sets of n^2)
     float *P = nullptr, *Q = nullptr, *E = nullptr, *pm = nullptr, *T = nullptr, *dm = nullptr, *lrow = nullptr;
     float *dP = nullptr, *dQ = nullptr, *dh2x = nullptr, *dummy = nullptr, *DB = nullptr;
+    // BF16-MFMA inference: transposed weights (W1^T, W2^T, Wp^T, Wq^T) and X^T
+    float *wt = nullptr, *XT = nullptr;
     int step = 0;
     std::string err;
 };
@@ -649,7 +740,8 @@ void free_ws(fem_setnet *s) {
     float **ptrs[] = {&s->X,  &s->Z1,  &s->H1, &s->Z2, &s->H2,  &s->g,    &s->q,  &s->alpha, &s->z,
                       &s->mu, &s->lv,  &s->M,  &s->V,  &s->pw,  &s->dmu,  &s->dlv, &s->loss, &s->dz,
                       &s->ds, &s->dq,  &s->dqs, &s->dg, &s->dZ2, &s->dZ1, &s->part, &s->P,   &s->Q,
-                      &s->E,  &s->pm,  &s->T,  &s->dm, &s->lrow, &s->dP,  &s->dQ,  &s->dh2x, &s->dummy, &s->DB};
+                      &s->E,  &s->pm,  &s->T,  &s->dm, &s->lrow, &s->dP,  &s->dQ,  &s->dh2x, &s->dummy, &s->DB,
+                      &s->wt, &s->XT};
     for (float **p : ptrs) {
         if (*p) (void)hipFree(*p);
         *p = nullptr;
@@ -680,7 +772,7 @@ int ensure_ws(fem_setnet *s, int B, int n) {
         A(&s->E, ecount) || A(&s->pm, (size_t)rows * d) || A(&s->T, (size_t)rows * d) ||
         A(&s->dm, (size_t)rows * d) || A(&s->lrow, (size_t)rows) || A(&s->dP, (size_t)rows * W) ||
         A(&s->dQ, (size_t)rows * W) || A(&s->dh2x, (size_t)rows * W) || A(&s->dummy, (size_t)W) ||
-        A(&s->DB, ecount))
+        A(&s->DB, ecount) || A(&s->wt, (size_t)W * d + 3 * W * W) || A(&s->XT, (size_t)rows * d))
         return serr(s, FEM_ERR_HIP, "setnet workspace allocation failed");
     s->cap_rows = rows;
     s->cap_sets = B;
@@ -722,6 +814,46 @@ void forward(fem_setnet *s, int B, int n) {
         constexpr int I = decltype(Ic)::value;
         hipLaunchKernelGGL((k_gemm_nn<0, I>), dim3(nt, B), dim3(256), 0, s->stream, s->E, s->X, n, s->pm);
     });
+}
+
+// Inference forward on s->X with the products on BF16 MFMA: the dense layers,
+// the point head's projections, its scores and its weighted sum; pooling, the
+// seed queries, their attention and the heads stay FP32 (K x n x 64 work).
+// Fills H1 H2 g q alpha z mu lv P Q E pm.
+void forward_bf16(fem_setnet *s, int B, int n) {
+    const int rows = B * n, d = s->d, K = s->K;
+    const float *th = s->theta;
+    float *w1t = s->wt, *w2t = w1t + W * d, *wpt = w2t + W * W, *wqt = wpt + W * W;
+    auto tr = [&](const float *in, int r, int c, float *out, int batch) {
+        hipLaunchKernelGGL(k_transpose, dim3((r * c + 255) / 256, batch), dim3(256), 0, s->stream, in, r, c, out);
+    };
+    tr(th + s->off.W1, d, W, w1t, 1);
+    tr(th + s->off.W2, W, W, w2t, 1);
+    tr(th + s->off.Wp, W, W, wpt, 1);
+    tr(th + s->off.Wq, W, W, wqt, 1);
+    tr(s->X, n, d, s->XT, B);
+    const dim3 gr(1, (rows + 63) / 64, 1);
+    hipLaunchKernelGGL(k_bgemm_nt<1>, gr, dim3(256), 0, s->stream, rows, W, d, s->X, d, (int64_t)0, w1t, d,
+                       (int64_t)0, th + s->off.b1, 1.0f, s->H1, W, (int64_t)0);
+    hipLaunchKernelGGL(k_bgemm_nt<1>, gr, dim3(256), 0, s->stream, rows, W, W, s->H1, W, (int64_t)0, w2t, W,
+                       (int64_t)0, th + s->off.b2, 1.0f, s->H2, W, (int64_t)0);
+    hipLaunchKernelGGL(k_pool, dim3(B), dim3(W), 0, s->stream, s->H2, n, s->g);
+    hipLaunchKernelGGL(k_query, dim3(B), dim3(256), 0, s->stream, s->g, th + s->off.S, th + s->off.Wg, K, s->q);
+    hipLaunchKernelGGL(k_attn_fwd, dim3(B * K), dim3(256), 0, s->stream, s->H2, s->q, n, K, s->alpha, s->z);
+    hipLaunchKernelGGL(k_heads, dim3((B * K * d + 255) / 256), dim3(256), 0, s->stream, s->z, B * K, d,
+                       th + s->off.Wm, th + s->off.bm, th + s->off.Wv, th + s->off.bv, s->mu, s->lv);
+    hipLaunchKernelGGL(k_bgemm_nt<0>, gr, dim3(256), 0, s->stream, rows, W, W, s->H2, W, (int64_t)0, wpt, W,
+                       (int64_t)0, (const float *)nullptr, 1.0f, s->P, W, (int64_t)0);
+    hipLaunchKernelGGL(k_bgemm_nt<0>, gr, dim3(256), 0, s->stream, rows, W, W, s->H2, W, (int64_t)0, wqt, W,
+                       (int64_t)0, (const float *)nullptr, 1.0f, s->Q, W, (int64_t)0);
+    const int nt = (n + 63) / 64;
+    hipLaunchKernelGGL(k_bgemm_nt<0>, dim3(nt, nt, B), dim3(256), 0, s->stream, n, n, W, s->P, W,
+                       (int64_t)n * W, s->Q, W, (int64_t)n * W, (const float *)nullptr, 0.125f, s->E, n,
+                       (int64_t)n * n);
+    hipLaunchKernelGGL(k_psoftmax, dim3(rows), dim3(256), 0, s->stream, s->E, n);
+    hipLaunchKernelGGL(k_bgemm_nt<0>, dim3((d + 63) / 64, nt, B), dim3(256), 0, s->stream, n, d, n, s->E, n,
+                       (int64_t)n * n, s->XT, n, (int64_t)d * n, (const float *)nullptr, 1.0f, s->pm, d,
+                       (int64_t)n * d);
 }
 
 // backward from s->dmu, s->dlv into s->grad
@@ -856,6 +988,21 @@ int fem_setnet_forward(fem_setnet *s, int B, int n, const float *X, float *mu, f
     if (int r = ensure_ws(s, B, n)) return r;
     SH(s, hipMemcpyAsync(s->X, X, sizeof(float) * (size_t)B * n * s->d, hipMemcpyHostToDevice, s->stream));
     forward(s, B, n);
+    SH(s, hipGetLastError());
+    SH(s, hipMemcpyAsync(mu, s->mu, sizeof(float) * B * s->K * s->d, hipMemcpyDeviceToHost, s->stream));
+    SH(s, hipMemcpyAsync(lv, s->lv, sizeof(float) * B * s->K * s->d, hipMemcpyDeviceToHost, s->stream));
+    if (m) SH(s, hipMemcpyAsync(m, s->pm, sizeof(float) * (size_t)B * n * s->d, hipMemcpyDeviceToHost, s->stream));
+    SH(s, hipStreamSynchronize(s->stream));
+    return FEM_OK;
+}
+
+// As fem_setnet_forward, with the products on BF16 MFMA (inference).
+int fem_setnet_forward_bf16(fem_setnet *s, int B, int n, const float *X, float *mu, float *lv, float *m) {
+    if (!s || B < 1 || n < 1 || !X || !mu || !lv) return FEM_ERR_ARG;
+    SH(s, hipSetDevice(s->device));
+    if (int r = ensure_ws(s, B, n)) return r;
+    SH(s, hipMemcpyAsync(s->X, X, sizeof(float) * (size_t)B * n * s->d, hipMemcpyHostToDevice, s->stream));
+    forward_bf16(s, B, n);
     SH(s, hipGetLastError());
     SH(s, hipMemcpyAsync(mu, s->mu, sizeof(float) * B * s->K * s->d, hipMemcpyDeviceToHost, s->stream));
     SH(s, hipMemcpyAsync(lv, s->lv, sizeof(float) * B * s->K * s->d, hipMemcpyDeviceToHost, s->stream));

@@ -15,6 +15,10 @@ moves every point of the subsample towards its cluster's mean; k-means++
 seeding over the moved points then picks one seed per cluster).  The point
 head's proposal is the one `init_context` uses.
 
+Inference (`propose_*`, `init_context`) runs the network's products on BF16
+MFMA (fem_setnet_forward_bf16) unless bf16=False; training uses the FP32
+forward.
+
 Coordinates: a point set X is normalised by its own centroid c and one
 scale s (root mean square deviation over all coordinates), so the network
 sees (X - c) / s and predicts means and log-variances in those units.
@@ -129,18 +133,20 @@ class SetNet:
         net.set_params(z["theta"])
         return net
 
-    def forward(self, Xs, points=False):
+    def forward(self, Xs, points=False, bf16=False):
         """Xs: (B, n, d) normalised sets -> (mu, lv), each (B, K, d), and with
-        points=True also the point head's outputs m (B, n, d)."""
+        points=True also the point head's outputs m (B, n, d).  bf16=True runs
+        the inference forward with its products on BF16 MFMA
+        (fem_setnet_forward_bf16); the default is the FP32 training forward."""
         Xs = np.ascontiguousarray(Xs, dtype=np.float32)
         B, n, d = Xs.shape
         assert d == self.d
         mu = np.empty((B, self.K, d), dtype=np.float32)
         lv = np.empty((B, self.K, d), dtype=np.float32)
         m = np.empty((B, n, d), dtype=np.float32) if points else None
-        self._check(lib().fem_setnet_forward(self._h, B, n, Xs.ctypes.data_as(_fp),
-                                             mu.ctypes.data_as(_fp), lv.ctypes.data_as(_fp),
-                                             m.ctypes.data_as(_fp) if points else None))
+        f = lib().fem_setnet_forward_bf16 if bf16 else lib().fem_setnet_forward
+        self._check(f(self._h, B, n, Xs.ctypes.data_as(_fp), mu.ctypes.data_as(_fp),
+                      lv.ctypes.data_as(_fp), m.ctypes.data_as(_fp) if points else None))
         return (mu, lv, m) if points else (mu, lv)
 
     def step(self, Xs, M, V, p, T=None, lam=0.5, gam=1.0, lr=0.0, want_grad=False):
@@ -168,43 +174,44 @@ class SetNet:
                                           ctypes.byref(loss)))
         return (loss.value, grad) if want_grad else loss.value
 
-    def propose_means(self, X, n_sub=1024, seed=0):
+    def propose_means(self, X, n_sub=1024, seed=0, bf16=True):
         """K starting means for dataset X (K x d, FP64): the point head moves
         each point of an n_sub subsample towards its cluster's mean, and
         k-means++ seeding (D^2 sampling, numpy generator `seed`) over the moved
-        points picks one per cluster."""
+        points picks one per cluster.  Inference on BF16 MFMA unless
+        bf16=False."""
         Xs = subsample(np.asarray(X), n_sub, seed)
         Z, c, s = normalise(Xs)
-        _, _, m = self.forward(Z[None].astype(np.float32), points=True)
+        _, _, m = self.forward(Z[None].astype(np.float32), points=True, bf16=bf16)
         return c + s * kmeanspp_points(m[0].astype(np.float64), self.K, seed)
 
-    def propose_query_means(self, X, n_sub=1024, seed=0):
+    def propose_query_means(self, X, n_sub=1024, seed=0, bf16=True):
         """The seed-query head's K means for dataset X (K x d, FP64)."""
         Xs = subsample(np.asarray(X), n_sub, seed)
         Z, c, s = normalise(Xs)
-        mu, _ = self.forward(Z[None].astype(np.float32))
+        mu, _ = self.forward(Z[None].astype(np.float32), bf16=bf16)
         return c + s * mu[0].astype(np.float64)
 
-    def init_context(self, ctx, X, n_sub=1024, seed=0, lloyd_sweeps=29, fallback_seed=None):
+    def init_context(self, ctx, X, n_sub=1024, seed=0, lloyd_sweeps=29, fallback_seed=None, bf16=True):
         """Learned initialisation of a device context holding X: the proposed
         means seed Lloyd sweeps and the cluster mixture on the device
         (fem_kmeans_init_seeded) in place of the k-means++ seeding pass over
         all n points.  If that leaves a cluster with fewer than two points,
         device k-means++ (fem_kmeans_init) is the fallback.  Returns
         (w, mu, cov, used)."""
-        C0 = self.propose_means(X, n_sub=n_sub, seed=seed)
+        C0 = self.propose_means(X, n_sub=n_sub, seed=seed, bf16=bf16)
         try:
             return ctx.kmeans_init_seeded(C0, seed=seed, max_sweeps=lloyd_sweeps) + ("learned",)
         except FemError:
             fs = seed if fallback_seed is None else fallback_seed
             return ctx.kmeans_init(fs, seeding=1, max_sweeps=lloyd_sweeps) + ("kmeans++",)
 
-    def propose(self, X, n_sub=1024, seed=0, var_floor=1e-6):
+    def propose(self, X, n_sub=1024, seed=0, var_floor=1e-6, bf16=True):
         """Starting mixture (w, mu, cov) for dataset X: uniform weights,
         network means, diagonal covariances exp(log-variance)."""
         Xs = subsample(np.asarray(X), n_sub, seed)
         Z, c, s = normalise(Xs)
-        mu, lv = self.forward(Z[None].astype(np.float32))
+        mu, lv = self.forward(Z[None].astype(np.float32), bf16=bf16)
         mu = c + s * mu[0].astype(np.float64)
         var = np.maximum(np.exp(np.clip(lv[0].astype(np.float64), -30, 30)) * s * s, var_floor)
         cov = np.zeros((self.K, self.d, self.d))

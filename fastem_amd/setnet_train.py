@@ -122,13 +122,14 @@ def train(net, data, steps, batch=64, lr=2e-3, gam=1.0, seed=0, val_frac=0.02, l
 
 
 def evaluate(net, seeds, n, d, K, gen, n_sub=1024, eps=1e-5, maxsweeps=1000, lloyd_iters=29, chunk=128,
-             g=0.0, log=print):
+             g=0.0, log=print, bf16=True):
     """EM from the learned initialisation vs EM from k-means++ on fresh seeds.
 
     Both start with Lloyd sweeps and the cluster mixture on the device; they
     differ in the seeds: the network's means (fem_kmeans_init_seeded) or the
     k-means++ seeding pass (fem_kmeans_init).  EM then runs to the reference's
-    stopping rule |dLL/LL| < eps for both in one batch."""
+    stopping rule |dLL/LL| < eps for both in one batch.  bf16: inference on
+    BF16 MFMA (False: the FP32 forward)."""
     lo, hi, diag = gen
     recs = []
     ctx = fem.Context(n, d, K)
@@ -147,13 +148,13 @@ def evaluate(net, seeds, n, d, K, gen, n_sub=1024, eps=1e-5, maxsweeps=1000, llo
             t = time.perf_counter()
             try:
                 w, mu, cov, used = net.init_context(ctx, X, n_sub=n_sub, seed=int(s),
-                                                    lloyd_sweeps=lloyd_iters)
+                                                    lloyd_sweeps=lloyd_iters, bf16=bf16)
                 lrn = (w, mu, cov)
             except fem.FemError:
                 lrn, used = None, "failed"
             t_lrn = time.perf_counter() - t
             try:
-                qry = ctx.kmeans_init_seeded(net.propose_query_means(X, n_sub=n_sub, seed=int(s)),
+                qry = ctx.kmeans_init_seeded(net.propose_query_means(X, n_sub=n_sub, seed=int(s), bf16=bf16),
                                              seed=int(s), max_sweeps=lloyd_iters)
             except fem.FemError:
                 qry = None
@@ -242,30 +243,43 @@ def main(argv=None):
     ap.add_argument("--out", default=None)
     ap.add_argument("--report", default=None)
     ap.add_argument("--net-seed", type=int, default=0)
+    ap.add_argument("--load", default=None,
+                    help="evaluate a saved network (.npz) instead of generating targets and training")
+    ap.add_argument("--fp32-inference", action="store_true",
+                    help="evaluate with the FP32 forward instead of the BF16-MFMA inference forward")
     a = ap.parse_args(argv)
 
     def log(*m):
         print(*m, flush=True)
 
     gen = GEN.get((a.d, a.K), (-10.0, 10.0, 0.1))
-    log(f"targets: {a.fits} fits n={a.n} d={a.d} K={a.K} gen={gen}")
-    data = make_targets(np.arange(1, a.fits + 1), a.n, a.d, a.K, gen, a.keep, g=a.g, log=log)
-    log(f"targets done: {data['sub'].shape[0]} fits, {data['failed']} failed, "
-        f"{data['seconds']:.1f} s, mean EM iters {data['iters'].mean():.1f}")
-    net = SetNet(a.d, a.K, seed=a.net_seed)
-    tr = train(net, data, a.steps, batch=a.batch, lr=a.lr, gam=a.gam, log=log)
-    if a.out:
-        net.save(a.out)
-        log(f"saved {a.out}")
+    if a.load:
+        net = SetNet.load(a.load)
+        assert (net.d, net.K) == (a.d, a.K), "saved network shape differs from --d/--K"
+        data, tr = None, None
+    else:
+        log(f"targets: {a.fits} fits n={a.n} d={a.d} K={a.K} gen={gen}")
+        data = make_targets(np.arange(1, a.fits + 1), a.n, a.d, a.K, gen, a.keep, g=a.g, log=log)
+        log(f"targets done: {data['sub'].shape[0]} fits, {data['failed']} failed, "
+            f"{data['seconds']:.1f} s, mean EM iters {data['iters'].mean():.1f}")
+        net = SetNet(a.d, a.K, seed=a.net_seed)
+        tr = train(net, data, a.steps, batch=a.batch, lr=a.lr, gam=a.gam, log=log)
+        if a.out:
+            net.save(a.out)
+            log(f"saved {a.out}")
     recs = evaluate(net, np.arange(a.eval_seed0, a.eval_seed0 + a.eval), a.eval_n or a.n, a.d, a.K, gen,
-                    n_sub=a.keep, eps=a.eval_eps, g=a.g, log=log) if a.eval > 0 else []
+                    n_sub=a.keep, eps=a.eval_eps, g=a.g, log=log,
+                    bf16=not a.fp32_inference) if a.eval > 0 else []
     summ = summarise(recs) if recs else {}
+    summ["inference"] = "fp32" if a.fp32_inference else "bf16 mfma"
     log(json.dumps(summ))
     if a.report:
+        rep = {"args": vars(a), "train": tr, "summary": summ, "records": recs}
+        if data is not None:
+            rep.update({"targets_seconds": data["seconds"], "targets_failed": data["failed"],
+                        "target_em_iters_mean": float(data["iters"].mean())})
         with open(a.report, "w") as f:
-            json.dump({"args": vars(a), "targets_seconds": data["seconds"],
-                       "targets_failed": data["failed"], "target_em_iters_mean": float(data["iters"].mean()),
-                       "train": tr, "summary": summ, "records": recs}, f, indent=1)
+            json.dump(rep, f, indent=1)
     return 0
 
 

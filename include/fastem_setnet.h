@@ -11,6 +11,10 @@
  * Parameters are one flat FP32 vector, in this order:
  *   W1[d][64] b1[64] W2[64][64] b2[64] S[K][64] Wg[64][64]
  *   Wm[64][d] bm[d] Wv[64][d] bv[d] Wp[64][64] Wq[64][64]
+ * Training arithmetic is FP32 (fem_setnet_forward, fem_setnet_step); the
+ * inference forward fem_setnet_forward_bf16 runs the per-point layers, the
+ * point head's projections, scores and weighted sum on BF16 MFMA
+ * (v_mfma_f32_16x16x32_bf16, operands rounded to bf16, FP32 accumulation).
  * Network, per set of n points x_i (width 64):
  *   h1 = gelu(x W1 + b1), h2 = gelu(h1 W2 + b2), g = mean_i h2_i,
  *   q_k = S_k + g Wg, a_ik = softmax_i(q_k . h2_i / 8), z_k = sum_i a_ik h2_i,
@@ -47,6 +51,11 @@ int fem_setnet_set_params(fem_setnet *net, const float *theta);
  * writes means and log-variances (B*K*d each, host) and, if m is not NULL,
  * the point head's outputs (B*n*d). */
 int fem_setnet_forward(fem_setnet *net, int B, int n, const float *X, float *mu, float *lv, float *m);
+
+/* The same forward for inference with the products on BF16 MFMA (FP32
+ * accumulation; pooling, seed-query attention, heads and the softmax in
+ * FP32).  Outputs as fem_setnet_forward. */
+int fem_setnet_forward_bf16(fem_setnet *net, int B, int n, const float *X, float *mu, float *lv, float *m);
 
 /* One training step over B sets: forward, loss against the targets
  * (M, V: B*J*d; p: B*J; point targets T: B*n*d, or NULL to leave the point

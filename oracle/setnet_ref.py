@@ -4,7 +4,9 @@ backward, and of its training loss.
 TEST INFRASTRUCTURE ONLY: the checker for the HIP training kernels
 (fastem_amd/csrc/fem_setnet.hip).  Pinned by central finite differences in
 tests/test_oracle_setnet.py.  The network has no counterpart in the fastem
-reference.
+reference.  forward_bf16 restates the inference forward whose products take
+operands rounded to bf16 (round_bf16, pinned on known values and against a
+bit-level definition of round-to-nearest-even in the same test file).
 
 Network (per point set of n normalised points x_i in R^d, width W = 64, K
 outputs):
@@ -83,6 +85,43 @@ def forward(p, X):
     m = np.einsum("bij,bjd->bid", bm_, X)
     cache = dict(X=X, z1=z1, h1=h1, z2=z2, h2=h2, g=g, q=q, a=a, z=z, P=P, Q=Q, beta=bm_, m=m)
     return mu, lv, cache
+
+
+def round_bf16(x):
+    """Values rounded to bfloat16 (8 significant bits, round to nearest, ties
+    to even) through their FP32 bit patterns; returned as float64."""
+    u = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def forward_bf16(p, X):
+    """The inference forward: every product of the per-point layers and of the
+    point head (x W1, h1 W2, h2 Wp, h2 Wq, the scores P Q^T and the weighted
+    sum beta X) takes operands rounded to bf16; accumulation, biases, GELU,
+    pooling, the seed-query attention, the heads and the softmax are exact.
+    Returns (mu, lv, m)."""
+    r = round_bf16
+    width = p["W2"].shape[0]
+    h1 = gelu(r(X) @ r(p["W1"]) + p["b1"])
+    h2 = gelu(r(h1) @ r(p["W2"]) + p["b2"])
+    g = h2.mean(axis=1)
+    q = p["S"][None] + (g @ p["Wg"])[:, None, :]
+    s = np.einsum("bkw,biw->bki", q, h2) / np.sqrt(width)
+    s = s - s.max(axis=2, keepdims=True)
+    a = np.exp(s)
+    a = a / a.sum(axis=2, keepdims=True)
+    z = np.einsum("bki,biw->bkw", a, h2)
+    mu = z @ p["Wm"] + p["bm"]
+    lv = z @ p["Wv"] + p["bv"]
+    P = r(h2) @ r(p["Wp"])
+    Q = r(h2) @ r(p["Wq"])
+    e = r(P) @ np.swapaxes(r(Q), 1, 2) / np.sqrt(width)
+    e = e - e.max(axis=2, keepdims=True)
+    be = np.exp(e)
+    be = be / be.sum(axis=2, keepdims=True)
+    m = r(be) @ r(X)
+    return mu, lv, m
 
 
 def point_loss_and_grad(m, T, gam=1.0):

@@ -58,6 +58,27 @@ def test_forward_matches_oracle(d, K, B, n):
         assert np.abs(m - cache["m"]).max() <= 2e-4 * np.abs(X).max()
 
 
+@pytest.mark.parametrize("d,K,B,n", [(8, 5, 3, 300), (32, 16, 4, 1024), (64, 64, 2, 700), (16, 8, 2, 77)])
+def test_forward_bf16_matches_oracle(d, K, B, n):
+    """The BF16-MFMA inference forward against the oracle's forward_bf16
+    (same bf16-rounded operands, exact accumulation): 1e-3 of the output
+    scale.  The bf16 rounding itself moves the outputs by a few 1e-3 from the
+    FP32 forward."""
+    print(f"setnet forward bf16 d={d} K={K} B={B} n={n}", flush=True)
+    with _net(d, K, seed=11) as net:
+        th = net.params().astype(np.float64)
+        X, _, _, _ = _case(d, K, B, n, 3, seed=d + K)
+        mu, lv, m = net.forward(X, points=True, bf16=True)
+        p = sr.unflatten(th, d, K)
+        mu_r, lv_r, m_r = sr.forward_bf16(p, X.astype(np.float64))
+        scale = max(np.abs(mu_r).max(), np.abs(lv_r).max())
+        assert np.abs(mu - mu_r).max() <= 1e-3 * scale
+        assert np.abs(lv - lv_r).max() <= 1e-3 * scale
+        assert np.abs(m - m_r).max() <= 1e-3 * np.abs(X).max()
+        mu32, lv32, m32 = net.forward(X, points=True)
+        assert np.abs(mu - mu32).max() <= 3e-2 * scale
+
+
 @pytest.mark.parametrize("d,K,B,n,J", [(8, 5, 3, 300, 4), (32, 16, 4, 1024, 16), (64, 64, 2, 500, 70)])
 def test_gradient_matches_oracle(d, K, B, n, J):
     """Targets are placed near the predictions (with the mean head scaled up so

@@ -108,3 +108,46 @@ def test_flatten_roundtrip():
     q = sr.unflatten(theta, 5, 6)
     for k in p:
         assert np.array_equal(p[k], q[k])
+
+
+def test_round_bf16_known_values():
+    """bf16 keeps 8 significant bits: 1 + 2^-8 is a tie (to even: 1), 1 +
+    3 * 2^-8 rounds up to 1 + 2^-6, 1 + 2^-8 + 2^-20 rounds up to 1 + 2^-7;
+    representable values pass unchanged; signs are kept."""
+    x = np.array([1.0, 1 + 2.0 ** -8, 1 + 3 * 2.0 ** -8, 1 + 2.0 ** -8 + 2.0 ** -20, -2.5, 0.15625,
+                  3.0 * 2.0 ** -130])
+    want = np.array([1.0, 1.0, 1 + 2.0 ** -6, 1 + 2.0 ** -7, -2.5, 0.15625, 3.0 * 2.0 ** -130])
+    np.testing.assert_array_equal(sr.round_bf16(x), want)
+
+
+def test_round_bf16_is_nearest_even_on_random_floats():
+    """Against the definition: the nearest of the two bf16 neighbours (FP32
+    with the low 16 bits cleared, and the next one up in magnitude), the one
+    with an even last bit on a tie."""
+    rng = np.random.default_rng(3)
+    f = (rng.standard_normal(20000) * np.exp(rng.uniform(-20, 20, 20000))).astype(np.float32)
+    u = f.view(np.uint32)
+    lo = (u & 0xFFFF0000).view(np.float32).astype(np.float64)
+    hi = ((u & 0xFFFF0000) + 0x10000).astype(np.uint32).view(np.float32).astype(np.float64)
+    x = f.astype(np.float64)
+    dlo, dhi = np.abs(x - lo), np.abs(hi - x)
+    even_lo = ((u >> 16) & 1) == 0
+    want = np.where(dlo < dhi, lo, np.where(dhi < dlo, hi, np.where(even_lo, lo, hi)))
+    np.testing.assert_array_equal(sr.round_bf16(f), want)
+
+
+def test_forward_bf16_reduces_to_forward_on_bf16_data():
+    """With parameters and inputs already bf16 values, only the rounding of
+    intermediate products separates forward_bf16 from forward: the outputs
+    agree to bf16 resolution of those intermediates, and differ from them
+    when the inputs are not bf16 values."""
+    d, K = 8, 4
+    p = sr.init_params(d, K, seed=2)
+    for k in p:
+        p[k] = sr.round_bf16(p[k])
+    X = sr.round_bf16(np.random.default_rng(4).standard_normal((2, 40, d)))
+    mu, lv, c = sr.forward(p, X)
+    mu_b, lv_b, m_b = sr.forward_bf16(p, X)
+    assert np.abs(mu_b - mu).max() <= 2e-2 * np.abs(mu).max()
+    assert np.abs(m_b - c["m"]).max() <= 2e-2 * np.abs(X).max()
+    assert np.abs(mu_b - mu).max() > 0.0
