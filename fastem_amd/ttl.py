@@ -8,10 +8,12 @@ LL* - tol*|LL*|.  Times are wall-clock from the start of the initialisation
 
 Forms:
 * plain: k-means++ init, then FP64 EM (the parity form).
-* mixed: the same init, reduced-precision EM stages (BF16x3, FP32) each until
-  the relative LL change falls below its switch threshold, then FP64
-  iterations on the same context until the target is met; the LL that decides
-  the criterion is always an FP64 E-step.
+* mixed: the same init, then reduced-precision EM (BF16x3, FP32) on the same
+  context.  Either the stage ends when the relative LL change falls below a
+  switch threshold (or the LL decreases) and FP64 iterations follow until the
+  target is met, or the stage continues and one FP64 E-step every m
+  iterations checks the target (its cost counted).  The LL that decides the
+  criterion is always an FP64 E-step.
 """
 import time
 
@@ -56,16 +58,26 @@ def run_plain(ctx, seed, tol=1e-6, cap=500, conv=1e-12, ll_star=None, init=None)
             "iters_to_reach": None if hit is None else hit + 1, "iters": len(trace)}
 
 
+def check_every(m):
+    """Stage rule: stay in the stage, check the target with an FP64 E-step
+    every m iterations."""
+    return ("check", int(m))
+
+
 def run_schedule(ctx, seed, ll_star, stages, tol=1e-6, cap=500, init=None):
     """Reduced-precision stages, then FP64 until the target is met.
 
     init: None (device k-means++), (w, mu, cov), or a callable that
     initialises the context itself (timed as part of the run).
-    stages: [(precision, switch_eps), ...]; each stage iterates in its form
-    until the relative LL change falls below switch_eps or the LL decreases
-    (the reference's convergence rule treats a decrease as converged), then
-    the next stage continues from the same parameters on the same context.
-    The LL that decides the criterion is always an FP64 E-step."""
+    stages: [(precision, rule), ...]; each stage iterates in its form until
+    its rule says switch, then the next stage continues from the same
+    parameters on the same context.  rule = a float eps: the relative LL
+    change falls below eps or the LL decreases (the reference's convergence
+    rule, which treats a decrease as converged), then the next stage; rule =
+    check_every(m): the stage runs to the end (or the sweep cap) and every m
+    iterations one FP64 E-step on the current parameters checks the target,
+    its cost included in the time.  The LL that decides whether the target is
+    met is always an FP64 E-step."""
     target = ll_star - tol * abs(ll_star)
     trace = []
     t0 = time.perf_counter()
@@ -78,18 +90,32 @@ def run_schedule(ctx, seed, ll_star, stages, tol=1e-6, cap=500, init=None):
         ctx.set_params(*init)
     t_init = time.perf_counter() - t0
     counts = []
-    for prec, switch_eps in stages:
+    for prec, rule in stages:
         ctx.set_precision(prec)
         ctx.factor()
         prev = None
         n = 0
+        checks = 0
         for _ in range(cap):
             ll = ctx.estep()
             trace.append((time.perf_counter() - t0, ll, prec))
             ctx.mstep(copy=False)
             ctx.factor()
             n += 1
-            if prev is not None and (ll < prev or abs((ll - prev) / ll) < switch_eps):
+            if isinstance(rule, tuple):
+                if n % rule[1] == 0:
+                    ctx.set_precision(fem.PREC_FP64)
+                    ctx.factor()
+                    ll64 = ctx.estep()
+                    t = time.perf_counter() - t0
+                    trace.append((t, ll64, fem.PREC_FP64))
+                    checks += 1
+                    if ll64 >= target:
+                        return {"trace": trace, "t_reach": t, "t_init": t_init, "stage_iters": counts + [n],
+                                "fp64_iters": 0, "fp64_checks": checks}
+                    ctx.set_precision(prec)
+                    ctx.factor()
+            elif prev is not None and (ll < prev or abs((ll - prev) / ll) < rule):
                 break
             prev = ll
         counts.append(n)

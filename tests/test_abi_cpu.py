@@ -63,3 +63,4 @@ def test_generator_fixed_bytes():
     h = hashlib.sha256(X.tobytes()).hexdigest()
     ref = open(os.path.join(ROOT, "tests", "golden", "gengmm_1000_4_3_7.sha256")).read().split()[0]
     assert h == ref
+

@@ -118,3 +118,23 @@ def test_bf16x3_d128_k128_per_iteration_errors(orc):
             print(f"iter {it}: LL {rows[-1][0]:.2e} mu {rows[-1][1]:.2e} cov {rows[-1][2]:.2e}", flush=True)
     for ll_e, mu_e, cov_e in rows:
         assert ll_e <= 1e-4 and mu_e <= 1e-4 and cov_e <= 1e-4
+
+
+def test_time_to_loglik_schedules_reach_the_fp64_target():
+    """The time-to-loglik harness: a BF16x3 stage with periodic FP64 checks
+    and an FP32 stage that hands over to FP64 both end on an FP64 E-step
+    within 1e-6 of the converged FP64 log-likelihood."""
+    from fastem_amd import ttl
+    print("time-to-loglik schedules", flush=True)
+    n, d, K = 40000, 16, 8
+    X = fem.gengmm(n, d, K, -10.0, 10.0, 0.1, 321)[0]
+    with fem.Context(n, d, K, fem.PREC_FP64, 0) as ctx:
+        ctx.upload(X)
+        plain = ttl.run_plain(ctx, seed=1, tol=1e-6, cap=300, conv=1e-12)
+        ll_star = plain["ll_final"]
+        target = ll_star - 1e-6 * abs(ll_star)
+        for stages in ([(fem.PREC_BF16X3, ttl.check_every(5))], [(fem.PREC_FP32, 1e-9)]):
+            r = ttl.run_schedule(ctx, seed=1, ll_star=ll_star, stages=stages, tol=1e-6, cap=300)
+            assert r["t_reach"] is not None
+            t, ll, prec = r["trace"][-1]
+            assert prec == fem.PREC_FP64 and ll >= target
