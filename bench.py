@@ -399,8 +399,12 @@ def main():
                 variants.append(("learned+bf16x3/switch1e-9", [(fem.PREC_BF16X3, 1e-9)], learned))
             for name, stages, init in variants:
                 log(f"time-to-loglik: {name} -> FP64 run")
-                r = ttl.run_schedule(ctx, seed=1, ll_star=ll_star, stages=stages, tol=1e-6,
-                                     cap=args.ttl_cap, init=init)
+                try:
+                    r = ttl.run_schedule(ctx, seed=1, ll_star=ll_star, stages=stages, tol=1e-6,
+                                         cap=args.ttl_cap, init=init)
+                except fem.FemError as e:   # e.g. a covariance that lost definiteness
+                    runs[name] = {"error": str(e)}
+                    continue
                 runs[name] = {"seconds": r["t_reach"], "init_s": r["t_init"],
                               "reduced_iters": r["stage_iters"][0], "fp64_iters": r["fp64_iters"],
                               "fp64_checks": r.get("fp64_checks", 0), "ll_final": r["trace"][-1][1],
@@ -417,8 +421,12 @@ def main():
                 for name, stages in (("learned+bf16x3", [(fem.PREC_BF16X3, chk)]),
                                      ("learned+bf16x3/switch1e-9", [(fem.PREC_BF16X3, 1e-9)])):
                     log(f"time-to-loglik: {name} -> FP64 run, own target")
-                    r = ttl.run_schedule(ctx, seed=1, ll_star=lp["ll_final"], stages=stages, tol=1e-6,
-                                         cap=args.ttl_cap, init=learned)
+                    try:
+                        r = ttl.run_schedule(ctx, seed=1, ll_star=lp["ll_final"], stages=stages,
+                                             tol=1e-6, cap=args.ttl_cap, init=learned)
+                    except fem.FemError as e:
+                        own[name] = {"error": str(e)}
+                        continue
                     t = r["t_reach"]
                     own[name] = {"seconds": t, "reduced_iters": r["stage_iters"][0],
                                  "fp64_iters": r["fp64_iters"], "fp64_checks": r.get("fp64_checks", 0),
@@ -435,13 +443,18 @@ def main():
                     def learned_s(c, Xs=Xs):
                         net.init_context(c, Xs, n_sub=2048, seed=1)
 
-                    pk = ttl.run_plain(ctx, seed=1, tol=1e-6, cap=args.ttl_cap, conv=1e-12)
-                    pl = ttl.run_plain(ctx, seed=1, tol=1e-6, cap=args.ttl_cap, conv=1e-12, init=learned_s)
-                    stages = [(fem.PREC_BF16X3, chk)]
-                    r1 = ttl.run_schedule(ctx, seed=1, ll_star=pk["ll_final"], stages=stages, tol=1e-6,
-                                          cap=args.ttl_cap, init=learned_s)
-                    r2 = ttl.run_schedule(ctx, seed=1, ll_star=pl["ll_final"], stages=stages, tol=1e-6,
-                                          cap=args.ttl_cap, init=learned_s)
+                    try:
+                        pk = ttl.run_plain(ctx, seed=1, tol=1e-6, cap=args.ttl_cap, conv=1e-12)
+                        pl = ttl.run_plain(ctx, seed=1, tol=1e-6, cap=args.ttl_cap, conv=1e-12,
+                                           init=learned_s)
+                        stages = [(fem.PREC_BF16X3, chk)]
+                        r1 = ttl.run_schedule(ctx, seed=1, ll_star=pk["ll_final"], stages=stages,
+                                              tol=1e-6, cap=args.ttl_cap, init=learned_s)
+                        r2 = ttl.run_schedule(ctx, seed=1, ll_star=pl["ll_final"], stages=stages,
+                                              tol=1e-6, cap=args.ttl_cap, init=learned_s)
+                    except fem.FemError as e:
+                        extra.append({"seed": s, "error": str(e)})
+                        continue
                     t1, t2 = r1["t_reach"], r2["t_reach"]
                     extra.append({
                         "seed": s, "ll_star_kpp": pk["ll_final"], "plain_kpp_s": pk["t_reach"],
@@ -480,7 +493,10 @@ def main():
                                       "to within tol of it"}
     if not args.no_precision:
         log("precision workload, N=4194304 d=128 K=128")
-        out["precision_d128_k128"] = precision_variant(log=log)
+        try:
+            out["precision_d128_k128"] = precision_variant(log=log)
+        except fem.FemError as e:
+            out["precision_d128_k128"] = {"error": str(e)}
     if not args.no_batch:
         log("batched fits")
         out["batch"] = batch_workload(args.batch_fits, compare=args.batch_compare, log=log)
