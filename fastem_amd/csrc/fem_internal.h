@@ -64,6 +64,7 @@ struct fem_ctx {
     int est_blocks = 0;         // capacity of ll_part (one per E-step workgroup)
     int estep_groups = 2;       // 16-point groups per wave in the FP64 E-step (2 or 4)
     int estep_groups32 = 2;     // same for the FP32 E-step
+    int estep_groupsb = 4;      // same for the BF16x3 E-step (4 measured faster at d <= 64)
     double *ll_part = nullptr;  // per E-step workgroup
     double *ll_dev = nullptr;   // 1
     int m_chunks = 0, c_chunks = 0;

@@ -299,9 +299,9 @@ static hipError_t estepb_t(fem_ctx *c) {
     return hipGetLastError();
 }
 
-// Point groups per wave: 2 by default, 4 for d <= 64 when estep_groups32 == 4.
+// Point groups per wave: 4 for d <= 64 unless estep_groupsb == 2, else 2.
 hipError_t launch_estepb(fem_ctx *c) {
-    const bool p4 = c->estep_groups32 == 4;
+    const bool p4 = c->estep_groupsb == 4;
     switch (c->NB) {
         case 1: return p4 ? estepb_t<1, 4>(c) : estepb_t<1, 2>(c);
         case 2: return p4 ? estepb_t<2, 4>(c) : estepb_t<2, 2>(c);
