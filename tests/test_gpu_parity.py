@@ -101,8 +101,9 @@ def test_estep_mstep_parity(orc, n, d, K):
 @pytest.mark.parametrize("g", [0.0, 0.01])
 def test_trajectory_config1(orc, g):
     """N=100,000, d=16, K=8, means U[-10,10], cov AA^T/d + 0.1 I, k-means++
-    init; every iteration's LL and the final parameters within 1e-10."""
-    n, d, K, iters = 100000, 16, 8, 40
+    init, 100 iterations; every iteration's LL and the final parameters
+    within 1e-10."""
+    n, d, K, iters = 100000, 16, 8, 100
     say(f"trajectory config1 g={g}")
     X, _, _, _ = mixture(n, d, K, seed=11)
     w0, mu0, cov0 = fem.kmeanspp_init(X, K, seed=5)
