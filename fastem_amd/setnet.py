@@ -7,7 +7,7 @@ k-means++ or file initializers (EM2/EMInitializer.cpp).  Here the network
 proposes the starting means and they replace the k-means++ seeding pass
 (`init_context`: Lloyd sweeps and the cluster mixture follow on the device as
 for k-means++); k-means++ stays the fallback, and the final EM
-log-likelihood judges each proposal.
+log-likelihood is the measure of each proposal.
 
 Two heads propose means: the seed-query head (K attention-pooled queries ->
 means and log-variances) and the point head (a learned mean-shift step that
