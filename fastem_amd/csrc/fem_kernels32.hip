@@ -363,7 +363,10 @@ __global__ __launch_bounds__(256) void k_cov32(const float *__restrict__ X, cons
 #pragma unroll
                     for (int fb = 0; fb < NB; fb++) t[q][fb] = (xv[h][q][fb] - mh[fb]) * sw;
                 }
-                if (i + 16 * (H + h) < i1) load16(h, i + 16 * (H + h));
+                // unconditional (past the chunk end it re-reads the last 16
+                // points, unused), so the buffers need no register copies
+                const int64_t ni = i + 16 * (H + h);
+                load16(h, ni < i1 ? ni : i1 - 16);
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
                     int u = 0;

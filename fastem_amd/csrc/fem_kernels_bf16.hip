@@ -147,9 +147,10 @@ __global__ __launch_bounds__(256) void k_estepb(const float *__restrict__ X, int
     constexpr int NS = (NB + 1) / 2;
     constexpr int NPAIR = pairs_bf16(NB);
     constexpr int FW = NPAIR * 512;  // fragment words per component
-    constexpr int STAGE = estepb_stage_words<NB>();
-    constexpr int PER = (STAGE + 255) / 256;
-    extern __shared__ unsigned int ldsb[];  // 2 * STAGE words
+    constexpr int STAGE = estepb_stage_words<NB>();  // a multiple of 4
+    constexpr int STAGE4 = STAGE / 4;
+    constexpr int PER = (STAGE4 + 255) / 256;
+    extern __shared__ __align__(16) unsigned int ldsb[];  // 2 * STAGE words
     __shared__ double wsum[4];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int g = lane >> 4, c = lane & 15;
@@ -172,10 +173,14 @@ __global__ __launch_bounds__(256) void k_estepb(const float *__restrict__ X, int
             }
         }
 
+    // stage blocks move in 16-byte pieces; piece indices past the block are
+    // clamped to its last piece (same value to the same address)
+    const uint4 *stage4 = reinterpret_cast<const uint4 *>(stage);
+    uint4 *lds4 = reinterpret_cast<uint4 *>(ldsb);
 #pragma unroll
     for (int q = 0; q < PER; q++) {
-        const int e = tid + 256 * q;
-        if (e < STAGE) ldsb[e] = stage[e];
+        const int e = min(tid + 256 * q, STAGE4 - 1);
+        lds4[e] = stage4[e];
     }
     __syncthreads();
 
@@ -187,13 +192,10 @@ __global__ __launch_bounds__(256) void k_estepb(const float *__restrict__ X, int
         const unsigned int *lf = ldsb + (k & 1) * STAGE;
         const float *lmh = reinterpret_cast<const float *>(lf + FW);
         const float *lml = lmh + 32 * NS;
-        unsigned int pre[PER];
-        const bool more = k + 1 < K;
+        const uint4 *next4 = stage4 + (size_t)(k + 1 < K ? k + 1 : k) * STAGE4;
+        uint4 pre[PER];
 #pragma unroll
-        for (int q = 0; q < PER; q++) {
-            const int e = tid + 256 * q;
-            pre[q] = (more && e < STAGE) ? stage[(size_t)(k + 1) * STAGE + e] : 0u;
-        }
+        for (int q = 0; q < PER; q++) pre[q] = next4[min(tid + 256 * q, STAGE4 - 1)];
         const float ck = (float)coef[k];
         v4f acc[P][NB];
 #pragma unroll
@@ -226,6 +228,7 @@ __global__ __launch_bounds__(256) void k_estepb(const float *__restrict__ X, int
                 for (int p = 0; p < P; p++) acc[p][jb] = mfma3(ah, al, bh[p], bl[p], acc[p][jb]);
             }
         }
+        float sv[P];
 #pragma unroll
         for (int p = 0; p < P; p++) {
             float q = 0.f;
@@ -234,17 +237,20 @@ __global__ __launch_bounds__(256) void k_estepb(const float *__restrict__ X, int
 #pragma unroll
                 for (int v = 0; v < 4; v++) q += acc[p][jb][v] * acc[p][jb][v];
             q = xor_sum_gb(q);
-            const float s = -0.5f * q + ck;
-            mx[p] = fmaxf(mx[p], s);
-            if ((k & 3) == g) Rt[(size_t)k * npad + base + 16 * p + c] = s;
+            sv[p] = -0.5f * q + ck;
+            mx[p] = fmaxf(mx[p], sv[p]);
         }
-        if (more) {
-            unsigned int *dst = ldsb + ((k + 1) & 1) * STAGE;
+        {
+            // unconditional (after the last component the idle buffer is
+            // dead) and ahead of the branch below, so that the loads of the
+            // next block stay in this basic block, ahead of the MFMAs
+            uint4 *dst = lds4 + ((k + 1) & 1) * STAGE4;
 #pragma unroll
-            for (int q = 0; q < PER; q++) {
-                const int e = tid + 256 * q;
-                if (e < STAGE) dst[e] = pre[q];
-            }
+            for (int q = 0; q < PER; q++) dst[min(tid + 256 * q, STAGE4 - 1)] = pre[q];
+        }
+        if ((k & 3) == g) {
+#pragma unroll
+            for (int p = 0; p < P; p++) Rt[(size_t)k * npad + base + 16 * p + c] = sv[p];
         }
         __syncthreads();
     }
