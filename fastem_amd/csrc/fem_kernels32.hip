@@ -10,9 +10,9 @@
 //
 // Centring: with mu = mu_hi + mu_lo split into two floats, the E-step forms
 // U^T (x - mu_hi) - U^T mu_lo (the second term precomputed in FP64 and used as
-// the accumulator seed) and the covariance pass forms (x - mu_hi) - mu_lo, so
-// the centred value is accurate to FP32 rounding of the centred value rather
-// than of |mu| (means reach |mu| ~ 20 sigma).
+// the accumulator seed), so the centred value is accurate to FP32 rounding of
+// the centred value rather than of |mu| (means reach |mu| ~ 20 sigma); the
+// covariance pass centres with mu_hi alone (see k_cov32).
 #include "fem_internal.h"
 #include "fem_tiles.h"
 
@@ -33,9 +33,9 @@ __device__ inline float xor_sum_g(float v) { return group_sum4(v); }
 // ---------------------------------------------------------------------------
 // Stage (a), FP32.  Workgroup = 4 waves x P groups of 16 points.  The
 // component's stage block (U_k fragments, mu_hi, -U_k^T mu_lo; written by the
-// factorisation) is double-buffered in LDS: the loads of component k+1 are
-// issued into registers before the MFMAs of component k and written to the
-// idle buffer after them, one barrier per component.  The B operand is
+// factorisation) is double-buffered in LDS: component k+1's block is read
+// into registers while component k is processed and written to the idle
+// buffer before the one barrier per component.  The B operand is
 // x - mu_hi and the accumulators start at -U_k^T mu_lo, so D = U_k^T (x - mu).
 template <int NB, int P>
 __global__ __launch_bounds__(256) void k_estep32(const float *__restrict__ X, int64_t n, int64_t npad,

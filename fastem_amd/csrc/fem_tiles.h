@@ -229,16 +229,17 @@ __device__ __attribute__((always_inline)) inline T lse_posteriors(T *__restrict_
 // ---------------------------------------------------------------------------
 // Stage (a), FP64, one tile of 64P points (EM2/MatrixEM.cpp:39-87).
 // Each wave owns P groups of 16 points.  Lane (g = l/16, c = l%16) holds, for
-// point c of each group, features 4*lb + g (lb = 0..4NB-1) as FP32, widened
-// to FP64 per use: the B operand of every MFMA.  Component k's stage block
+// point c of each group, features 4*lb + g (lb = 0..4NB-1), loaded as FP32
+// and widened to FP64 once (the compiler keeps them widened across the
+// component loop): the B operand of every MFMA.  Component k's stage block
 // (fragments of U_k^T, then -m_k = -U_k^T mu_k) is double-buffered in LDS
-// (2 x STAGE doubles): the global loads of component k+1 are issued into
-// registers before the MFMAs of component k and stored to the idle buffer after
-// them, one barrier per component.  Accumulators start at -m_k so that
+// (2 x STAGE doubles): component k+1's block is read into registers while
+// component k is processed and stored to the idle buffer before the one
+// barrier per component.  Accumulators start at -m_k so that
 // D = U^T (x - mu).  log-densities go to the component-major Rt; the row max is
 // tracked as they are produced; a second sweep over the lane's own entries
-// forms the log-sum-exp and the posteriors.  The tile's LL is written to
-// *ll_out by thread 0.  wsum: 4 LDS doubles.
+// forms the log-sum-exp and the posteriors (lse_posteriors).  The tile's LL is
+// written to *ll_out by thread 0.  wsum: 4 LDS doubles.
 template <int NB>
 __device__ __host__ constexpr int estep64_stage() { return pairs_of(NB) * 64 + 16 * NB; }
 
