@@ -302,13 +302,18 @@ extern "C" int fem_kmeans_init_rng(fem_ctx *c, double (*rnd)(void *), void *rnd_
                     sum += dm[i] * 1.0;
                     agg[i] = sum;
                 }
-                for (int64_t i = 0; i < n; i++) agg[i] /= sum;
+                // first i with r <= agg[i] / sum: agg is non-decreasing and
+                // rounded division by sum > 0 keeps that order, so a binary
+                // search over the quotients finds the same row as the
+                // reference's scan over the normalised prefix sums
                 const double r = draw();
-                for (int64_t i = 0; i < n; i++)
-                    if (r <= agg[i]) {
-                        fetch_row(c, i, C.data() + (size_t)k * d);
-                        break;
-                    }
+                int64_t lo = 0, hi = n;  // answer in [lo, hi); hi = n: none
+                while (lo < hi) {
+                    const int64_t mid = lo + (hi - lo) / 2;
+                    if (r <= agg[mid] / sum) hi = mid;
+                    else lo = mid + 1;
+                }
+                if (lo < n) fetch_row(c, lo, C.data() + (size_t)k * d);
             }
         } else {
             // Forgy random partition (Clust/KMeansInitializer.cpp:33-49): one
