@@ -293,9 +293,10 @@ hipError_t launch_means32_partials(fem_ctx *c) {
 // sum_i r_ik (x_i - mu_k), both below 1e-9 of D.  Strided feature blocks as
 // k_cov64 (a lane's NB features of a point in one load); lane group g takes
 // points 4g..4g+3 of each 16, so its four R values are one 16-byte load.  For
-// d <= 64 two register buffers of 16 points: the loads for 32 points ahead are
-// issued once the current 16 are formed, before their MFMAs (one buffer and
-// 16 points ahead for larger d).  Same grid and XCD remapping
+// 48 < d <= 64 four register buffers of 16 points: the loads for 64 points
+// ahead are issued once the current 16 are formed, before their MFMAs (two
+// buffers measured 5% slower there; d <= 48 keeps two, where four measured no
+// faster, and d > 64 one buffer and 16 points ahead).  Same grid and XCD remapping
 // as k_cov64.
 template <int NB>
 __global__ __launch_bounds__(256) void k_cov32(const float *__restrict__ X, const float *__restrict__ Rt,
@@ -332,9 +333,8 @@ __global__ __launch_bounds__(256) void k_cov32(const float *__restrict__ X, cons
 #pragma unroll
             for (int v = 0; v < 4; v++) accd[u][v] = 0.0;
     }
-    // H register buffers of 16 points each (two for d <= 64): the loads run H
-    // steps ahead
-    constexpr int H = NB <= 4 ? 2 : 1;
+    // H register buffers of 16 points each: the loads run H steps ahead
+    constexpr int H = NB == 4 ? 4 : NB < 4 ? 2 : 1;
     float rv[H][4], xv[H][4][NB];
     auto load16 = [&](int h, int64_t i) {
         const float4 r4 = *reinterpret_cast<const float4 *>(rrow + i);
