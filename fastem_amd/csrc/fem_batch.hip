@@ -274,7 +274,7 @@ int grid_of(int mult, int cus, int nitems) { return std::max(1, std::min(nitems,
 
 // one EM iteration of all active fits (the captured sequence)
 hipError_t launch_iteration(fem_batch *b) {
-    const size_t lds_e = (size_t)2 * fem::estep64_stage<4>() * sizeof(double);
+    const size_t lds_e = (size_t)fem::estep64_lds<4>() * sizeof(double);
     const size_t lds_f = (size_t)64 * 64 * sizeof(double);
     const Item *est = b->est_groups == 4 ? b->est4 : b->est2;
     const int nest = b->est_groups == 4 ? b->n_est4 : b->n_est2;
@@ -423,7 +423,7 @@ int fem_batch_create(fem_batch **out, int nfits, const int64_t *n, const int *d,
         (e = hipMemset(b->dpool, 0, sizeof(double) * std::max<int64_t>(1, b->nd))) != hipSuccess ||
         (e = hipMemset(b->ipool, 0, sizeof(int) * std::max<int64_t>(1, b->ni))) != hipSuccess)
         return bail(FEM_ERR_HIP, hipGetErrorString(e));
-    const size_t lds_e = (size_t)2 * fem::estep64_stage<4>() * sizeof(double);
+    const size_t lds_e = (size_t)fem::estep64_lds<4>() * sizeof(double);
     (void)hipFuncSetAttribute((const void *)fem::kb_estep<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_e);
     (void)hipFuncSetAttribute((const void *)fem::kb_estep<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_e);
     b->grid_est = grid_of(2, cus, b->n_est2);
@@ -581,7 +581,7 @@ int fem_batch_tune(fem_batch *b, char *report, int report_len) {
         (void)hipEventElapsedTime(&ms, e0, e1);
         return ms;
     };
-    const size_t lds_e = (size_t)2 * fem::estep64_stage<4>() * sizeof(double);
+    const size_t lds_e = (size_t)fem::estep64_lds<4>() * sizeof(double);
     std::string rep;
     char line[256];
     float best = 1e30f;

@@ -55,7 +55,7 @@ __global__ __launch_bounds__(256) void k_estep64(const float *__restrict__ X, in
                                                  const double *__restrict__ coef,
                                                  double *__restrict__ Rt,
                                                  double *__restrict__ ll_part) {
-    extern __shared__ double lds[];  // 2 * estep64_stage<NB>()
+    extern __shared__ double lds[];  // estep64_lds<NB>()
     __shared__ double wsum[4];
     estep64_unit<NB, P>(blockIdx.x, X, n, npad, K, stage, coef, Rt, ll_part + blockIdx.x, lds, wsum);
 }
@@ -71,7 +71,7 @@ __global__ __launch_bounds__(256) void k_reduce_sum(const double *__restrict__ p
 
 template <int NB, int P>
 static hipError_t estep64_t(fem_ctx *c) {
-    const size_t lds = (size_t)2 * estep64_stage<NB>() * sizeof(double);
+    const size_t lds = (size_t)estep64_lds<NB>() * sizeof(double);
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void *)k_estep64<NB, P>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

@@ -233,15 +233,23 @@ __device__ __attribute__((always_inline)) inline T lse_posteriors(T *__restrict_
 // and widened to FP64 once (the compiler keeps them widened across the
 // component loop): the B operand of every MFMA.  Component k's stage block
 // (fragments of U_k^T, then -m_k = -U_k^T mu_k) is double-buffered in LDS
-// (2 x STAGE doubles): component k+1's block is read into registers while
-// component k is processed and stored to the idle buffer before the one
-// barrier per component.  Accumulators start at -m_k so that
+// (estep64_lds doubles): component k+1's block is copied from global memory
+// straight into the idle buffer (global_load_lds_dwordx4) while component k
+// is processed, and the one barrier per component publishes it.  Copied
+// through registers instead, the compiler sank those loads behind the MFMAs
+// and waited on each, which one wave per SIMD (d > 80, where the buffers fill
+// the LDS) cannot hide: the direct copy measured 30% faster at d = 96..128
+// and 3% at d = 64.  Accumulators start at -m_k so that
 // D = U^T (x - mu).  log-densities go to the component-major Rt; the row max is
 // tracked as they are produced; a second sweep over the lane's own entries
 // forms the log-sum-exp and the posteriors (lse_posteriors).  The tile's LL is
 // written to *ll_out by thread 0.  wsum: 4 LDS doubles.
 template <int NB>
 __device__ __host__ constexpr int estep64_stage() { return pairs_of(NB) * 64 + 16 * NB; }
+// LDS doubles of the two stage buffers, each padded to whole rounds of 256
+// 16-byte pieces (every lane of a direct copy writes one piece per round)
+template <int NB>
+__device__ __host__ constexpr int estep64_lds() { return 2 * ((estep64_stage<NB>() / 2 + 255) / 256) * 512; }
 
 template <int NB, int P>
 __device__ inline void estep64_unit(int64_t tile, const float *__restrict__ X, int64_t n, int64_t npad, int K,
@@ -253,6 +261,7 @@ __device__ inline void estep64_unit(int64_t tile, const float *__restrict__ X, i
     constexpr int STAGE = NPAIR * 64 + DP;  // even
     constexpr int STAGE2 = STAGE / 2;
     constexpr int PER = (STAGE2 + 255) / 256;
+    constexpr int BUF2 = PER * 256;  // 16-byte pieces per LDS buffer
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int g = lane >> 4, c = lane & 15;
     const int64_t base = tile * (64 * P) + wave * (16 * P);
@@ -279,12 +288,20 @@ __device__ inline void estep64_unit(int64_t tile, const float *__restrict__ X, i
     for (int p = 0; p < P; p++) mx[p] = -INFINITY;
 
     for (int k = 0; k < K; k++) {
-        const double *lu = lds + (k & 1) * STAGE;
+        const double *lu = lds + (k & 1) * (2 * BUF2);
         const double *lm = lu + NPAIR * 64;
-        const double2 *next2 = stage2 + (size_t)(k + 1 < K ? k + 1 : k) * STAGE2;
-        double2 pre[PER];
+        if (k + 1 < K) {
+            // component k+1's block straight into the idle buffer (each lane
+            // one 16-byte piece per round: global_load_lds_dwordx4), so no
+            // registers hold it and the copy overlaps this component's MFMAs
+            const double2 *next2 = stage2 + (size_t)(k + 1) * STAGE2;
+            double2 *dst = lds2 + ((k + 1) & 1) * BUF2 + 64 * wave;
 #pragma unroll
-        for (int q = 0; q < PER; q++) pre[q] = next2[min(tid + 256 * q, STAGE2 - 1)];
+            for (int q = 0; q < PER; q++)
+                __builtin_amdgcn_global_load_lds(
+                    (const __attribute__((address_space(1))) void *)(next2 + min(tid + 256 * q, STAGE2 - 1)),
+                    (__attribute__((address_space(3))) void *)(dst + 256 * q), 16, 0, 0);
+        }
         const double ck = coef[k];
         v4d acc[P][NB];
 #pragma unroll
@@ -326,19 +343,16 @@ __device__ inline void estep64_unit(int64_t tile, const float *__restrict__ X, i
             sv[p] = -0.5 * q + ck;
             mx[p] = fmax(mx[p], sv[p]);
         }
-        {
-            // unconditional (after the last component the idle buffer is
-            // dead) and ahead of the branch below, so that the loads of the
-            // next block stay in this basic block, ahead of the MFMAs
-            double2 *dst = lds2 + ((k + 1) & 1) * STAGE2;
-#pragma unroll
-            for (int q = 0; q < PER; q++) dst[min(tid + 256 * q, STAGE2 - 1)] = pre[q];
-        }
+        // this wave's pieces of the next block have landed and the barrier
+        // publishes all of them; the log-densities are stored after it, so
+        // that the wait does not include those stores (a lane reads back only
+        // the entries it stored itself)
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
         if ((k & 3) == g) {
 #pragma unroll
             for (int p = 0; p < P; p++) Rt[(size_t)k * npad + base + 16 * p + c] = sv[p];
         }
-        __syncthreads();
     }
 
     double llsum = 0.0;
