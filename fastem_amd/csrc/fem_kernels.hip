@@ -60,6 +60,17 @@ __global__ __launch_bounds__(256) void k_estep64(const float *__restrict__ X, in
     estep64_unit<NB, P>(blockIdx.x, X, n, npad, K, stage, coef, Rt, ll_part + blockIdx.x, lds, wsum);
 }
 
+// d = 65..80: LDS leaves room for two workgroups per CU; registers capped
+// so that two waves per SIMD fit
+template <int NB, int P>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_estep64_w2(
+    const float *__restrict__ X, int64_t n, int64_t npad, int K, const double *__restrict__ stage,
+    const double *__restrict__ coef, double *__restrict__ Rt, double *__restrict__ ll_part) {
+    extern __shared__ double lds[];  // estep64_lds<NB>()
+    __shared__ double wsum[4];
+    estep64_unit<NB, P>(blockIdx.x, X, n, npad, K, stage, coef, Rt, ll_part + blockIdx.x, lds, wsum);
+}
+
 // Deterministic sum of n doubles into *out: fixed strided partials, then a
 // fixed tree.  One workgroup of 256 threads.
 __global__ __launch_bounds__(256) void k_reduce_sum(const double *__restrict__ parts, int64_t n,
@@ -72,14 +83,20 @@ __global__ __launch_bounds__(256) void k_reduce_sum(const double *__restrict__ p
 template <int NB, int P>
 static hipError_t estep64_t(fem_ctx *c) {
     const size_t lds = (size_t)estep64_lds<NB>() * sizeof(double);
+    const void *kern;
+    if constexpr (NB == 5) kern = (const void *)k_estep64_w2<NB, P>;
+    else kern = (const void *)k_estep64<NB, P>;
     if (lds > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_estep64<NB, P>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
     const int blocks = (int)(c->npad / (64 * P));
-    hipLaunchKernelGGL((k_estep64<NB, P>), dim3(blocks), dim3(256), lds, c->stream, c->X, c->n,
-                       c->npad, c->K, c->stage64, c->coef, c->Rt, c->ll_part);
+    if constexpr (NB == 5)
+        hipLaunchKernelGGL((k_estep64_w2<NB, P>), dim3(blocks), dim3(256), lds, c->stream, c->X, c->n, c->npad,
+                           c->K, c->stage64, c->coef, c->Rt, c->ll_part);
+    else
+        hipLaunchKernelGGL((k_estep64<NB, P>), dim3(blocks), dim3(256), lds, c->stream, c->X, c->n, c->npad,
+                           c->K, c->stage64, c->coef, c->Rt, c->ll_part);
     hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(256), 0, c->stream, c->ll_part,
                        (int64_t)blocks, c->ll_dev);
     return hipGetLastError();
