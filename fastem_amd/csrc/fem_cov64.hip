@@ -23,12 +23,15 @@ __global__ __launch_bounds__(256) void k_cov64(const float *__restrict__ X,
     const int nblk = gridDim.x;
     const int b = blockIdx.x;
     const int logical = (nblk % 8 == 0) ? (b % 8) * (nblk / 8) + b / 8 : b;
-    cov64_unit<NB>(logical % KG, logical / KG, X, Rt, npad, K, KP, d, mu, nk, chunk_rows, cpart);
+    // the part (d > 96) varies fastest, then the component group
+    constexpr int NPART = cov64_parts(NB);
+    const int part = logical % NPART, rest = logical / NPART;
+    cov64_unit<NB>(rest % KG, rest / KG, X, Rt, npad, K, KP, d, mu, nk, chunk_rows, cpart, part);
 }
 
 template <int NB>
 static hipError_t cov_t(fem_ctx *c) {
-    const int blocks = (c->KP / 4) * c->c_chunks;
+    const int blocks = (c->KP / 4) * c->c_chunks * cov64_parts(NB);
     hipLaunchKernelGGL(k_cov64<NB>, dim3(blocks), dim3(256), 0, c->stream, c->X, c->Rt, c->npad,
                        c->K, c->KP, c->d, c->mu, c->nk, c->c_chunk_rows, c->cpart);
     return hipGetLastError();

@@ -428,31 +428,18 @@ hipError_t launch_meansb_partials(fem_ctx *c) {
 // are split, before their MFMAs.
 __device__ __host__ constexpr int cov_parts(int NB) { return NB > 4 ? 2 : 1; }
 
-// Part of upper block u (row-major over ib <= jb) among NPART parts.
-__device__ __host__ constexpr int cov_part_of(int u, int UB, int NPART) { return (u * NPART) / UB; }
+// Part of upper block u (row-major over ib <= jb) among NPART parts, the
+// first block of part PART, its block count, and whether it uses feature
+// block fb (fem_tiles.h's upper_part_* with NPART = cov_parts(NB)).
+__device__ __host__ constexpr int cov_part_of(int u, int UB, int NPART) { return upper_part_of(u, UB, NPART); }
 
 __device__ __host__ constexpr bool cov_needs_fb(int NB, int PART, int fb) {
-    const int UB = NB * (NB + 1) / 2, NPART = cov_parts(NB);
-    int u = 0;
-    for (int ib = 0; ib < NB; ib++)
-        for (int jb = ib; jb < NB; jb++, u++)
-            if (cov_part_of(u, UB, NPART) == PART && (ib == fb || jb == fb)) return true;
-    return false;
+    return upper_part_uses(NB, cov_parts(NB), PART, fb, false);
 }
 
-__device__ __host__ constexpr int cov_part_begin(int NB, int PART) {
-    const int UB = NB * (NB + 1) / 2, NPART = cov_parts(NB);
-    for (int u = 0; u < UB; u++)
-        if (cov_part_of(u, UB, NPART) == PART) return u;
-    return UB;
-}
+__device__ __host__ constexpr int cov_part_begin(int NB, int PART) { return upper_part_begin(NB, cov_parts(NB), PART); }
 
-__device__ __host__ constexpr int cov_part_size(int NB, int PART) {
-    const int UB = NB * (NB + 1) / 2, NPART = cov_parts(NB);
-    int s = 0;
-    for (int u = 0; u < UB; u++) s += cov_part_of(u, UB, NPART) == PART;
-    return s;
-}
+__device__ __host__ constexpr int cov_part_size(int NB, int PART) { return upper_part_size(NB, cov_parts(NB), PART); }
 
 // LDS doubles per wave for the FP64 accumulators (0 when they are registers).
 __device__ __host__ constexpr int cov_lds_doubles(int NB) {

@@ -18,6 +18,35 @@ namespace fem {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
+// The upper block pairs (ib <= jb, row-major index u) of one component split
+// into NPART contiguous parts of equal count, one wave each.
+__device__ __host__ constexpr int upper_part_of(int u, int UB, int NPART) { return (u * NPART) / UB; }
+
+__device__ __host__ constexpr int upper_part_begin(int NB, int NPART, int PART) {
+    const int UB = NB * (NB + 1) / 2;
+    for (int u = 0; u < UB; u++)
+        if (upper_part_of(u, UB, NPART) == PART) return u;
+    return UB;
+}
+
+__device__ __host__ constexpr int upper_part_size(int NB, int NPART, int PART) {
+    const int UB = NB * (NB + 1) / 2;
+    int s = 0;
+    for (int u = 0; u < UB; u++) s += upper_part_of(u, UB, NPART) == PART;
+    return s;
+}
+
+// Whether feature block fb is an operand of a block pair of part PART: as
+// the row block ib (row_only) or as either block.
+__device__ __host__ constexpr bool upper_part_uses(int NB, int NPART, int PART, int fb, bool row_only) {
+    const int UB = NB * (NB + 1) / 2;
+    int u = 0;
+    for (int ib = 0; ib < NB; ib++)
+        for (int jb = ib; jb < NB; jb++, u++)
+            if (upper_part_of(u, UB, NPART) == PART && (ib == fb || (!row_only && jb == fb))) return true;
+    return false;
+}
+
 __device__ __host__ constexpr int pairs_of(int NB) {
     int p = 0;
     for (int lb = 0; lb < 4 * NB; lb++) p += NB - lb / 4;
@@ -473,9 +502,9 @@ __device__ inline void means_finalize_unit(int k, int ct, int d, int dpad, int K
 // loads run two steps (16 points) ahead in two register buffers, which
 // measured 10% faster at d <= 48 and 1% at d = 64.
 template <int NB>
-__device__ inline void cov64_unit(int cg, int chunk, const float *__restrict__ X, const double *__restrict__ Rt,
-                                  int64_t npad, int K, int KP, int d, const double *__restrict__ mu,
-                                  const double *__restrict__ nk, int64_t chunk_rows, double *__restrict__ cpart) {
+__device__ inline void cov64_whole(int cg, int chunk, const float *__restrict__ X, const double *__restrict__ Rt,
+                                   int64_t npad, int K, int KP, int d, const double *__restrict__ mu,
+                                   const double *__restrict__ nk, int64_t chunk_rows, double *__restrict__ cpart) {
     constexpr int DP = 16 * NB;
     constexpr int UB = NB * (NB + 1) / 2;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -580,6 +609,86 @@ __device__ inline void cov64_unit(int cg, int chunk, const float *__restrict__ X
     for (int u = 0; u < UB; u++)
 #pragma unroll
         for (int v = 0; v < 4; v++) out[(size_t)u * 256 + (4 * v + g) * 16 + c] = acc[u][v];
+}
+
+// d > 96: the 28 or 36 block pairs' accumulators of a component (224 or 288
+// registers) would leave one wave per SIMD, so the pairs are split over two
+// waves (cov64_parts); part PART forms t only for the feature blocks its
+// pairs use and w t only for its row blocks.
+__device__ __host__ constexpr int cov64_parts(int NB) { return NB >= 7 ? 2 : 1; }
+
+template <int NB, int PART>
+__device__ inline void cov64_part(int cg, int chunk, const float *__restrict__ X, const double *__restrict__ Rt,
+                                  int64_t npad, int K, int KP, int d, const double *__restrict__ mu,
+                                  const double *__restrict__ nk, int64_t chunk_rows, double *__restrict__ cpart) {
+    constexpr int DP = 16 * NB;
+    constexpr int UB = NB * (NB + 1) / 2;
+    constexpr int NPART = cov64_parts(NB);
+    constexpr int U0 = upper_part_begin(NB, NPART, PART);
+    constexpr int NU = upper_part_size(NB, NPART, PART);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int g = lane >> 4, c = lane & 15;
+    const int k = 4 * cg + wave;
+    if (k >= K) return;
+    const int64_t i0 = (int64_t)chunk * chunk_rows;
+    const int64_t i1 = (i0 + chunk_rows < npad) ? i0 + chunk_rows : npad;
+    double muv[NB];
+#pragma unroll
+    for (int fb = 0; fb < NB; fb++) muv[fb] = (NB * c + fb < d) ? mu[(size_t)k * d + NB * c + fb] : 0.0;
+    const double invn = 1.0 / nk[k];
+    v4d acc[NU];
+#pragma unroll
+    for (int u = 0; u < NU; u++) acc[u] = (v4d){0.0, 0.0, 0.0, 0.0};
+    const double *rrow = Rt + (size_t)k * npad + g;
+    const float *xrow = X + g * DP + NB * c;
+    for (int64_t i = i0; i < i1; i += 8) {
+        double rv[2];
+        float xv[2][NB];
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            rv[q] = rrow[i + 4 * q];
+            load_consecutive<NB>(xrow + (i + 4 * q) * DP, xv[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const double wq = rv[q] * invn;
+            double t[NB], wt[NB];
+#pragma unroll
+            for (int fb = 0; fb < NB; fb++) {
+                if (!upper_part_uses(NB, NPART, PART, fb, false)) continue;
+                t[fb] = (double)xv[q][fb] - muv[fb];
+                if (upper_part_uses(NB, NPART, PART, fb, true)) wt[fb] = t[fb] * wq;
+            }
+            int u = 0;
+#pragma unroll
+            for (int ib = 0; ib < NB; ib++)
+#pragma unroll
+                for (int jb = ib; jb < NB; jb++, u++) {
+                    if (upper_part_of(u, UB, NPART) != PART) continue;
+                    acc[u - U0] = __builtin_amdgcn_mfma_f64_16x16x4f64(wt[ib], t[jb], acc[u - U0], 0, 0, 0);
+                }
+        }
+    }
+    double *out = cpart + ((size_t)chunk * KP + k) * UB * 256;
+#pragma unroll
+    for (int u = 0; u < NU; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) out[(size_t)(U0 + u) * 256 + (4 * v + g) * 16 + c] = acc[u][v];
+}
+
+// One workgroup's share of stage (c), FP64: components 4cg..4cg+3 over one
+// chunk of points, and for d > 96 one of the two parts of their block pairs.
+template <int NB>
+__device__ inline void cov64_unit(int cg, int chunk, const float *__restrict__ X, const double *__restrict__ Rt,
+                                  int64_t npad, int K, int KP, int d, const double *__restrict__ mu,
+                                  const double *__restrict__ nk, int64_t chunk_rows, double *__restrict__ cpart,
+                                  int part = 0) {
+    if constexpr (cov64_parts(NB) > 1) {
+        if (part == 0) cov64_part<NB, 0>(cg, chunk, X, Rt, npad, K, KP, d, mu, nk, chunk_rows, cpart);
+        else cov64_part<NB, 1>(cg, chunk, X, Rt, npad, K, KP, d, mu, nk, chunk_rows, cpart);
+    } else {
+        cov64_whole<NB>(cg, chunk, X, Rt, npad, K, KP, d, mu, nk, chunk_rows, cpart);
+    }
 }
 
 // Sum the chunk partials of block pair u = (ib <= jb) of component k (fixed
