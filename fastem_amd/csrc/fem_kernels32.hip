@@ -298,23 +298,22 @@ hipError_t launch_means32_partials(fem_ctx *c) {
 // buffers measured 5% slower there; d <= 48 keeps two, where four measured no
 // faster, and d > 64 one buffer and 16 points ahead).  Same grid and XCD remapping
 // as k_cov64.
-template <int NB>
-__global__ __launch_bounds__(256) void k_cov32(const float *__restrict__ X, const float *__restrict__ Rt,
-                                               int64_t npad, int K, int KP, int d,
-                                               const double *__restrict__ mu,
-                                               const double *__restrict__ nk, int64_t chunk_rows,
-                                               double *__restrict__ cpart) {
+__device__ __host__ constexpr int cov32_parts(int NB) { return NB == 5 || NB == 6 ? 2 : 1; }
+
+template <int NB, int PART>
+__device__ inline void cov32_body(int cg, int chunk, const float *__restrict__ X, const float *__restrict__ Rt,
+                                  int64_t npad, int K, int d, const double *__restrict__ mu,
+                                  const double *__restrict__ nk, int64_t chunk_rows, int KP,
+                                  double *__restrict__ cpart) {
     constexpr int DP = 16 * NB;
     constexpr int UB = NB * (NB + 1) / 2;
+    constexpr int NPART = cov32_parts(NB);
+    constexpr int U0 = upper_part_begin(NB, NPART, PART);
+    constexpr int NU = upper_part_size(NB, NPART, PART);
     constexpr int FLUSH = 2048;
     constexpr bool REG = NB <= 4;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int g = lane >> 4, c = lane & 15;
-    const int KG = KP / 4;
-    const int nblk = gridDim.x;
-    const int b = blockIdx.x;
-    const int logical = (nblk % 8 == 0) ? (b % 8) * (nblk / 8) + b / 8 : b;
-    const int cg = logical % KG, chunk = logical / KG;
     const int k = 4 * cg + wave;
     if (k >= K) return;
     const int64_t i0 = (int64_t)chunk * chunk_rows;
@@ -326,10 +325,10 @@ __global__ __launch_bounds__(256) void k_cov32(const float *__restrict__ X, cons
     double *out = cpart + ((size_t)chunk * KP + k) * UB * 256;
     const float *rrow = Rt + (size_t)k * npad + 4 * g;
     const float *xrow = X + 4 * g * DP + NB * c;
-    double accd[REG ? UB : 1][4];
+    double accd[REG ? NU : 1][4];
     if (REG) {
 #pragma unroll
-        for (int u = 0; u < (REG ? UB : 1); u++)
+        for (int u = 0; u < (REG ? NU : 1); u++)
 #pragma unroll
             for (int v = 0; v < 4; v++) accd[u][v] = 0.0;
     }
@@ -350,9 +349,9 @@ __global__ __launch_bounds__(256) void k_cov32(const float *__restrict__ X, cons
     bool first = true;
     for (int64_t s0 = i0; s0 < i1; s0 += FLUSH) {
         const int64_t s1 = (s0 + FLUSH < i1) ? s0 + FLUSH : i1;
-        v4f acc[UB];
+        v4f acc[NU];
 #pragma unroll
-        for (int u = 0; u < UB; u++) acc[u] = (v4f){0.f, 0.f, 0.f, 0.f};
+        for (int u = 0; u < NU; u++) acc[u] = (v4f){0.f, 0.f, 0.f, 0.f};
         for (int64_t i = s0; i < s1; i += 16 * H) {
 #pragma unroll
             for (int h = 0; h < H; h++) {
@@ -361,7 +360,8 @@ __global__ __launch_bounds__(256) void k_cov32(const float *__restrict__ X, cons
                 for (int q = 0; q < 4; q++) {
                     const float sw = __builtin_amdgcn_sqrtf(rv[h][q]) * inv;
 #pragma unroll
-                    for (int fb = 0; fb < NB; fb++) t[q][fb] = (xv[h][q][fb] - mh[fb]) * sw;
+                    for (int fb = 0; fb < NB; fb++)
+                        if (upper_part_uses(NB, NPART, PART, fb, false)) t[q][fb] = (xv[h][q][fb] - mh[fb]) * sw;
                 }
                 // unconditional (past the chunk end it re-reads the last 16
                 // points, unused), so the buffers need no register copies
@@ -373,25 +373,26 @@ __global__ __launch_bounds__(256) void k_cov32(const float *__restrict__ X, cons
 #pragma unroll
                     for (int ib = 0; ib < NB; ib++)
 #pragma unroll
-                        for (int jb = ib; jb < NB; jb++) {
-                            acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(t[q][ib], t[q][jb], acc[u], 0, 0, 0);
-                            u++;
+                        for (int jb = ib; jb < NB; jb++, u++) {
+                            if (upper_part_of(u, UB, NPART) != PART) continue;
+                            acc[u - U0] =
+                                __builtin_amdgcn_mfma_f32_16x16x4f32(t[q][ib], t[q][jb], acc[u - U0], 0, 0, 0);
                         }
                 }
             }
         }
         if (REG) {
 #pragma unroll
-            for (int u = 0; u < (REG ? UB : 1); u++)
+            for (int u = 0; u < (REG ? NU : 1); u++)
 #pragma unroll
                 for (int v = 0; v < 4; v++) accd[u][v] += (double)acc[u][v];
         } else {
             // each wave owns its partial slice: plain read-add-write
 #pragma unroll
-            for (int u = 0; u < UB; u++)
+            for (int u = 0; u < NU; u++)
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
-                    double *o = out + (size_t)u * 256 + (4 * g + v) * 16 + c;
+                    double *o = out + (size_t)(U0 + u) * 256 + (4 * g + v) * 16 + c;
                     *o = (first ? 0.0 : *o) + (double)acc[u][v];
                 }
         }
@@ -399,15 +400,38 @@ __global__ __launch_bounds__(256) void k_cov32(const float *__restrict__ X, cons
     }
     if (REG) {
 #pragma unroll
-        for (int u = 0; u < (REG ? UB : 1); u++)
+        for (int u = 0; u < (REG ? NU : 1); u++)
 #pragma unroll
-            for (int v = 0; v < 4; v++) out[(size_t)u * 256 + (4 * g + v) * 16 + c] = accd[u][v];
+            for (int v = 0; v < 4; v++) out[(size_t)(U0 + u) * 256 + (4 * g + v) * 16 + c] = accd[u][v];
     }
+}
+
+// For d = 65..96 the upper blocks of a component are split over two waves
+// (cov32_parts; the part varies fastest in the grid), so that each wave's
+// accumulators and FP64 flushes fit two waves per SIMD (16-18% faster at
+// d = 80, 96).  At d > 96 a half still needs the whole register file and
+// the split measured 15-20% slower (more X reads and centring), so those
+// keep one wave per component.
+template <int NB>
+__global__ __launch_bounds__(256) void k_cov32(const float *__restrict__ X, const float *__restrict__ Rt,
+                                               int64_t npad, int K, int KP, int d,
+                                               const double *__restrict__ mu,
+                                               const double *__restrict__ nk, int64_t chunk_rows,
+                                               double *__restrict__ cpart) {
+    const int KG = KP / 4;
+    const int nblk = gridDim.x;
+    const int b = blockIdx.x;
+    const int logical = (nblk % 8 == 0) ? (b % 8) * (nblk / 8) + b / 8 : b;
+    constexpr int NPART = cov32_parts(NB);
+    const int part = logical % NPART, rest = logical / NPART;
+    const int cg = rest % KG, chunk = rest / KG;
+    if (part == 0) cov32_body<NB, 0>(cg, chunk, X, Rt, npad, K, d, mu, nk, chunk_rows, KP, cpart);
+    else if constexpr (NPART > 1) cov32_body<NB, 1>(cg, chunk, X, Rt, npad, K, d, mu, nk, chunk_rows, KP, cpart);
 }
 
 template <int NB>
 static hipError_t cov32_t(fem_ctx *c) {
-    const int blocks = (c->KP / 4) * c->c_chunks;
+    const int blocks = (c->KP / 4) * c->c_chunks * cov32_parts(NB);
     hipLaunchKernelGGL(k_cov32<NB>, dim3(blocks), dim3(256), 0, c->stream, c->X, c->Rt32, c->npad, c->K,
                        c->KP, c->d, c->mu, c->nk, c->c_chunk_rows, c->cpart);
     return hipGetLastError();
