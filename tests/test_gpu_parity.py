@@ -73,7 +73,8 @@ def test_factor_not_pd(orc):
 
 @pytest.mark.parametrize("n,d,K", [(1000, 3, 2), (5003, 17, 7), (4096, 16, 8), (3000, 64, 5),
                                    (2500, 128, 3), (777, 1, 1), (9000, 8, 64), (4099, 2, 16),
-                                   (3001, 90, 4), (2600, 100, 3), (2200, 112, 2), (6000, 12, 100)])
+                                   (3001, 90, 4), (2600, 100, 3), (2200, 112, 2), (6000, 12, 100),
+                                   (2800, 72, 3), (2300, 80, 6), (2400, 97, 3), (2100, 120, 2)])
 def test_estep_mstep_parity(orc, n, d, K):
     say(f"estep/mstep n={n} d={d} K={K}")
     X, _, _, _ = mixture(n, d, K, seed=n + d + K)
