@@ -10,7 +10,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libfastem.so")
 
-HIP_SOURCES = ["fem_kernels.hip", "fem_cov64.hip", "fem_kernels32.hip", "fem_kernels_bf16.hip", "fem_kmeans.hip",
+HIP_SOURCES = ["fem_kernels.hip", "fem_cov64.hip", "fem_cov64_narrow.hip", "fem_kernels32.hip", "fem_kernels_bf16.hip", "fem_kmeans.hip",
                "fem_batch.hip", "fem_setnet.hip", "fem_abi.hip"]
 HOST_SOURCES = ["fem_tools.cpp", "host/linalg.cpp", "host/rand.cpp", "host/block_manager.cpp",
                 "host/normal_density.cpp", "host/gaussian_mixture.cpp", "host/dataset.cpp",
@@ -18,9 +18,9 @@ HOST_SOURCES = ["fem_tools.cpp", "host/linalg.cpp", "host/rand.cpp", "host/block
 TOOLS = {"em2": "host/em2_main.cpp", "eminit": "host/eminit_main.cpp"}
 # MFMA accumulators in VGPRs (no accumulator-read moves when results are
 # squared, reduced or flushed) for every file but fem_cov64.hip, whose FP64
-# covariance kernel measured faster with the default AGPR form.
+# covariance kernel for d > 48 measured faster with the default AGPR form.
 VGPR_FORM = ["-mllvm", "-amdgpu-mfma-vgpr-form"]
-FILE_FLAGS = {"fem_kernels.hip": VGPR_FORM, "fem_kernels32.hip": VGPR_FORM, "fem_kernels_bf16.hip": VGPR_FORM,
+FILE_FLAGS = {"fem_cov64_narrow.hip": VGPR_FORM, "fem_kernels.hip": VGPR_FORM, "fem_kernels32.hip": VGPR_FORM, "fem_kernels_bf16.hip": VGPR_FORM,
               "fem_kmeans.hip": VGPR_FORM, "fem_batch.hip": VGPR_FORM, "fem_setnet.hip": VGPR_FORM}
 
 # Packed FP32 VALU ops are disabled for all device code.

@@ -102,6 +102,7 @@ hipError_t copy_rows_d2h(void *dst, size_t dpitch, const void *src, size_t spitc
 hipError_t launch_means64_partials(fem_ctx *c);
 hipError_t launch_means_finalize(fem_ctx *c, double *mu_out, double *nk_out, int raw);
 hipError_t launch_cov64_partials(fem_ctx *c);
+hipError_t launch_cov64_narrow(fem_ctx *c);  // NB <= 3 (fem_cov64_narrow.hip)
 hipError_t launch_cov_finalize(fem_ctx *c, double g);
 // FP32 form (fem_kernels32.hip)
 hipError_t launch_estep32(fem_ctx *c);
