@@ -298,7 +298,7 @@ hipError_t launch_means32_partials(fem_ctx *c) {
 // buffers measured 5% slower there; d <= 48 keeps two, where four measured no
 // faster, and d > 64 one buffer and 16 points ahead).  Same grid and XCD remapping
 // as k_cov64.
-__device__ __host__ constexpr int cov32_parts(int NB) { return NB == 5 || NB == 6 ? 2 : 1; }
+__device__ __host__ constexpr int cov32_parts(int NB) { return NB > 4 ? 2 : 1; }
 
 template <int NB, int PART>
 __device__ inline void cov32_body(int cg, int chunk, const float *__restrict__ X, const float *__restrict__ Rt,
@@ -387,14 +387,19 @@ __device__ inline void cov32_body(int cg, int chunk, const float *__restrict__ X
 #pragma unroll
                 for (int v = 0; v < 4; v++) accd[u][v] += (double)acc[u][v];
         } else {
-            // each wave owns its partial slice: plain read-add-write
+            // each wave owns its partial slice: plain read-add-write, four
+            // blocks at a time (the compiler barrier keeps the compiler from
+            // loading every block's partials at once, which would take the
+            // register file and leave one wave per SIMD)
 #pragma unroll
-            for (int u = 0; u < NU; u++)
+            for (int u = 0; u < NU; u++) {
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
                     double *o = out + (size_t)(U0 + u) * 256 + (4 * g + v) * 16 + c;
                     *o = (first ? 0.0 : *o) + (double)acc[u][v];
                 }
+                if ((u & 3) == 3) asm volatile("" ::: "memory");
+            }
         }
         first = false;
     }
@@ -406,14 +411,13 @@ __device__ inline void cov32_body(int cg, int chunk, const float *__restrict__ X
     }
 }
 
-// For d = 65..96 the upper blocks of a component are split over two waves
-// (cov32_parts; the part varies fastest in the grid), so that each wave's
-// accumulators and FP64 flushes fit two waves per SIMD (16-18% faster at
-// d = 80, 96).  At d > 96 a half still needs the whole register file and
-// the split measured 15-20% slower (more X reads and centring), so those
-// keep one wave per component.
+// For d > 64 the upper blocks of a component are split over two waves
+// (cov32_parts; the part varies fastest in the grid), and with the flush
+// taken four blocks at a time each wave fits in half the register file
+// (capped for two waves per SIMD): 20% faster at d = 80 and 96, 14% at
+// d = 112 and 4% at d = 128 than one wave per component.
 template <int NB>
-__global__ __launch_bounds__(256) void k_cov32(const float *__restrict__ X, const float *__restrict__ Rt,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_cov32(const float *__restrict__ X, const float *__restrict__ Rt,
                                                int64_t npad, int K, int KP, int d,
                                                const double *__restrict__ mu,
                                                const double *__restrict__ nk, int64_t chunk_rows,
