@@ -526,10 +526,14 @@ __device__ inline void covb_wave(const float *__restrict__ X, const float *__res
 #pragma unroll
                 for (int v = 0; v < 4; v++) accd[u][v] += (double)acc[u][v];
         } else {
+            // four blocks at a time (the compiler barrier keeps it from
+            // loading every block's FP64 sums at once)
 #pragma unroll
-            for (int u = 0; u < NU; u++)
+            for (int u = 0; u < NU; u++) {
 #pragma unroll
                 for (int v = 0; v < 4; v++) lacc[(u * 4 + v) * 64 + lane] += (double)acc[u][v];
+                if ((u & 3) == 3) asm volatile("" ::: "memory");
+            }
         }
     }
 #pragma unroll
