@@ -62,7 +62,7 @@ struct fem_ctx {
 
     // reductions
     int est_blocks = 0;         // capacity of ll_part (one per E-step workgroup)
-    int estep_groups = 2;       // 16-point groups per wave in the FP64 E-step (2 or 4)
+    int estep_groups = 2;       // 16-point groups per wave in the FP64 E-step (2 or 4; 4 for d <= 32)
     int estep_groups32 = 2;     // same for the FP32 E-step
     int estep_groupsb = 4;      // same for the BF16x3 E-step (4 measured faster at d <= 64)
     double *ll_part = nullptr;  // per E-step workgroup
