@@ -129,7 +129,7 @@ __global__ __launch_bounds__(256) void k_means64(const float *__restrict__ X,
 }
 
 // raw != 0 writes the sums M_k themselves (k-means centroid sums).
-__global__ __launch_bounds__(128) void k_means_finalize(int K, int d, int dpad, int KP, int chunks,
+__global__ __launch_bounds__(256) void k_means_finalize(int K, int d, int dpad, int KP, int chunks,
                                                         const double *__restrict__ mpart,
                                                         double *__restrict__ mu,
                                                         double *__restrict__ nk, int raw) {
@@ -158,7 +158,7 @@ hipError_t launch_means64_partials(fem_ctx *c) {
 }
 
 hipError_t launch_means_finalize(fem_ctx *c, double *mu_out, double *nk_out, int raw) {
-    hipLaunchKernelGGL(k_means_finalize, dim3(c->K, (c->d + 127) / 128), dim3(128), 0, c->stream, c->K,
+    hipLaunchKernelGGL(k_means_finalize, dim3(c->K, (c->d + 63) / 64), dim3(256), 0, c->stream, c->K,
                        c->d, c->dpad, c->KP, c->m_chunks, c->mpart, mu_out, nk_out, raw);
     return hipGetLastError();
 }

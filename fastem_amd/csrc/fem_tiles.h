@@ -472,21 +472,30 @@ __device__ inline void means64_unit(int chunk, const float *__restrict__ X, cons
     }
 }
 
-// mu_k (or the raw sums M_k when raw != 0) and N_k from the chunk partials;
-// thread `tid` of column tile `ct` handles column j = 128 ct + tid (128
-// threads per unit).
+// mu_k (or the raw sums M_k when raw != 0) and N_k from the chunk partials
+// for the 64 columns of column tile ct (256 threads per unit).
 __device__ inline void means_finalize_unit(int k, int ct, int d, int dpad, int KP, int chunks,
                                            const double *__restrict__ mpart, double *__restrict__ mu,
                                            double *__restrict__ nk, int raw) {
-    const int tid = threadIdx.x;
+    // 256 threads: wave w sums its quarter of the chunks for features
+    // ct*64 + lane (lane 0 also the N_k column), then the quarters are added
+    // in a fixed order; the caller's whole block calls this together
+    __shared__ double quarter[4][65];
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const size_t stride = (size_t)KP * (dpad + 1);
-    const double sn = chunk_sum4(mpart + (size_t)k * (dpad + 1) + dpad, stride, chunks);
-    if (ct == 0 && tid == 0) nk[k] = sn;
-    const int j = ct * 128 + tid;
-    if (tid < 128 && j < d) {
-        const double s = chunk_sum4(mpart + (size_t)k * (dpad + 1) + j, stride, chunks);
-        mu[(size_t)k * d + j] = raw ? s : s / sn;
+    const int c0 = (chunks * w) / 4, c1 = (chunks * (w + 1)) / 4;
+    const double *col = mpart + (size_t)k * (dpad + 1) + (size_t)c0 * stride;
+    const int j = ct * 64 + lane;
+    quarter[w][lane] = j < d ? chunk_sum4(col + j, stride, c1 - c0) : 0.0;
+    if (lane == 0) quarter[w][64] = chunk_sum4(col + dpad, stride, c1 - c0);
+    __syncthreads();
+    if (w == 0) {
+        const double sn = (quarter[0][64] + quarter[1][64]) + (quarter[2][64] + quarter[3][64]);
+        if (ct == 0 && lane == 0) nk[k] = sn;
+        const double s = (quarter[0][lane] + quarter[1][lane]) + (quarter[2][lane] + quarter[3][lane]);
+        if (j < d) mu[(size_t)k * d + j] = raw ? s : s / sn;
     }
+    __syncthreads();  // quarter is reused by the caller's next call
 }
 
 // ---------------------------------------------------------------------------
