@@ -15,6 +15,7 @@ EM2_PATH = os.path.join(_PKG, "em2")
 EMINIT_PATH = os.path.join(_PKG, "eminit")
 
 PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
+SEED_FORGY, SEED_KMEANSPP, SEED_KMEANSOROR = 0, 1, 2
 FEM_OK, FEM_ERR_ARG, FEM_ERR_HIP, FEM_ERR_NOT_PD, FEM_ERR_STATE = 0, -1, -2, -3, -4
 
 _lib = None
@@ -203,8 +204,9 @@ class Context:
         return R
 
     def kmeans_init(self, seed, seeding=1, max_sweeps=29):
-        """k-means++ (seeding=1) or Forgy (0) seeding, Lloyd sweeps and the
-        mixture of the clusters on the device data; becomes the parameters."""
+        """Forgy (SEED_FORGY), k-means++ (SEED_KMEANSPP) or k-means||
+        (SEED_KMEANSOROR) seeding, Lloyd sweeps and the mixture of the
+        clusters on the device data; becomes the parameters."""
         w = np.empty(self.K)
         mu = np.empty((self.K, self.d))
         cov = np.empty((self.K, self.d, self.d))

@@ -4,6 +4,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
@@ -43,18 +44,31 @@ def _run(cmd):
     subprocess.check_call(cmd)
 
 
+def _jobs():
+    try:
+        n = int(os.environ.get("MAX_JOBS", "0"))
+    except ValueError:
+        n = 0
+    return max(1, min(n or (os.cpu_count() or 1), 16))
+
+
 def build(verbose=True):
     objdir = os.path.join(PKG, "_objs")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
+    cmds, objs = [], []
     for s in HIP_SOURCES + HOST_SOURCES:
         src = os.path.join(CSRC, s)
         obj = os.path.join(objdir, os.path.splitext(s.replace("/", "_"))[0] + ".o")
         flags = list(HIP_FLAGS) + FILE_FLAGS.get(s, [])
         if s.endswith(".cpp"):
             flags += ["-fopenmp"]
-        _run([hipcc()] + flags + ["-I", os.path.join(ROOT, "include"), "-c", src, "-o", obj])
+        cmds.append([hipcc()] + flags + ["-I", os.path.join(ROOT, "include"), "-c", src, "-o", obj])
         objs.append(obj)
+    # the largest translation units first, so the pool drains evenly
+    order = sorted(range(len(cmds)), key=lambda i: -os.path.getsize(cmds[i][-3]))
+    with ThreadPoolExecutor(_jobs()) as pool:
+        for f in [pool.submit(_run, cmds[i]) for i in order]:
+            f.result()
     tmp = LIB + ".tmp"
     _run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-fopenmp", "-o", tmp] + objs)
     os.replace(tmp, LIB)

@@ -1,6 +1,7 @@
 // fem_kmeans.hip -- k-means initialisation on the device data of a context:
-// k-means++ seeding (Clust/PlusPlusInitializer/PlusPlusInitializer.cpp:33-84),
-// Lloyd sweeps (Clust/KMAlgorithm.cpp:44-87,205-255) and the mixture of the
+// Forgy seeding (Clust/KMeansInitializer.cpp:33-49), k-means++ seeding
+// (Clust/PlusPlusInitializer/PlusPlusInitializer.cpp:33-84) or k-means||
+// seeding (Clust/KMeansOrOrInitializer.cpp:143-174), Lloyd sweeps (Clust/KMAlgorithm.cpp:44-87,205-255) and the mixture of the
 // final clustering (EM2/EMInitializer.cpp:103-134).
 //
 // Squared distances are FP64 sums in ascending feature order with no
@@ -105,6 +106,61 @@ __global__ __launch_bounds__(256) void k_assign(const float *__restrict__ X, int
         __syncthreads();
     }
     if (threadIdx.x == 0) fit_part[blockIdx.x] = red[0];
+}
+
+// Squared distance of two float rows as Clust/CentroidVector.h:79-87 forms
+// it: each difference and square rounded to float, the sum in double,
+// ascending feature order.
+template <int DP>
+__device__ inline double sqdist_ff(const float *c, const float (&x)[DP], int d) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < DP; j++) {
+        if (j < d) {
+            const float t = x[j] - c[j];
+            const float t2 = t * t;
+            s = s + (double)t2;
+        }
+    }
+    return s;
+}
+
+// k-means|| closest-center update (Clust/KMeansOrOrInitializer.cpp:38-70):
+// nc new float centers (row-major, stride d) staged in LDS, numbered from
+// `base`; a row moves to a center only if strictly closer, centers in
+// order, so ties keep the earlier center.  reset starts from +inf / -1.
+template <int DP>
+__global__ __launch_bounds__(256) void k_oror_update(const float *__restrict__ X, int64_t n, int dpad, int d,
+                                                     const float *__restrict__ cen, int nc, int base,
+                                                     double *__restrict__ cd, int *__restrict__ cc, int reset) {
+    extern __shared__ float lcf[];  // nc * d
+    for (int e = threadIdx.x; e < nc * d; e += 256) lcf[e] = cen[e];
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float x[DP];
+    load_row<DP>(X, i, dpad, x);
+    double best = reset ? __builtin_inf() : cd[i];
+    int bi = reset ? -1 : cc[i];
+    for (int j = 0; j < nc; j++) {
+        const double s = sqdist_ff<DP>(lcf + (size_t)j * d, x, d);
+        if (s < best) {
+            best = s;
+            bi = base + j;
+        }
+    }
+    cd[i] = best;
+    cc[i] = bi;
+}
+
+// rows idx[0..m) of X packed densely (m x d floats)
+__global__ __launch_bounds__(256) void k_gather_rows(const float *__restrict__ X, int dpad, int d,
+                                                     const int64_t *__restrict__ idx, int m,
+                                                     float *__restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)m * d) return;
+    const int64_t r = e / d, q = e % d;
+    out[e] = X[idx[r] * dpad + q];
 }
 #pragma clang fp contract(on)
 
@@ -247,6 +303,128 @@ int lloyd_and_mixture(fem_ctx *c, std::vector<double> &C, Draw &&draw, int max_s
     return rc;
 }
 
+#pragma clang fp contract(off)
+// k-means|| (Clust/KMeansOrOrInitializer.cpp:143-174) with oversampling 1.4
+// and 5 rounds (EM2/EMInitializer.cpp:158-161), in the order of a one-thread
+// run of the reference: one draw per row per round in row order, selected
+// rows become centers in row order.  The closest-center distances and
+// indices are kept on the device; each round's cost is summed sequentially
+// on the host from a copy of the distances, as are the draws (the shared
+// Rand() stream).  The candidate centers, weighted by how many rows each is
+// closest to, then go through the weighted k-means++ of
+// Clust/PlusPlusInitializer/PlusPlusInitializer.cpp:23-84 on the host (a few
+// hundred candidates).  Writes K x d centroids into C.
+template <typename Draw>
+void kmeansoror_seed(fem_ctx *c, std::vector<double> &C, Draw &&draw) {
+    const int K = c->K, d = c->d;
+    const int64_t n = c->n;
+    const double os = 1.4;
+    const int rounds = os <= 0.1 ? 10 : 5;
+    const int chunk = std::max(1, 16384 / d);  // centers per launch: 64 KB of LDS
+    const int blocks = (int)((n + 255) / 256);
+    struct Dev {
+        void *p = nullptr;
+        ~Dev() {
+            if (p) (void)hipFree(p);
+        }
+    } dcen, didx;
+    KM_HIP(hipMalloc(&dcen.p, sizeof(float) * (size_t)chunk * d));
+    KM_HIP(hipMalloc(&didx.p, sizeof(int64_t) * (size_t)chunk));
+    double *cd = nullptr;
+    KM_HIP(hipHostMalloc((void **)&cd, sizeof(double) * n, hipHostMallocDefault));
+    struct Pinned {
+        double *p;
+        ~Pinned() { (void)hipHostFree(p); }
+    } pin{cd};
+    std::vector<int64_t> rows;
+    // Inserts rows[from, to) as centers numbered from `from`.
+    auto insert = [&](size_t from, size_t to, int reset) {
+        for (size_t b = from; b < to; b += chunk) {
+            const int nc = (int)std::min<size_t>(chunk, to - b);
+            KM_HIP(hipMemcpyAsync(didx.p, rows.data() + b, sizeof(int64_t) * nc, hipMemcpyHostToDevice,
+                                  c->stream));
+            hipLaunchKernelGGL(fem::k_gather_rows, dim3((nc * d + 255) / 256), dim3(256), 0, c->stream, c->X,
+                               c->dpad, d, (const int64_t *)didx.p, nc, (float *)dcen.p);
+            KM_HIP(hipGetLastError());
+            KM_HIP(fem::by_dpad(c->dpad, [&](auto DPc) -> hipError_t {
+                constexpr int DP = decltype(DPc)::value;
+                hipLaunchKernelGGL(fem::k_oror_update<DP>, dim3(blocks), dim3(256), sizeof(float) * nc * d,
+                                   c->stream, c->X, n, c->dpad, d, (const float *)dcen.p, nc, (int)b, c->dmin,
+                                   c->cls, reset && b == from ? 1 : 0);
+                return hipGetLastError();
+            }));
+        }
+    };
+    auto cost_of = [&]() {
+        KM_HIP(hipMemcpyAsync(cd, c->dmin, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+        KM_HIP(hipStreamSynchronize(c->stream));
+        double s = 0.0;
+        for (int64_t i = 0; i < n; i++) s += cd[i];
+        return s;
+    };
+    rows.push_back((int64_t)(draw() * (double)n));
+    insert(0, 1, 1);
+    double cost = cost_of();
+    for (int r = 0; r < rounds; r++) {
+        const size_t base = rows.size();
+        for (int64_t i = 0; i < n; i++) {
+            const double prob = os * (double)K * cd[i] / cost;
+            if (draw() < prob) rows.push_back(i);
+        }
+        insert(base, rows.size(), 0);
+        cost = cost_of();
+    }
+    // weights: rows closest to each candidate (Clust/KMeansOrOrInitializer.cpp:114-126)
+    const int64_t m = (int64_t)rows.size();
+    std::vector<int> cls(n);
+    KM_HIP(hipMemcpyAsync(cls.data(), c->cls, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
+    KM_HIP(hipStreamSynchronize(c->stream));
+    std::vector<double> wt(m, 0.0);
+    for (int64_t i = 0; i < n; i++) wt[cls[i]] += 1.0;
+    // candidate rows to the host
+    std::vector<float> cand((size_t)m * d);
+    for (int64_t b = 0; b < m; b += chunk) {
+        const int nc = (int)std::min<int64_t>(chunk, m - b);
+        KM_HIP(hipMemcpyAsync(didx.p, rows.data() + b, sizeof(int64_t) * nc, hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(fem::k_gather_rows, dim3((nc * d + 255) / 256), dim3(256), 0, c->stream, c->X, c->dpad,
+                           d, (const int64_t *)didx.p, nc, (float *)dcen.p);
+        KM_HIP(hipGetLastError());
+        KM_HIP(hipMemcpyAsync(cand.data() + (size_t)b * d, dcen.p, sizeof(float) * nc * d, hipMemcpyDeviceToHost,
+                              c->stream));
+        KM_HIP(hipStreamSynchronize(c->stream));
+    }
+    // weighted k-means++ over the candidates; distances: double centroid
+    // minus float row, ascending feature order
+    std::vector<double> cd2(m, std::numeric_limits<double>::infinity()), agg(m);
+    const int64_t r0 = (int64_t)(draw() * (double)m);
+    for (int j = 0; j < d; j++) C[j] = (double)cand[(size_t)r0 * d + j];
+    for (int k = 1; k < K; k++) {
+        const double *pc = C.data() + (size_t)(k - 1) * d;
+        for (int64_t j = 0; j < m; j++) {
+            double s = 0.0;
+            for (int q = 0; q < d; q++) {
+                const double t = pc[q] - (double)cand[(size_t)j * d + q];
+                s += t * t;
+            }
+            if (cd2[j] > s) cd2[j] = s;
+        }
+        double sum = 0.0;
+        for (int64_t j = 0; j < m; j++) {
+            sum += cd2[j] * wt[j];
+            agg[j] = sum;
+        }
+        for (int64_t j = 0; j < m; j++) agg[j] /= sum;
+        const double rr = draw();
+        for (int64_t j = 0; j < m; j++) {
+            if (rr <= agg[j]) {
+                for (int q = 0; q < d; q++) C[(size_t)k * d + q] = (double)cand[(size_t)j * d + q];
+                break;
+            }
+        }
+    }
+}
+#pragma clang fp contract(on)
+
 }  // namespace
 
 extern "C" int fem_kmeans_init_rng(fem_ctx *c, double (*rnd)(void *), void *rnd_state, int seeding,
@@ -263,7 +441,8 @@ extern "C" int fem_kmeans_init(fem_ctx *c, uint64_t seed, int seeding, int max_s
 extern "C" int fem_kmeans_init_rng(fem_ctx *c, double (*rnd)(void *), void *rnd_state, int seeding,
                                    int max_sweeps, double *w, double *mu, double *cov) {
     if (!c) return FEM_ERR_ARG;
-    if (!rnd || !w || !mu || !cov || max_sweeps < 0 || (seeding != 0 && seeding != 1)) {
+    if (!rnd || !w || !mu || !cov || max_sweeps < 0 ||
+        (seeding != FEM_SEED_FORGY && seeding != FEM_SEED_KMEANSPP && seeding != FEM_SEED_KMEANSOROR)) {
         c->err = "fem_kmeans_init: bad argument";
         return FEM_ERR_ARG;
     }
@@ -274,7 +453,7 @@ extern "C" int fem_kmeans_init_rng(fem_ctx *c, double (*rnd)(void *), void *rnd_
         const int64_t n = c->n;
         auto draw = [&]() { return rnd(rnd_state); };
         std::vector<double> C((size_t)K * d, 0.0);
-        if (seeding == 1) {
+        if (seeding == FEM_SEED_KMEANSPP) {
             // Clust/PlusPlusInitializer/PlusPlusInitializer.cpp:33-84
             const int64_t r0 = (int64_t)(draw() * (double)n);
             fetch_row(c, r0, C.data());
@@ -315,28 +494,15 @@ extern "C" int fem_kmeans_init_rng(fem_ctx *c, double (*rnd)(void *), void *rnd_
                 }
                 if (lo < n) fetch_row(c, lo, C.data() + (size_t)k * d);
             }
-        } else {
-            // Forgy random partition (Clust/KMeansInitializer.cpp:33-49): one
-            // Rand() per row in row order; partition sums by the means pass.
-            std::vector<int> part(n);
-            for (int64_t i = 0; i < n; i++) part[i] = (int)(draw() * K);
-            std::vector<double> onehot_col(c->npad);
-            for (int k = 0; k < c->KP; k++) {
-                for (int64_t i = 0; i < c->npad; i++) onehot_col[i] = (i < n && part[i] == k) ? 1.0 : 0.0;
-                KM_HIP(hipMemcpyAsync(c->Rt + (size_t)k * c->npad, onehot_col.data(), sizeof(double) * c->npad,
-                                      hipMemcpyHostToDevice, c->stream));
-                KM_HIP(hipStreamSynchronize(c->stream));
-            }
-            std::vector<double> sums((size_t)K * d), counts(K);
-            KM_HIP(fem::launch_means64_partials(c));
-            KM_HIP(fem::launch_means_finalize(c, c->kC, c->nk, 1));
-            KM_HIP(hipMemcpyAsync(sums.data(), c->kC, sizeof(double) * K * d, hipMemcpyDeviceToHost, c->stream));
-            KM_HIP(hipMemcpyAsync(counts.data(), c->nk, sizeof(double) * K, hipMemcpyDeviceToHost, c->stream));
-            KM_HIP(hipStreamSynchronize(c->stream));
+        } else if (seeding == FEM_SEED_FORGY) {
+            // Clust/KMeansInitializer.cpp:33-49: K rows at (long)(Rand()*N),
+            // in component order, repeats allowed
             for (int k = 0; k < K; k++) {
-                const double f = 1.0 / counts[k];
-                for (int j = 0; j < d; j++) C[(size_t)k * d + j] = sums[(size_t)k * d + j] * f;
+                const int64_t r = (int64_t)(draw() * (double)n);
+                fetch_row(c, r, C.data() + (size_t)k * d);
             }
+        } else {
+            kmeansoror_seed(c, C, draw);
         }
         return lloyd_and_mixture(c, C, draw, max_sweeps, w, mu, cov);
     } catch (KmErr &e) {

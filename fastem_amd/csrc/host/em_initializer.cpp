@@ -67,14 +67,15 @@ void FileEMInitializer::Init(GaussianMixture &G) {
 
 static double global_rand(void *) { return Rand(); }
 
-void KMeansEMInitializer::Seed(DynamicArray<OPTFLOAT> &) {}
+int KMeansEMInitializer::Seeding() const { return FEM_SEED_FORGY; }
 
 void KMeansEMInitializer::PrintInfo() { printf("K-means EM initializer\n"); }
 
-// Forgy seeding, NaiveKMA::RunKMeans(vec, 0, 1e-4, 30), classification and the
-// per-cluster mixture (EM2/EMInitializer.cpp:103-134).
+// The seeding, NaiveKMA::RunKMeans(vec, 0, 1e-4, 30), classification and the
+// per-cluster mixture (EM2/EMInitializer.cpp:103-134), on the global Rand()
+// stream.
 void KMeansEMInitializer::Init(GaussianMixture &G) {
-    const int seeding = dynamic_cast<KMeansPPEMInitializer *>(this) ? 1 : 0;
+    const int seeding = Seeding();
     std::vector<double> w(Components), mu((size_t)Components * ncols),
         cov((size_t)Components * ncols * ncols);
     fem_ctx *c = nullptr;
@@ -94,13 +95,18 @@ void KMeansEMInitializer::Init(GaussianMixture &G) {
     G.Init(Weights, Covs, Means);
 }
 
-void KMeansPPEMInitializer::Seed(DynamicArray<OPTFLOAT> &) {}
+int KMeansPPEMInitializer::Seeding() const { return FEM_SEED_KMEANSPP; }
 
 void KMeansPPEMInitializer::PrintInfo() { printf("K-means++ EM initializer\n"); }
+
+int KMeansOrOrEMInitializer::Seeding() const { return FEM_SEED_KMEANSOROR; }
+
+void KMeansOrOrEMInitializer::PrintInfo() { printf("K-means|| EM initializer\n"); }
 
 EMInitializer *CreateEMInitializer(const char *name, MultithreadedDataset &Data, int nC) {
     if (name == nullptr || !std::strcmp(name, "jain")) return new JainEMInitializer(Data, nC);
     if (!std::strcmp(name, "kmeans")) return new KMeansEMInitializer(Data, nC);
+    if (!std::strcmp(name, "kmeansoror")) return new KMeansOrOrEMInitializer(Data, nC);
     if (!std::strcmp(name, "kmeanspp")) return new KMeansPPEMInitializer(Data, nC);
     return new FileEMInitializer(Data, nC, name);
 }

@@ -1,6 +1,7 @@
 // EMInitializer.h -- initial mixtures (EM2/EMInitializer.h:16-80).
 // CreateEMInitializer names: "jain" (default), "kmeans" (Forgy + Lloyd),
-// "kmeanspp" (k-means++ + Lloyd), or a path to a .cmp file.
+// "kmeansoror" (k-means|| + Lloyd), "kmeanspp" (k-means++ + Lloyd), or a path
+// to a .cmp file.
 #pragma once
 #include "GaussianMixture.h"
 #include "MultithreadedDataset.h"
@@ -39,11 +40,13 @@ public:
 };
 
 // Lloyd k-means from a seeding, then the mixture of the clusters
-// (EM2/EMInitializer.cpp:103-134).  Seeding: Forgy random partition
-// (Clust/KMeansInitializer.cpp:33-49).
+// (EM2/EMInitializer.cpp:82-134), on the GPU.  Seeding: Forgy
+// (Clust/KMeansInitializer.cpp:33-49), as the reference's constructor picks
+// (EM2/EMInitializer.cpp:86); subclasses pick another through Seeding().
 class KMeansEMInitializer : public EMInitializer {
 protected:
-    virtual void Seed(DynamicArray<OPTFLOAT> &vec);
+    // enum fem_seeding value (include/fastem_abi.h)
+    virtual int Seeding() const;
 
 public:
     KMeansEMInitializer(MultithreadedDataset &D, int nC) : EMInitializer(D, nC) {}
@@ -51,13 +54,25 @@ public:
     void Init(GaussianMixture &G) override;
 };
 
-// k-means++ seeding (Clust/PlusPlusInitializer/PlusPlusInitializer.cpp:33-84).
+// k-means++ seeding (Clust/PlusPlusInitializer/PlusPlusInitializer.cpp:33-84);
+// no reference EM initializer uses it, the name "kmeanspp" is this build's.
 class KMeansPPEMInitializer : public KMeansEMInitializer {
 protected:
-    void Seed(DynamicArray<OPTFLOAT> &vec) override;
+    int Seeding() const override;
 
 public:
     KMeansPPEMInitializer(MultithreadedDataset &D, int nC) : KMeansEMInitializer(D, nC) {}
+    void PrintInfo() override;
+};
+
+// k-means|| seeding with oversampling 1.4 (EM2/EMInitializer.cpp:158-166,
+// Clust/KMeansOrOrInitializer.cpp:143-174), in one-thread draw order.
+class KMeansOrOrEMInitializer : public KMeansEMInitializer {
+protected:
+    int Seeding() const override;
+
+public:
+    KMeansOrOrEMInitializer(MultithreadedDataset &D, int nC) : KMeansEMInitializer(D, nC) {}
     void PrintInfo() override;
 };
 

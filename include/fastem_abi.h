@@ -104,16 +104,27 @@ int fem_mstep(fem_ctx *ctx, double *w, double *mu, double *cov);
  * (EM2/EMAlgorithm.h:41) read by RetrieveClasses (EM2/EMAlgorithm.cpp:95-110). */
 int fem_get_posteriors(fem_ctx *ctx, double *R, int64_t ld);
 
+/* Seeding of the k-means initialisation (the reference's KMeansInitializer
+ * subclasses). */
+enum fem_seeding {
+    FEM_SEED_FORGY = 0,      /* K rows at (long)(Rand()*N), Clust/KMeansInitializer.cpp:33-49 */
+    FEM_SEED_KMEANSPP = 1,   /* Clust/PlusPlusInitializer/PlusPlusInitializer.cpp:33-84 */
+    FEM_SEED_KMEANSOROR = 2  /* k-means||, oversampling 1.4, 5 rounds, then weighted k-means++
+                                over the candidates: Clust/KMeansOrOrInitializer.cpp:15-174,
+                                EM2/EMInitializer.cpp:158-161 (one-thread draw order) */
+};
+
 /* k-means initialisation on the context's data, the step before the EM path:
- * seeding (1 = k-means++, Clust/PlusPlusInitializer/PlusPlusInitializer.cpp:33-84;
- * 0 = Forgy random partition, Clust/KMeansInitializer.cpp:33-49), up to
- * max_sweeps Lloyd sweeps with MinRel 1e-4 (Clust/KMAlgorithm.cpp:44-87,205-255;
- * the reference's MaxIter = max_sweeps + 1), then the mixture of the final
- * clustering (EM2/EMInitializer.cpp:103-134: covariances about the centroids
- * divided by n_k - 1, weights n_k / n).  Replaces KMeansEMInitializer::Init.
+ * a seeding (enum fem_seeding), up to max_sweeps Lloyd sweeps with MinRel
+ * 1e-4 (Clust/KMAlgorithm.cpp:44-87,205-255; the reference's MaxIter =
+ * max_sweeps + 1; empty clusters re-seeded by Clust/CentroidRepair.cpp:21-28),
+ * then the mixture of the final clustering (EM2/EMInitializer.cpp:103-134:
+ * covariances about the centroids divided by n_k - 1, weights n_k / n).
+ * Replaces KMeansEMInitializer::Init and KMeansOrOrEMInitializer::Init.
  * Distances are FP64 in ascending feature order without multiply-add
- * contraction, ties go to the lower index, the k-means++ prefix sum and draw
- * are sequential FP64 on the host.  The random stream is
+ * contraction (k-means||: float differences and squares summed in FP64, as
+ * Clust/CentroidVector.h:79-87), ties go to the lower index, prefix sums,
+ * costs and draws are sequential FP64 on the host.  The random stream is
  * std::mt19937_64(seed) through generate_canonical (Util/Rand.cpp:7-14), or
  * rnd(rnd_state) for the _rng variant.  The result becomes the context's
  * parameters and is copied to w[K], mu[K][d], cov[K][d][d].  Returns

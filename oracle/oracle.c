@@ -576,3 +576,154 @@ void orc_gmm_from_clusters(int64_t n, int d, int K, const float *X, const double
     }
     free(cnt); free(row);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Clust/KMeansInitializer.cpp:33-49 ForgyInitializer::Init: K rows drawn at
+ * (long)(Rand()*N), in component order (repeats allowed).                   */
+void orc_forgy(int64_t n, int d, int K, const float *X, orc_mt64 *rng, double *C) {
+    for (int k = 0; k < K; k++) {
+        int64_t r = (int64_t)(orc_mt64_canonical(rng) * (double)n);
+        for (int j = 0; j < d; j++) C[(size_t)k * d + j] = (double)X[(size_t)r * d + j];
+    }
+}
+
+/* Util/MultithreadedDataset.cpp:179-283 ComputeStatistics -> TraceCov: column
+ * means, then sum of (x - mean)^2 / (N - 1), summed over columns.  Sequential
+ * ascending row order (the reference sums per thread, then over threads).   */
+double orc_data_trace(int64_t n, int d, const float *X) {
+    double *m = (double *)calloc((size_t)d, sizeof(double));
+    double *s = (double *)calloc((size_t)d, sizeof(double));
+    for (int64_t i = 0; i < n; i++)
+        for (int j = 0; j < d; j++) m[j] += (double)X[(size_t)i * d + j];
+    for (int j = 0; j < d; j++) m[j] /= (double)n;
+    for (int64_t i = 0; i < n; i++)
+        for (int j = 0; j < d; j++) {
+            double t = (double)X[(size_t)i * d + j] - m[j];
+            s[j] += t * t;
+        }
+    double tr = 0.0;
+    for (int j = 0; j < d; j++) tr += s[j] / ((double)n - 1.0);
+    free(m); free(s);
+    return tr;
+}
+
+/* EM2/EMInitializer.cpp:43-80 JainEMInitializer::Init.  For component i one
+ * index is drawn and kept; the duplicate check draws further indices into a
+ * shadowed variable (:57) until one differs from the kept indices of earlier
+ * components, and discards them.  Weights 1/K, means = kept rows,
+ * covariances (trace / d / 10) I.                                           */
+void orc_jain(int64_t n, int d, int K, const float *X, double trace, orc_mt64 *rng,
+              double *w, double *mu, double *cov) {
+    int64_t *obj = (int64_t *)malloc((size_t)K * sizeof(int64_t));
+    const double wt = 1.0 / (double)K;
+    const double cv = trace / (double)d / 10.0;
+    memset(cov, 0, (size_t)K * d * d * sizeof(double));
+    for (int i = 0; i < K; i++) {
+        int64_t idx = (int64_t)(orc_mt64_canonical(rng) * (double)n);
+        int found;
+        do {
+            int64_t idx2 = (int64_t)(orc_mt64_canonical(rng) * (double)n);
+            found = 0;
+            for (int j = 0; j < i; j++)
+                if (obj[j] == idx2) { found = 1; break; }
+        } while (found);
+        obj[i] = idx;
+        w[i] = wt;
+        for (int j = 0; j < d; j++) mu[(size_t)i * d + j] = (double)X[(size_t)idx * d + j];
+        for (int j = 0; j < d; j++) cov[(size_t)i * d * d + (size_t)j * d + j] = cv;
+    }
+    free(obj);
+}
+
+/* Clust/CentroidVector.h:79-87: squared distance of two float rows; each
+ * difference and square in float, accumulated in double, ascending order.   */
+static inline double sqdist_ff(int d, const float *a, const float *b) {
+    double s = 0.0;
+    for (int j = 0; j < d; j++) {
+        float t = b[j] - a[j];
+        float t2 = t * t;
+        s += (double)t2;
+    }
+    return s;
+}
+
+/* Clust/KMeansOrOrInitializer.cpp:15-174 (k-means||, oversampling factor os,
+ * 5 rounds for os > 0.1 else 10) followed by the weighted k-means++ of
+ * Clust/PlusPlusInitializer/PlusPlusInitializer.cpp:23-84 over the collected
+ * centers.  Restated in the order of a one-thread run: one Rand() per row per
+ * round in row order (the reference draws from one shared stream inside an
+ * OpenMP loop, so its multi-thread result is not reproducible), new centers
+ * in row order, ties keep the earlier center.  Returns the number of
+ * collected centers (its candidate set), or -1 on allocation failure.       */
+int64_t orc_kmeansoror(int64_t n, int d, int K, const float *X, double os, orc_mt64 *rng,
+                       double *C) {
+    const int rounds = os <= 0.1 ? 10 : 5;
+    double *cd = (double *)malloc((size_t)n * sizeof(double));
+    int *cc = (int *)malloc((size_t)n * sizeof(int));
+    int64_t cap = 1024, m = 0;
+    int64_t *rows = (int64_t *)malloc((size_t)cap * sizeof(int64_t));
+    if (!cd || !cc || !rows) { free(cd); free(cc); free(rows); return -1; }
+    for (int64_t i = 0; i < n; i++) { cd[i] = INFINITY; cc[i] = -1; }
+    /* first center and InsertCentroid (:38-51) */
+    rows[m++] = (int64_t)(orc_mt64_canonical(rng) * (double)n);
+    for (int64_t i = 0; i < n; i++) {
+        double s = sqdist_ff(d, X + (size_t)rows[0] * d, X + (size_t)i * d);
+        if (s < cd[i]) { cd[i] = s; cc[i] = 0; }
+    }
+    double cost = 0.0;
+    for (int64_t i = 0; i < n; i++) cost += cd[i];
+    for (int r = 0; r < rounds; r++) {
+        /* ScanDataset (:83-103) */
+        int64_t base = m;
+        for (int64_t i = 0; i < n; i++) {
+            double prob = os * (double)K * cd[i] / cost;
+            if (orc_mt64_canonical(rng) < prob) {
+                if (m == cap) {
+                    cap *= 2;
+                    int64_t *nr = (int64_t *)realloc(rows, (size_t)cap * sizeof(int64_t));
+                    if (!nr) { free(cd); free(cc); free(rows); return -1; }
+                    rows = nr;
+                }
+                rows[m++] = i;
+            }
+        }
+        /* InsertCentroids (:53-70) */
+        for (int64_t i = 0; i < n; i++) {
+            const float *x = X + (size_t)i * d;
+            for (int64_t j = base; j < m; j++) {
+                double s = sqdist_ff(d, X + (size_t)rows[j] * d, x);
+                if (s < cd[i]) { cd[i] = s; cc[i] = (int)j; }
+            }
+        }
+        cost = 0.0;
+        for (int64_t i = 0; i < n; i++) cost += cd[i];
+    }
+    /* ComputeWeights (:114-126) */
+    double *wt = (double *)calloc((size_t)m, sizeof(double));
+    for (int64_t i = 0; i < n; i++) wt[cc[i]] += 1.0;
+    /* weighted k-means++ over the m centers (PlusPlusInitializer.cpp:33-84) */
+    double *cd2 = (double *)malloc((size_t)m * sizeof(double));
+    double *agg = (double *)malloc((size_t)m * sizeof(double));
+    int64_t r0 = (int64_t)(orc_mt64_canonical(rng) * (double)m);
+    for (int j = 0; j < d; j++) C[j] = (double)X[(size_t)rows[r0] * d + j];
+    for (int64_t j = 0; j < m; j++) cd2[j] = INFINITY;
+    for (int c = 1; c < K; c++) {
+        const double *prevc = C + (size_t)(c - 1) * d;
+        for (int64_t j = 0; j < m; j++) {
+            double s = sqdist(d, prevc, X + (size_t)rows[j] * d);
+            if (cd2[j] > s) cd2[j] = s;
+        }
+        double sum = 0.0;
+        for (int64_t j = 0; j < m; j++) { sum += cd2[j] * wt[j]; agg[j] = sum; }
+        for (int64_t j = 0; j < m; j++) agg[j] /= sum;
+        double rr = orc_mt64_canonical(rng);
+        for (int64_t j = 0; j < m; j++) {
+            if (rr <= agg[j]) {
+                for (int q = 0; q < d; q++) C[(size_t)c * d + q] = (double)X[(size_t)rows[j] * d + q];
+                break;
+            }
+        }
+    }
+    free(cd); free(cc); free(rows); free(wt); free(cd2); free(agg);
+    return m;
+}

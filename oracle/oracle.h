@@ -75,6 +75,18 @@ int  orc_lloyd(int64_t n, int d, int K, const float *X, double *C, double minrel
 void orc_gmm_from_clusters(int64_t n, int d, int K, const float *X, const double *C,
                            const int *cls, double *w, double *mu, double *cov);
 
+/* ---- Clust/KMeansInitializer.cpp:33-49 ForgyInitializer::Init ---- */
+void orc_forgy(int64_t n, int d, int K, const float *X, orc_mt64 *rng, double *C);
+/* ---- Util/MultithreadedDataset.cpp:179-283 TraceCov (sequential row order) ---- */
+double orc_data_trace(int64_t n, int d, const float *X);
+/* ---- EM2/EMInitializer.cpp:43-80 JainEMInitializer::Init ---- */
+void orc_jain(int64_t n, int d, int K, const float *X, double trace, orc_mt64 *rng,
+              double *w, double *mu, double *cov);
+/* ---- Clust/KMeansOrOrInitializer.cpp:143-174 + weighted k-means++ (one-thread
+ * order); returns the number of candidate centers collected ---- */
+int64_t orc_kmeansoror(int64_t n, int d, int K, const float *X, double os, orc_mt64 *rng,
+                       double *C);
+
 #ifdef __cplusplus
 }
 #endif

@@ -63,6 +63,12 @@ def lib():
                                          ctypes.c_double, ctypes.c_int, mp, _dp]),
             "orc_gmm_from_clusters": (None, [_i64, ctypes.c_int, ctypes.c_int, _fp, _dp, _ip,
                                              _dp, _dp, _dp]),
+            "orc_forgy": (None, [_i64, ctypes.c_int, ctypes.c_int, _fp, mp, _dp]),
+            "orc_data_trace": (ctypes.c_double, [_i64, ctypes.c_int, _fp]),
+            "orc_jain": (None, [_i64, ctypes.c_int, ctypes.c_int, _fp, ctypes.c_double, mp, _dp,
+                                _dp, _dp]),
+            "orc_kmeansoror": (_i64, [_i64, ctypes.c_int, ctypes.c_int, _fp, ctypes.c_double, mp,
+                                      _dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -265,3 +271,64 @@ def kmeanspp_init(X, K, seed):
     C, _, _ = lloyd(X, C, rng)
     cls = classify(X, C)
     return gmm_from_clusters(X, C, cls)
+
+
+def forgy(X, K, rng):
+    """Clust/KMeansInitializer.cpp:33-49: K rows at (long)(Rand()*N)."""
+    X = _c32(X)
+    n, d = X.shape
+    C = np.zeros((K, d))
+    lib().orc_forgy(n, d, K, _f(X), ctypes.byref(rng.state), _d(C))
+    return C
+
+
+def data_trace(X):
+    """Util/MultithreadedDataset.cpp:179-283 TraceCov."""
+    X = _c32(X)
+    n, d = X.shape
+    return lib().orc_data_trace(n, d, _f(X))
+
+
+def jain_init(X, K, rng, trace=None):
+    """EM2/EMInitializer.cpp:43-80 -> (w, mu, cov)."""
+    X = _c32(X)
+    n, d = X.shape
+    tr = data_trace(X) if trace is None else float(trace)
+    w = np.zeros(K)
+    mu = np.zeros((K, d))
+    cov = np.zeros((K, d, d))
+    lib().orc_jain(n, d, K, _f(X), tr, ctypes.byref(rng.state), _d(w), _d(mu), _d(cov))
+    return w, mu, cov
+
+
+def kmeansoror(X, K, rng, oversample=1.4):
+    """Clust/KMeansOrOrInitializer.cpp:143-174 in one-thread order -> (C, m)
+    with m the number of candidate centers."""
+    X = _c32(X)
+    n, d = X.shape
+    C = np.zeros((K, d))
+    m = lib().orc_kmeansoror(n, d, K, _f(X), oversample, ctypes.byref(rng.state), _d(C))
+    if m < 0:
+        raise MemoryError("orc_kmeansoror")
+    return C, m
+
+
+def _lloyd_mixture(X, C, rng):
+    C, _, _ = lloyd(X, C, rng)
+    cls = classify(X, C)
+    return gmm_from_clusters(X, C, cls)
+
+
+def kmeans_init(X, K, seed):
+    """KMeansEMInitializer::Init (EM2/EMInitializer.cpp:82-134): Forgy seeding,
+    Lloyd (MinRel 1e-4, MaxIter 30), classify, cluster mixture; one stream."""
+    rng = Rand(seed)
+    return _lloyd_mixture(X, forgy(X, K, rng), rng)
+
+
+def kmeansoror_init(X, K, seed):
+    """KMeansOrOrEMInitializer (EM2/EMInitializer.cpp:158-166): k-means||
+    (oversampling 1.4), then the KMeansEMInitializer tail."""
+    rng = Rand(seed)
+    C, _ = kmeansoror(X, K, rng)
+    return _lloyd_mixture(X, C, rng)

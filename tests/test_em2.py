@@ -49,6 +49,24 @@ def test_em2_argument_errors(small_bin, args, msg):
     assert msg in out
 
 
+@pytest.mark.parametrize("n,d,K,spec", [(20000, 16, 50, "automax:1"), (20000, 16, 50, "automax:18"),
+                                        (30000, 8, 200, "automax:2"), (5000, 4, 3, "7:3"),
+                                        (5000, 4, 3, "2:9"), (5000, 4, 3, None)])
+def test_em2_block_counts(orc, tmp_path, n, d, K, spec):
+    """em2 -N prints max(nDens, nMStep) blocks (EM2/main.cpp:60-69) as the
+    oracle's EM2/BlockManager.cpp:10-28 computes them.  The run stops at the
+    missing initializer file, before any device work."""
+    X = np.zeros((n, d), dtype=np.float32)
+    formats.write_bin(tmp_path / "x.bin", X)
+    args = ["-c", K, "-I", "/nonexistent.cmp"] + (["-N", spec] if spec else []) + [tmp_path / "x.bin"]
+    rc, out = run_em2(args)
+    m = re.search(r"Loading done in (\d+) blocks", out)
+    assert m, out
+    assert int(m.group(1)) == max(orc.parse_block_params(spec, n, d, K))
+    if spec == "automax:1":
+        assert int(m.group(1)) > 1
+
+
 def test_em2_usage_without_file():
     rc, out = run_em2([])
     assert "Arguments: filename" in out

@@ -196,3 +196,32 @@ def test_full_size_properties(orc):
     U, ld = orc.factor_all(cov0)
     _, R_ref = orc.estep(X[sample], w0, mu0, U, ld)
     np.testing.assert_allclose(R_s, R_ref, rtol=1e-9, atol=1e-13)
+
+
+@pytest.mark.timeout(900)
+def test_config2_full_size_vs_oracle(orc):
+    """The benchmark workload at full size against the oracle: N=4,194,304,
+    d=64, K=64 (means U[-20,20], cov AA^T/d + 0.05 I), device k-means++ +
+    Lloyd init, then 3 EM iterations of the FP64 form and of the oracle's
+    TrainNoCopy restatement from that identical init.  Every iteration's LL
+    and the final weights, means and covariances agree within 1e-10."""
+    n, d, K, iters = 4194304, 64, 64, 3
+    say("config2 generate")
+    X, _, _, _ = mixture(n, d, K, seed=2025, lo=-20.0, hi=20.0, diag=0.05)
+    say("config2 device kmeans++ init")
+    w0, mu0, cov0 = fem.kmeanspp_init(X, K, seed=1)
+    trace = []
+    with fem.Context(n, d, K) as ctx:
+        ctx.upload(X)
+        say("config2 device EM")
+        it, ll, (w, mu, cov) = fem.train(ctx, w0, mu0, cov0, maxsweeps=iters, eps=1e-300,
+                                         ll_trace=trace)
+    say("config2 oracle EM (host, FP64)")
+    ref = orc.train(X, w0, mu0, cov0, maxsweeps=iters, eps=1e-300, svd=False)
+    assert ref.rc == iters + 1 and it == ref.rc
+    ll_err = np.abs(np.array(trace) - ref.ll_trace) / np.abs(ref.ll_trace)
+    errs = {"ll": ll_err.max(), "w": rel_err(w, ref.w), "mu": rel_err(mu, ref.mu),
+            "cov": rel_err(cov, ref.cov)}
+    say("config2 max rel errors vs oracle:", errs, "LL trace", list(ref.ll_trace))
+    assert errs["ll"] < LL_RTOL
+    assert errs["w"] < PARAM_RTOL and errs["mu"] < PARAM_RTOL and errs["cov"] < PARAM_RTOL
