@@ -55,6 +55,8 @@ _SIGS = {
     "fem_kmeans_init_seeded": (ctypes.c_int, [_vp, _dp, ctypes.c_uint64, ctypes.c_int, _dp, _dp,
                                               _dp]),
     "fem_last_error": (ctypes.c_char_p, [_vp]),
+    "fem_estep_noalloc": (ctypes.c_int, [_vp, _i64, _i64, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp,
+                                         _vp, _i64, _dp, _ip, _vp]),
     "fem_version": (ctypes.c_char_p, []),
     "fem_batch_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int,
                                         ctypes.POINTER(ctypes.c_int64), _ip, _ip, ctypes.c_int]),

@@ -151,6 +151,8 @@ int set_device(fem_ctx *c) {
 
 }  // namespace
 
+int fem::fail_global(int code, const std::string &msg) { return fail(nullptr, code, msg); }
+
 extern "C" {
 
 const char *fem_version(void) { return "fastem-mi355x 0.1 (gfx950)"; }

@@ -85,6 +85,8 @@ struct fem_ctx {
 };
 
 namespace fem {
+// Records msg as the error fem_last_error(NULL) reports; returns code.
+int fail_global(int code, const std::string &msg);
 hipError_t launch_factor(fem_ctx *c);
 hipError_t launch_estep(fem_ctx *c);
 hipError_t launch_means(fem_ctx *c);

@@ -170,6 +170,31 @@ int fem_batch_get_result(fem_batch *batch, int fit, double *w, double *mu, doubl
                          int *iter_counter, int *status);
 const char *fem_batch_last_error(const fem_batch *batch);
 
+/* ---- No allocation ----------------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): fem_factor + fem_estep of the FP64 form over
+ * caller-owned device buffers, with no context.  It allocates nothing: its
+ * working memory (factor products, a zero-padded tile of rows, the tile's
+ * posteriors, log-likelihood partials) is one static 64 MiB device array per
+ * GPU, held until the process ends.  Limits: 1 <= d <= 128, 1 <= K <= 128
+ * (FEM_ERR_ARG beyond).  Every call on a GPU shares that GPU's array, so
+ * calls that overlap on one GPU race: serialise them.  Runs on the current
+ * device, on `stream` (a hipStream_t; NULL = the null stream); synchronous at
+ * return; errors through fem_last_error(NULL).
+ *   X        device, n rows of ldx floats (ldx >= d)
+ *   w,mu,cov device FP64: K; K x d; K x d x d (covariances read from their
+ *            lower triangle, as fem_factor does)
+ *   R        device FP64, n rows of ldr (ldr >= K): the posteriors, equal bit
+ *            for bit to fem_estep's (fem_get_posteriors layout)
+ *   loglik   host, may be NULL: the log-likelihood (fem_estep's up to
+ *            summation order: tiles of at most 65,536 rows are summed in order)
+ *   info     host, K ints, may be NULL: as fem_factor; FEM_ERR_NOT_PD if any
+ *            component is not positive definite (R is then not written). */
+int fem_estep_noalloc(const float *X, int64_t n, int64_t ldx, int d, int K, const double *w,
+                      const double *mu, const double *cov, double *R, int64_t ldr, double *loglik,
+                      int *info, void *stream);
+
 /* Device time in milliseconds of the last fem_estep / fem_mstep (means and
  * covariance passes) / fem_factor, from HIP events on the context's stream.
  * Any pointer may be NULL. */
