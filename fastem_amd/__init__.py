@@ -55,6 +55,9 @@ _SIGS = {
     "fem_kmeans_init_seeded": (ctypes.c_int, [_vp, _dp, ctypes.c_uint64, ctypes.c_int, _dp, _dp,
                                               _dp]),
     "fem_last_error": (ctypes.c_char_p, [_vp]),
+    "fem_get_classes": (ctypes.c_int, [_vp, _ip]),
+    "fem_condition_number": (ctypes.c_int, [ctypes.c_int, _dp, _dp]),
+    "fem_bhatt_dist": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp, _dp]),
     "fem_estep_noalloc": (ctypes.c_int, [_vp, _i64, _i64, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp,
                                          _vp, _i64, _dp, _ip, _vp]),
     "fem_version": (ctypes.c_char_p, []),
@@ -228,11 +231,38 @@ class Context:
                                                  _d(cov)))
         return w, mu, cov
 
+    def classes(self):
+        """Component of largest posterior per row (fem_get_classes)."""
+        cls = np.empty(self.n, dtype=np.int32)
+        self._check(lib().fem_get_classes(self._h, cls.ctypes.data_as(_ip)))
+        return cls
+
     def times(self):
         t = [ctypes.c_float() for _ in range(4)]
         self._check(lib().fem_last_times(self._h, *[ctypes.byref(x) for x in t]))
         return {"estep_ms": t[0].value, "means_ms": t[1].value, "cov_ms": t[2].value,
                 "factor_ms": t[3].value}
+
+
+def condition_number(C):
+    """|lambda_max / lambda_min| of a covariance, as em2's default check
+    computes it (NormalDensity::ComputeConditionNumber); host only."""
+    C = np.ascontiguousarray(C, dtype=np.float64)
+    out = ctypes.c_double()
+    rc = lib().fem_condition_number(C.shape[0], _d(C), ctypes.byref(out))
+    if rc != 0:
+        raise FemError(rc, "fem_condition_number failed")
+    return out.value
+
+
+def bhatt_dist(mu1, C1, mu2, C2):
+    """Bhattacharyya distance of two normal densities (BhattDist); host only."""
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (mu1, C1, mu2, C2)]
+    out = ctypes.c_double()
+    rc = lib().fem_bhatt_dist(len(a[0]), *[_d(x) for x in a], ctypes.byref(out))
+    if rc != 0:
+        raise FemError(rc, "fem_bhatt_dist failed")
+    return out.value
 
 
 def regularize(cov, g):

@@ -443,6 +443,21 @@ int fem_get_posteriors(fem_ctx *c, double *R, int64_t ld) {
     return FEM_OK;
 }
 
+int fem_get_classes(fem_ctx *c, int *cls) {
+    if (int r = check_ctx(c)) return r;
+    if (!cls) return fail(c, FEM_ERR_ARG, "bad class pointer");
+    if (!c->have_post) return fail(c, FEM_ERR_STATE, "no posteriors yet");
+    if (int r = set_device(c)) return r;
+    int *tmp = nullptr;
+    FEM_HIP(c, hipMalloc((void **)&tmp, (size_t)c->n * sizeof(int)));
+    hipError_t e = fem::launch_classes(c, tmp);
+    if (e == hipSuccess) e = hipMemcpyAsync(cls, tmp, (size_t)c->n * sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(c, FEM_ERR_HIP, std::string("fem_get_classes: ") + hipGetErrorString(e));
+    return FEM_OK;
+}
+
 int fem_last_times(fem_ctx *c, float *estep_ms, float *means_ms, float *cov_ms, float *factor_ms) {
     if (int r = check_ctx(c)) return r;
     if (estep_ms) *estep_ms = c->t_estep;

@@ -104,6 +104,30 @@ __global__ __launch_bounds__(256) void kb_factor(const BFit *__restrict__ F, BSt
     }
 }
 
+// RegularizeCovariances before the loop (EM2/EMAlgorithm.cpp:145,237-244):
+// C <- C (1 - g) + (g tr(C) / d) I for every component of every fit, the
+// trace summed in ascending diagonal order.  One workgroup per component.
+__global__ __launch_bounds__(256) void kb_regularize(const BFit *__restrict__ F, double *__restrict__ dp,
+                                                     const Item *__restrict__ items, int nitems, double g) {
+    __shared__ double diag;
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const Item itm = items[it];
+        const BFit &f = F[itm.fit];
+        double *C = dp + f.cov + (size_t)itm.a * f.d * f.d;
+        if (threadIdx.x == 0) {
+            double tr = 0.0;
+            for (int i = 0; i < f.d; i++) tr += C[(size_t)i * f.d + i];
+            diag = g * tr / (double)f.d;
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < f.d * f.d; e += 256) {
+            const int i = e / f.d, j = e % f.d;
+            C[e] = C[e] * (1.0 - g) + (i == j ? diag : 0.0);
+        }
+        __syncthreads();
+    }
+}
+
 template <int P>
 __global__ __launch_bounds__(256) void kb_estep(const BFit *__restrict__ F, const BState *__restrict__ S,
                                                 const float *__restrict__ xp, double *__restrict__ dp,
@@ -162,7 +186,9 @@ __global__ __launch_bounds__(256) void kb_means_finalize(const BFit *__restrict_
         const Item itm = items[it];
         if (!S[itm.fit].active) continue;
         const BFit &f = F[itm.fit];
-        means_finalize_unit(itm.a, 0, f.d, f.dpad, f.KP, f.m_chunks, dp + f.mpart, dp + f.mu, dp + f.nk, 0);
+        // 64 features per call (its column tile ct); any d
+        for (int ct = 0; ct * 64 < f.d; ct++)
+            means_finalize_unit(itm.a, ct, f.d, f.dpad, f.KP, f.m_chunks, dp + f.mpart, dp + f.mu, dp + f.nk, 0);
     }
 }
 
@@ -484,9 +510,13 @@ int fem_batch_get_result(fem_batch *b, int fit, double *w, double *mu, double *c
     return FEM_OK;
 }
 
-// Factorises every fit's current parameters and marks the fits active.
+// Regularises (g > 0) and factorises every fit's current parameters and
+// marks the fits active.
 static int batch_start(fem_batch *b) {
     const size_t lds_f = (size_t)64 * 64 * sizeof(double);
+    if (b->g > 0.0)
+        hipLaunchKernelGGL(fem::kb_regularize, dim3(grid_of(4, b->cus, b->n_comp)), dim3(256), 0, b->stream,
+                           b->dfits, b->dpool, b->comp, b->n_comp, b->g);
     hipLaunchKernelGGL(fem::kb_factor, dim3(grid_of(4, b->cus, b->n_comp)), dim3(256), lds_f, b->stream, b->dfits,
                        b->dstate, b->dpool, b->ipool, b->comp, b->n_comp, 0);
     hipLaunchKernelGGL(fem::kb_start, dim3((b->F + 63) / 64), dim3(64), 0, b->stream, b->dfits, b->dstate, b->ipool,
@@ -506,8 +536,8 @@ static int count_active(fem_batch *b, int *active) {
 }
 
 // Runs EM on all fits from their current parameters: regularisation
-// coefficient g (applied after every M-step; the initial parameters are used as
-// given), stopping rule e < eps or maxsweeps E-steps.  use_graph != 0 replays a
+// coefficient g (applied to the initial covariances, then after every M-step,
+// as TrainNoCopy does), stopping rule e < eps or maxsweeps E-steps.  use_graph != 0 replays a
 // captured iteration, else launches the same kernels eagerly.  Blocks until
 // every fit has stopped.
 int fem_batch_run(fem_batch *b, int maxsweeps, double eps, double g, int use_graph) {

@@ -93,6 +93,7 @@ hipError_t launch_means(fem_ctx *c);
 hipError_t launch_cov(fem_ctx *c);
 hipError_t launch_zero(double *p, int64_t n, hipStream_t s);
 hipError_t launch_transpose_post(const fem_ctx *c, double *out_dev, int64_t ld);
+hipError_t launch_classes(const fem_ctx *c, int *cls_dev);
 // Row copies between pageable host memory and pitched device memory (bytes),
 // staged through two pinned buffers so the DMA engine moves large contiguous
 // chunks while the host packs or unpacks rows; device pad bytes beyond

@@ -220,6 +220,33 @@ __global__ void k_transpose_post(const double *__restrict__ Rt, int64_t npad, in
     for (int k = 0; k < K; k++) out[i * ld + k] = Rt[(size_t)k * npad + i];
 }
 
+// Class of each row: the first component of largest posterior, compared with
+// > from -1 as EMAlgorithm::RetrieveClasses does (EM2/EMAlgorithm.cpp:107-122).
+template <typename T>
+__global__ void k_classes(const T *__restrict__ Rt, int64_t npad, int64_t n, int K, int *__restrict__ cls) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double best = -1.0;
+    int bj = -1;
+    for (int k = 0; k < K; k++) {
+        const double r = (double)Rt[(size_t)k * npad + i];
+        if (r > best) {
+            best = r;
+            bj = k;
+        }
+    }
+    cls[i] = bj;
+}
+
+hipError_t launch_classes(const fem_ctx *c, int *cls_dev) {
+    const dim3 grid((unsigned)((c->n + 255) / 256));
+    if (c->prec == FEM_PREC_FP64)
+        hipLaunchKernelGGL(k_classes<double>, grid, dim3(256), 0, c->stream, c->Rt, c->npad, c->n, c->K, cls_dev);
+    else
+        hipLaunchKernelGGL(k_classes<float>, grid, dim3(256), 0, c->stream, c->Rt32, c->npad, c->n, c->K, cls_dev);
+    return hipGetLastError();
+}
+
 hipError_t launch_transpose_post(const fem_ctx *c, double *out_dev, int64_t ld) {
     if (c->prec != FEM_PREC_FP64) return launch_transpose_post32(c, out_dev, ld);
     hipLaunchKernelGGL(k_transpose_post, dim3((unsigned)((c->n + 255) / 256)), dim3(256), 0,

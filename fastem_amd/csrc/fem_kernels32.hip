@@ -187,7 +187,8 @@ static hipError_t estep32_t(fem_ctx *c) {
     return hipGetLastError();
 }
 
-// Point groups per wave: 2 by default, 4 for d <= 64 when estep_groups32 == 4.
+// Point groups per wave: estep_groups32 (fem_create picks 4 for d <= 32, else 2;
+// 4 is available for d <= 64).
 hipError_t launch_estep32(fem_ctx *c) {
     const bool p2 = c->estep_groups32 == 2;
     switch (c->NB) {

@@ -9,6 +9,8 @@
 #include <random>
 #include <vector>
 
+#include "../../include/fastem/EMException.h"
+#include "../../include/fastem/NormalDensity.h"
 #include "../../include/fastem_abi.h"
 #include "../../include/fastem_tools.h"
 
@@ -127,4 +129,46 @@ extern "C" int fem_kmeanspp_init(int64_t n, int d, int K, const float *X, uint64
     if (rc == FEM_OK) rc = fem_kmeans_init(c, seed, 1, lloyd_iters, w, mu, cov);
     fem_destroy(c);
     return rc;
+}
+
+namespace {
+
+void load_density(NormalDensity &D, int d, const double *mu, const double *C) {
+    EigenMatrix M(d, d);
+    std::memcpy(M.data(), C, sizeof(double) * d * d);
+    D.Init(M, EigenRowVector(mu, d));
+}
+
+}  // namespace
+
+extern "C" int fem_condition_number(int d, const double *C, double *cond) {
+    if (d < 1 || !C || !cond) return FEM_ERR_ARG;
+    std::vector<double> zero(d, 0.0);
+    NormalDensity D(d);
+    try {
+        load_density(D, d, zero.data(), C);
+    } catch (EMException &) {
+        return FEM_ERR_NOT_PD;
+    }
+    try {
+        *cond = D.ComputeConditionNumber();
+    } catch (EMException &) {
+        return FEM_ERR_ARG;
+    }
+    return FEM_OK;
+}
+
+extern "C" int fem_bhatt_dist(int d, const double *mu1, const double *C1, const double *mu2, const double *C2,
+                              double *dist) {
+    if (d < 1 || !mu1 || !C1 || !mu2 || !C2 || !dist) return FEM_ERR_ARG;
+    NormalDensity A(d), B(d);
+    BhattDist Bh(d);
+    try {
+        load_density(A, d, mu1, C1);
+        load_density(B, d, mu2, C2);
+        *dist = Bh.ComputeDist(A, B);
+    } catch (EMException &) {
+        return FEM_ERR_NOT_PD;
+    }
+    return FEM_OK;
 }

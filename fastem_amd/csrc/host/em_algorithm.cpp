@@ -45,7 +45,7 @@ int EMAlgorithm::GetDefaultPrecision() { return default_precision(); }
 
 EMAlgorithm::EMAlgorithm(int ncols, long nrows, int ncl)
     : nCols(ncols), K(ncl), IterCounter(0), TotalRowCount(nrows), nRows(nrows), MaxSweeps(1000000),
-      Eps(1e-5), AbortThr(100000.0), RegCoef(0.0), Means(ncl, ncols), Probs(ncl), LogLike(0.0),
+      Eps(1e-5), AbortThr(100000.0), RegCoef(0.0), Means(ncl, ncols), Probs(ncl), PLLs(ncl), LogLike(0.0),
       Ctx(nullptr), Precision(default_precision()), HostParamsNewer(false),
       DeviceParamsNewer(false), DeviceRegularized(false) {
     Covs.SetSize(K);
@@ -144,12 +144,16 @@ OPTFLOAT EMAlgorithm::FindLargestConditionNumber() {
 #pragma omp parallel for schedule(dynamic, 1)
     for (int i = 0; i < K; i++) {
         std::vector<double> ev(nCols);
-        fastem::SymmetricEigenvalues(nCols, Covs[i].data(), ev.data());
-        conds[i] = std::fabs(ev[nCols - 1] / ev[0]);
+        // a QL iteration that does not converge leaves no usable eigenvalues
+        conds[i] = fastem::SymmetricEigenvalues(nCols, Covs[i].data(), ev.data())
+                       ? std::fabs(ev[nCols - 1] / ev[0])
+                       : std::numeric_limits<double>::quiet_NaN();
     }
     double m = 0.0;
-    for (int i = 0; i < K; i++)
+    for (int i = 0; i < K; i++) {
+        if (std::isnan(conds[i])) throw EMException("Eigenvalue computation of a covariance matrix failed");
         if (conds[i] > m) m = conds[i];
+    }
     return m;
 }
 
@@ -221,21 +225,9 @@ OPTFLOAT EMAlgorithm::TrainNoCopy(GaussianMixture &G, int Verbosity, bool SVD) {
     return LogL;
 }
 
-void EMAlgorithm::RetrieveClasses(std::vector<int> &Classes) const {
-    std::vector<double> R((size_t)nRows * K);
-    Check(fem_get_posteriors(Ctx, R.data(), K), "fem_get_posteriors");
-    Classes.assign(nRows, 0);
-#pragma omp parallel for schedule(static)
-    for (long i = 0; i < nRows; i++) {
-        double maxP = -1.0;
-        int maxj = -1;
-        for (int j = 0; j < K; j++)
-            if (R[(size_t)i * K + j] > maxP) {
-                maxP = R[(size_t)i * K + j];
-                maxj = j;
-            }
-        Classes[i] = maxj;
-    }
+void EMAlgorithm::RetrieveClasses(LargeVector<int> &Classes) const {
+    Classes.SetSize(nRows);
+    Check(fem_get_classes(Ctx, Classes.GetData()), "fem_get_classes");
 }
 
 void EMAlgorithm::ComputePosterriorCorrelation(EigenMatrix &Corr) {

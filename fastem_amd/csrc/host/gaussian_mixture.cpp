@@ -154,6 +154,10 @@ OPTFLOAT GaussianMixture::FindLargestConditionNumber() const {
     return m;
 }
 
+OPTFLOAT GaussianMixture::ComponentBhattDist(BhattDist &Dist, int Comp, const GaussianMixture &G, int CG) const {
+    return Dist.ComputeDist(*Densities[Comp], *G.Densities[CG]);
+}
+
 void GaussianMixture::Dump() const {
     printf("Number of components: %d\n", nComponents);
     for (int i = 0; i < nComponents; i++) {

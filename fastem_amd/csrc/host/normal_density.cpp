@@ -31,9 +31,12 @@ void NormalDensity::Init(const EigenMatrix &aC, const EigenRowVector &aM) {
         }
 }
 
+// |lambda_max / lambda_min| (EM2/NormalDensity.cpp:67-71); throws if the
+// QL iteration does not converge.
 OPTFLOAT NormalDensity::ComputeConditionNumber() const {
     std::vector<double> ev(Dim);
-    fastem::SymmetricEigenvalues(Dim, C.data(), ev.data());
+    if (!fastem::SymmetricEigenvalues(Dim, C.data(), ev.data()))
+        throw EMException("Eigenvalue computation of a covariance matrix failed");
     return std::fabs(ev[Dim - 1] / ev[0]);
 }
 
@@ -61,3 +64,22 @@ OPTFLOAT NormalDensity::GetDensity(const float *row) const {
 }
 
 OPTFLOAT NormalDensity::GetLogDensity(const OPTFLOAT *row) const { return LogCoeff - ComputeDot(row); }
+
+BhattDist::BhattDist(int aDim) : Dens(aDim), Sum(aDim, aDim) {}
+
+// Bhattacharyya distance between two normal densities
+// (EM2/NormalDensity.cpp:128-138): with S = (C1 + C2) / 2,
+// D = (m2 - m1)^T S^-1 (m2 - m1) / 8 + ln(det S / sqrt(det C1 det C2)) / 2,
+// clamped below at 0.
+OPTFLOAT BhattDist::ComputeDist(const NormalDensity &Dens1, const NormalDensity &Dens2) {
+    const EigenMatrix C1 = Dens1.GetCovariance(), C2 = Dens2.GetCovariance();
+    const int d = Dens1.GetDimension();
+    for (int i = 0; i < d; i++)
+        for (int j = 0; j < d; j++) Sum(i, j) = 0.5 * (C1(i, j) + C2(i, j));
+    Dens.Init(Sum, Dens1.GetMean());
+    const OPTFLOAT D1 = 0.125 * Dens.MahalanobisDist(Dens2.GetMean());
+    const OPTFLOAT D2 =
+        0.5 * std::log(Dens.GetDeterminant() / std::sqrt(Dens1.GetDeterminant() * Dens2.GetDeterminant()));
+    const OPTFLOAT D = D1 + D2;
+    return D < 0.0 ? 0.0 : D;
+}

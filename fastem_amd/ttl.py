@@ -26,18 +26,30 @@ def kmeanspp(ctx, seed, sweeps=29):
     return ctx.kmeans_init(seed=seed, seeding=1, max_sweeps=sweeps)
 
 
-def run_plain(ctx, seed, tol=1e-6, cap=500, conv=1e-12, ll_star=None, init=None):
-    """FP64 EM; returns dict(trace=[(t, LL)], ll_final, t_reach, iters).
-    init as for run_schedule."""
-    ctx.set_precision(fem.PREC_FP64)
-    trace = []
-    t0 = time.perf_counter()
+def _initialise(ctx, seed, init, g):
+    """Initial parameters (device k-means++, given, or a callable), the
+    regularisation coefficient g for every M-step, and, for g > 0, the
+    regularisation TrainNoCopy applies before the first factorisation
+    (EM2/EMAlgorithm.cpp:145)."""
+    ctx.set_regularization(g)
     if init is None:
         kmeanspp(ctx, seed)
     elif callable(init):
         init(ctx)
     else:
         ctx.set_params(*init)
+    if g > 0.0:
+        w, mu, cov = ctx.get_params()
+        ctx.set_params(w, mu, fem.regularize(cov, g))
+
+
+def run_plain(ctx, seed, tol=1e-6, cap=500, conv=1e-12, ll_star=None, init=None, g=0.0):
+    """FP64 EM; returns dict(trace=[(t, LL)], ll_final, t_reach, iters).
+    init and g as for run_schedule."""
+    ctx.set_precision(fem.PREC_FP64)
+    trace = []
+    t0 = time.perf_counter()
+    _initialise(ctx, seed, init, g)
     t_init = time.perf_counter() - t0
     ctx.factor()
     prev = -np.finfo(np.float64).max
@@ -64,11 +76,13 @@ def check_every(m):
     return ("check", int(m))
 
 
-def run_schedule(ctx, seed, ll_star, stages, tol=1e-6, cap=500, init=None):
+def run_schedule(ctx, seed, ll_star, stages, tol=1e-6, cap=500, init=None, g=0.0):
     """Reduced-precision stages, then FP64 until the target is met.
 
     init: None (device k-means++), (w, mu, cov), or a callable that
-    initialises the context itself (timed as part of the run).
+    initialises the context itself (timed as part of the run).  g: the
+    covariance regularisation coefficient (after every M-step, and once on
+    the initial covariances when g > 0, as TrainNoCopy does).
     stages: [(precision, rule), ...]; each stage iterates in its form until
     its rule says switch, then the next stage continues from the same
     parameters on the same context.  rule = a float eps: the relative LL
@@ -82,12 +96,7 @@ def run_schedule(ctx, seed, ll_star, stages, tol=1e-6, cap=500, init=None):
     trace = []
     t0 = time.perf_counter()
     ctx.set_precision(fem.PREC_FP64)
-    if init is None:
-        kmeanspp(ctx, seed)
-    elif callable(init):
-        init(ctx)
-    else:
-        ctx.set_params(*init)
+    _initialise(ctx, seed, init, g)
     t_init = time.perf_counter() - t0
     counts = []
     for prec, rule in stages:
@@ -135,8 +144,9 @@ def run_schedule(ctx, seed, ll_star, stages, tol=1e-6, cap=500, init=None):
     return res
 
 
-def run_mixed(ctx, seed, ll_star, tol=1e-6, switch_eps=1e-7, cap=500, init=None):
+def run_mixed(ctx, seed, ll_star, tol=1e-6, switch_eps=1e-7, cap=500, init=None, g=0.0):
     """FP32-form iterations, then FP64 polish on the same context."""
-    r = run_schedule(ctx, seed, ll_star, [(fem.PREC_FP32, switch_eps)], tol=tol, cap=cap, init=init)
+    r = run_schedule(ctx, seed, ll_star, [(fem.PREC_FP32, switch_eps)], tol=tol, cap=cap, init=init,
+                     g=g)
     r["fp32_iters"] = r["stage_iters"][0]
     return r

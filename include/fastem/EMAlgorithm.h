@@ -27,6 +27,10 @@ protected:
     DynamicArray<EigenMatrix> Covs;
     EigenMatrix Means;
     EigenVector Probs;
+    // Per-component partial log-likelihoods.  The reference sizes them
+    // (EM2/EMAlgorithm.cpp:53,63) but never writes them and ComputePLLs has
+    // no caller, so they stay zero; the same holds here.
+    DynamicArray<OPTFLOAT> PLLs;
     OPTFLOAT LogLike;
 
     fem_ctx *Ctx;
@@ -42,6 +46,7 @@ protected:
     OPTFLOAT FindLargestConditionNumber();
     OPTFLOAT FindMinTrace();
     void DecodeMixture(const GaussianMixture &G);
+    void ComputePLLs() {}
     virtual void EStep() {}
     virtual void InitDensities();
     virtual void PrecomputeDensityMatrix();
@@ -60,11 +65,14 @@ public:
     static int GetDefaultPrecision();
 
     void ComputePosterriorCorrelation(EigenMatrix &Corr);
-    void RetrieveClasses(std::vector<int> &Classes) const;
+    // Class of each row = the component of largest posterior from the last
+    // E-step, ties to the lower index (EM2/EMAlgorithm.cpp:107-122); sizes
+    // Classes to the row count.
+    void RetrieveClasses(LargeVector<int> &Classes) const;
     int GetIterCounter() const { return IterCounter; }
     void ClearIterCounter() { IterCounter = 0; }
     int GetK() const { return K; }
-    OPTFLOAT GetPLL(int) const { return 0.0; }
+    OPTFLOAT GetPLL(int k) const { return PLLs[k]; }
     void SetParameters(int aMaxSweeps, double aEps, double aAbortThr, double aRegCoef = 0.0);
     virtual OPTFLOAT Train(const MultithreadedDataset &Data, GaussianMixture &G, int Verbosity, bool SVD);
     virtual OPTFLOAT TrainNoCopy(GaussianMixture &G, int Verbosity, bool SVD);

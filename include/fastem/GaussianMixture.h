@@ -43,6 +43,9 @@ public:
     void ClassifyData(const MultithreadedDataset &D, DynamicArray<int> &ClassNums);
     void WriteClasses(const char *fname, const MultithreadedDataset &D);
     OPTFLOAT FindLargestConditionNumber() const;
+    // Bhattacharyya distance of component Comp to component CG of G
+    // (EM2/GaussianMixture.cpp:81-83).
+    OPTFLOAT ComponentBhattDist(BhattDist &Dist, int Comp, const GaussianMixture &G, int CG) const;
     void Dump() const;
     void Write(FILE *pF);
     void Write(const char *fname);

@@ -78,6 +78,22 @@ public:
     void RemoveAt(long i) { v_.erase(v_.begin() + i); }
 };
 
+// Zero-initialised vector with the reference's LargeVector interface
+// (Util/LargeVector.h:15-95): SetSize discards the old contents and zeroes.
+template <typename T>
+class LargeVector {
+    std::vector<T> v_;
+
+public:
+    LargeVector() = default;
+    void SetSize(long n) { v_.assign((size_t)n, T(0)); }
+    long GetSize() const { return (long)v_.size(); }
+    T &operator[](long i) { return v_[(size_t)i]; }
+    const T &operator[](long i) const { return v_[(size_t)i]; }
+    T *GetData() { return v_.data(); }
+    const T *GetData() const { return v_.data(); }
+};
+
 // Cholesky-Crout of the lower triangle of C (d x d): L lower.  False if a
 // pivot is not positive.
 bool CholeskyLower(int d, const double *C, double *L);
@@ -92,3 +108,4 @@ typedef fastem::RowVector EigenRowVector;
 typedef fastem::RowVector EigenVector;
 typedef double OPTFLOAT;
 using fastem::DynamicArray;
+using fastem::LargeVector;

@@ -25,6 +25,9 @@ public:
     OPTFLOAT GetDeterminant() const { return Determinant; }
     OPTFLOAT GetLogDeterminant() const { return LogDeterminant; }
     int GetDimension() const { return Dim; }
+    // Declared by the reference (EM2/NormalDensity.h:47) but defined nowhere
+    // in it; declared here for source compatibility, with no definition.
+    int ScanMatrix();
     // Symmetrises aC from its lower triangle, factorises; throws
     // EMException("Cholesky decomposition failed") (EM2/NormalDensity.cpp:43-65).
     void Init(const EigenMatrix &aC, const EigenRowVector &aM);
@@ -34,4 +37,15 @@ public:
     OPTFLOAT MahalanobisDist(const EigenRowVector &row) const;
     OPTFLOAT ComputeTrace() const;
     OPTFLOAT ComputeConditionNumber() const;
+};
+
+// Bhattacharyya distance between two components (EM2/NormalDensity.h:60-69).
+class BhattDist {
+protected:
+    NormalDensity Dens;
+    EigenMatrix Sum;
+
+public:
+    explicit BhattDist(int aDim);
+    OPTFLOAT ComputeDist(const NormalDensity &Dens1, const NormalDensity &Dens2);
 };

@@ -114,6 +114,11 @@ enum fem_seeding {
                                 EM2/EMInitializer.cpp:158-161 (one-thread draw order) */
 };
 
+/* Class of each of the n rows (cls[n], host): the component of largest
+ * posterior from the last E-step, ties to the lower index; computed on the
+ * device.  Replaces EMAlgorithm::RetrieveClasses (EM2/EMAlgorithm.cpp:107-122). */
+int fem_get_classes(fem_ctx *ctx, int *cls);
+
 /* k-means initialisation on the context's data, the step before the EM path:
  * a seeding (enum fem_seeding), up to max_sweeps Lloyd sweeps with MinRel
  * 1e-4 (Clust/KMAlgorithm.cpp:44-87,205-255; the reference's MaxIter =
@@ -162,7 +167,8 @@ int fem_batch_set_params(fem_batch *batch, int fit, const double *w, const doubl
  * fastest (report: text, may be NULL). */
 int fem_batch_tune(fem_batch *batch, char *report, int report_len);
 /* EM on all fits until each converges (|dLL/LL| < eps), fails (Cholesky) or
- * reaches maxsweeps E-steps; g = regularisation after every M-step. */
+ * reaches maxsweeps E-steps; g = the regularisation coefficient, applied to the
+ * initial covariances (g > 0) and after every M-step, as TrainNoCopy does. */
 int fem_batch_run(fem_batch *batch, int maxsweeps, double eps, double g, int use_graph);
 /* status: 1 converged, 2 sweep cap, -1 Cholesky failure; ll = LL of the last
  * E-step; iter_counter = E-steps + 1 (the reference's IterCounter at exit). */

@@ -31,6 +31,19 @@ int fem_gengmm(int64_t n, int d, int K, double mean_lo, double mean_hi, double d
 int fem_kmeanspp_init(int64_t n, int d, int K, const float *X, uint64_t seed, int lloyd_iters,
                       double *w, double *mu, double *cov);
 
+/* Condition number |lambda_max / lambda_min| of the lower-symmetrised d x d
+ * matrix C, through NormalDensity::Init + ComputeConditionNumber
+ * (EM2/NormalDensity.cpp:43-71), the check em2 runs on every component
+ * unless -S is given.  Host only.  FEM_ERR_NOT_PD if C is not positive
+ * definite, FEM_ERR_ARG if the eigenvalue iteration does not converge. */
+int fem_condition_number(int d, const double *C, double *cond);
+
+/* Bhattacharyya distance between N(mu1, C1) and N(mu2, C2) through
+ * BhattDist::ComputeDist (EM2/NormalDensity.cpp:128-138).  Host only.
+ * FEM_ERR_NOT_PD if a covariance (or their mean) is not positive definite. */
+int fem_bhatt_dist(int d, const double *mu1, const double *C1, const double *mu2, const double *C2,
+                   double *dist);
+
 #ifdef __cplusplus
 }
 #endif

@@ -28,13 +28,13 @@ def fits():
     return make_fits(7, 6)
 
 
-def run_batch(fits, use_graph, tune=False, maxsweeps=30, eps=1e-8, also_eager=False):
+def run_batch(fits, use_graph, tune=False, maxsweeps=30, eps=1e-8, also_eager=False, g=0.0):
     b = fem.Batch([(X.shape[0], X.shape[1], K) for X, K, _ in fits])
     for i, (X, K, init) in enumerate(fits):
         b.upload(i, X)
         b.set_params(i, *init)
     report = b.tune() if tune else None
-    b.run(maxsweeps, eps=eps, use_graph=use_graph)
+    b.run(maxsweeps, eps=eps, g=g, use_graph=use_graph)
     res = [b.result(i) for i in range(len(fits))]
     eager = None
     if also_eager:
@@ -74,6 +74,54 @@ def test_batch_matches_oracle(orc, fits):
         assert r["iter_counter"] == ref.rc
         assert abs(r["ll"] - ref.ll) <= 1e-10 * abs(ref.ll)
         assert np.max(np.abs(r["mu"] - ref.mu)) <= 1e-10 * np.max(np.abs(ref.mu))
+
+
+def test_batch_regularized_matches_single_and_oracle(orc, fits):
+    """g = 1e-3: the batch regularises the initial covariances before the
+    first factorisation and after every M-step (EM2/EMAlgorithm.cpp:145,158),
+    as the single-fit path and the oracle do."""
+    print("batch g=1e-3", flush=True)
+    g = 1e-3
+    res, _, _ = run_batch(fits, use_graph=True, maxsweeps=15, eps=1e-300, g=g)
+    for i, ((X, K, init), r) in enumerate(zip(fits, res)):
+        with fem.Context(X.shape[0], X.shape[1], K) as ctx:
+            ctx.upload(X)
+            it, ll, (w, mu, cov) = fem.train(ctx, *init, maxsweeps=15, eps=1e-300, g=g)
+        assert r["iter_counter"] == it
+        assert abs(r["ll"] - ll) <= 1e-12 * abs(ll)
+        assert np.max(np.abs(r["cov"] - cov)) <= 1e-12 * np.max(np.abs(cov))
+        if i < 2:
+            ref = orc.train(X, *init, maxsweeps=15, eps=1e-300, g=g, svd=False)
+            assert abs(r["ll"] - ref.ll) <= 1e-10 * abs(ref.ll)
+            assert np.max(np.abs(r["cov"] - ref.cov)) <= 1e-10 * np.max(np.abs(ref.cov))
+
+
+def test_batch_config4_ranges(orc):
+    """Fits at the edges of the many-fit workload's ranges (N up to 1e6,
+    K up to 64, every d in {8, 16, 32, 64}) in one batch: each equals the fit
+    run alone (1e-12) and the smaller ones the oracle (1e-10)."""
+    shapes = [(1000000, 64, 64), (10000, 8, 4), (300000, 32, 40), (50000, 16, 64), (20000, 64, 7)]
+    fits = []
+    for i, (n, d, K) in enumerate(shapes):
+        print(f"config4 fit n={n} d={d} K={K}", flush=True)
+        X = fem.gengmm(n, d, K, -20.0, 20.0, 0.05, 4000 + i)[0]
+        fits.append((X, K, fem.kmeanspp_init(X, K, seed=i, lloyd_iters=29)))
+    res, report, _ = run_batch(fits, use_graph=True, tune=True, maxsweeps=6, eps=1e-300, g=1e-3)
+    print(report, flush=True)
+    for i, ((X, K, init), r) in enumerate(zip(fits, res)):
+        with fem.Context(X.shape[0], X.shape[1], K) as ctx:
+            ctx.upload(X)
+            it, ll, (w, mu, cov) = fem.train(ctx, *init, maxsweeps=6, eps=1e-300, g=1e-3)
+        # regularised EM need not increase the LL; a decrease stops the fit
+        # (e = 0 < eps, EM2/EMAlgorithm.cpp:167-172) in both paths alike
+        assert r["iter_counter"] == it and 2 <= it <= 7, (i, r["iter_counter"], it)
+        assert abs(r["ll"] - ll) <= 1e-12 * abs(ll)
+        for key, ref in (("w", w), ("mu", mu), ("cov", cov)):
+            assert np.max(np.abs(r[key] - ref)) <= 1e-12 * np.max(np.abs(ref))
+        if X.shape[0] <= 50000:
+            o = orc.train(X, *init, maxsweeps=6, eps=1e-300, g=1e-3, svd=False)
+            assert abs(r["ll"] - o.ll) <= 1e-10 * abs(o.ll)
+            assert np.max(np.abs(r["cov"] - o.cov)) <= 1e-10 * np.max(np.abs(o.cov))
 
 
 def test_batch_failure_is_isolated(fits):
