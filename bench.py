@@ -7,9 +7,9 @@ initialisation.  One step = one EM iteration on the device: E-step (fused
 log-densities + log-sum-exp + posteriors), M-step means, M-step covariances,
 regularisation and Cholesky/log-det of the new covariances.
 
-Prints one JSON line.  The CPU baseline is the FP64 OpenMP oracle (a
-restatement of the reference's SimpleMatrixEM) timed on a bounded sample of the
-same workload on this host.
+Prints one JSON line.  The CPU baseline is a blocked FP64 restatement of the
+reference's SimpleMatrixEM (BLAS on row blocks, one worker per host core)
+timed on a quarter of the same workload's rows on this host.
 """
 import argparse
 import json
@@ -40,31 +40,68 @@ def algorithmic_flops(n, d, k):
 
 
 def cpu_baseline(X, w0, mu0, cov0, rows, iters):
-    """Times the FP64 oracle's E-step + M-step + factorisation on the first
-    `rows` rows and scales to the full N (the work is linear in N)."""
-    from oracle import pyoracle as orc
+    """Times the reference's algorithm shape on this host's cores: the blocked
+    FP64 restatement of SimpleMatrixEM (dtrmm / dgemm / dsyrk on row blocks
+    sized by -N automax:<L3 MB>, one worker per core, oracle/blocked_em.py) on
+    the first `rows` rows, one burn-in iteration then `iters` timed ones, and
+    scales to the full N (the work is linear in N)."""
+    from oracle import blocked_em as be
+    host = be.host_description()
+    workers = max(1, min(host["affinity"], 16))
+    l3 = host["l3_mb_per_cpu0"] or 32.0
+    nblocks = be.automax_blocks(rows, D, K, l3)
     sub = np.ascontiguousarray(X[:rows])
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    w, mu, cov = w0, mu0, cov0
-    U, ld = orc.factor_all(cov)
-    orc.estep(sub[:4096], w, mu, U, ld, nvt=threads)  # warm the library
-    done = 0
-    t0 = time.perf_counter()
-    for _ in range(iters):
-        _, R = orc.estep(sub, w, mu, U, ld, nvt=threads)
-        w, mu, cov = orc.mstep(sub, R, nvt=threads)
-        done += 1
+    done, t = 0, 0.0
+    with be.BlockedEM(sub, K, threads=workers, nblocks=nblocks) as em:
         try:
-            U, ld = orc.factor_all(cov)
-        except np.linalg.LinAlgError:
-            break
-    t = time.perf_counter() - t0
-    per_iter_full = (t / done) * (N / rows)
-    return {"value": 1.0 / per_iter_full, "unit": "EM iters/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{done} FP64 EM iterations (oracle restatement of SimpleMatrixEM, "
-                      f"OpenMP, {threads} threads) on the first {rows} of {N} rows, scaled "
-                      f"by {N}/{rows}; {t:.1f} s measured"}
+            w, mu, cov = em.iterate(w0, mu0, cov0)[1:]  # burn-in
+            for _ in range(iters):
+                t0 = time.perf_counter()
+                w, mu, cov = em.iterate(w, mu, cov)[1:]
+                t += time.perf_counter() - t0
+                done += 1
+        except np.linalg.LinAlgError:  # a covariance of the sample lost definiteness
+            pass
+    if done == 0:
+        return {"value": None, "unit": "EM iters/s", "cores": workers, "kind": "port", "host": host,
+                "sample": "no timed iteration: a covariance of the sample was not positive definite"}
+    iters = done
+    per_iter_full = (t / iters) * (N / rows)
+    return {"value": 1.0 / per_iter_full, "unit": "EM iters/s", "cores": workers, "kind": "port",
+            "host": host,
+            "sample": f"{iters} FP64 EM iterations after 1 burn-in of the blocked SimpleMatrixEM "
+                      f"restatement (dtrmm/dgemm/dsyrk through numpy's OpenBLAS, {workers} worker "
+                      f"processes, -N automax:{l3:g} -> {nblocks} blocks per partition, naive "
+                      f"reducer) on the first {rows} of {N} rows, scaled by {N}/{rows}; "
+                      f"{t:.1f} s measured"}
+
+
+def em2_modes(X, w0, mu0, cov0, iters=10, log=print):
+    """em2 at the benchmark workload in its default mode (condition-number
+    check of every covariance after each factorisation, EM2/EMAlgorithm.cpp:
+    147-151,159-163) and with -S (check off): em2's own "Iteration time"."""
+    import re
+    import subprocess
+    import tempfile
+    import fastem_amd as fem
+    from fastem_amd import formats
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        xb, ic = os.path.join(tmp, "x.bin"), os.path.join(tmp, "init.cmp")
+        formats.write_bin(xb, X)
+        formats.write_cmp(ic, w0, mu0, cov0)
+        for name, extra in (("default", []), ("no_svd", ["-S"])):
+            log(f"em2 {name}")
+            p = subprocess.run([fem.EM2_PATH, "-c", str(K), "-I", ic, "-m", str(iters), "-f", "1e-300",
+                                "-v", "0"] + extra + [xb], capture_output=True, text=True, timeout=600)
+            m = re.search(r"Iteration time: (\S+) miliseconds", p.stdout)
+            out[name + "_ms_per_iter"] = float(m.group(1)) if m else None
+            if not m:
+                out[name + "_tail"] = (p.stdout + p.stderr)[-400:]
+    a, b = out.get("default_ms_per_iter"), out.get("no_svd_ms_per_iter")
+    out["check_overhead_frac"] = (a - b) / b if a and b else None
+    out["iters"] = iters
+    return out
 
 
 def batch_workload(nfits=256, seed=7, maxsweeps=100, eps=1e-6, g=1e-3, compare=False, log=print):
@@ -257,8 +294,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the FP32-form measurement")
-    ap.add_argument("--cpu-rows", type=int, default=262144)
+    ap.add_argument("--cpu-rows", type=int, default=N // 4)
     ap.add_argument("--cpu-iters", type=int, default=2)
+    ap.add_argument("--no-em2", action="store_true",
+                    help="skip the em2 default-mode (condition-number check) measurement")
     ap.add_argument("--no-ttl", action="store_true",
                     help="skip the time-to-log-likelihood measurement")
     ap.add_argument("--no-ttl-extra", action="store_true",
@@ -500,6 +539,8 @@ def main():
     if not args.no_batch:
         log("batched fits")
         out["batch"] = batch_workload(args.batch_fits, compare=args.batch_compare, log=log)
+    if not args.no_em2:
+        out["em2_modes"] = em2_modes(X, w0, mu0, cov0, log=log)
     if not args.no_cpu_baseline:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(X, w0, mu0, cov0, args.cpu_rows, args.cpu_iters)
