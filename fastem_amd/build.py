@@ -15,7 +15,7 @@ HIP_SOURCES = ["fem_kernels.hip", "fem_cov64.hip", "fem_cov64_narrow.hip", "fem_
                "fem_batch.hip", "fem_setnet.hip", "fem_abi.hip", "fem_noalloc.hip"]
 HOST_SOURCES = ["fem_tools.cpp", "host/linalg.cpp", "host/rand.cpp", "host/block_manager.cpp",
                 "host/normal_density.cpp", "host/gaussian_mixture.cpp", "host/dataset.cpp",
-                "host/em_algorithm.cpp", "host/em_initializer.cpp"]
+                "host/em_algorithm.cpp", "host/em_initializer.cpp", "host/setnet_init.cpp"]
 TOOLS = {"em2": "host/em2_main.cpp", "eminit": "host/eminit_main.cpp"}
 # MFMA accumulators in VGPRs (no accumulator-read moves when results are
 # squared, reduced or flushed) for every file but fem_cov64.hip, whose FP64

@@ -87,6 +87,8 @@ struct fem_ctx {
 namespace fem {
 // Records msg as the error fem_last_error(NULL) reports; returns code.
 int fail_global(int code, const std::string &msg);
+// Rows idx[0..m) of the context's data to host memory (m x d floats).
+int gather_rows_host(fem_ctx *c, const int64_t *idx, int m, float *out);
 hipError_t launch_factor(fem_ctx *c);
 hipError_t launch_estep(fem_ctx *c);
 hipError_t launch_means(fem_ctx *c);

@@ -513,11 +513,11 @@ extern "C" int fem_kmeans_init_rng(fem_ctx *c, double (*rnd)(void *), void *rnd_
 
 // Lloyd sweeps and the cluster mixture from caller-supplied centroids
 // (K x d, e.g. the learned initializer's proposals); empty clusters are
-// re-seeded from a mt19937_64(seed) row draw.
-extern "C" int fem_kmeans_init_seeded(fem_ctx *c, const double *C0, uint64_t seed, int max_sweeps, double *w,
-                                      double *mu, double *cov) {
+// re-seeded from rows drawn with rnd(rnd_state), or a mt19937_64(seed) stream.
+extern "C" int fem_kmeans_init_seeded_rng(fem_ctx *c, const double *C0, double (*rnd)(void *), void *rnd_state,
+                                          int max_sweeps, double *w, double *mu, double *cov) {
     if (!c) return FEM_ERR_ARG;
-    if (!C0 || !w || !mu || !cov || max_sweeps < 0) {
+    if (!C0 || !rnd || !w || !mu || !cov || max_sweeps < 0) {
         c->err = "fem_kmeans_init_seeded: bad argument";
         return FEM_ERR_ARG;
     }
@@ -525,11 +525,50 @@ extern "C" int fem_kmeans_init_seeded(fem_ctx *c, const double *C0, uint64_t see
         KM_HIP(hipSetDevice(c->device));
         ensure_scratch(c);
         std::vector<double> C(C0, C0 + (size_t)c->K * c->d);
-        std::mt19937_64 rng(seed);
-        auto draw = [&]() { return canonical(rng); };
+        auto draw = [&]() { return rnd(rnd_state); };
         return lloyd_and_mixture(c, C, draw, max_sweeps, w, mu, cov);
     } catch (KmErr &e) {
         c->err = e.msg;
         return e.code;
     }
+}
+
+extern "C" int fem_kmeans_init_seeded(fem_ctx *c, const double *C0, uint64_t seed, int max_sweeps, double *w,
+                                      double *mu, double *cov) {
+    std::mt19937_64 rng(seed);
+    return fem_kmeans_init_seeded_rng(c, C0, mt_rand, &rng, max_sweeps, w, mu, cov);
+}
+
+// Rows idx[0..m) of the context's data copied to host memory (m x d floats).
+int fem::gather_rows_host(fem_ctx *c, const int64_t *idx, int m, float *out) {
+    if (!c || m < 0 || (m > 0 && (!idx || !out))) return FEM_ERR_ARG;
+    if (m == 0) return FEM_OK;
+    for (int j = 0; j < m; j++)
+        if (idx[j] < 0 || idx[j] >= c->n) {
+            c->err = "gather_rows_host: row index out of range";
+            return FEM_ERR_ARG;
+        }
+    int64_t *didx = nullptr;
+    float *dout = nullptr;
+    auto done = [&](int rc, const char *what, hipError_t e) {
+        if (didx) (void)hipFree(didx);
+        if (dout) (void)hipFree(dout);
+        if (rc != FEM_OK) c->err = std::string(what) + ": " + hipGetErrorString(e);
+        return rc;
+    };
+    hipError_t e;
+    if ((e = hipSetDevice(c->device)) != hipSuccess) return done(FEM_ERR_HIP, "hipSetDevice", e);
+    if ((e = hipMalloc((void **)&didx, sizeof(int64_t) * m)) != hipSuccess ||
+        (e = hipMalloc((void **)&dout, sizeof(float) * (size_t)m * c->d)) != hipSuccess)
+        return done(FEM_ERR_HIP, "hipMalloc", e);
+    if ((e = hipMemcpyAsync(didx, idx, sizeof(int64_t) * m, hipMemcpyHostToDevice, c->stream)) != hipSuccess)
+        return done(FEM_ERR_HIP, "copy", e);
+    hipLaunchKernelGGL(fem::k_gather_rows, dim3((unsigned)(((int64_t)m * c->d + 255) / 256)), dim3(256), 0,
+                       c->stream, c->X, c->dpad, c->d, didx, m, dout);
+    if ((e = hipGetLastError()) != hipSuccess) return done(FEM_ERR_HIP, "gather", e);
+    if ((e = hipMemcpyAsync(out, dout, sizeof(float) * (size_t)m * c->d, hipMemcpyDeviceToHost, c->stream)) !=
+            hipSuccess ||
+        (e = hipStreamSynchronize(c->stream)) != hipSuccess)
+        return done(FEM_ERR_HIP, "copy", e);
+    return done(FEM_OK, "", hipSuccess);
 }

@@ -965,6 +965,13 @@ const char *fem_setnet_last_error(const fem_setnet *s) { return s ? s->err.c_str
 
 int fem_setnet_num_params(const fem_setnet *s) { return s ? s->off.total : 0; }
 
+int fem_setnet_shape(const fem_setnet *s, int *d, int *K) {
+    if (!s) return FEM_ERR_ARG;
+    if (d) *d = s->d;
+    if (K) *K = s->K;
+    return FEM_OK;
+}
+
 int fem_setnet_get_params(fem_setnet *s, float *theta) {
     if (!s || !theta) return FEM_ERR_ARG;
     SH(s, hipMemcpy(theta, s->theta, sizeof(float) * s->off.total, hipMemcpyDeviceToHost));

@@ -29,7 +29,7 @@ static void ExitUsage() {
     printf("-O filename : writes classes in a binary format to the file filename\n");
     printf("-r seed : sets the seed of pseudo-random number generator\n");
     printf("-c components : selects the number of Gaussian mixture components\n");
-    printf("-I init : jain | kmeans | kmeansoror | kmeanspp | file.cmp\n");
+    printf("-I init : jain | kmeans | kmeansoror | kmeanspp | learned:net.bin | learnedbest:net.bin | file.cmp\n");
     printf("-m emiter : sets the maximal number of EM iterations\n");
     printf("-b burnin : sets the number of burn-in EM iterations (default is 0)\n");
     printf("-f eps : sets the Eps EM stopping threshold (default 1e-5)\n");

@@ -4,20 +4,23 @@ include/fastem_setnet.h; kernels in csrc/fem_setnet.hip).
 
 The reference has no such initializer: it starts EM from the Jain, k-means,
 k-means++ or file initializers (EM2/EMInitializer.cpp).  Here the network
-proposes the starting means and they replace the k-means++ seeding pass
-(`init_context`: Lloyd sweeps and the cluster mixture follow on the device as
-for k-means++); k-means++ stays the fallback, and the final EM
-log-likelihood is the measure of each proposal.
+proposes the starting means and they replace the k-means++ seeding pass; the
+proposal itself runs in the library's C entries (fem_setnet_propose,
+fem_setnet_judge; `init_context`, `judge`): Lloyd sweeps and the cluster
+mixture follow on the device as for k-means++, k-means++ stays the fallback,
+and `judge` keeps whichever of the learned and the k-means++ start reaches
+the higher final EM log-likelihood.  em2 and eminit reach the same entries
+through `-I learned:<file>` and `-I learnedbest:<file>`.
 
 Two heads propose means: the seed-query head (K attention-pooled queries ->
 means and log-variances) and the point head (a learned mean-shift step that
 moves every point of the subsample towards its cluster's mean; k-means++
 seeding over the moved points then picks one seed per cluster).  The point
-head's proposal is the one `init_context` uses.
+head's proposal is the one the initializer uses.
 
-Inference (`propose_*`, `init_context`) runs the network's products on BF16
-MFMA (fem_setnet_forward_bf16) unless bf16=False; training uses the FP32
-forward.
+Inference runs the network's products on BF16 MFMA (fem_setnet_forward_bf16);
+training uses the FP32 forward.  Weights are stored as raw files
+(fem_setnet_save / fem_setnet_load).
 
 Coordinates: a point set X is normalised by its own centroid c and one
 scale s (root mean square deviation over all coordinates), so the network
@@ -27,7 +30,28 @@ import ctypes
 
 import numpy as np
 
-from . import FemError, _fp, _vp, lib
+from . import FemError, _d, _fp, _ip, _vp, lib
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_SIGS = {
+    "fem_setnet_save": (ctypes.c_int, [_vp, ctypes.c_char_p]),
+    "fem_setnet_load": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_char_p, ctypes.c_int]),
+    "fem_setnet_shape": (ctypes.c_int, [_vp, _ip, _ip]),
+    "fem_setnet_propose_means": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_uint64, _dp]),
+    "fem_setnet_propose": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, _dp, _dp,
+                                          _dp, _ip]),
+    "fem_setnet_judge": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_double, ctypes.c_double, _dp, _dp, _dp, _ip, _dp]),
+}
+
+
+def _lib():
+    L = lib()
+    for name, (res, args) in _SIGS.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
 
 
 def normalise(X):
@@ -50,22 +74,6 @@ def subsample(X, n, seed):
     return X[idx]
 
 
-def kmeanspp_points(P, K, seed):
-    """k-means++ seeding (first point uniform, then D^2 sampling) over the
-    rows of P with numpy generator `seed`; K x d."""
-    rng = np.random.default_rng(seed)
-    n = P.shape[0]
-    C = np.empty((K, P.shape[1]))
-    C[0] = P[rng.integers(n)]
-    d2 = ((P - C[0]) ** 2).sum(axis=1)
-    for k in range(1, K):
-        tot = d2.sum()
-        i = rng.choice(n, p=d2 / tot) if tot > 0 else rng.integers(n)
-        C[k] = P[i]
-        d2 = np.minimum(d2, ((P - C[k]) ** 2).sum(axis=1))
-    return C
-
-
 def point_targets(Z, M):
     """Point-head targets: the nearest of the fitted means M (K x d) to every
     row of Z (same normalised units)."""
@@ -83,14 +91,16 @@ def fit_to_targets(w, mu, cov, c, s):
 class SetNet:
     """FP32 set network on one HIP device (fem_setnet)."""
 
-    def __init__(self, d, K, seed=0, device=0):
+    def __init__(self, d, K, seed=0, device=0, _handle=None):
         self.d, self.K = int(d), int(K)
-        h = _vp()
-        rc = lib().fem_setnet_create(ctypes.byref(h), self.d, self.K, int(seed), device)
-        if rc != 0:
-            raise FemError(rc, "fem_setnet_create failed")
-        self._h = h
-        self.nparams = lib().fem_setnet_num_params(h)
+        if _handle is None:
+            h = _vp()
+            rc = lib().fem_setnet_create(ctypes.byref(h), self.d, self.K, int(seed), device)
+            if rc != 0:
+                raise FemError(rc, "fem_setnet_create failed")
+            _handle = h
+        self._h = _handle
+        self.nparams = lib().fem_setnet_num_params(self._h)
 
     def _check(self, rc):
         if rc != 0:
@@ -124,14 +134,20 @@ class SetNet:
         self._check(lib().fem_setnet_set_params(self._h, th.ctypes.data_as(_fp)))
 
     def save(self, path):
-        np.savez(path, d=self.d, K=self.K, theta=self.params())
+        """Raw weight file (fem_setnet_save)."""
+        self._check(_lib().fem_setnet_save(self._h, str(path).encode()))
 
     @classmethod
     def load(cls, path, device=0):
-        z = np.load(path, allow_pickle=False)
-        net = cls(int(z["d"]), int(z["K"]), device=device)
-        net.set_params(z["theta"])
-        return net
+        """Network from a raw weight file (fem_setnet_load)."""
+        L = _lib()
+        h = _vp()
+        rc = L.fem_setnet_load(ctypes.byref(h), str(path).encode(), device)
+        if rc != 0:
+            raise FemError(rc, L.fem_last_error(None).decode())
+        d, K = ctypes.c_int(), ctypes.c_int()
+        L.fem_setnet_shape(h, ctypes.byref(d), ctypes.byref(K))
+        return cls(d.value, K.value, _handle=h)
 
     def forward(self, Xs, points=False, bf16=False):
         """Xs: (B, n, d) normalised sets -> (mu, lv), each (B, K, d), and with
@@ -174,16 +190,17 @@ class SetNet:
                                           ctypes.byref(loss)))
         return (loss.value, grad) if want_grad else loss.value
 
-    def propose_means(self, X, n_sub=1024, seed=0, bf16=True):
-        """K starting means for dataset X (K x d, FP64): the point head moves
-        each point of an n_sub subsample towards its cluster's mean, and
-        k-means++ seeding (D^2 sampling, numpy generator `seed`) over the moved
-        points picks one per cluster.  Inference on BF16 MFMA unless
-        bf16=False."""
-        Xs = subsample(np.asarray(X), n_sub, seed)
-        Z, c, s = normalise(Xs)
-        _, _, m = self.forward(Z[None].astype(np.float32), points=True, bf16=bf16)
-        return c + s * kmeanspp_points(m[0].astype(np.float64), self.K, seed)
+    def propose_means(self, ctx, n_sub=2048, seed=0):
+        """The network's K starting means (K x d, FP64) for the dataset held by
+        ctx: the point head moves each point of an n_sub subsample towards its
+        cluster's mean and k-means++ seeding over the moved points picks one
+        per cluster (fem_setnet_propose_means; std::mt19937_64(seed) as the
+        Rand() stream)."""
+        C0 = np.empty((self.K, self.d))
+        rc = _lib().fem_setnet_propose_means(self._h, ctx._h, int(n_sub), int(seed), _d(C0))
+        if rc != 0:
+            raise FemError(rc, lib().fem_last_error(ctx._h).decode())
+        return C0
 
     def propose_query_means(self, X, n_sub=1024, seed=0, bf16=True):
         """The seed-query head's K means for dataset X (K x d, FP64)."""
@@ -192,19 +209,34 @@ class SetNet:
         mu, _ = self.forward(Z[None].astype(np.float32), bf16=bf16)
         return c + s * mu[0].astype(np.float64)
 
-    def init_context(self, ctx, X, n_sub=1024, seed=0, lloyd_sweeps=29, fallback_seed=None, bf16=True):
-        """Learned initialisation of a device context holding X: the proposed
-        means seed Lloyd sweeps and the cluster mixture on the device
-        (fem_kmeans_init_seeded) in place of the k-means++ seeding pass over
-        all n points.  If that leaves a cluster with fewer than two points,
-        device k-means++ (fem_kmeans_init) is the fallback.  Returns
-        (w, mu, cov, used)."""
-        C0 = self.propose_means(X, n_sub=n_sub, seed=seed, bf16=bf16)
-        try:
-            return ctx.kmeans_init_seeded(C0, seed=seed, max_sweeps=lloyd_sweeps) + ("learned",)
-        except FemError:
-            fs = seed if fallback_seed is None else fallback_seed
-            return ctx.kmeans_init(fs, seeding=1, max_sweeps=lloyd_sweeps) + ("kmeans++",)
+    def init_context(self, ctx, X=None, n_sub=2048, seed=0, lloyd_sweeps=29):
+        """Learned initialisation of a device context (fem_setnet_propose):
+        the proposed means seed Lloyd sweeps and the cluster mixture on the
+        device in place of the k-means++ seeding pass; k-means++ on the same
+        Rand() stream is the fallback if a cluster keeps fewer than two
+        points.  X (ignored) is accepted for older callers.  Returns
+        (w, mu, cov, used) with used "learned" or "kmeans++"."""
+        w, mu, cov = np.empty(self.K), np.empty((self.K, self.d)), np.empty((self.K, self.d, self.d))
+        used = ctypes.c_int()
+        rc = _lib().fem_setnet_propose(self._h, ctx._h, int(n_sub), int(seed), int(lloyd_sweeps), _d(w), _d(mu),
+                                       _d(cov), ctypes.byref(used))
+        if rc != 0:
+            raise FemError(rc, lib().fem_last_error(ctx._h).decode())
+        return w, mu, cov, ("learned" if used.value == 1 else "kmeans++")
+
+    def judge(self, ctx, n_sub=2048, seed=0, lloyd_sweeps=29, em_sweeps=1000, eps=1e-5, g=0.0):
+        """EM from the learned and from the k-means++ initialisation, the
+        higher final log-likelihood kept (fem_setnet_judge).  Returns
+        (w, mu, cov, winner, (ll_learned, ll_kmeanspp)) with the converged
+        mixture; winner "learned" or "kmeans++"."""
+        w, mu, cov = np.empty(self.K), np.empty((self.K, self.d)), np.empty((self.K, self.d, self.d))
+        used = ctypes.c_int()
+        ll = np.empty(2)
+        rc = _lib().fem_setnet_judge(self._h, ctx._h, int(n_sub), int(seed), int(lloyd_sweeps), int(em_sweeps),
+                                     float(eps), float(g), _d(w), _d(mu), _d(cov), ctypes.byref(used), _d(ll))
+        if rc != 0:
+            raise FemError(rc, lib().fem_last_error(ctx._h).decode())
+        return w, mu, cov, ("learned" if used.value == 1 else "kmeans++"), (ll[0], ll[1])
 
     def propose(self, X, n_sub=1024, seed=0, var_floor=1e-6, bf16=True):
         """Starting mixture (w, mu, cov) for dataset X: uniform weights,

@@ -1,7 +1,7 @@
 """Training and evaluation pipeline of the learned initializer.
 
   python -m fastem_amd.setnet_train --d 32 --K 16 --n 65536 --fits 20000 \\
-      --eval 1000 --out fastem_amd/data/setnet_d32_k16.npz --report report.json
+      --eval 1000 --out fastem_amd/data/setnet_d32_k16.bin --report report.json
 
 1. Targets: `--fits` synthetic mixtures (fem_gengmm, seeds 1..fits) are fitted
    to convergence by the batched device EM engine (fem_batch_*), started from
@@ -128,8 +128,9 @@ def evaluate(net, seeds, n, d, K, gen, n_sub=1024, eps=1e-5, maxsweeps=1000, llo
     Both start with Lloyd sweeps and the cluster mixture on the device; they
     differ in the seeds: the network's means (fem_kmeans_init_seeded) or the
     k-means++ seeding pass (fem_kmeans_init).  EM then runs to the reference's
-    stopping rule |dLL/LL| < eps for both in one batch.  bf16: inference on
-    BF16 MFMA (False: the FP32 forward)."""
+    stopping rule |dLL/LL| < eps for both in one batch.  The learned start is
+    the library's proposal (BF16 inference, fem_setnet_propose); bf16 selects
+    the forward of the seed-query head's proposal (False: the FP32 forward)."""
     lo, hi, diag = gen
     recs = []
     ctx = fem.Context(n, d, K)
@@ -147,8 +148,7 @@ def evaluate(net, seeds, n, d, K, gen, n_sub=1024, eps=1e-5, maxsweeps=1000, llo
             t_kpp = time.perf_counter() - t
             t = time.perf_counter()
             try:
-                w, mu, cov, used = net.init_context(ctx, X, n_sub=n_sub, seed=int(s),
-                                                    lloyd_sweeps=lloyd_iters, bf16=bf16)
+                w, mu, cov, used = net.init_context(ctx, n_sub=n_sub, seed=int(s), lloyd_sweeps=lloyd_iters)
                 lrn = (w, mu, cov)
             except fem.FemError:
                 lrn, used = None, "failed"
@@ -244,7 +244,7 @@ def main(argv=None):
     ap.add_argument("--report", default=None)
     ap.add_argument("--net-seed", type=int, default=0)
     ap.add_argument("--load", default=None,
-                    help="evaluate a saved network (.npz) instead of generating targets and training")
+                    help="evaluate a saved network (.bin) instead of generating targets and training")
     ap.add_argument("--fp32-inference", action="store_true",
                     help="evaluate with the FP32 forward instead of the BF16-MFMA inference forward")
     a = ap.parse_args(argv)

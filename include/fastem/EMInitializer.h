@@ -1,8 +1,10 @@
 // EMInitializer.h -- initial mixtures (EM2/EMInitializer.h:16-80).
 // CreateEMInitializer names: "jain" (default), "kmeans" (Forgy + Lloyd),
-// "kmeansoror" (k-means|| + Lloyd), "kmeanspp" (k-means++ + Lloyd), or a path
-// to a .cmp file.
+// "kmeansoror" (k-means|| + Lloyd), "kmeanspp" (k-means++ + Lloyd),
+// "learned:<file>", "learnedbest:<file>", or a path to a .cmp file.
 #pragma once
+#include <string>
+
 #include "GaussianMixture.h"
 #include "MultithreadedDataset.h"
 
@@ -74,6 +76,24 @@ protected:
 public:
     KMeansOrOrEMInitializer(MultithreadedDataset &D, int nC) : KMeansEMInitializer(D, nC) {}
     void PrintInfo() override;
+};
+
+// The learned initializer (include/fastem_setnet.h), no reference
+// counterpart: "learned:<file>" proposes with the set network stored in
+// <file> (2048-row subsample; k-means++ if a cluster keeps fewer than two
+// points); "learnedbest:<file>" runs EM (eps 1e-5, at most 1000 sweeps, no
+// regularisation) from that proposal and from k-means++ and returns the
+// converged mixture with the higher final log-likelihood.  Draws come from
+// the global Rand() stream.
+class LearnedEMInitializer : public EMInitializer {
+    std::string File;
+    bool Judged;
+
+public:
+    static constexpr int SubsampleRows = 2048;
+    LearnedEMInitializer(MultithreadedDataset &D, int nC, const char *file, bool judged);
+    void PrintInfo() override;
+    void Init(GaussianMixture &G) override;
 };
 
 EMInitializer *CreateEMInitializer(const char *name, MultithreadedDataset &Data, int nC);

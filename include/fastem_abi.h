@@ -143,8 +143,11 @@ int fem_kmeans_init_rng(fem_ctx *ctx, double (*rnd)(void *), void *rnd_state, in
  * centroids C0[K][d] instead of a seeding pass (the reference's
  * FileEMInitializer path reads a finished mixture, EM2/EMInitializer.cpp:148-151;
  * this variant is used by the learned initializer, whose proposals replace the
- * k-means++ seeding).  Empty clusters are re-seeded from rows drawn with
- * std::mt19937_64(seed).  Same result and error convention as fem_kmeans_init. */
+ * seeding).  Empty clusters are re-seeded from rows drawn with
+ * std::mt19937_64(seed), or rnd(rnd_state) for the _rng variant.  Same result
+ * and error convention as fem_kmeans_init. */
+int fem_kmeans_init_seeded_rng(fem_ctx *ctx, const double *C0, double (*rnd)(void *), void *rnd_state,
+                               int max_sweeps, double *w, double *mu, double *cov);
 int fem_kmeans_init_seeded(fem_ctx *ctx, const double *C0, uint64_t seed, int max_sweeps, double *w,
                            double *mu, double *cov);
 

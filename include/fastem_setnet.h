@@ -31,6 +31,8 @@
 #define FASTEM_SETNET_H
 #include <stdint.h>
 
+#include "fastem_abi.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -43,6 +45,7 @@ int fem_setnet_create(fem_setnet **net, int d, int K, uint64_t seed, int device)
 int fem_setnet_destroy(fem_setnet *net);
 const char *fem_setnet_last_error(const fem_setnet *net);
 int fem_setnet_num_params(const fem_setnet *net);
+int fem_setnet_shape(const fem_setnet *net, int *d, int *K);
 int fem_setnet_get_params(fem_setnet *net, float *theta);
 /* Loads parameters and resets the Adam state. */
 int fem_setnet_set_params(fem_setnet *net, const float *theta);
@@ -65,6 +68,54 @@ int fem_setnet_forward_bf16(fem_setnet *net, int B, int n, const float *X, float
  * gradient of the batch-mean loss. */
 int fem_setnet_step(fem_setnet *net, int B, int n, const float *X, int J, const float *M, const float *V,
                     const float *p, const float *T, float lam, float gam, float lr, float *grad, float *loss);
+
+/* ---- Weight files ----------------------------------------------------------
+ * Raw little-endian file: 8 bytes "FEMSNET1", int32 d, int32 K, int32 count,
+ * then count FP32 parameters in the order above.  fem_setnet_load creates the
+ * network on `device` (errors through fem_last_error(NULL)). */
+int fem_setnet_save(fem_setnet *net, const char *path);
+int fem_setnet_load(fem_setnet **net, const char *path, int device);
+
+/* ---- The learned initializer -------------------------------------------------
+ * For the dataset held by a fem_ctx of the network's d and K.  One Rand()
+ * stream, rnd(rnd_state) (or std::mt19937_64(seed) through generate_canonical,
+ * Util/Rand.cpp:7-14), drives every draw, in this order:
+ *  1. n_sub (>= K) row indices (long)(Rand() * N), with replacement;
+ *  2. the rows are normalised by their column means c and one scale s (the
+ *     root mean square deviation over all coordinates), the BF16 forward runs
+ *     on them, and the point head's outputs m_i (each point moved towards its
+ *     cluster's mean) are seeded by the k-means++ rule of
+ *     Clust/PlusPlusInitializer/PlusPlusInitializer.cpp:33-84 (first row
+ *     uniform, then D^2 draws);
+ *  3. the proposal C0 = c + s * seeds (K x d) goes through the device Lloyd
+ *     sweeps and cluster mixture of fem_kmeans_init_seeded_rng.
+ * fem_setnet_propose_means[_rng] stop after step 2 and write C0.
+ * fem_setnet_propose[_rng] runs all three; if the proposal leaves a cluster
+ * with fewer than two points, k-means++ (fem_kmeans_init_rng) on the same
+ * stream is the fallback.  *used = 1 for the network's proposal, 0 for the
+ * fallback.  The result becomes the context's parameters.
+ * fem_setnet_judge[_rng] judges the proposal by the final EM log-likelihood:
+ * it takes the learned initialisation and then the k-means++ one (same
+ * stream), runs EM from each on the context to the reference's stopping rule
+ * (regularisation g before the loop and after each M-step, stop at
+ * |dLL/LL| < eps, or an LL decrease, or em_sweeps E-steps;
+ * EM2/EMAlgorithm.cpp:139-191), and keeps the converged mixture with the
+ * higher final log-likelihood (the learned one on a tie): *used = 1 learned,
+ * 0 learned path's k-means++ fallback, 2 the k-means++ start; ll[2] (may be
+ * NULL) = final LLs from the learned and the k-means++ starts (-inf where EM
+ * failed).  Errors through fem_last_error(ctx). */
+int fem_setnet_propose_means_rng(fem_setnet *net, fem_ctx *ctx, int n_sub, double (*rnd)(void *),
+                                 void *rnd_state, double *C0);
+int fem_setnet_propose_means(fem_setnet *net, fem_ctx *ctx, int n_sub, uint64_t seed, double *C0);
+int fem_setnet_propose_rng(fem_setnet *net, fem_ctx *ctx, int n_sub, double (*rnd)(void *), void *rnd_state,
+                           int max_sweeps, double *w, double *mu, double *cov, int *used);
+int fem_setnet_propose(fem_setnet *net, fem_ctx *ctx, int n_sub, uint64_t seed, int max_sweeps, double *w,
+                       double *mu, double *cov, int *used);
+int fem_setnet_judge_rng(fem_setnet *net, fem_ctx *ctx, int n_sub, double (*rnd)(void *), void *rnd_state,
+                         int max_sweeps, int em_sweeps, double eps, double g, double *w, double *mu, double *cov,
+                         int *used, double *ll);
+int fem_setnet_judge(fem_setnet *net, fem_ctx *ctx, int n_sub, uint64_t seed, int max_sweeps, int em_sweeps,
+                     double eps, double g, double *w, double *mu, double *cov, int *used, double *ll);
 
 #ifdef __cplusplus
 }

@@ -169,7 +169,7 @@ def test_trained_network_gains_over_kmeanspp():
     from fastem_amd import setnet_train as st
     from fastem_amd.setnet import SetNet
     print("setnet gain over k-means++", flush=True)
-    path = os.path.join(os.path.dirname(__file__), "..", "fastem_amd", "data", "setnet_d32_k16.npz")
+    path = os.path.join(os.path.dirname(__file__), "..", "fastem_amd", "data", "setnet_d32_k16.bin")
     with SetNet.load(path) as net:
         recs = st.evaluate(net, np.arange(20_000_000, 20_000_024), 65536, 32, 16, st.GEN[(32, 16)],
                            n_sub=1024, eps=1e-5, chunk=24, log=lambda *a: None)
