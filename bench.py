@@ -424,7 +424,7 @@ def main():
             variants = [("fp32", [(fem.PREC_FP32, chk)], None),
                         ("bf16x3", [(fem.PREC_BF16X3, chk)], None),
                         ("bf16x3/switch1e-9", [(fem.PREC_BF16X3, 1e-9)], None)]
-            net_path = os.path.join(ROOT, "fastem_amd", "data", f"setnet_d{D}_k{K}.npz")
+            net_path = os.path.join(ROOT, "fastem_amd", "data", f"setnet_d{D}_k{K}.bin")
             learned = None
             if os.path.exists(net_path):
                 from fastem_amd.setnet import SetNet

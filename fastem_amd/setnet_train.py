@@ -238,6 +238,8 @@ def main(argv=None):
     ap.add_argument("--eval-seed0", type=int, default=10_000_000)
     ap.add_argument("--eval-eps", type=float, default=1e-5)
     ap.add_argument("--eval-n", type=int, default=None, help="points per evaluation dataset (default --n)")
+    ap.add_argument("--eval-chunk", type=int, default=128,
+                    help="evaluation seeds per batched EM run (3 fits each; bounds device memory)")
     ap.add_argument("--g", type=float, default=0.0,
                     help="covariance regularisation of the target fits and of the evaluation EM")
     ap.add_argument("--out", default=None)
@@ -268,7 +270,7 @@ def main(argv=None):
             net.save(a.out)
             log(f"saved {a.out}")
     recs = evaluate(net, np.arange(a.eval_seed0, a.eval_seed0 + a.eval), a.eval_n or a.n, a.d, a.K, gen,
-                    n_sub=a.keep, eps=a.eval_eps, g=a.g, log=log,
+                    n_sub=a.keep, eps=a.eval_eps, g=a.g, log=log, chunk=a.eval_chunk,
                     bf16=not a.fp32_inference) if a.eval > 0 else []
     summ = summarise(recs) if recs else {}
     summ["inference"] = "fp32" if a.fp32_inference else "bf16 mfma"
