@@ -387,8 +387,8 @@ class Batch:
         self._check(lib().fem_batch_set_params(self._h, fit, _d(w), _d(mu), _d(cov)))
 
     def tune(self):
-        buf = ctypes.create_string_buffer(4096)
-        self._check(lib().fem_batch_tune(self._h, buf, 4096))
+        buf = ctypes.create_string_buffer(65536)
+        self._check(lib().fem_batch_tune(self._h, buf, 65536))
         return buf.value.decode()
 
     def run(self, maxsweeps, eps=1e-5, g=0.0, use_graph=True):

@@ -14,6 +14,8 @@
 // fastest per phase.
 #include <hip/hip_runtime.h>
 
+#include <map>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -32,7 +34,8 @@ struct BFit {
     int64_t n, npad, ntotal;
     int d, dpad, NB, K, KP, npairs, ub;
     int m_chunks, c_chunks;
-    int64_t c_chunk_rows;
+    int eg, cmul;                       // tile shape: E-step point groups (2, 4), chunk multiplier (1, 2, 4)
+    int64_t c_chunk_base, c_chunk_rows; // covariance chunk rows: base (<= 256 chunks) times cmul
     int64_t x;                          // float pool
     int64_t r;                          // double pool: posteriors KP x npad
     int64_t w, mu, cov, nk, logdet, coef, U, stage64, llp, mpart, cpart;  // double pool
@@ -69,13 +72,15 @@ struct fem_batch {
     int *ipool = nullptr;
     int64_t nx = 0, nd = 0, ni = 0;
     // work lists
-    Item *est2 = nullptr, *est4 = nullptr, *means = nullptr, *cov = nullptr, *comp = nullptr, *covred = nullptr;
-    int n_est2 = 0, n_est4 = 0, n_means = 0, n_cov = 0, n_comp = 0, n_covred = 0;
+    // E-step tiles of the fits with 2 (estA) and 4 (estB) point groups per
+    // wave, covariance (fit, component group, chunk) items, and the
+    // shape-independent lists
+    Item *estA = nullptr, *estB = nullptr, *means = nullptr, *cov = nullptr, *comp = nullptr, *covred = nullptr;
+    int n_estA = 0, n_estB = 0, n_means = 0, n_cov = 0, n_comp = 0, n_covred = 0;
     int maxNB = 1;
     int cus = 256;
-    // tuned launch shape
-    int est_groups = 2;
-    int grid_est = 0, grid_means = 0, grid_cov = 0;
+    // tuned launch grids
+    int grid_estA = 0, grid_estB = 0, grid_means = 0, grid_cov = 0;
     // run parameters
     int maxsweeps = 100;
     double eps = 1e-5, g = 0.0;
@@ -153,12 +158,12 @@ __global__ __launch_bounds__(256) void kb_estep(const BFit *__restrict__ F, cons
 
 // LL of each active fit: fixed-order sum of its E-step tile partials.
 __global__ __launch_bounds__(256) void kb_ll(const BFit *__restrict__ F, BState *__restrict__ S,
-                                             const double *__restrict__ dp, int est_groups) {
+                                             const double *__restrict__ dp) {
     __shared__ double s[256];
     const int fi = blockIdx.x;
     if (!S[fi].active) return;
     const BFit &f = F[fi];
-    const double r = block_sum(dp + f.llp, f.npad / (64 * est_groups), s);
+    const double r = block_sum(dp + f.llp, f.npad / (64 * f.eg), s);
     if (threadIdx.x == 0) S[fi].ll = r;
 }
 
@@ -302,16 +307,13 @@ int grid_of(int mult, int cus, int nitems) { return std::max(1, std::min(nitems,
 hipError_t launch_iteration(fem_batch *b) {
     const size_t lds_e = (size_t)fem::estep64_lds<4>() * sizeof(double);
     const size_t lds_f = (size_t)64 * 64 * sizeof(double);
-    const Item *est = b->est_groups == 4 ? b->est4 : b->est2;
-    const int nest = b->est_groups == 4 ? b->n_est4 : b->n_est2;
-    if (b->est_groups == 4)
-        hipLaunchKernelGGL(fem::kb_estep<4>, dim3(b->grid_est), dim3(256), lds_e, b->stream, b->dfits, b->dstate,
-                           b->xpool, b->dpool, est, nest);
-    else
-        hipLaunchKernelGGL(fem::kb_estep<2>, dim3(b->grid_est), dim3(256), lds_e, b->stream, b->dfits, b->dstate,
-                           b->xpool, b->dpool, est, nest);
-    hipLaunchKernelGGL(fem::kb_ll, dim3(b->F), dim3(256), 0, b->stream, b->dfits, b->dstate, b->dpool,
-                       b->est_groups);
+    if (b->n_estA > 0)
+        hipLaunchKernelGGL(fem::kb_estep<2>, dim3(b->grid_estA), dim3(256), lds_e, b->stream, b->dfits, b->dstate,
+                           b->xpool, b->dpool, b->estA, b->n_estA);
+    if (b->n_estB > 0)
+        hipLaunchKernelGGL(fem::kb_estep<4>, dim3(b->grid_estB), dim3(256), lds_e, b->stream, b->dfits, b->dstate,
+                           b->xpool, b->dpool, b->estB, b->n_estB);
+    hipLaunchKernelGGL(fem::kb_ll, dim3(b->F), dim3(256), 0, b->stream, b->dfits, b->dstate, b->dpool);
     hipLaunchKernelGGL(fem::kb_means, dim3(b->grid_means), dim3(256), 0, b->stream, b->dfits, b->dstate, b->xpool,
                        b->dpool, b->means, b->n_means);
     hipLaunchKernelGGL(fem::kb_means_finalize, dim3(grid_of(4, b->cus, b->n_comp)), dim3(256), 0, b->stream,
@@ -332,7 +334,7 @@ hipError_t launch_iteration(fem_batch *b) {
 void free_batch(fem_batch *b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
-    void *ptrs[] = {b->dfits, b->dstate, b->xpool, b->dpool, b->ipool, b->est2, b->est4,
+    void *ptrs[] = {b->dfits, b->dstate, b->xpool, b->dpool, b->ipool, b->estA, b->estB,
                     b->means, b->cov, b->comp, b->covred};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -345,6 +347,58 @@ hipError_t upload_vec(T **dst, const std::vector<T> &v) {
     if (e != hipSuccess) return e;
     if (v.empty()) return hipSuccess;
     return hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+// Large work lists interleaved across fits (ordered by item index, then
+// fit) so that a persistent grid sees a mix of fits.
+void interleave(std::vector<Item> &v) {
+    std::stable_sort(v.begin(), v.end(), [](const Item &x, const Item &y) {
+        if (x.a != y.a) return x.a < y.a;
+        return x.fit < y.fit;
+    });
+}
+
+// E-step tiles of fit f under P point groups per wave; covariance items of
+// fit f under its current chunk rows.
+void estep_items(std::vector<Item> &v, int i, const BFit &f, int P) {
+    for (int64_t t = 0; t < f.npad / (64 * P); t++) v.push_back({i, (int)t, 0, 0});
+}
+void cov_items(std::vector<Item> &v, int i, const BFit &f) {
+    for (int ch = 0; ch < f.c_chunks; ch++)
+        for (int cg = 0; cg < f.KP / 4; cg++)
+            if (4 * cg < f.K) v.push_back({i, cg, ch, 0});
+}
+
+void set_chunks(BFit &f) {
+    f.c_chunk_rows = f.c_chunk_base * f.cmul;
+    f.c_chunks = (int)((f.npad + f.c_chunk_rows - 1) / f.c_chunk_rows);
+}
+
+template <typename T>
+hipError_t replace_list(T **dst, int *count, const std::vector<T> &v) {
+    if (*dst) (void)hipFree(*dst);
+    *dst = nullptr;
+    *count = (int)v.size();
+    return upload_vec(dst, v);
+}
+
+// (Re)builds the lists that depend on the fits' tile shapes from b->fits
+// and uploads the descriptors.
+hipError_t build_shape_lists(fem_batch *b) {
+    std::vector<Item> a, c4, cv;
+    for (int i = 0; i < b->F; i++) {
+        BFit &f = b->fits[i];
+        set_chunks(f);
+        estep_items(f.eg == 4 ? c4 : a, i, f, f.eg);
+        cov_items(cv, i, f);
+    }
+    interleave(a);
+    interleave(c4);
+    hipError_t e;
+    if ((e = replace_list(&b->estA, &b->n_estA, a)) != hipSuccess) return e;
+    if ((e = replace_list(&b->estB, &b->n_estB, c4)) != hipSuccess) return e;
+    if ((e = replace_list(&b->cov, &b->n_cov, cv)) != hipSuccess) return e;
+    return hipMemcpy(b->dfits, b->fits.data(), sizeof(BFit) * b->F, hipMemcpyHostToDevice);
 }
 
 }  // namespace
@@ -369,7 +423,7 @@ int fem_batch_create(fem_batch **out, int nfits, const int64_t *n, const int *d,
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
     b->cus = cus;
-    std::vector<Item> est2, est4, means, cov, comp, covred;
+    std::vector<Item> means, comp, covred;
     for (int i = 0; i < nfits; i++) {
         BFit f{};
         f.n = n[i];
@@ -384,8 +438,10 @@ int fem_batch_create(fem_batch **out, int nfits, const int64_t *n, const int *d,
         f.ub = fem::upper_blocks(f.NB);
         f.m_chunks = (int)(f.npad / 4096);
         const int64_t units = f.npad / 4096, per = (units + 255) / 256;
-        f.c_chunk_rows = per * 4096;
-        f.c_chunks = (int)((f.npad + f.c_chunk_rows - 1) / f.c_chunk_rows);
+        f.eg = 2;
+        f.cmul = 1;
+        f.c_chunk_base = per * 4096;
+        set_chunks(f);  // the most chunks of any multiplier: cpart is sized for it
         f.x = b->nx;
         b->nx += f.npad * f.dpad;
         auto take = [&](int64_t cnt) {
@@ -409,30 +465,13 @@ int fem_batch_create(fem_batch **out, int nfits, const int64_t *n, const int *d,
         b->ni += f.K;
         b->fits.push_back(f);
         b->maxNB = std::max(b->maxNB, f.NB);
-        for (int64_t t = 0; t < f.npad / 128; t++) est2.push_back({i, (int)t, 0, 0});
-        for (int64_t t = 0; t < f.npad / 256; t++) est4.push_back({i, (int)t, 0, 0});
         for (int c = 0; c < f.m_chunks; c++) means.push_back({i, c, 0, 0});
-        for (int ch = 0; ch < f.c_chunks; ch++)
-            for (int cg = 0; cg < f.KP / 4; cg++)
-                if (4 * cg < f.K) cov.push_back({i, cg, ch, 0});
         for (int k = 0; k < f.K; k++) comp.push_back({i, k, 0, 0});
         for (int k = 0; k < f.K; k++)
             for (int u = 0; u < f.ub; u++) covred.push_back({i, k, u, 0});
     }
-    // interleave large work lists across fits so a persistent grid sees a mix
-    auto interleave = [](std::vector<Item> &v) {
-        std::stable_sort(v.begin(), v.end(), [](const Item &x, const Item &y) {
-            if (x.a != y.a) return x.a < y.a;
-            return x.fit < y.fit;
-        });
-    };
-    interleave(est2);
-    interleave(est4);
     interleave(means);
-    b->n_est2 = (int)est2.size();
-    b->n_est4 = (int)est4.size();
     b->n_means = (int)means.size();
-    b->n_cov = (int)cov.size();
     b->n_comp = (int)comp.size();
     b->n_covred = (int)covred.size();
     hipError_t e;
@@ -442,8 +481,7 @@ int fem_batch_create(fem_batch **out, int nfits, const int64_t *n, const int *d,
         (e = hipMalloc((void **)&b->xpool, sizeof(float) * std::max<int64_t>(1, b->nx))) != hipSuccess ||
         (e = hipMalloc((void **)&b->dpool, sizeof(double) * std::max<int64_t>(1, b->nd))) != hipSuccess ||
         (e = hipMalloc((void **)&b->ipool, sizeof(int) * std::max<int64_t>(1, b->ni))) != hipSuccess ||
-        (e = upload_vec(&b->est2, est2)) != hipSuccess || (e = upload_vec(&b->est4, est4)) != hipSuccess ||
-        (e = upload_vec(&b->means, means)) != hipSuccess || (e = upload_vec(&b->cov, cov)) != hipSuccess ||
+        (e = upload_vec(&b->means, means)) != hipSuccess || (e = build_shape_lists(b)) != hipSuccess ||
         (e = upload_vec(&b->comp, comp)) != hipSuccess || (e = upload_vec(&b->covred, covred)) != hipSuccess ||
         (e = hipMemset(b->xpool, 0, sizeof(float) * std::max<int64_t>(1, b->nx))) != hipSuccess ||
         (e = hipMemset(b->dpool, 0, sizeof(double) * std::max<int64_t>(1, b->nd))) != hipSuccess ||
@@ -452,7 +490,8 @@ int fem_batch_create(fem_batch **out, int nfits, const int64_t *n, const int *d,
     const size_t lds_e = (size_t)fem::estep64_lds<4>() * sizeof(double);
     (void)hipFuncSetAttribute((const void *)fem::kb_estep<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_e);
     (void)hipFuncSetAttribute((const void *)fem::kb_estep<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_e);
-    b->grid_est = grid_of(2, cus, b->n_est2);
+    b->grid_estA = grid_of(2, cus, b->n_estA);
+    b->grid_estB = grid_of(2, cus, b->n_estB);
     b->grid_means = grid_of(2, cus, b->n_means);
     b->grid_cov = grid_of(2, cus, b->n_cov);
     *out = b;
@@ -579,11 +618,15 @@ int fem_batch_run(fem_batch *b, int maxsweeps, double eps, double g, int use_gra
     return FEM_OK;
 }
 
-// Autotuner: from the current parameters, times one eager iteration of each
-// phase under candidate launch shapes (E-step point groups {2, 4} x grids
-// {1, 2, 4} x CUs; means and covariance grids {1, 2, 4, 8} x CUs) and keeps
-// the fastest.  Parameters are restored afterwards.  The report is written to
-// `report` (may be NULL).
+// Autotuner.  From the current parameters it picks, per class of fits with
+// the same (16-feature blocks, padded K) -- the shape the kernels' work
+// depends on -- the E-step point groups per wave (2 or 4) and the
+// covariance chunk rows (1, 2 or 4 times the base length of at most 256
+// chunks), each by timing that class's own work items under every candidate
+// (mean of three launches after a warm one); then the persistent grids of the
+// combined lists ({1, 2, 4} x CUs for the E-steps, {1, 2, 4, 8} x CUs for the
+// means and covariance passes).  Parameters are restored afterwards.  The
+// report is written to `report` (may be NULL).
 int fem_batch_tune(fem_batch *b, char *report, int report_len) {
     if (!b) return FEM_ERR_ARG;
     BH(b, hipSetDevice(b->device));
@@ -604,72 +647,135 @@ int fem_batch_tune(fem_batch *b, char *report, int report_len) {
     auto timeit = [&](auto &&fn) -> float {
         fn();  // warm
         (void)hipEventRecord(e0, b->stream);
-        fn();
+        for (int r = 0; r < 3; r++) fn();
         (void)hipEventRecord(e1, b->stream);
         (void)hipEventSynchronize(e1);
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, e0, e1);
-        return ms;
+        return ms / 3.0f;
     };
     const size_t lds_e = (size_t)fem::estep64_lds<4>() * sizeof(double);
+    auto run_estep = [&](int P, const Item *items, int n, int grid) {
+        if (n <= 0) return;
+        if (P == 4)
+            hipLaunchKernelGGL(fem::kb_estep<4>, dim3(grid), dim3(256), lds_e, b->stream, b->dfits, b->dstate,
+                               b->xpool, b->dpool, items, n);
+        else
+            hipLaunchKernelGGL(fem::kb_estep<2>, dim3(grid), dim3(256), lds_e, b->stream, b->dfits, b->dstate,
+                               b->xpool, b->dpool, items, n);
+    };
     std::string rep;
     char line[256];
-    float best = 1e30f;
-    for (int groups : {2, 4}) {
-        const Item *est = groups == 4 ? b->est4 : b->est2;
-        const int nest = groups == 4 ? b->n_est4 : b->n_est2;
-        for (int mult : {1, 2, 4}) {
-            const int grid = grid_of(mult, b->cus, nest);
-            const float ms = timeit([&]() {
-                if (groups == 4)
-                    hipLaunchKernelGGL(fem::kb_estep<4>, dim3(grid), dim3(256), lds_e, b->stream, b->dfits, b->dstate,
-                                       b->xpool, b->dpool, est, nest);
-                else
-                    hipLaunchKernelGGL(fem::kb_estep<2>, dim3(grid), dim3(256), lds_e, b->stream, b->dfits, b->dstate,
-                                       b->xpool, b->dpool, est, nest);
-            });
-            snprintf(line, sizeof line, "estep groups=%d grid=%dxCU: %.3f ms\n", groups, mult, ms);
+    std::map<std::pair<int, int>, std::vector<int>> classes;
+    for (int i = 0; i < b->F; i++) classes[{b->fits[i].NB, b->fits[i].KP}].push_back(i);
+    Item *tmp = nullptr;
+    int ntmp = 0;
+    struct FreeTmp {
+        Item **p;
+        ~FreeTmp() {
+            if (*p) (void)hipFree(*p);
+        }
+    } free_tmp{&tmp};
+    // E-step point groups per class
+    for (auto &cl : classes) {
+        float best = 1e30f;
+        int bestP = 2;
+        for (int P : {2, 4}) {
+            std::vector<Item> v;
+            for (int i : cl.second) estep_items(v, i, b->fits[i], P);
+            interleave(v);
+            BH(b, replace_list(&tmp, &ntmp, v));
+            const int grid = grid_of(2, b->cus, ntmp);
+            const float ms = timeit([&]() { run_estep(P, tmp, ntmp, grid); });
+            snprintf(line, sizeof line, "class d<=%d K<=%d (%zu fits): estep groups=%d %.3f ms\n", 16 * cl.first.first,
+                     cl.first.second, cl.second.size(), P, ms);
             rep += line;
             if (ms < best) {
                 best = ms;
-                b->est_groups = groups;
-                b->grid_est = grid;
+                bestP = P;
             }
         }
+        for (int i : cl.second) b->fits[i].eg = bestP;
     }
-    hipLaunchKernelGGL(fem::kb_ll, dim3(b->F), dim3(256), 0, b->stream, b->dfits, b->dstate, b->dpool, b->est_groups);
-    best = 1e30f;
-    for (int mult : {1, 2, 4, 8}) {
-        const int grid = grid_of(mult, b->cus, b->n_means);
-        const float ms = timeit([&]() {
-            hipLaunchKernelGGL(fem::kb_means, dim3(grid), dim3(256), 0, b->stream, b->dfits, b->dstate, b->xpool,
-                               b->dpool, b->means, b->n_means);
-        });
-        snprintf(line, sizeof line, "means grid=%dxCU: %.3f ms\n", mult, ms);
-        rep += line;
-        if (ms < best) {
-            best = ms;
-            b->grid_means = grid;
-        }
-    }
+    BH(b, build_shape_lists(b));
+    // one E-step and means pass with the chosen shapes, so that the timed
+    // covariance passes see this iteration's means
+    run_estep(2, b->estA, b->n_estA, grid_of(2, b->cus, b->n_estA));
+    run_estep(4, b->estB, b->n_estB, grid_of(2, b->cus, b->n_estB));
+    hipLaunchKernelGGL(fem::kb_ll, dim3(b->F), dim3(256), 0, b->stream, b->dfits, b->dstate, b->dpool);
+    hipLaunchKernelGGL(fem::kb_means, dim3(grid_of(2, b->cus, b->n_means)), dim3(256), 0, b->stream, b->dfits,
+                       b->dstate, b->xpool, b->dpool, b->means, b->n_means);
     hipLaunchKernelGGL(fem::kb_means_finalize, dim3(grid_of(4, b->cus, b->n_comp)), dim3(256), 0, b->stream,
                        b->dfits, b->dstate, b->dpool, b->comp, b->n_comp);
-    best = 1e30f;
-    for (int mult : {1, 2, 4, 8}) {
-        const int grid = grid_of(mult, b->cus, b->n_cov);
-        const float ms = timeit([&]() {
-            hipLaunchKernelGGL(fem::kb_cov, dim3(grid), dim3(256), 0, b->stream, b->dfits, b->dstate, b->xpool,
-                               b->dpool, b->cov, b->n_cov);
-        });
-        snprintf(line, sizeof line, "cov grid=%dxCU: %.3f ms\n", mult, ms);
-        rep += line;
-        if (ms < best) {
-            best = ms;
-            b->grid_cov = grid;
+    BH(b, hipGetLastError());
+    // covariance chunk rows per class
+    for (auto &cl : classes) {
+        float best = 1e30f;
+        int bestM = 1;
+        for (int mul : {1, 2, 4}) {
+            std::vector<Item> v;
+            for (int i : cl.second) {
+                b->fits[i].cmul = mul;
+                set_chunks(b->fits[i]);
+                cov_items(v, i, b->fits[i]);
+            }
+            BH(b, hipMemcpy(b->dfits, b->fits.data(), sizeof(BFit) * b->F, hipMemcpyHostToDevice));
+            BH(b, replace_list(&tmp, &ntmp, v));
+            const int grid = grid_of(2, b->cus, ntmp);
+            const float ms = timeit([&]() {
+                hipLaunchKernelGGL(fem::kb_cov, dim3(grid), dim3(256), 0, b->stream, b->dfits, b->dstate, b->xpool,
+                                   b->dpool, tmp, ntmp);
+            });
+            snprintf(line, sizeof line, "class d<=%d K<=%d: cov chunk rows=%lld %.3f ms\n", 16 * cl.first.first,
+                     cl.first.second, (long long)b->fits[cl.second[0]].c_chunk_rows, ms);
+            rep += line;
+            if (ms < best) {
+                best = ms;
+                bestM = mul;
+            }
         }
+        for (int i : cl.second) b->fits[i].cmul = bestM;
     }
-    snprintf(line, sizeof line, "chosen: estep groups=%d grid=%d, means grid=%d, cov grid=%d (CUs=%d)\n",
-             b->est_groups, b->grid_est, b->grid_means, b->grid_cov, b->cus);
+    BH(b, build_shape_lists(b));
+    // persistent grids of the combined lists
+    auto pick_grid = [&](const char *name, std::initializer_list<int> mults, int n, int *grid, auto &&launch) {
+        float best = 1e30f;
+        for (int mult : mults) {
+            const int g = grid_of(mult, b->cus, n);
+            const float ms = timeit([&]() { launch(g); });
+            snprintf(line, sizeof line, "%s grid=%dxCU: %.3f ms\n", name, mult, ms);
+            rep += line;
+            if (ms < best) {
+                best = ms;
+                *grid = g;
+            }
+        }
+    };
+    if (b->n_estA > 0)
+        pick_grid("estep groups=2", {1, 2, 4}, b->n_estA, &b->grid_estA,
+                  [&](int g) { run_estep(2, b->estA, b->n_estA, g); });
+    if (b->n_estB > 0)
+        pick_grid("estep groups=4", {1, 2, 4}, b->n_estB, &b->grid_estB,
+                  [&](int g) { run_estep(4, b->estB, b->n_estB, g); });
+    hipLaunchKernelGGL(fem::kb_ll, dim3(b->F), dim3(256), 0, b->stream, b->dfits, b->dstate, b->dpool);
+    pick_grid("means", {1, 2, 4, 8}, b->n_means, &b->grid_means, [&](int g) {
+        hipLaunchKernelGGL(fem::kb_means, dim3(g), dim3(256), 0, b->stream, b->dfits, b->dstate, b->xpool, b->dpool,
+                           b->means, b->n_means);
+    });
+    hipLaunchKernelGGL(fem::kb_means_finalize, dim3(grid_of(4, b->cus, b->n_comp)), dim3(256), 0, b->stream,
+                       b->dfits, b->dstate, b->dpool, b->comp, b->n_comp);
+    pick_grid("cov", {1, 2, 4, 8}, b->n_cov, &b->grid_cov, [&](int g) {
+        hipLaunchKernelGGL(fem::kb_cov, dim3(g), dim3(256), 0, b->stream, b->dfits, b->dstate, b->xpool, b->dpool,
+                           b->cov, b->n_cov);
+    });
+    for (auto &cl : classes) {
+        const BFit &f = b->fits[cl.second[0]];
+        snprintf(line, sizeof line, "chosen for class d<=%d K<=%d: estep groups=%d, cov chunk rows=%lld\n",
+                 16 * cl.first.first, cl.first.second, f.eg, (long long)f.c_chunk_rows);
+        rep += line;
+    }
+    snprintf(line, sizeof line, "chosen grids: estep %d/%d, means %d, cov %d (CUs=%d)\n", b->grid_estA, b->grid_estB,
+             b->grid_means, b->grid_cov, b->cus);
     rep += line;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

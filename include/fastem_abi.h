@@ -166,8 +166,12 @@ int fem_batch_destroy(fem_batch *batch);
 int fem_batch_upload(fem_batch *batch, int fit, const float *X, int64_t ld);
 int fem_batch_set_params(fem_batch *batch, int fit, const double *w, const double *mu,
                          const double *cov);
-/* Times each phase under candidate launch shapes on this batch and keeps the
- * fastest (report: text, may be NULL). */
+/* Autotuner: per class of fits with the same (16-feature blocks, padded K)
+ * it times that class's work under each candidate tile shape -- E-step point
+ * groups per wave (2, 4), covariance chunk rows (1, 2, 4 times the base
+ * length) -- and keeps the fastest, then times the persistent grids of the
+ * combined work lists.  The current parameters are restored (report: text,
+ * may be NULL). */
 int fem_batch_tune(fem_batch *batch, char *report, int report_len);
 /* EM on all fits until each converges (|dLL/LL| < eps), fails (Cholesky) or
  * reaches maxsweeps E-steps; g = the regularisation coefficient, applied to the
