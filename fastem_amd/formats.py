@@ -11,11 +11,13 @@ import struct
 import numpy as np
 
 
-def write_bin(path, X):
+def write_bin(path, X, long_header=False):
+    """long_header forces the 16-byte header (-1, d, int64 N) that files with
+    more than 2^31 - 1 rows need."""
     X = np.ascontiguousarray(X, dtype=np.float32)
     n, d = X.shape
     with open(path, "wb") as f:
-        if n > 2**31 - 1:
+        if long_header or n > 2**31 - 1:
             f.write(struct.pack("<iiq", -1, d, n))
         else:
             f.write(struct.pack("<ii", n, d))

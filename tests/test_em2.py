@@ -67,6 +67,44 @@ def test_em2_block_counts(orc, tmp_path, n, d, K, spec):
         assert int(m.group(1)) > 1
 
 
+def test_bin_long_header_roundtrip_and_em2_load(tmp_path):
+    """The 16-byte .bin header (int32 -1, int32 d, int64 N;
+    Util/MultithreadedDataset.cpp:119-134) reads back the same rows, and em2
+    loads it as the short form (same row count, features and covariance
+    trace; the run stops at the missing initializer file, before device
+    work)."""
+    X = fem.gengmm(5000, 7, 3, -5.0, 5.0, 0.1, 23)[0]
+    formats.write_bin(tmp_path / "s.bin", X)
+    formats.write_bin(tmp_path / "l.bin", X, long_header=True)
+    raw = open(tmp_path / "l.bin", "rb").read()
+    assert raw[:16] == np.array([-1, 7], "<i4").tobytes() + np.array([5000], "<i8").tobytes()
+    np.testing.assert_array_equal(formats.read_bin(tmp_path / "l.bin"), X)
+    outs = []
+    for name in ("s.bin", "l.bin"):
+        rc, out = run_em2(["-c", 3, "-I", "/nonexistent.cmp", tmp_path / name])
+        assert "5000 vectors, 7 features" in out, out
+        outs.append(re.search(r"Covariance matrix trace: (\S+)", out).group(1))
+    assert outs[0] == outs[1]
+
+
+@pytest.mark.gpu
+def test_em2_long_header_same_trajectory(orc, tmp_path):
+    print("em2 long header", flush=True)
+    n, d, K = 20000, 6, 4
+    X = fem.gengmm(n, d, K, -10.0, 10.0, 0.1, 24)[0]
+    w0, mu0, cov0 = orc.kmeanspp_init(X, K, seed=3)
+    formats.write_cmp(tmp_path / "init.cmp", w0, mu0, cov0)
+    traces = []
+    for name, long_header in (("s.bin", False), ("l.bin", True)):
+        formats.write_bin(tmp_path / name, X, long_header=long_header)
+        rc, out = run_em2(["-c", K, "-I", tmp_path / "init.cmp", "-m", 8, "-f", "1e-300", "-S", "-v", "3",
+                           tmp_path / name])
+        assert rc == 0, out
+        traces.append(parse_trace(out))
+    assert len(traces[0]) == 8
+    np.testing.assert_array_equal(traces[0], traces[1])
+
+
 def test_em2_usage_without_file():
     rc, out = run_em2([])
     assert "Arguments: filename" in out
