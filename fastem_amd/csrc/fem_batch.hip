@@ -708,6 +708,8 @@ int fem_batch_tune(fem_batch *b, char *report, int report_len) {
     hipLaunchKernelGGL(fem::kb_means_finalize, dim3(grid_of(4, b->cus, b->n_comp)), dim3(256), 0, b->stream,
                        b->dfits, b->dstate, b->dpool, b->comp, b->n_comp);
     BH(b, hipGetLastError());
+    // the descriptors are rewritten below by copies outside the stream
+    BH(b, hipStreamSynchronize(b->stream));
     // covariance chunk rows per class
     for (auto &cl : classes) {
         float best = 1e30f;
