@@ -15,6 +15,7 @@ EM2_PATH = os.path.join(_PKG, "em2")
 EMINIT_PATH = os.path.join(_PKG, "eminit")
 
 PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
+ESTEP_FORMS = ("fp64", "fp32", "bf16x3", "bf16x3-ref")  # fem_estep_form
 SEED_FORGY, SEED_KMEANSPP, SEED_KMEANSOROR = 0, 1, 2
 FEM_OK, FEM_ERR_ARG, FEM_ERR_HIP, FEM_ERR_NOT_PD, FEM_ERR_STATE = 0, -1, -2, -3, -4
 
@@ -48,6 +49,7 @@ _SIGS = {
     "fem_mstep": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
     "fem_get_posteriors": (ctypes.c_int, [_vp, _dp, _i64]),
     "fem_last_times": (ctypes.c_int, [_vp, _fp, _fp, _fp, _fp]),
+    "fem_last_estep_form": (ctypes.c_int, [_vp, _ip]),
     "fem_kmeans_init": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, _dp, _dp,
                                        _dp]),
     "fem_kmeans_init_rng": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _dp, _dp,
@@ -242,6 +244,13 @@ class Context:
         self._check(lib().fem_last_times(self._h, *[ctypes.byref(x) for x in t]))
         return {"estep_ms": t[0].value, "means_ms": t[1].value, "cov_ms": t[2].value,
                 "factor_ms": t[3].value}
+
+    def estep_form(self):
+        """Kernel of the last estep(): "fp64", "fp32", "bf16x3" (centred on
+        each component) or "bf16x3-ref" (centred on a reference component)."""
+        f = ctypes.c_int()
+        self._check(lib().fem_last_estep_form(self._h, ctypes.byref(f)))
+        return ESTEP_FORMS[f.value]
 
 
 def condition_number(C):

@@ -130,13 +130,26 @@ hipError_t dmalloc(T **p, size_t count) {
 
 void free_all(fem_ctx *c) {
     void *ptrs[] = {c->X, c->Rt, c->Rt32, c->w, c->mu, c->cov, c->nk, c->logdet, c->info, c->coef,
-                    c->U, c->stage64, c->stage32, c->stageb, c->ll_part, c->ll_dev,
+                    c->U, c->stage64, c->stage32, c->stageb, c->stager, c->refc, c->ll_part, c->ll_dev,
                     c->mpart, c->cpart, c->kC, c->dmin, c->cls, c->kfit};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+// Buffers of the reference-centred BF16x3 E-step where it fits (unless
+// FASTEM_ESTEPB_REF=0 keeps the per-component form throughout); the
+// references start at component 0.
+hipError_t alloc_ref(fem_ctx *c) {
+    if (c->stager || !fem::estepr_fits(c->NB, c->K)) return hipSuccess;
+    if (const char *v = std::getenv("FASTEM_ESTEPB_REF"))
+        if (std::atoi(v) == 0) return hipSuccess;
+    hipError_t e = hipMalloc((void **)&c->stager, fem::estepr_words(c->NB, c->K) * sizeof(unsigned int));
+    if (e == hipSuccess) e = hipMalloc((void **)&c->refc, (size_t)c->npad * sizeof(int));
+    if (e == hipSuccess) e = hipMemsetAsync(c->refc, 0, (size_t)c->npad * sizeof(int), c->stream);
+    return e;
 }
 
 int check_ctx(fem_ctx *c) {
@@ -225,6 +238,7 @@ int fem_create(fem_ctx **out, int64_t n, int d, int K, int precision, int device
     if (precision == FEM_PREC_BF16X3 &&
         (e = dmalloc(&c->stageb, (size_t)K * fem::estepb_stage(c->NB))) != hipSuccess)
         return bail(e, "hipMalloc(bf16 factors)");
+    if (precision == FEM_PREC_BF16X3 && (e = alloc_ref(c)) != hipSuccess) return bail(e, "hipMalloc(bf16 references)");
     if ((e = dmalloc(&c->w, K)) != hipSuccess ||
         (e = dmalloc(&c->mu, (size_t)K * d)) != hipSuccess ||
         (e = dmalloc(&c->cov, (size_t)K * d * d)) != hipSuccess ||
@@ -268,6 +282,7 @@ static int upload(fem_ctx *c, const float *X, int64_t ld, hipMemcpyKind kind) {
         FEM_HIP(c, hipMemcpy2DAsync(c->X, (size_t)c->dpad * sizeof(float), X, (size_t)ld * sizeof(float),
                                     (size_t)c->d * sizeof(float), (size_t)c->n, kind, c->stream));
     FEM_HIP(c, hipStreamSynchronize(c->stream));
+    c->ref_ok = false;
     return FEM_OK;
 }
 
@@ -300,11 +315,13 @@ int fem_set_precision(fem_ctx *c, int precision) {
     if (precision == FEM_PREC_BF16X3) {
         if (!c->stageb)
             FEM_HIP(c, hipMalloc((void **)&c->stageb, sizeof(unsigned int) * c->K * fem::estepb_stage(c->NB)));
+        FEM_HIP(c, alloc_ref(c));
     }
     FEM_HIP(c, hipStreamSynchronize(c->stream));
     c->prec = precision;
     c->factored = false;
     c->have_post = false;
+    c->ref_ok = false;
     return FEM_OK;
 }
 
@@ -334,6 +351,7 @@ int fem_set_params(fem_ctx *c, const double *w, const double *mu, const double *
     FEM_HIP(c, hipStreamSynchronize(c->stream));
     c->have_params = true;
     c->factored = false;
+    c->ref_ok = false;
     return FEM_OK;
 }
 
@@ -398,6 +416,7 @@ int fem_estep(fem_ctx *c, double *loglik) {
     if (!c->factored) return fail(c, FEM_ERR_STATE, "fem_estep needs a successful fem_factor");
     if (int r = set_device(c)) return r;
     FEM_HIP(c, hipEventRecord(c->ev[0], c->stream));
+    c->last_form = c->prec != FEM_PREC_BF16X3 ? c->prec : c->stage_ref ? FEM_ESTEP_BF16X3_REF : FEM_ESTEP_BF16X3;
     FEM_HIP(c, fem::launch_estep(c));
     FEM_HIP(c, hipEventRecord(c->ev[1], c->stream));
     double ll = 0.0;
@@ -455,6 +474,13 @@ int fem_get_classes(fem_ctx *c, int *cls) {
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(tmp);
     if (e != hipSuccess) return fail(c, FEM_ERR_HIP, std::string("fem_get_classes: ") + hipGetErrorString(e));
+    return FEM_OK;
+}
+
+int fem_last_estep_form(fem_ctx *c, int *form) {
+    if (int r = check_ctx(c)) return r;
+    if (!form) return fail(c, FEM_ERR_ARG, "null form pointer");
+    *form = c->last_form;
     return FEM_OK;
 }
 

@@ -59,6 +59,12 @@ struct fem_ctx {
     double *stage64 = nullptr; // K x (pairs x 64 + dpad): FP64 E-step fragments, -U_k^T mu_k
     float *stage32 = nullptr;  // K x stage32_words: FP32 fragments, mu_hi, -U_k^T mu_lo
     unsigned int *stageb = nullptr;  // K x stage words: BF16 hi/lo E-step fragments, mu_hi, U_k^T mu_lo
+    // reference-centred BF16x3 E-step (allocated where it fits, d <= 64)
+    unsigned int *stager = nullptr;  // K stage blocks (fragments, seed table), then the mu_hi table
+    int *refc = nullptr;             // npad: reference component of each point
+    bool ref_ok = false;   // refc was written by an E-step since the data or means were last set
+    bool stage_ref = false;  // the last BF16x3 prep packed stager (else stageb)
+    int last_form = 0;       // fem_estep_form of the last fem_estep
 
     // reductions
     int est_blocks = 0;         // capacity of ll_part (one per E-step workgroup)
@@ -118,6 +124,9 @@ hipError_t launch_transpose_post32(const fem_ctx *c, double *out_dev, int64_t ld
 int estepb_stage(int NB);
 hipError_t launch_prep_bf16(fem_ctx *c);
 hipError_t launch_estepb(fem_ctx *c);
+// the reference-centred BF16x3 E-step handles (NB, K); words of its stage
+bool estepr_fits(int NB, int K);
+size_t estepr_words(int NB, int K);
 hipError_t launch_meansb_partials(fem_ctx *c);
 hipError_t launch_covb_partials(fem_ctx *c);
 }  // namespace fem

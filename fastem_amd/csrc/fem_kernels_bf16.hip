@@ -88,13 +88,10 @@ __device__ __host__ inline int stage_words_of(int NB) {
     return pairs_bf16(NB) * 512 + 32 * ((NB + 1) / 2) + 16 * NB;
 }
 
-__global__ __launch_bounds__(256) void k_prep_bf16(int d, int dpad, int NB, const double *__restrict__ U,
-                                                   const double *__restrict__ mu, unsigned int *__restrict__ stage) {
-    const int k = blockIdx.x;
-    const int np = pairs_bf16(NB), NS = (NB + 1) / 2;
-    const int FW = np * 512;
-    const double *Uk = U + (size_t)k * dpad * dpad;
-    unsigned int *st = stage + (size_t)k * stage_words_of(NB);
+// The hi/lo fragments of U_k^T for every (ls, jb) pair into st[0, 512 pairs),
+// by the threads of one workgroup.
+__device__ void pack_fragments_bf16(const double *__restrict__ Uk, int dpad, int NB, unsigned int *__restrict__ st) {
+    const int np = pairs_bf16(NB);
     for (int e = threadIdx.x; e < np * 64 * 4; e += blockDim.x) {
         const int p = e / 256, lane = (e / 4) % 64, w = e % 4;
         int ls = 0, rem = p;
@@ -118,6 +115,16 @@ __global__ __launch_bounds__(256) void k_prep_bf16(int d, int dpad, int NB, cons
         dst[lane * 4 + w] = hw;
         dst[256 + lane * 4 + w] = lw;
     }
+}
+
+__global__ __launch_bounds__(256) void k_prep_bf16(int d, int dpad, int NB, const double *__restrict__ U,
+                                                   const double *__restrict__ mu, unsigned int *__restrict__ stage) {
+    const int k = blockIdx.x;
+    const int np = pairs_bf16(NB), NS = (NB + 1) / 2;
+    const int FW = np * 512;
+    const double *Uk = U + (size_t)k * dpad * dpad;
+    unsigned int *st = stage + (size_t)k * stage_words_of(NB);
+    pack_fragments_bf16(Uk, dpad, NB, st);
     for (int j = threadIdx.x; j < 32 * NS; j += blockDim.x)
         st[FW + j] = __builtin_bit_cast(unsigned int, (j < d) ? (float)mu[(size_t)k * d + j] : 0.0f);
     for (int j = threadIdx.x; j < dpad; j += blockDim.x) {
@@ -270,6 +277,201 @@ __global__ __launch_bounds__(256) void k_estepb(const float *__restrict__ X, int
     if (tid == 0) ll_part[blockIdx.x] = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
 }
 
+// ---------------------------------------------------------------------------
+// Reference-centred form of stage (a), BF16x3.  The form above centres and
+// splits every (point, component, feature): three VALU operations per value
+// against 3/16 of an MFMA, which leaves it VALU-bound.  Here each point is
+// centred once, on a reference component a(i) (the component of its largest
+// log-density in the previous E-step of this form), and split once:
+//   U_k^T (x - mu_k) = U_k^T (x - mu_a,hi) + s_k[a],
+//   s_k[a] = U_k^T (mu_a,hi - mu_k)   (FP64, rounded once to FP32),
+// with mu_a,hi = float(mu_a).  The accumulators start at s_k[a(i)], read
+// from a per-component table in LDS, and the MFMAs take the point's split
+// operands held in registers for all K components.  For k = a this is the
+// other form's arithmetic; for k != a the split error is relative to
+// |x - mu_a| instead of |x - mu_k|, no larger for a point near its
+// reference.  Any reference gives the same densities up to rounding; a
+// distant one only makes them less precise.
+//
+// Stage block of component k (estepr_buffer_pieces(NB, K) 16-byte pieces,
+// padded with zeros to whole copy rounds of 256 pieces):
+//   [0, FW)                 fragments as in the form above;
+//   [FW, FW + K SR)         s_k[a][j] as FP32, row a at FW + a SR, SR = 16 NB + 4
+//                           (the 4-word pad spreads the rows' banks), zero for
+//                           j >= d.
+// After the K blocks: the K x 16 NB table of mu_a,hi (zero beyond d).
+__device__ __host__ inline int estepr_row(int NB) { return 16 * NB + 4; }
+__device__ __host__ inline int estepr_stage_words(int NB, int K) {
+    return pairs_bf16(NB) * 512 + K * estepr_row(NB);
+}
+// 16-byte pieces of one LDS stage buffer: whole rounds of 256 (every lane of
+// a direct copy writes one piece per round)
+__device__ __host__ inline int estepr_buffer_pieces(int NB, int K) {
+    return ((estepr_stage_words(NB, K) / 4 + 255) / 256) * 256;
+}
+
+__global__ __launch_bounds__(256) void k_prep_ref(int d, int dpad, int NB, int K, const double *__restrict__ U,
+                                                  const double *__restrict__ mu, unsigned int *__restrict__ stage) {
+    const int k = blockIdx.x;
+    const int FW = pairs_bf16(NB) * 512, SR = estepr_row(NB);
+    const int BW = 4 * estepr_buffer_pieces(NB, K);  // words per padded block
+    const double *Uk = U + (size_t)k * dpad * dpad;
+    unsigned int *st = stage + (size_t)k * BW;
+    pack_fragments_bf16(Uk, dpad, NB, st);
+    for (int e = threadIdx.x; e < BW - FW; e += blockDim.x) {
+        const int a = e / SR, j = e % SR;
+        double s = 0.0;
+        if (a < K && j < d)
+            for (int f = 0; f <= j; f++)
+                s += Uk[(size_t)f * dpad + j] * ((double)(float)mu[(size_t)a * d + f] - mu[(size_t)k * d + f]);
+        st[FW + e] = __builtin_bit_cast(unsigned int, (float)s);
+    }
+    float *muh = reinterpret_cast<float *>(stage + (size_t)K * BW) + (size_t)k * dpad;
+    for (int j = threadIdx.x; j < dpad; j += blockDim.x) muh[j] = (j < d) ? (float)mu[(size_t)k * d + j] : 0.0f;
+}
+
+// Workgroup = 4 waves x P groups of 16 points, lanes as in the form above.
+// ref[i] is read at the start (clamped into [0, K)) and replaced at the end by
+// the first component of largest log-density.  The stage blocks move straight
+// from global memory into the idle LDS buffer (global_load_lds_dwordx4: a
+// scalar base, the lane's piece as the vector offset, no registers held), one
+// barrier per component.
+template <int NB, int P>
+__global__ __launch_bounds__(256) void k_estepr(const float *__restrict__ X, int64_t n, int64_t npad, int K,
+                                                const unsigned int *__restrict__ stage,
+                                                const double *__restrict__ coef, int *__restrict__ ref,
+                                                float *__restrict__ Rt, double *__restrict__ ll_part) {
+    constexpr int DP = 16 * NB;
+    constexpr int NS = (NB + 1) / 2;
+    constexpr int NPAIR = pairs_bf16(NB);
+    constexpr int FW = NPAIR * 512;
+    constexpr int SR = 16 * NB + 4;
+    const int BUF4 = estepr_buffer_pieces(NB, K);  // 16-byte pieces per (padded) block
+    const int rounds = BUF4 / 256;
+    extern __shared__ __align__(16) unsigned int ldsr[];  // 2 * BUF4 pieces
+    __shared__ double wsum[4];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int g = lane >> 4, c = lane & 15;
+    const int64_t base = (int64_t)blockIdx.x * (64 * P) + wave * (16 * P);
+    const uint4 *stage4 = reinterpret_cast<const uint4 *>(stage);
+    uint4 *lds4 = reinterpret_cast<uint4 *>(ldsr);
+    const float *muh = reinterpret_cast<const float *>(stage4 + (size_t)K * BUF4);
+
+    // block 0 into buffer 0
+    for (int q = 0; q < rounds; q++)
+        __builtin_amdgcn_global_load_lds(
+            (const __attribute__((address_space(1))) void *)(stage4 + 256 * q + tid),
+            (__attribute__((address_space(3))) void *)(lds4 + 256 * q + 64 * wave), 16, 0, 0);
+
+    int ar[P];
+    v8bf bh[P][NS], bl[P][NS];
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        const int64_t pt = base + 16 * p + c;
+        const int a = min(max(ref[pt], 0), K - 1);
+        ar[p] = a;
+#pragma unroll
+        for (int ls = 0; ls < NS; ls++) {
+            const int f0 = 32 * ls + 8 * g;
+            float t[8];
+            if (f0 < DP) {
+                const float4 *src = reinterpret_cast<const float4 *>(X + pt * DP + f0);
+                const float4 *mh = reinterpret_cast<const float4 *>(muh + (size_t)a * DP + f0);
+                const float4 u0 = src[0], u1 = src[1], m0 = mh[0], m1 = mh[1];
+                t[0] = u0.x - m0.x; t[1] = u0.y - m0.y; t[2] = u0.z - m0.z; t[3] = u0.w - m0.w;
+                t[4] = u1.x - m1.x; t[5] = u1.y - m1.y; t[6] = u1.z - m1.z; t[7] = u1.w - m1.w;
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; e++) t[e] = 0.0f;
+            }
+            split8(t, bh[p][ls], bl[p][ls]);
+        }
+    }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+
+    float mx[P];
+    int best[P];
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        mx[p] = -INFINITY;
+        best[p] = ar[p];
+    }
+
+    for (int k = 0; k < K; k++) {
+        const uint4 *cur4 = lds4 + (k & 1) * BUF4;
+        const float *lsd = reinterpret_cast<const float *>(cur4) + FW;
+        if (k + 1 < K) {
+            const uint4 *next4 = stage4 + (size_t)(k + 1) * BUF4;
+            uint4 *dst = lds4 + ((k + 1) & 1) * BUF4 + 64 * wave;
+            for (int q = 0; q < rounds; q++)
+                __builtin_amdgcn_global_load_lds(
+                    (const __attribute__((address_space(1))) void *)(next4 + 256 * q + tid),
+                    (__attribute__((address_space(3))) void *)(dst + 256 * q), 16, 0, 0);
+        }
+        const float ck = (float)coef[k];
+        v4f acc[P][NB];
+#pragma unroll
+        for (int p = 0; p < P; p++)
+#pragma unroll
+            for (int jb = 0; jb < NB; jb++) {
+                const float4 s = *reinterpret_cast<const float4 *>(lsd + ar[p] * SR + 16 * jb + 4 * g);
+                acc[p][jb] = (v4f){s.x, s.y, s.z, s.w};
+            }
+        int pr = 0;
+#pragma unroll
+        for (int ls = 0; ls < NS; ls++)
+#pragma unroll
+            for (int jb = 2 * ls; jb < NB; jb++) {
+                const v4u *fp = reinterpret_cast<const v4u *>(cur4 + pr * 128);
+                const v8bf ah = __builtin_bit_cast(v8bf, fp[lane]);
+                const v8bf al = __builtin_bit_cast(v8bf, fp[64 + lane]);
+                pr++;
+#pragma unroll
+                for (int p = 0; p < P; p++) acc[p][jb] = mfma3(ah, al, bh[p][ls], bl[p][ls], acc[p][jb]);
+            }
+        float sv[P];
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            float q = 0.f;
+#pragma unroll
+            for (int jb = 0; jb < NB; jb++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) q += acc[p][jb][v] * acc[p][jb][v];
+            q = xor_sum_gb(q);
+            sv[p] = -0.5f * q + ck;
+            if (sv[p] > mx[p]) best[p] = k;
+            mx[p] = fmaxf(mx[p], sv[p]);
+        }
+        // the next block has landed (this wave's pieces) and the barrier
+        // publishes it; the log-densities are stored after the wait
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+        if ((k & 3) == g) {
+#pragma unroll
+            for (int p = 0; p < P; p++) Rt[(size_t)k * npad + base + 16 * p + c] = sv[p];
+        }
+    }
+
+    double llsum = 0.0;
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        const int64_t pt = base + 16 * p + c;
+        const bool valid = pt < n;
+        const float lse = lse_posteriors(Rt, npad, pt, K, g, mx[p], valid);
+        if (g == 0) {
+            ref[pt] = best[p];
+            if (valid) llsum += (double)lse;
+        }
+    }
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) llsum += __shfl_xor(llsum, o);
+    if (lane == 0) wsum[wave] = llsum;
+    __syncthreads();
+    if (tid == 0) ll_part[blockIdx.x] = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
+}
+
 __global__ __launch_bounds__(256) void k_reduce_sumb(const double *__restrict__ parts, int64_t n,
                                                      double *__restrict__ out) {
     __shared__ double s[256];
@@ -284,9 +486,24 @@ __global__ __launch_bounds__(256) void k_reduce_sumb(const double *__restrict__ 
     if (threadIdx.x == 0) *out = s[0];
 }
 
+bool estepr_fits(int NB, int K) {
+    // two LDS stage buffers of at most 80 KB together: two workgroups per CU
+    return NB <= 4 && K >= 1 && 2 * (size_t)estepr_buffer_pieces(NB, K) * 16 <= 80 * 1024;
+}
+
+size_t estepr_words(int NB, int K) { return (size_t)K * 4 * estepr_buffer_pieces(NB, K) + (size_t)K * 16 * NB; }
+
+// Packs the reference-centred stage when the context has it and its
+// references come from an E-step since the data or means were last set;
+// otherwise the per-component form's stage.
 hipError_t launch_prep_bf16(fem_ctx *c) {
-    hipLaunchKernelGGL(k_prep_bf16, dim3(c->K), dim3(256), 0, c->stream, c->d, c->dpad, c->NB, c->U, c->mu,
-                       c->stageb);
+    c->stage_ref = c->stager && c->refc && c->ref_ok && estepr_fits(c->NB, c->K);
+    if (c->stage_ref)
+        hipLaunchKernelGGL(k_prep_ref, dim3(c->K), dim3(256), 0, c->stream, c->d, c->dpad, c->NB, c->K, c->U,
+                           c->mu, c->stager);
+    else
+        hipLaunchKernelGGL(k_prep_bf16, dim3(c->K), dim3(256), 0, c->stream, c->d, c->dpad, c->NB, c->U, c->mu,
+                           c->stageb);
     return hipGetLastError();
 }
 
@@ -306,7 +523,47 @@ static hipError_t estepb_t(fem_ctx *c) {
 }
 
 // Point groups per wave: 4 for d <= 64 unless estep_groupsb == 2, else 2.
+template <int NB, int P>
+static hipError_t estepr_t(fem_ctx *c) {
+    const size_t lds = (size_t)2 * estepr_buffer_pieces(NB, c->K) * 16;
+    if (lds + sizeof(double) * 4 > 64 * 1024) {  // with the static LDS
+        hipError_t e = hipFuncSetAttribute((const void *)k_estepr<NB, P>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    const int blocks = (int)(c->npad / (64 * P));
+    hipLaunchKernelGGL((k_estepr<NB, P>), dim3(blocks), dim3(256), lds, c->stream, c->X, c->n, c->npad, c->K,
+                       c->stager, c->coef, c->refc, c->Rt32, c->ll_part);
+    hipLaunchKernelGGL(k_reduce_sumb, dim3(1), dim3(256), 0, c->stream, c->ll_part, (int64_t)blocks, c->ll_dev);
+    return hipGetLastError();
+}
+
+static hipError_t estepb_ref(fem_ctx *c) {
+    switch (c->NB) {
+        case 1: return estepr_t<1, 4>(c);
+        case 2: return estepr_t<2, 4>(c);
+        case 3: return estepr_t<3, 4>(c);
+        case 4: return estepr_t<4, 4>(c);
+    }
+    return hipErrorInvalidValue;
+}
+
+static hipError_t estepb_plain(fem_ctx *c);
+
+// The reference-centred kernel when the last prep packed its stage; else the
+// per-component form, after which (where the reference form fits) the
+// posterior argmax becomes the references of the next E-step.
 hipError_t launch_estepb(fem_ctx *c) {
+    if (c->stage_ref) return estepb_ref(c);
+    hipError_t e = estepb_plain(c);
+    if (e == hipSuccess && c->stager && c->refc && estepr_fits(c->NB, c->K)) {
+        e = launch_classes(c, c->refc);
+        if (e == hipSuccess) c->ref_ok = true;
+    }
+    return e;
+}
+
+static hipError_t estepb_plain(fem_ctx *c) {
     const bool p4 = c->estep_groupsb == 4;
     switch (c->NB) {
         case 1: return p4 ? estepb_t<1, 4>(c) : estepb_t<1, 2>(c);

@@ -299,6 +299,7 @@ int lloyd_and_mixture(fem_ctx *c, std::vector<double> &C, Draw &&draw, int max_s
     c->have_params = true;
     c->factored = false;
     c->have_post = false;
+    c->ref_ok = false;
     if (rc != FEM_OK) c->err = "k-means left a cluster with fewer than two points";
     return rc;
 }

@@ -214,6 +214,23 @@ int fem_estep_noalloc(const float *X, int64_t n, int64_t ldx, int d, int K, cons
 int fem_last_times(fem_ctx *ctx, float *estep_ms, float *means_ms, float *cov_ms,
                    float *factor_ms);
 
+/* Which E-step kernel the last fem_estep ran (no counterpart in the
+ * reference; for tests and profiling).  In the BF16x3 form the first E-step
+ * after the data, the means (fem_set_params, an initializer) or the precision
+ * were set centres every (point, component) pair on mu_k; later ones, for
+ * d <= 64 and the component counts whose stage fits two workgroups per CU
+ * (K <= 105 at d = 64), centre each point once on the component of its
+ * largest log-density in the previous E-step and add U_k^T (mu_a - mu_k)
+ * from a table (FASTEM_ESTEPB_REF=0 at fem_create / fem_set_precision keeps
+ * the first kind throughout). */
+enum fem_estep_form {
+    FEM_ESTEP_FP64 = 0,
+    FEM_ESTEP_FP32 = 1,
+    FEM_ESTEP_BF16X3 = 2,      /* centred on each component's mean */
+    FEM_ESTEP_BF16X3_REF = 3   /* centred on a reference component per point */
+};
+int fem_last_estep_form(fem_ctx *ctx, int *form);
+
 /* Message of the last failed call on ctx (or of the last failed fem_create
  * when ctx is NULL). */
 const char *fem_last_error(const fem_ctx *ctx);
