@@ -31,6 +31,7 @@ SEED = 30_000_001
 TTL_EXTRA_SEEDS = (30_000_002, 30_000_003)
 PEAK_FP64_MFMA = 78.6e12   # MI355X dense FP64 matrix, FLOP/s
 PEAK_FP32_MFMA = 157.3e12  # MI355X dense FP32 matrix, FLOP/s
+PEAK_BF16_MFMA = 2.5e15    # MI355X dense BF16 matrix, FLOP/s
 PEAK_HBM = 8.0e12          # B/s
 
 
@@ -375,11 +376,23 @@ def main():
                 / PEAK_FP32_MFMA}
         log("bf16x3 form")
         rb = run_form(fem.PREC_BF16X3, args.steps, args.warmup)
+        # the same form with every (point, component) pair centred on its own
+        # mean in the E-step (the reference-centred kernel switched off)
+        log("bf16x3 form, component-centred E-step")
+        os.environ["FASTEM_ESTEPB_REF"] = "0"
+        try:
+            rbc = run_form(fem.PREC_BF16X3, args.steps, args.warmup)
+        finally:
+            del os.environ["FASTEM_ESTEPB_REF"]
+        ebf = algorithmic_flops(N, D, K)["estep"]
         variants["bf16x3"] = {
             "iters_per_s": rb["iters_per_s"], "phases_ms": rb["phases_ms"],
             "estep_fp32_equivalent_frac_of_fp32_mfma_peak":
-                algorithmic_flops(N, D, K)["estep"] / (rb["phases_ms"]["estep"] * 1e-3)
-                / PEAK_FP32_MFMA}
+                ebf / (rb["phases_ms"]["estep"] * 1e-3) / PEAK_FP32_MFMA,
+            "estep_frac_of_bf16_peak_3_products":
+                3 * ebf / (rb["phases_ms"]["estep"] * 1e-3) / PEAK_BF16_MFMA,
+            "estep_ms_component_centred": rbc["phases_ms"]["estep"],
+            "iters_per_s_component_centred": rbc["iters_per_s"]}
 
     fl = algorithmic_flops(N, D, K)
     ph = r64["phases_ms"]

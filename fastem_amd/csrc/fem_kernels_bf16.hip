@@ -304,11 +304,14 @@ __device__ __host__ inline int estepr_row(int NB) { return 16 * NB + 4; }
 __device__ __host__ inline int estepr_stage_words(int NB, int K) {
     return pairs_bf16(NB) * 512 + K * estepr_row(NB);
 }
-// 16-byte pieces of one LDS stage buffer: whole rounds of 256 (every lane of
-// a direct copy writes one piece per round)
-__device__ __host__ inline int estepr_buffer_pieces(int NB, int K) {
-    return ((estepr_stage_words(NB, K) / 4 + 255) / 256) * 256;
+// Copy rounds of 256 16-byte pieces (one per thread) per block, rounded up
+// to an even count (the kernel is instantiated per count); at most 8: two
+// 32 KB LDS buffers, two workgroups per CU.
+__device__ __host__ inline int estepr_rounds(int NB, int K) {
+    const int r = (estepr_stage_words(NB, K) / 4 + 255) / 256;
+    return r + (r & 1);
 }
+__device__ __host__ inline int estepr_buffer_pieces(int NB, int K) { return 256 * estepr_rounds(NB, K); }
 
 __global__ __launch_bounds__(256) void k_prep_ref(int d, int dpad, int NB, int K, const double *__restrict__ U,
                                                   const double *__restrict__ mu, unsigned int *__restrict__ stage) {
@@ -330,46 +333,160 @@ __global__ __launch_bounds__(256) void k_prep_ref(int d, int dpad, int NB, int K
     for (int j = threadIdx.x; j < dpad; j += blockDim.x) muh[j] = (j < d) ? (float)mu[(size_t)k * d + j] : 0.0f;
 }
 
-// Workgroup = 4 waves x P groups of 16 points, lanes as in the form above.
+// Sums of q[p] over the four 16-lane groups for p = 0..3 at once, with the
+// gfx950 lane swaps: lane (g, c) gets the total of q[g] over lanes c, c + 16,
+// c + 32, c + 48, added as (g0 + g1) + (g2 + g3) like group_sum4.
+// permlane16_swap(x, y) returns x with its odd rows (16 lanes) replaced by y's
+// even rows and y with its even rows replaced by x's odd rows;
+// permlane32_swap(x, y) returns x with its upper half replaced by y's lower
+// half and y with its lower half replaced by x's upper half.
+__device__ inline float group_sums4_spread(const float (&q)[4]) {
+    auto u = [](float v) { return __builtin_bit_cast(unsigned, v); };
+    auto f = [](unsigned v) { return __builtin_bit_cast(float, v); };
+    const auto s01 = __builtin_amdgcn_permlane16_swap(u(q[0]), u(q[1]), false, false);
+    const auto s23 = __builtin_amdgcn_permlane16_swap(u(q[2]), u(q[3]), false, false);
+    // rows of a: q0 (r0 + r1), q1 (r0 + r1), q0 (r2 + r3), q1 (r2 + r3); b likewise
+    const float a = f(s01[0]) + f(s01[1]);
+    const float b = f(s23[0]) + f(s23[1]);
+    const auto t = __builtin_amdgcn_permlane32_swap(u(a), u(b), false, false);
+    return f(t[0]) + f(t[1]);
+}
+
+// One component of the reference-centred E-step: accumulators from the seed
+// rows, 3 MFMAs per (strip, block) pair and point group, squared norms summed
+// over the lane groups so that lane (g, c) holds the log-density of point
+// (g, c) of its wave, and that point's running max / argmax; component k+1's stage block is copied straight
+// from global memory into the other LDS buffer meanwhile (one 16-byte piece
+// per thread and round: global_load_lds_dwordx4, no registers held).  The two
+// buffers are distinct LDS arrays, so the compiler knows the copy does not
+// write what this component reads and waits for it only where the caller
+// asks (a single shared array made it wait before the first LDS read).
+template <int NB, int R>
+__device__ __attribute__((always_inline)) inline void estepr_component(
+    int k, int K, const uint4 *__restrict__ cur4, uint4 *__restrict__ nxt4, const uint4 *__restrict__ stage4,
+    const double *__restrict__ coef, int tid, int wave, int lane, int g, const int (&ar)[4],
+    const v8bf (&bh)[4][(NB + 1) / 2], const v8bf (&bl)[4][(NB + 1) / 2], float &mx, int &best, float &sv) {
+    constexpr int NS = (NB + 1) / 2;
+    constexpr int FW = pairs_bf16(NB) * 512;
+    constexpr int SR = 16 * NB + 4;
+    constexpr int P = 4;
+    constexpr int BUF4 = 256 * R;
+    const float ck = (float)coef[k];
+    constexpr int NPAIR = pairs_bf16(NB);
+    const float *lsd = reinterpret_cast<const float *>(cur4) + FW;
+    // every LDS read of the component first (fragments in pair order, then
+    // the seed rows), so that one LDS latency precedes the MFMAs instead of
+    // one per pair: 16-cycle MFMAs do not hide a read issued just ahead
+    v8bf fh[NPAIR], fl[NPAIR];
+#pragma unroll
+    for (int pr = 0; pr < NPAIR; pr++) {
+        const v4u *fp = reinterpret_cast<const v4u *>(cur4 + pr * 128);
+        fh[pr] = __builtin_bit_cast(v8bf, fp[lane]);
+        fl[pr] = __builtin_bit_cast(v8bf, fp[64 + lane]);
+    }
+    v4f acc[P][NB];
+#pragma unroll
+    for (int jb = 0; jb < NB; jb++)
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            const float4 s = *reinterpret_cast<const float4 *>(lsd + ar[p] * SR + 16 * jb + 4 * g);
+            acc[p][jb] = (v4f){s.x, s.y, s.z, s.w};
+        }
+    // the copy is issued after the reads: a wait the compiler places before
+    // an LDS read then finds no copy in flight
+    __builtin_amdgcn_sched_barrier(0);
+    if (k + 1 < K) {
+        const uint4 *next4 = stage4 + (size_t)(k + 1) * BUF4;
+#pragma unroll
+        for (int q = 0; q < R; q++)
+            __builtin_amdgcn_global_load_lds(
+                (const __attribute__((address_space(1))) void *)(next4 + 256 * q + tid),
+                (__attribute__((address_space(3))) void *)(nxt4 + 256 * q + 64 * wave), 16, 0, 0);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    int pr = 0;
+#pragma unroll
+    for (int ls = 0; ls < NS; ls++)
+#pragma unroll
+        for (int jb = 2 * ls; jb < NB; jb++) {
+#pragma unroll
+            for (int p = 0; p < P; p++) acc[p][jb] = mfma3(fh[pr], fl[pr], bh[p][ls], bl[p][ls], acc[p][jb]);
+            pr++;
+        }
+    float q[P];
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        q[p] = 0.f;
+#pragma unroll
+        for (int jb = 0; jb < NB; jb++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) q[p] += acc[p][jb][v] * acc[p][jb][v];
+    }
+    sv = -0.5f * group_sums4_spread(q) + ck;
+    if (sv > mx) best = k;
+    mx = fmaxf(mx, sv);
+}
+
+// Workgroup = 4 waves x 4 groups of 16 points, lanes as in the form above; R
+// copy rounds per stage block (estepr_rounds), two LDS buffers of 4R KB.
 // ref[i] is read at the start (clamped into [0, K)) and replaced at the end by
-// the first component of largest log-density.  The stage blocks move straight
-// from global memory into the idle LDS buffer (global_load_lds_dwordx4: a
-// scalar base, the lane's piece as the vector offset, no registers held), one
-// barrier per component.
-template <int NB, int P>
+// the first component of largest log-density.  The workgroup's 256 points are
+// ranked by (reference, row) and dealt to the lanes in that order, so a
+// 16-point group holds few distinct references, consecutive ones: its reads
+// of the seed rows are mostly broadcasts over distinct LDS banks instead of
+// 16 rows that collide in the banks.  Log-densities, posteriors and
+// references stay in point order.  One barrier per component.  Lane (g, c)
+// stores the log-densities of point (g, c) of its wave; the log-sum-exp
+// sweep (4 lanes per point, components g, g + 4, ...) reads them after a
+// barrier.
+template <int NB, int R>
 __global__ __launch_bounds__(256) void k_estepr(const float *__restrict__ X, int64_t n, int64_t npad, int K,
                                                 const unsigned int *__restrict__ stage,
                                                 const double *__restrict__ coef, int *__restrict__ ref,
                                                 float *__restrict__ Rt, double *__restrict__ ll_part) {
     constexpr int DP = 16 * NB;
     constexpr int NS = (NB + 1) / 2;
-    constexpr int NPAIR = pairs_bf16(NB);
-    constexpr int FW = NPAIR * 512;
-    constexpr int SR = 16 * NB + 4;
-    const int BUF4 = estepr_buffer_pieces(NB, K);  // 16-byte pieces per (padded) block
-    const int rounds = BUF4 / 256;
-    extern __shared__ __align__(16) unsigned int ldsr[];  // 2 * BUF4 pieces
+    constexpr int P = 4;  // one point per thread of the 256-thread workgroup
+    constexpr int BUF4 = 256 * R;  // 16-byte pieces per (padded) block
+    __shared__ __align__(16) uint4 bufA[BUF4];
+    __shared__ __align__(16) uint4 bufB[BUF4];
     __shared__ double wsum[4];
+    __shared__ __align__(16) int skey[256];
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int g = lane >> 4, c = lane & 15;
-    const int64_t base = (int64_t)blockIdx.x * (64 * P) + wave * (16 * P);
+    const int64_t blk0 = (int64_t)blockIdx.x * 256;
     const uint4 *stage4 = reinterpret_cast<const uint4 *>(stage);
-    uint4 *lds4 = reinterpret_cast<uint4 *>(ldsr);
     const float *muh = reinterpret_cast<const float *>(stage4 + (size_t)K * BUF4);
 
-    // block 0 into buffer 0
-    for (int q = 0; q < rounds; q++)
-        __builtin_amdgcn_global_load_lds(
-            (const __attribute__((address_space(1))) void *)(stage4 + 256 * q + tid),
-            (__attribute__((address_space(3))) void *)(lds4 + 256 * q + 64 * wave), 16, 0, 0);
+    // block 0 into buffer A
+#pragma unroll
+    for (int q = 0; q < R; q++) bufA[256 * q + tid] = stage4[256 * q + tid];
 
-    int ar[P];
+    // rank of each point's key (reference, row) among the workgroup's: a
+    // permutation, the same on every run
+    {
+        const int key = min(max(ref[blk0 + tid], 0), K - 1) * 256 + tid;
+        skey[tid] = key;
+        __syncthreads();
+        int rank = 0;
+        for (int i = 0; i < 256; i += 4) {
+            const int4 q = *reinterpret_cast<const int4 *>(skey + i);
+            rank += (q.x < key) + (q.y < key) + (q.z < key) + (q.w < key);
+        }
+        __syncthreads();
+        skey[rank] = key;
+        __syncthreads();
+    }
+
+    int ar[P], loc[P];
     v8bf bh[P][NS], bl[P][NS];
 #pragma unroll
     for (int p = 0; p < P; p++) {
-        const int64_t pt = base + 16 * p + c;
-        const int a = min(max(ref[pt], 0), K - 1);
+        const int key = skey[wave * 64 + 16 * p + c];
+        loc[p] = key & 255;
+        const int64_t pt = blk0 + loc[p];
+        const int a = key >> 8;
         ar[p] = a;
 #pragma unroll
         for (int ls = 0; ls < NS; ls++) {
@@ -388,82 +505,51 @@ __global__ __launch_bounds__(256) void k_estepr(const float *__restrict__ X, int
             split8(t, bh[p][ls], bl[p][ls]);
         }
     }
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
 
-    float mx[P];
-    int best[P];
+    // lane (g, c) follows point (g, c) of its wave: its log-densities, max
+    // and argmax
+    int myloc = loc[0], best = ar[0];
 #pragma unroll
-    for (int p = 0; p < P; p++) {
-        mx[p] = -INFINITY;
-        best[p] = ar[p];
-    }
-
-    for (int k = 0; k < K; k++) {
-        const uint4 *cur4 = lds4 + (k & 1) * BUF4;
-        const float *lsd = reinterpret_cast<const float *>(cur4) + FW;
-        if (k + 1 < K) {
-            const uint4 *next4 = stage4 + (size_t)(k + 1) * BUF4;
-            uint4 *dst = lds4 + ((k + 1) & 1) * BUF4 + 64 * wave;
-            for (int q = 0; q < rounds; q++)
-                __builtin_amdgcn_global_load_lds(
-                    (const __attribute__((address_space(1))) void *)(next4 + 256 * q + tid),
-                    (__attribute__((address_space(3))) void *)(dst + 256 * q), 16, 0, 0);
+    for (int p = 1; p < P; p++)
+        if (p == g) {
+            myloc = loc[p];
+            best = ar[p];
         }
-        const float ck = (float)coef[k];
-        v4f acc[P][NB];
-#pragma unroll
-        for (int p = 0; p < P; p++)
-#pragma unroll
-            for (int jb = 0; jb < NB; jb++) {
-                const float4 s = *reinterpret_cast<const float4 *>(lsd + ar[p] * SR + 16 * jb + 4 * g);
-                acc[p][jb] = (v4f){s.x, s.y, s.z, s.w};
-            }
-        int pr = 0;
-#pragma unroll
-        for (int ls = 0; ls < NS; ls++)
-#pragma unroll
-            for (int jb = 2 * ls; jb < NB; jb++) {
-                const v4u *fp = reinterpret_cast<const v4u *>(cur4 + pr * 128);
-                const v8bf ah = __builtin_bit_cast(v8bf, fp[lane]);
-                const v8bf al = __builtin_bit_cast(v8bf, fp[64 + lane]);
-                pr++;
-#pragma unroll
-                for (int p = 0; p < P; p++) acc[p][jb] = mfma3(ah, al, bh[p][ls], bl[p][ls], acc[p][jb]);
-            }
-        float sv[P];
-#pragma unroll
-        for (int p = 0; p < P; p++) {
-            float q = 0.f;
-#pragma unroll
-            for (int jb = 0; jb < NB; jb++)
-#pragma unroll
-                for (int v = 0; v < 4; v++) q += acc[p][jb][v] * acc[p][jb][v];
-            q = xor_sum_gb(q);
-            sv[p] = -0.5f * q + ck;
-            if (sv[p] > mx[p]) best[p] = k;
-            mx[p] = fmaxf(mx[p], sv[p]);
-        }
-        // the next block has landed (this wave's pieces) and the barrier
-        // publishes it; the log-densities are stored after the wait
+    float mx = -INFINITY, sv = 0.f;
+    float *rt = Rt + blk0 + myloc;
+    auto store = [&](int k) { rt[(size_t)k * npad] = sv; };
+    // components in pairs: even ones read A and fill B, odd ones the reverse;
+    // at each barrier this wave's copy has landed and the barrier publishes
+    // every wave's; the log-densities are stored after the wait
+    // (the scheduling barriers keep the component's MFMAs ahead of the wait)
+    for (int k = 0; k < K; k += 2) {
+        estepr_component<NB, R>(k, K, bufA, bufB, stage4, coef, tid, wave, lane, g, ar, bh, bl, mx, best, sv);
+        __builtin_amdgcn_sched_barrier(0);
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
-        if ((k & 3) == g) {
-#pragma unroll
-            for (int p = 0; p < P; p++) Rt[(size_t)k * npad + base + 16 * p + c] = sv[p];
+        store(k);
+        if (k + 1 < K) {
+            estepr_component<NB, R>(k + 1, K, bufB, bufA, stage4, coef, tid, wave, lane, g, ar, bh, bl, mx, best,
+                                    sv);
+            __builtin_amdgcn_sched_barrier(0);
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            __syncthreads();
+            store(k + 1);
         }
     }
+    // the sweep below reads entries other lanes stored
+    __syncthreads();
 
+    ref[blk0 + myloc] = best;
     double llsum = 0.0;
 #pragma unroll
     for (int p = 0; p < P; p++) {
-        const int64_t pt = base + 16 * p + c;
+        const int64_t pt = blk0 + loc[p];
         const bool valid = pt < n;
-        const float lse = lse_posteriors(Rt, npad, pt, K, g, mx[p], valid);
-        if (g == 0) {
-            ref[pt] = best[p];
-            if (valid) llsum += (double)lse;
-        }
+        const float mxp = __shfl(mx, 16 * p + c);
+        const float lse = lse_posteriors(Rt, npad, pt, K, g, mxp, valid);
+        if (g == 0 && valid) llsum += (double)lse;
     }
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) llsum += __shfl_xor(llsum, o);
@@ -487,8 +573,8 @@ __global__ __launch_bounds__(256) void k_reduce_sumb(const double *__restrict__ 
 }
 
 bool estepr_fits(int NB, int K) {
-    // two LDS stage buffers of at most 80 KB together: two workgroups per CU
-    return NB <= 4 && K >= 1 && 2 * (size_t)estepr_buffer_pieces(NB, K) * 16 <= 80 * 1024;
+    // two LDS stage buffers of at most 64 KB together: two workgroups per CU
+    return NB <= 4 && K >= 1 && estepr_rounds(NB, K) <= 8;
 }
 
 size_t estepr_words(int NB, int K) { return (size_t)K * 4 * estepr_buffer_pieces(NB, K) + (size_t)K * 16 * NB; }
@@ -523,27 +609,32 @@ static hipError_t estepb_t(fem_ctx *c) {
 }
 
 // Point groups per wave: 4 for d <= 64 unless estep_groupsb == 2, else 2.
-template <int NB, int P>
+template <int NB, int R>
 static hipError_t estepr_t(fem_ctx *c) {
-    const size_t lds = (size_t)2 * estepr_buffer_pieces(NB, c->K) * 16;
-    if (lds + sizeof(double) * 4 > 64 * 1024) {  // with the static LDS
-        hipError_t e = hipFuncSetAttribute((const void *)k_estepr<NB, P>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-    }
-    const int blocks = (int)(c->npad / (64 * P));
-    hipLaunchKernelGGL((k_estepr<NB, P>), dim3(blocks), dim3(256), lds, c->stream, c->X, c->n, c->npad, c->K,
+    const int blocks = (int)(c->npad / 256);
+    hipLaunchKernelGGL((k_estepr<NB, R>), dim3(blocks), dim3(256), 0, c->stream, c->X, c->n, c->npad, c->K,
                        c->stager, c->coef, c->refc, c->Rt32, c->ll_part);
     hipLaunchKernelGGL(k_reduce_sumb, dim3(1), dim3(256), 0, c->stream, c->ll_part, (int64_t)blocks, c->ll_dev);
     return hipGetLastError();
 }
 
+template <int NB>
+static hipError_t estepr_rounds_t(fem_ctx *c) {
+    switch (estepr_rounds(NB, c->K)) {
+        case 2: return estepr_t<NB, 2>(c);
+        case 4: return estepr_t<NB, 4>(c);
+        case 6: return estepr_t<NB, 6>(c);
+        case 8: return estepr_t<NB, 8>(c);
+    }
+    return hipErrorInvalidValue;
+}
+
 static hipError_t estepb_ref(fem_ctx *c) {
     switch (c->NB) {
-        case 1: return estepr_t<1, 4>(c);
-        case 2: return estepr_t<2, 4>(c);
-        case 3: return estepr_t<3, 4>(c);
-        case 4: return estepr_t<4, 4>(c);
+        case 1: return estepr_rounds_t<1>(c);
+        case 2: return estepr_rounds_t<2>(c);
+        case 3: return estepr_rounds_t<3>(c);
+        case 4: return estepr_rounds_t<4>(c);
     }
     return hipErrorInvalidValue;
 }

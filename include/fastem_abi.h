@@ -219,7 +219,7 @@ int fem_last_times(fem_ctx *ctx, float *estep_ms, float *means_ms, float *cov_ms
  * after the data, the means (fem_set_params, an initializer) or the precision
  * were set centres every (point, component) pair on mu_k; later ones, for
  * d <= 64 and the component counts whose stage fits two workgroups per CU
- * (K <= 105 at d = 64), centre each point once on the component of its
+ * (K <= 75 at d = 64, 118 at d = 48), centre each point once on the component of its
  * largest log-density in the previous E-step and add U_k^T (mu_a - mu_k)
  * from a table (FASTEM_ESTEPB_REF=0 at fem_create / fem_set_precision keeps
  * the first kind throughout). */
