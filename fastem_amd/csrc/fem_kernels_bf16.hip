@@ -313,15 +313,20 @@ __device__ __host__ inline int estepr_rounds(int NB, int K) {
 }
 __device__ __host__ inline int estepr_buffer_pieces(int NB, int K) { return 256 * estepr_rounds(NB, K); }
 
+// Grid (K, kPrepParts): workgroup (k, part) writes every kPrepParts-th group
+// of 256 seed words of component k's block; part 0 also packs the fragments
+// and mu_k,hi.
+constexpr int kPrepParts = 8;
+
 __global__ __launch_bounds__(256) void k_prep_ref(int d, int dpad, int NB, int K, const double *__restrict__ U,
                                                   const double *__restrict__ mu, unsigned int *__restrict__ stage) {
-    const int k = blockIdx.x;
+    const int k = blockIdx.x, part = blockIdx.y;
     const int FW = pairs_bf16(NB) * 512, SR = estepr_row(NB);
     const int BW = 4 * estepr_buffer_pieces(NB, K);  // words per padded block
     const double *Uk = U + (size_t)k * dpad * dpad;
     unsigned int *st = stage + (size_t)k * BW;
-    pack_fragments_bf16(Uk, dpad, NB, st);
-    for (int e = threadIdx.x; e < BW - FW; e += blockDim.x) {
+    if (part == 0) pack_fragments_bf16(Uk, dpad, NB, st);
+    for (int e = part * 256 + threadIdx.x; e < BW - FW; e += kPrepParts * 256) {
         const int a = e / SR, j = e % SR;
         double s = 0.0;
         if (a < K && j < d)
@@ -330,7 +335,8 @@ __global__ __launch_bounds__(256) void k_prep_ref(int d, int dpad, int NB, int K
         st[FW + e] = __builtin_bit_cast(unsigned int, (float)s);
     }
     float *muh = reinterpret_cast<float *>(stage + (size_t)K * BW) + (size_t)k * dpad;
-    for (int j = threadIdx.x; j < dpad; j += blockDim.x) muh[j] = (j < d) ? (float)mu[(size_t)k * d + j] : 0.0f;
+    if (part == 0)
+        for (int j = threadIdx.x; j < dpad; j += blockDim.x) muh[j] = (j < d) ? (float)mu[(size_t)k * d + j] : 0.0f;
 }
 
 // Sums of q[p] over the four 16-lane groups for p = 0..3 at once, with the
@@ -585,7 +591,7 @@ size_t estepr_words(int NB, int K) { return (size_t)K * 4 * estepr_buffer_pieces
 hipError_t launch_prep_bf16(fem_ctx *c) {
     c->stage_ref = c->stager && c->refc && c->ref_ok && estepr_fits(c->NB, c->K);
     if (c->stage_ref)
-        hipLaunchKernelGGL(k_prep_ref, dim3(c->K), dim3(256), 0, c->stream, c->d, c->dpad, c->NB, c->K, c->U,
+        hipLaunchKernelGGL(k_prep_ref, dim3(c->K, kPrepParts), dim3(256), 0, c->stream, c->d, c->dpad, c->NB, c->K, c->U,
                            c->mu, c->stager);
     else
         hipLaunchKernelGGL(k_prep_bf16, dim3(c->K), dim3(256), 0, c->stream, c->d, c->dpad, c->NB, c->U, c->mu,
