@@ -69,6 +69,42 @@ __device__ inline v4f mfma3(v8bf ah, v8bf al, v8bf bh, v8bf bl, v4f acc) {
 
 __device__ inline float xor_sum_gb(float v) { return group_sum4(v); }
 
+// Sums of q[p] over the four 16-lane groups for p = 0..3 at once, with the
+// gfx950 lane swaps: lane (g, c) gets the total of q[g] over lanes c, c + 16,
+// c + 32, c + 48, added as (g0 + g1) + (g2 + g3) like group_sum4.
+// permlane16_swap(x, y) returns x with its odd rows (16 lanes) replaced by y's
+// even rows and y with its even rows replaced by x's odd rows;
+// permlane32_swap(x, y) returns x with its upper half replaced by y's lower
+// half and y with its lower half replaced by x's upper half.
+__device__ inline float group_sums4_spread(const float (&q)[4]) {
+    auto u = [](float v) { return __builtin_bit_cast(unsigned, v); };
+    auto f = [](unsigned v) { return __builtin_bit_cast(float, v); };
+    const auto s01 = __builtin_amdgcn_permlane16_swap(u(q[0]), u(q[1]), false, false);
+    const auto s23 = __builtin_amdgcn_permlane16_swap(u(q[2]), u(q[3]), false, false);
+    // rows of a: q0 (r0 + r1), q1 (r0 + r1), q0 (r2 + r3), q1 (r2 + r3); b likewise
+    const float a = f(s01[0]) + f(s01[1]);
+    const float b = f(s23[0]) + f(s23[1]);
+    const auto t = __builtin_amdgcn_permlane32_swap(u(a), u(b), false, false);
+    return f(t[0]) + f(t[1]);
+}
+
+// The same for two values: lane (g, c) gets the total of q[g & 1].  The
+// second swap pairs a with a copy of itself in another register (a plain
+// move: passed the same value twice, the swap builtin's two results were
+// combined into one register by this compiler).
+__device__ inline float group_sums2_spread(const float (&q)[2]) {
+    auto u = [](float v) { return __builtin_bit_cast(unsigned, v); };
+    auto f = [](unsigned v) { return __builtin_bit_cast(float, v); };
+    const auto s01 = __builtin_amdgcn_permlane16_swap(u(q[0]), u(q[1]), false, false);
+    // rows of a: q0 (r0 + r1), q1 (r0 + r1), q0 (r2 + r3), q1 (r2 + r3)
+    const float a = f(s01[0]) + f(s01[1]);
+    float b;
+    asm("v_mov_b32 %0, %1" : "=v"(b) : "v"(a));
+    const auto t = __builtin_amdgcn_permlane32_swap(u(a), u(b), false, false);
+    return f(t[0]) + f(t[1]);
+}
+
+
 // ---------------------------------------------------------------------------
 // E-step operands of component k, packed as one contiguous stage block of
 // estepb_stage_words(NB) words so the E-step copies it with plain strided
@@ -191,9 +227,7 @@ __global__ __launch_bounds__(256) void k_estepb(const float *__restrict__ X, int
     }
     __syncthreads();
 
-    float mx[P];
-#pragma unroll
-    for (int p = 0; p < P; p++) mx[p] = -INFINITY;
+    float mx = -INFINITY;  // of point (g mod P, c)
 
     for (int k = 0; k < K; k++) {
         const unsigned int *lf = ldsb + (k & 1) * STAGE;
@@ -235,18 +269,23 @@ __global__ __launch_bounds__(256) void k_estepb(const float *__restrict__ X, int
                 for (int p = 0; p < P; p++) acc[p][jb] = mfma3(ah, al, bh[p], bl[p], acc[p][jb]);
             }
         }
-        float sv[P];
+        // squared norms summed over the lane groups so that lane (g, c)
+        // holds the log-density of point (g mod P, c) of its wave
+        float q[P];
 #pragma unroll
         for (int p = 0; p < P; p++) {
-            float q = 0.f;
+            q[p] = 0.f;
 #pragma unroll
             for (int jb = 0; jb < NB; jb++)
 #pragma unroll
-                for (int v = 0; v < 4; v++) q += acc[p][jb][v] * acc[p][jb][v];
-            q = xor_sum_gb(q);
-            sv[p] = -0.5f * q + ck;
-            mx[p] = fmaxf(mx[p], sv[p]);
+                for (int v = 0; v < 4; v++) q[p] += acc[p][jb][v] * acc[p][jb][v];
         }
+        float sv;
+        if constexpr (P == 4)
+            sv = -0.5f * group_sums4_spread(q) + ck;
+        else
+            sv = -0.5f * group_sums2_spread(q) + ck;
+        mx = fmaxf(mx, sv);
         {
             // unconditional (after the last component the idle buffer is
             // dead) and ahead of the branch below, so that the loads of the
@@ -255,10 +294,9 @@ __global__ __launch_bounds__(256) void k_estepb(const float *__restrict__ X, int
 #pragma unroll
             for (int q = 0; q < PER; q++) dst[min(tid + 256 * q, STAGE4 - 1)] = pre[q];
         }
-        if ((k & 3) == g) {
-#pragma unroll
-            for (int p = 0; p < P; p++) Rt[(size_t)k * npad + base + 16 * p + c] = sv[p];
-        }
+        if (g < P) Rt[(size_t)k * npad + base + 16 * g + c] = sv;
+        // (the barrier also orders these stores before the sweep below,
+        // whose lanes read entries other lanes stored)
         __syncthreads();
     }
 
@@ -267,7 +305,8 @@ __global__ __launch_bounds__(256) void k_estepb(const float *__restrict__ X, int
     for (int p = 0; p < P; p++) {
         const int64_t pt = base + 16 * p + c;
         const bool valid = pt < n;
-        const float lse = lse_posteriors(Rt, npad, pt, K, g, mx[p], valid);
+        const float mxp = __shfl(mx, 16 * p + c);
+        const float lse = lse_posteriors(Rt, npad, pt, K, g, mxp, valid);
         if (g == 0 && valid) llsum += (double)lse;
     }
 #pragma unroll
@@ -337,25 +376,6 @@ __global__ __launch_bounds__(256) void k_prep_ref(int d, int dpad, int NB, int K
     float *muh = reinterpret_cast<float *>(stage + (size_t)K * BW) + (size_t)k * dpad;
     if (part == 0)
         for (int j = threadIdx.x; j < dpad; j += blockDim.x) muh[j] = (j < d) ? (float)mu[(size_t)k * d + j] : 0.0f;
-}
-
-// Sums of q[p] over the four 16-lane groups for p = 0..3 at once, with the
-// gfx950 lane swaps: lane (g, c) gets the total of q[g] over lanes c, c + 16,
-// c + 32, c + 48, added as (g0 + g1) + (g2 + g3) like group_sum4.
-// permlane16_swap(x, y) returns x with its odd rows (16 lanes) replaced by y's
-// even rows and y with its even rows replaced by x's odd rows;
-// permlane32_swap(x, y) returns x with its upper half replaced by y's lower
-// half and y with its lower half replaced by x's upper half.
-__device__ inline float group_sums4_spread(const float (&q)[4]) {
-    auto u = [](float v) { return __builtin_bit_cast(unsigned, v); };
-    auto f = [](unsigned v) { return __builtin_bit_cast(float, v); };
-    const auto s01 = __builtin_amdgcn_permlane16_swap(u(q[0]), u(q[1]), false, false);
-    const auto s23 = __builtin_amdgcn_permlane16_swap(u(q[2]), u(q[3]), false, false);
-    // rows of a: q0 (r0 + r1), q1 (r0 + r1), q0 (r2 + r3), q1 (r2 + r3); b likewise
-    const float a = f(s01[0]) + f(s01[1]);
-    const float b = f(s23[0]) + f(s23[1]);
-    const auto t = __builtin_amdgcn_permlane32_swap(u(a), u(b), false, false);
-    return f(t[0]) + f(t[1]);
 }
 
 // One component of the reference-centred E-step: accumulators from the seed
