@@ -28,7 +28,6 @@ __device__ __host__ constexpr int pairs32(int NB) {
     return p;
 }
 
-__device__ inline float xor_sum_g(float v) { return group_sum4(v); }
 
 // ---------------------------------------------------------------------------
 // Stage (a), FP32.  Workgroup = 4 waves x P groups of 16 points.  The
@@ -70,9 +69,7 @@ __global__ __launch_bounds__(256) void k_estep32(const float *__restrict__ X, in
     }
     __syncthreads();
 
-    float mx[P];
-#pragma unroll
-    for (int p = 0; p < P; p++) mx[p] = -INFINITY;
+    float mx = -INFINITY;  // of point (g mod P, c)
 
     for (int k = 0; k < K; k++) {
         const float *lu = lds32 + (k & 1) * STAGE;
@@ -114,18 +111,23 @@ __global__ __launch_bounds__(256) void k_estep32(const float *__restrict__ X, in
                     acc[p][jb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b[p], acc[p][jb], 0, 0, 0);
             }
         }
-        float sv[P];
+        // squared norms summed over the lane groups with lane swaps, so that
+        // lane (g, c) holds the log-density of point (g mod P, c)
+        float q[P];
 #pragma unroll
         for (int p = 0; p < P; p++) {
-            float q = 0.f;
+            q[p] = 0.f;
 #pragma unroll
             for (int jb = 0; jb < NB; jb++)
 #pragma unroll
-                for (int v = 0; v < 4; v++) q += acc[p][jb][v] * acc[p][jb][v];
-            q = xor_sum_g(q);
-            sv[p] = -0.5f * q + ck;
-            mx[p] = fmaxf(mx[p], sv[p]);
+                for (int v = 0; v < 4; v++) q[p] += acc[p][jb][v] * acc[p][jb][v];
         }
+        float sv;
+        if constexpr (P == 4)
+            sv = -0.5f * group_sums4_spread(q) + ck;
+        else
+            sv = -0.5f * group_sums2_spread(q) + ck;
+        mx = fmaxf(mx, sv);
         {
             // unconditional (after the last component the idle buffer is
             // dead) and ahead of the branches below, so that the loads of the
@@ -134,10 +136,9 @@ __global__ __launch_bounds__(256) void k_estep32(const float *__restrict__ X, in
 #pragma unroll
             for (int q = 0; q < PER; q++) dst[min(tid + 256 * q, STAGE4 - 1)] = pre[q];
         }
-        if ((k & 3) == g) {
-#pragma unroll
-            for (int p = 0; p < P; p++) Rt[(size_t)k * npad + base + 16 * p + c] = sv[p];
-        }
+        if (g < P) Rt[(size_t)k * npad + base + 16 * g + c] = sv;
+        // (the barrier also orders these stores before the sweep below,
+        // whose lanes read entries other lanes stored)
         __syncthreads();
     }
 
@@ -146,7 +147,8 @@ __global__ __launch_bounds__(256) void k_estep32(const float *__restrict__ X, in
     for (int p = 0; p < P; p++) {
         const int64_t pt = base + 16 * p + c;
         const bool valid = pt < n;
-        const float lse = lse_posteriors(Rt, npad, pt, K, g, mx[p], valid);
+        const float mxp = __shfl(mx, 16 * p + c);
+        const float lse = lse_posteriors(Rt, npad, pt, K, g, mxp, valid);
         if (g == 0 && valid) llsum += (double)lse;
     }
 #pragma unroll

@@ -59,8 +59,10 @@ __device__ __host__ constexpr int pairs_of(int NB) {
 // permlane16_swap(v, v) returns (v of the even group, v of the odd group) of
 // each 32-lane half, permlane32_swap(v, v) returns (v of the lower half, v of
 // the upper half), so each step is one swap of each 32-bit half and one add.
-// (The 32-bit swap builtin is not used for FP32 sums: this compiler combined
-// its two results into one for a single-register operand.)
+// (The 32-bit swap builtin is not used for a single FP32 value: this
+// compiler combined its two results into one for a single-register operand.
+// The FP32 E-steps sum two or four values at once with group_sums2/4_spread
+// below; this form remains for the log-sum-exp sweep.)
 __device__ inline float group_sum4(float v) {
     v += __shfl_xor(v, 16);
     v += __shfl_xor(v, 32);
@@ -83,6 +85,41 @@ __device__ inline double group_sum4(double v) {
 }
 
 __device__ inline double wave_xor_sum_g(double v) { return group_sum4(v); }
+
+// Sums of q[p] over the four 16-lane groups for p = 0..3 at once, with the
+// gfx950 lane swaps: lane (g, c) gets the total of q[g] over lanes c, c + 16,
+// c + 32, c + 48, added as (g0 + g1) + (g2 + g3) like group_sum4.
+// permlane16_swap(x, y) returns x with its odd rows (16 lanes) replaced by y's
+// even rows and y with its even rows replaced by x's odd rows;
+// permlane32_swap(x, y) returns x with its upper half replaced by y's lower
+// half and y with its lower half replaced by x's upper half.
+__device__ inline float group_sums4_spread(const float (&q)[4]) {
+    auto u = [](float v) { return __builtin_bit_cast(unsigned, v); };
+    auto f = [](unsigned v) { return __builtin_bit_cast(float, v); };
+    const auto s01 = __builtin_amdgcn_permlane16_swap(u(q[0]), u(q[1]), false, false);
+    const auto s23 = __builtin_amdgcn_permlane16_swap(u(q[2]), u(q[3]), false, false);
+    // rows of a: q0 (r0 + r1), q1 (r0 + r1), q0 (r2 + r3), q1 (r2 + r3); b likewise
+    const float a = f(s01[0]) + f(s01[1]);
+    const float b = f(s23[0]) + f(s23[1]);
+    const auto t = __builtin_amdgcn_permlane32_swap(u(a), u(b), false, false);
+    return f(t[0]) + f(t[1]);
+}
+
+// The same for two values: lane (g, c) gets the total of q[g & 1].  The
+// second swap pairs a with a copy of itself in another register (a plain
+// move: passed the same value twice, the swap builtin's two results were
+// combined into one register by this compiler).
+__device__ inline float group_sums2_spread(const float (&q)[2]) {
+    auto u = [](float v) { return __builtin_bit_cast(unsigned, v); };
+    auto f = [](unsigned v) { return __builtin_bit_cast(float, v); };
+    const auto s01 = __builtin_amdgcn_permlane16_swap(u(q[0]), u(q[1]), false, false);
+    // rows of a: q0 (r0 + r1), q1 (r0 + r1), q0 (r2 + r3), q1 (r2 + r3)
+    const float a = f(s01[0]) + f(s01[1]);
+    float b;
+    asm("v_mov_b32 %0, %1" : "=v"(b) : "v"(a));
+    const auto t = __builtin_amdgcn_permlane32_swap(u(a), u(b), false, false);
+    return f(t[0]) + f(t[1]);
+}
 
 // Fixed-order sum of p[0], p[stride], ..., with 4 interleaved accumulators.
 __device__ inline double chunk_sum4(const double *__restrict__ p, size_t stride, int chunks) {
