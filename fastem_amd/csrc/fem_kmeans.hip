@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "fem_internal.h"
+#include "fem_tiles.h"
 
 namespace fem {
 
@@ -106,6 +107,161 @@ __global__ __launch_bounds__(256) void k_assign(const float *__restrict__ X, int
         __syncthreads();
     }
     if (threadIdx.x == 0) fit_part[blockIdx.x] = red[0];
+}
+
+// The same assignment with the distances' cross terms on FP64 MFMA.  Phase 1
+// (the FP64 E-step's tile: lane (g, c) holds features 4 lb + g of point c of
+// each of its P groups; A = 16 centroids x 4 features from LDS, B = the point
+// tile): a_k = (|x|^2 + |c_k|^2) - 2 x.c_k for every centroid, and per point
+// the smallest a (lowest index on ties) and the second smallest.  Every a_k
+// is within E = 2 (4 DP + 16) u (|x|^2 + max_k |c_k|^2) of the exact
+// distance s_k that sqdist_reg computes (u = 2^-53: the rounding of the two
+// norms, of the MFMA dot product over DP features and of the final sum,
+// plus that of sqdist_reg itself; both bounded by (2 DP + 8) u (|x|^2 +
+// |c_k|^2), doubled for margin), so a gap above 2E between the smallest and
+// the second smallest a proves that exact argmin is the MFMA one, strictly.
+// Phase 2 (one point per thread): the exact distance to that centroid, or,
+// for points without such a gap, the exact scan over all centroids as in
+// k_assign; outputs as k_assign's, so both kernels give identical results.
+// LDS: KP centroid rows of DP + 4 doubles (zero beyond d and K), then the KP
+// squared norms.
+template <int DP>
+__global__ __launch_bounds__(256) void k_assign_mm(const float *__restrict__ X, int64_t n, int64_t npad,
+                                                   int dpad, int d, int K, int KP, const double *__restrict__ C,
+                                                   double *__restrict__ Rt, int *__restrict__ cls,
+                                                   double *__restrict__ fit_part) {
+    constexpr int NL = DP / 4;                 // 4-feature strips
+    constexpr int P = DP <= 64 ? 4 : 2;        // 16-point groups per wave and pass
+    constexpr int H = 4 / P;                   // passes: 64 points per wave
+    constexpr int LD = DP + 4;
+    constexpr double kBound = 2.0 * (4.0 * DP + 16.0) * 0x1p-53;
+    extern __shared__ double lc[];
+    __shared__ double red[256];
+    __shared__ int sb[256];  // per point: the proven argmin, or -1
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int g = lane >> 4, c = lane & 15;
+    double *ccl = lc + (size_t)KP * LD;
+    for (int e = tid; e < KP * LD; e += 256) {
+        const int k = e / LD, j = e % LD;
+        lc[e] = (k < K && j < d) ? C[(size_t)k * d + j] : 0.0;
+    }
+    __syncthreads();
+    for (int k = tid; k < KP; k += 256) {
+        double s = 0.0;
+        for (int j = 0; j < d; j++) s += lc[(size_t)k * LD + j] * lc[(size_t)k * LD + j];
+        ccl[k] = s;
+    }
+    __syncthreads();
+    double ccmax = 0.0;
+    for (int k = 0; k < K; k++) ccmax = fmax(ccmax, ccl[k]);
+    const int64_t blk0 = (int64_t)blockIdx.x * 256;
+
+    for (int h = 0; h < H; h++) {
+        const int loc0 = wave * 64 + h * 16 * P;
+        float xr[P][NL];
+#pragma unroll
+        for (int p = 0; p < P; p++)
+#pragma unroll
+            for (int lb = 0; lb < NL; lb++) xr[p][lb] = X[(blk0 + loc0 + 16 * p + c) * DP + 4 * lb + g];
+        double xx[P], best[P], second[P];
+        int bidx[P];
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            double s = 0.0;
+#pragma unroll
+            for (int lb = 0; lb < NL; lb++) s += (double)xr[p][lb] * (double)xr[p][lb];
+            xx[p] = group_sum4(s);
+            best[p] = second[p] = __builtin_inf();
+            bidx[p] = 0x7fffffff;
+        }
+        for (int kb = 0; kb < KP / 16; kb++) {
+            v4d acc[P];
+#pragma unroll
+            for (int p = 0; p < P; p++) acc[p] = (v4d){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int lb = 0; lb < NL; lb++) {
+                const double a = lc[(size_t)(16 * kb + c) * LD + 4 * lb + g];  // A[c][g]
+#pragma unroll
+                for (int p = 0; p < P; p++)
+                    acc[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, (double)xr[p][lb], acc[p], 0, 0, 0);
+            }
+            // element v of lane (g, c) holds x_c . c_k for k = 16 kb + 4 v + g
+            // (the FP64 16x16x4 accumulator layout)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int k = 16 * kb + 4 * v + g;
+                if (k < K) {
+                    const double ck = ccl[k];
+#pragma unroll
+                    for (int p = 0; p < P; p++) {
+                        const double dist = (xx[p] + ck) - 2.0 * acc[p][v];
+                        if (dist < best[p]) {
+                            second[p] = best[p];
+                            best[p] = dist;
+                            bidx[p] = k;
+                        } else if (dist < second[p]) {
+                            second[p] = dist;
+                        }
+                    }
+                }
+            }
+        }
+        // merge the four lane groups of each point (a set function: the
+        // smallest value with the lowest index, and the second smallest)
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+#pragma unroll
+            for (int o = 16; o < 64; o <<= 1) {
+                const double ob = __shfl_xor(best[p], o), os = __shfl_xor(second[p], o);
+                const int oi = __shfl_xor(bidx[p], o);
+                if (ob < best[p] || (ob == best[p] && oi < bidx[p])) {
+                    second[p] = fmin(best[p], os);
+                    best[p] = ob;
+                    bidx[p] = oi;
+                } else {
+                    second[p] = fmin(second[p], ob);
+                }
+            }
+            if (g == 0) {
+                const bool proven = second[p] - best[p] > 2.0 * kBound * (xx[p] + ccmax);
+                sb[loc0 + 16 * p + c] = proven ? bidx[p] : -1;
+            }
+        }
+    }
+    __syncthreads();
+
+    const int64_t i = blk0 + tid;
+    double bestd = 0.0;
+    if (i < npad) {
+        int bj = -1;
+        if (i < n) {
+            float x[DP];
+            load_row<DP>(X, i, dpad, x);
+            const int cand = sb[tid];
+            if (cand >= 0) {
+                bj = cand;
+                bestd = sqdist_reg<DP>(lc + (size_t)cand * LD, x, d);
+            } else {
+                bestd = std::numeric_limits<double>::max();
+                for (int k = 0; k < K; k++) {
+                    const double s = sqdist_reg<DP>(lc + (size_t)k * LD, x, d);
+                    if (s < bestd) {
+                        bestd = s;
+                        bj = k;
+                    }
+                }
+            }
+            cls[i] = bj;
+        }
+        for (int k = 0; k < KP; k++) Rt[(size_t)k * npad + i] = (k == bj) ? 1.0 : 0.0;
+    }
+    red[tid] = (i < n) ? bestd : 0.0;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (tid < st) red[tid] += red[tid + st];
+        __syncthreads();
+    }
+    if (tid == 0) fit_part[blockIdx.x] = red[0];
 }
 
 // Squared distance of two float rows as Clust/CentroidVector.h:79-87 forms
@@ -224,8 +380,20 @@ double assign(fem_ctx *c, const std::vector<double> &C, std::vector<double> *sum
     KM_HIP(hipMemcpyAsync(c->kC, C.data(), sizeof(double) * c->K * c->d, hipMemcpyHostToDevice, c->stream));
     const int blocks = (int)(c->npad / 256);
     const size_t lds = sizeof(double) * c->K * c->d;
+    // MFMA form where its LDS (centroid rows padded by 4, their norms) fits
+    const size_t lds_mm = sizeof(double) * (size_t)c->KP * (c->dpad + 5);
     KM_HIP(fem::by_dpad(c->dpad, [&](auto DPc) -> hipError_t {
         constexpr int DP = decltype(DPc)::value;
+        if (lds_mm <= 150 * 1024) {
+            if (lds_mm > 64 * 1024) {
+                hipError_t e = hipFuncSetAttribute((const void *)fem::k_assign_mm<DP>,
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_mm);
+                if (e != hipSuccess) return e;
+            }
+            hipLaunchKernelGGL(fem::k_assign_mm<DP>, dim3(blocks), dim3(256), lds_mm, c->stream, c->X, c->n,
+                               c->npad, c->dpad, c->d, c->K, c->KP, c->kC, c->Rt, c->cls, c->kfit);
+            return hipGetLastError();
+        }
         if (lds > 64 * 1024) {
             hipError_t e = hipFuncSetAttribute((const void *)fem::k_assign<DP>,
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

@@ -48,3 +48,50 @@ def test_kmeans_init_on_context_sets_params(orc):
     U, ld = orc.factor_all(cov)
     ll_ref, _ = orc.estep(X, w, mu, U, ld)
     assert abs(ll - ll_ref) <= 1e-10 * abs(ll_ref)
+
+
+def grid_points(n, d, seed):
+    """n rows drawn from the integer grid {-2..2}^d (many rows equidistant
+    from symmetric centroids)."""
+    rng = np.random.default_rng(seed)
+    return rng.integers(-2, 3, size=(n, d)).astype(np.float32)
+
+
+@pytest.mark.parametrize("case", ["exact_ties", "near_ties", "random_d64_k64", "separated"])
+def test_assignment_matches_oracle_classification(orc, case):
+    """The assignment pass (distances' cross terms on FP64 MFMA, the exact
+    sequential distance wherever the MFMA gap does not prove the argmin)
+    against the oracle's classification by the same centroids: exact ties
+    (lowest index wins), near-ties 1e-15 apart (below the MFMA error bound,
+    decided by the exact distances), a random d = 64, K = 64 case, and
+    the generating means of well-separated clusters.  Zero Lloyd sweeps: the mixture is that of the
+    classification by C0; counts must be equal and covariances within
+    1e-12."""
+    print(f"assignment vs oracle: {case}", flush=True)
+    if case in ("exact_ties", "near_ties"):
+        n, d = 40000, 4
+        X = grid_points(n, d, 3)
+        C0 = np.array([[1, 0, 0, 0], [-1, 0, 0, 0], [0, 1, 0, 0], [0, -1, 0, 0], [0, 0, 1, 1],
+                       [0, 0, -1, -1]], dtype=np.float64)
+        if case == "near_ties":
+            C0[1, 0] -= 1e-15
+            C0[3, 1] += 3e-16
+            C0[5, 2] -= 2e-15
+    elif case == "random_d64_k64":
+        n, d, K = 100000, 64, 64
+        X = fem.gengmm(n, d, K, -5.0, 5.0, 0.5, 41)[0]
+        C0 = X[np.random.default_rng(2).choice(n, K, replace=False)].astype(np.float64)
+    else:
+        n, d, K = 30000, 17, 9
+        X, _, C0, _ = fem.gengmm(n, d, K, -10.0, 10.0, 0.1, 42)
+        C0 = np.ascontiguousarray(C0, dtype=np.float64)
+    K = C0.shape[0]
+    cls = orc.classify(X, C0)
+    assert np.bincount(cls, minlength=K).min() >= 2
+    with fem.Context(n, d, K) as ctx:
+        ctx.upload(X)
+        w, mu, cov = ctx.kmeans_init_seeded(C0, seed=1, max_sweeps=0)
+    w_ref, mu_ref, cov_ref = orc.gmm_from_clusters(X, C0, cls)
+    np.testing.assert_array_equal(np.round(w * n), np.bincount(cls, minlength=K))
+    np.testing.assert_array_equal(mu, C0)
+    assert np.max(np.abs(cov - cov_ref)) <= 1e-12 * np.max(np.abs(cov_ref))
