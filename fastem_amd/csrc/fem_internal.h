@@ -82,6 +82,7 @@ struct fem_ctx {
     double *kC = nullptr;       // K x d centroids
     double *dmin = nullptr;     // npad  k-means++ closest squared distances
     int *cls = nullptr;         // npad  cluster of each point
+    int64_t *kpick = nullptr;   // 2     k-means++ draw decided on the device (row, decided)
     double *kfit = nullptr;     // per-workgroup fit partials
     bool owns_rt64 = false;     // Rt allocated for k-means in a non-FP64 context
 

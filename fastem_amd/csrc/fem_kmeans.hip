@@ -5,15 +5,19 @@
 // final clustering (EM2/EMInitializer.cpp:103-134).
 //
 // Squared distances are FP64 sums in ascending feature order with no
-// multiply-add contraction (double centroid minus float row widened), the
-// argmin keeps the lowest index on ties, and the D^2 prefix sum and draw of
-// k-means++ run on the host in sequential FP64 order, so seeds and assignment
-// counts equal the FP64 oracle's on inputs without near-ties.  Centroid sums
+// multiply-add contraction (double centroid minus float row widened) and the
+// argmin keeps the lowest index on ties; the assignment pass takes its
+// argmin from FP64 MFMA distances only where a rounding bound proves it
+// (k_assign_mm).  The k-means++ D^2 draw follows the sequential FP64 prefix
+// sums: decided on the device where a rounding bound proves the sequential
+// rule's row (k_kpp_pick), by those sums on the host otherwise.  So seeds and
+// assignment counts equal the FP64 oracle's.  Centroid sums
 // and counts come from the MFMA means pass applied to the one-hot assignment
 // (products by 1.0 are exact; sums in FP64, fixed order).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <random>
@@ -54,19 +58,108 @@ __device__ inline void load_row(const float *__restrict__ X, int64_t i, int dpad
     }
 }
 
+// k-means++ distance update: dmin[i] = min(dmin[i], |x_i - c|^2) (or the
+// distance itself on reset), and bsum[block] = the workgroup's sum of the
+// updated dmin (fixed tree order).
 template <int DP>
 __global__ __launch_bounds__(256) void k_kpp_update(const float *__restrict__ X, int64_t n, int dpad, int d,
                                                     const double *__restrict__ cen, double *__restrict__ dmin,
-                                                    int reset) {
+                                                    int reset, double *__restrict__ bsum) {
     __shared__ double lc[DP];
+    __shared__ double red[256];
     for (int j = threadIdx.x; j < d; j += 256) lc[j] = cen[j];
     __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    float x[DP];
-    load_row<DP>(X, i, dpad, x);
-    const double s = sqdist_reg<DP>(lc, x, d);
-    if (reset || dmin[i] > s) dmin[i] = s;
+    double v = 0.0;
+    if (i < n) {
+        float x[DP];
+        load_row<DP>(X, i, dpad, x);
+        const double s = sqdist_reg<DP>(lc, x, d);
+        v = (reset || dmin[i] > s) ? s : dmin[i];
+        dmin[i] = v;
+    }
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = red[0];
+}
+
+// The k-means++ D^2 draw (Clust/PlusPlusInitializer/PlusPlusInitializer.cpp:
+// 33-84): the first row i with r <= agg_i / agg_{n-1}, agg the prefix sums of
+// dmin, decided on the device where that provably agrees with the host's
+// sequential FP64 sums.  Any summation order of m non-negative terms is
+// within (m - 1) u of the exact sum relatively (u = 2^-53), so every prefix
+// sum formed here and every one the sequential loop forms is within
+// (n + terms here) u of the exact one; bnd bounds the resulting difference of
+// the quotients.  The device finds the first row whose prefix reaches
+// (r - bnd) S; if it also reaches (r + bnd) S, the sequential rule picks the
+// same row, and if none reaches (r - bnd) S, it picks none.  out = {row (n:
+// none), 1} then; {-1, 0} asks the host to decide with the sequential sums
+// (a prefix within bnd of r S, or S not positive and finite).  One workgroup;
+// bsum: nb = ceil(n / 256) workgroup sums from k_kpp_update.
+__global__ __launch_bounds__(256) void k_kpp_pick(const double *__restrict__ bsum, int nb,
+                                                  const double *__restrict__ dmin, int64_t n, double r,
+                                                  int64_t *__restrict__ out) {
+    __shared__ double ts[256], tp[256];
+    __shared__ double S_s;
+    __shared__ int first;
+    const int t = threadIdx.x;
+    const int per = (nb + 255) / 256;
+    const int b0 = min(t * per, nb), b1 = min(b0 + per, nb);
+    double s = 0.0;
+    for (int b = b0; b < b1; b++) s += bsum[b];
+    ts[t] = s;
+    __syncthreads();
+    if (t == 0) {
+        double a = 0.0;
+        for (int u = 0; u < 256; u++) {
+            tp[u] = a;
+            a += ts[u];
+        }
+        S_s = a;
+        first = 0x7fffffff;
+    }
+    __syncthreads();
+    const double S = S_s;
+    if (!(S > 0.0) || !(S < __builtin_inf())) {
+        if (t == 0) out[0] = -1, out[1] = 0;
+        return;
+    }
+    // terms per device prefix: tree 8 + range 2 per + offsets 256 + block 256
+    const double bnd = 8.0 * ((double)n + 2.0 * per + 1024.0) * 0x1p-53;
+    const double lo = (r - bnd) * S, hi = (r + bnd) * S;
+    double a = tp[t];
+    int fb = 0x7fffffff;
+    for (int b = b0; b < b1; b++) {
+        a += bsum[b];
+        if (a >= lo) {
+            fb = b;
+            break;
+        }
+    }
+    atomicMin(&first, fb);
+    __syncthreads();
+    if (t != 0) return;
+    const int b = first;
+    if (b == 0x7fffffff) {
+        out[0] = n, out[1] = 1;
+        return;
+    }
+    const int tb = b / per;
+    double ex = tp[tb];
+    for (int q = tb * per; q < b; q++) ex += bsum[q];
+    const int64_t i0 = (int64_t)b * 256, i1 = min(n, i0 + 256);
+    for (int64_t i = i0; i < i1; i++) {
+        ex += dmin[i];
+        if (ex >= lo) {
+            out[0] = i, out[1] = (ex >= hi) ? 1 : 0;
+            return;
+        }
+    }
+    out[0] = -1, out[1] = 0;
 }
 
 // One point per thread, its row in registers; centroids staged in LDS.
@@ -360,6 +453,7 @@ void ensure_scratch(fem_ctx *c) {
     if (!c->dmin) KM_HIP(hipMalloc((void **)&c->dmin, sizeof(double) * c->npad));
     if (!c->cls) KM_HIP(hipMalloc((void **)&c->cls, sizeof(int) * c->npad));
     if (!c->kfit) KM_HIP(hipMalloc((void **)&c->kfit, sizeof(double) * (c->npad / 256)));
+    if (!c->kpick) KM_HIP(hipMalloc((void **)&c->kpick, sizeof(int64_t) * 2));
     if (!c->Rt) {
         KM_HIP(hipMalloc((void **)&c->Rt, sizeof(double) * c->KP * c->npad));
         c->owns_rt64 = true;
@@ -634,15 +728,33 @@ extern "C" int fem_kmeans_init_rng(fem_ctx *c, double (*rnd)(void *), void *rnd_
                 ~Pinned() { (void)hipHostFree(p); }
             } pin{dm};
             const int blocks = (int)((n + 255) / 256);
+            // FASTEM_KPP_DRAW=host decides every draw with the host's
+            // sequential sums (the path the device falls back to)
+            const char *kd = std::getenv("FASTEM_KPP_DRAW");
+            const bool host_draw = kd && std::strcmp(kd, "host") == 0;
             for (int k = 1; k < K; k++) {
                 KM_HIP(hipMemcpyAsync(c->kC, C.data() + (size_t)(k - 1) * d, sizeof(double) * d,
                                       hipMemcpyHostToDevice, c->stream));
                 KM_HIP(fem::by_dpad(c->dpad, [&](auto DPc) -> hipError_t {
                     constexpr int DP = decltype(DPc)::value;
                     hipLaunchKernelGGL(fem::k_kpp_update<DP>, dim3(blocks), dim3(256), 0, c->stream, c->X, n,
-                                       c->dpad, d, c->kC, c->dmin, k == 1 ? 1 : 0);
+                                       c->dpad, d, c->kC, c->dmin, k == 1 ? 1 : 0, c->kfit);
                     return hipGetLastError();
                 }));
+                const double r = draw();
+                if (!host_draw) {
+                    hipLaunchKernelGGL(fem::k_kpp_pick, dim3(1), dim3(256), 0, c->stream, c->kfit, blocks, c->dmin,
+                                       n, r, c->kpick);
+                    KM_HIP(hipGetLastError());
+                    int64_t pick[2];
+                    KM_HIP(hipMemcpyAsync(pick, c->kpick, sizeof pick, hipMemcpyDeviceToHost, c->stream));
+                    KM_HIP(hipStreamSynchronize(c->stream));
+                    if (pick[1] == 1) {
+                        if (pick[0] < n) fetch_row(c, pick[0], C.data() + (size_t)k * d);
+                        continue;
+                    }
+                }
+                // undecided on the device: the sequential FP64 sums
                 KM_HIP(hipMemcpyAsync(dm, c->dmin, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
                 KM_HIP(hipStreamSynchronize(c->stream));
                 double sum = 0.0;
@@ -654,7 +766,6 @@ extern "C" int fem_kmeans_init_rng(fem_ctx *c, double (*rnd)(void *), void *rnd_
                 // rounded division by sum > 0 keeps that order, so a binary
                 // search over the quotients finds the same row as the
                 // reference's scan over the normalised prefix sums
-                const double r = draw();
                 int64_t lo = 0, hi = n;  // answer in [lo, hi); hi = n: none
                 while (lo < hi) {
                     const int64_t mid = lo + (hi - lo) / 2;

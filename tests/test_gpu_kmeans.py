@@ -95,3 +95,18 @@ def test_assignment_matches_oracle_classification(orc, case):
     np.testing.assert_array_equal(np.round(w * n), np.bincount(cls, minlength=K))
     np.testing.assert_array_equal(mu, C0)
     assert np.max(np.abs(cov - cov_ref)) <= 1e-12 * np.max(np.abs(cov_ref))
+
+
+@pytest.mark.parametrize("n,d,K,seed", [(30000, 6, 5, 9), (200000, 64, 64, 3), (50001, 17, 12, 4)])
+def test_kmeanspp_device_draw_equals_host_draw(monkeypatch, n, d, K, seed):
+    """The k-means++ D^2 draw decided on the device (prefix sums in any
+    order, a rounding bound that proves the sequential rule's row) against
+    the host's sequential FP64 prefix sums (FASTEM_KPP_DRAW=host, the path
+    the device falls back to): the same seeds, hence the same mixture."""
+    print(f"kmeans++ device vs host draw n={n} d={d} K={K}", flush=True)
+    X = fem.gengmm(n, d, K, -20.0, 20.0, 0.05, seed)[0]
+    dev = fem.kmeanspp_init(X, K, seed=seed, lloyd_iters=0)
+    monkeypatch.setenv("FASTEM_KPP_DRAW", "host")
+    host = fem.kmeanspp_init(X, K, seed=seed, lloyd_iters=0)
+    for a, b in zip(dev, host):
+        np.testing.assert_array_equal(a, b)
