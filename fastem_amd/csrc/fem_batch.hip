@@ -483,9 +483,13 @@ int fem_batch_create(fem_batch **out, int nfits, const int64_t *n, const int *d,
         (e = hipMalloc((void **)&b->ipool, sizeof(int) * std::max<int64_t>(1, b->ni))) != hipSuccess ||
         (e = upload_vec(&b->means, means)) != hipSuccess || (e = build_shape_lists(b)) != hipSuccess ||
         (e = upload_vec(&b->comp, comp)) != hipSuccess || (e = upload_vec(&b->covred, covred)) != hipSuccess ||
-        (e = hipMemset(b->xpool, 0, sizeof(float) * std::max<int64_t>(1, b->nx))) != hipSuccess ||
-        (e = hipMemset(b->dpool, 0, sizeof(double) * std::max<int64_t>(1, b->nd))) != hipSuccess ||
-        (e = hipMemset(b->ipool, 0, sizeof(int) * std::max<int64_t>(1, b->ni))) != hipSuccess)
+        // cleared on the batch's own stream, and finished before return: the
+        // stream is non-blocking, so a clear queued on the null stream could
+        // land after the uploads and parameters the caller enqueues next
+        (e = hipMemsetAsync(b->xpool, 0, sizeof(float) * std::max<int64_t>(1, b->nx), b->stream)) != hipSuccess ||
+        (e = hipMemsetAsync(b->dpool, 0, sizeof(double) * std::max<int64_t>(1, b->nd), b->stream)) != hipSuccess ||
+        (e = hipMemsetAsync(b->ipool, 0, sizeof(int) * std::max<int64_t>(1, b->ni), b->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(b->stream)) != hipSuccess)
         return bail(FEM_ERR_HIP, hipGetErrorString(e));
     const size_t lds_e = (size_t)fem::estep64_lds<4>() * sizeof(double);
     (void)hipFuncSetAttribute((const void *)fem::kb_estep<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_e);

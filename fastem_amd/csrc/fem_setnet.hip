@@ -939,7 +939,10 @@ int fem_setnet_create(fem_setnet **out, int d, int K, uint64_t seed, int device)
         hipMalloc((void **)&s->m, P * sizeof(float)) != hipSuccess ||
         hipMalloc((void **)&s->v, P * sizeof(float)) != hipSuccess ||
         hipMemcpy(s->theta, th.data(), P * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemset(s->m, 0, P * sizeof(float)) != hipSuccess || hipMemset(s->v, 0, P * sizeof(float)) != hipSuccess) {
+        // on the net's own (non-blocking) stream, finished before return
+        hipMemsetAsync(s->m, 0, P * sizeof(float), s->stream) != hipSuccess ||
+        hipMemsetAsync(s->v, 0, P * sizeof(float), s->stream) != hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess) {
         delete s;
         return FEM_ERR_HIP;
     }
@@ -972,18 +975,23 @@ int fem_setnet_shape(const fem_setnet *s, int *d, int *K) {
     return FEM_OK;
 }
 
+// Parameter copies wait for the net's stream (non-blocking: the null-stream
+// copies would not order against its kernels).
 int fem_setnet_get_params(fem_setnet *s, float *theta) {
     if (!s || !theta) return FEM_ERR_ARG;
+    SH(s, hipStreamSynchronize(s->stream));
     SH(s, hipMemcpy(theta, s->theta, sizeof(float) * s->off.total, hipMemcpyDeviceToHost));
     return FEM_OK;
 }
 
 int fem_setnet_set_params(fem_setnet *s, const float *theta) {
     if (!s || !theta) return FEM_ERR_ARG;
+    SH(s, hipStreamSynchronize(s->stream));
     SH(s, hipMemcpy(s->theta, theta, sizeof(float) * s->off.total, hipMemcpyHostToDevice));
     s->step = 0;
-    SH(s, hipMemset(s->m, 0, sizeof(float) * s->off.total));
-    SH(s, hipMemset(s->v, 0, sizeof(float) * s->off.total));
+    SH(s, hipMemsetAsync(s->m, 0, sizeof(float) * s->off.total, s->stream));
+    SH(s, hipMemsetAsync(s->v, 0, sizeof(float) * s->off.total, s->stream));
+    SH(s, hipStreamSynchronize(s->stream));
     return FEM_OK;
 }
 

@@ -145,7 +145,7 @@ void LearnedEMInitializer::Init(GaussianMixture &G) {
         printf("Learned initializer: final LL %.17g from the network, %.17g from k-means++; kept %s\n", ll[0], ll[1],
                used == 2 ? "k-means++" : "the network's");
     else if (used == 0)
-        printf("Learned initializer: a proposed cluster kept fewer than two points, k-means++ used\n");
+        printf("Learned initializer: a proposed cluster kept fewer than d + 1 points, k-means++ used\n");
     for (int i = 0; i < Components; i++) {
         Weights[i] = w[i];
         for (int j = 0; j < ncols; j++) Means(i, j) = mu[(size_t)i * ncols + j];

@@ -120,11 +120,21 @@ int propose(fem_setnet *net, fem_ctx *c, int n_sub, double (*rnd)(void *), void 
     if (int rc = propose_means(net, c, n_sub, rnd, st, C0.data())) return rc;
     int rc = fem_kmeans_init_seeded_rng(c, C0.data(), rnd, st, max_sweeps, w, mu, cov);
     if (rc == FEM_OK) {
-        if (used) *used = 1;
-        return FEM_OK;
+        // a cluster of fewer than d + 1 points has a singular covariance:
+        // without regularisation (the context's g) EM could not start from
+        // this mixture, so k-means++ instead
+        bool full_rank = true;
+        for (int k = 0; k < c->K; k++)
+            if (std::llround(w[k] * (double)c->n) < (long long)c->d + 1) full_rank = false;
+        if (full_rank || c->g > 0.0) {
+            if (used) *used = 1;
+            return FEM_OK;
+        }
+    } else if (rc != FEM_ERR_ARG) {
+        return rc;
     }
-    if (rc != FEM_ERR_ARG) return rc;
-    // a cluster of the proposal kept fewer than two points: k-means++
+    // a cluster of the proposal kept fewer than two points, or (g = 0) fewer
+    // than d + 1: k-means++
     if (used) *used = 0;
     return fem_kmeans_init_rng(c, rnd, st, FEM_SEED_KMEANSPP, max_sweeps, w, mu, cov);
 }
@@ -245,6 +255,9 @@ int fem_setnet_judge_rng(fem_setnet *net, fem_ctx *c, int n_sub, double (*rnd)(v
     const double lost = -std::numeric_limits<double>::infinity();
     double lll = lost, llk = lost;
     int ul = 0;
+    // the EM below regularises with g: the proposal may keep rank-deficient
+    // clusters then
+    if (int r = fem_set_regularization(c, g)) return r;
     int rc = propose(net, c, n_sub, rnd, rnd_state, max_sweeps, wl.data(), ml.data(), cl.data(), &ul);
     if (rc != FEM_OK && rc != FEM_ERR_ARG) return rc;
     const bool have_l = rc == FEM_OK;

@@ -214,7 +214,9 @@ class SetNet:
         the proposed means seed Lloyd sweeps and the cluster mixture on the
         device in place of the k-means++ seeding pass; k-means++ on the same
         Rand() stream is the fallback if a cluster keeps fewer than two
-        points.  X (ignored) is accepted for older callers.  Returns
+        points, or, when the context has no covariance regularisation
+        (Context.set_regularization), fewer than d + 1 (a singular
+        covariance).  X (ignored) is accepted for older callers.  Returns
         (w, mu, cov, used) with used "learned" or "kmeans++"."""
         w, mu, cov = np.empty(self.K), np.empty((self.K, self.d)), np.empty((self.K, self.d, self.d))
         used = ctypes.c_int()

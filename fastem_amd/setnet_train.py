@@ -134,6 +134,7 @@ def evaluate(net, seeds, n, d, K, gen, n_sub=1024, eps=1e-5, maxsweeps=1000, llo
     lo, hi, diag = gen
     recs = []
     ctx = fem.Context(n, d, K)
+    ctx.set_regularization(g)  # the proposal then keeps rank-deficient clusters when g > 0
     for c0 in range(0, len(seeds), chunk):
         cs = seeds[c0:c0 + chunk]
         inits = []

@@ -91,8 +91,10 @@ int fem_setnet_load(fem_setnet **net, const char *path, int device);
  *     sweeps and cluster mixture of fem_kmeans_init_seeded_rng.
  * fem_setnet_propose_means[_rng] stop after step 2 and write C0.
  * fem_setnet_propose[_rng] runs all three; if the proposal leaves a cluster
- * with fewer than two points, k-means++ (fem_kmeans_init_rng) on the same
- * stream is the fallback.  *used = 1 for the network's proposal, 0 for the
+ * with fewer than two points, or, when the context has no covariance
+ * regularisation (fem_set_regularization g = 0), fewer than d + 1 (a singular
+ * covariance, from which unregularised EM cannot start), k-means++
+ * (fem_kmeans_init_rng) on the same stream is the fallback.  *used = 1 for the network's proposal, 0 for the
  * fallback.  The result becomes the context's parameters.
  * fem_setnet_judge[_rng] judges the proposal by the final EM log-likelihood:
  * it takes the learned initialisation and then the k-means++ one (same

@@ -156,3 +156,34 @@ def test_bic_k_sweep_prefers_true_order():
     b.close()
     print("BIC", dict(zip(Ks, scores)), flush=True)
     assert Ks[int(np.argmin(scores))] == Ktrue
+
+
+def test_consecutive_batches_keep_every_fit():
+    """Batches created one after another, each filled right after creation
+    (the pattern of the learned initializer's evaluation): every fit of every
+    batch, all on the same data and start, ends exactly like one run of the
+    single-fit context.  The pools are cleared on the batch's own stream at
+    creation; a clear queued on another stream could land after the first
+    fits' uploads and wipe them."""
+    n, d, K, fits = 65536, 32, 16, 96
+    X = fem.gengmm(n, d, K, -10.0, 10.0, 0.1, 10000256)[0]
+    w0, mu0, cov0 = fem.kmeanspp_init(X, K, seed=3, lloyd_iters=29)
+    with fem.Context(n, d, K) as ctx:
+        ctx.upload(X)
+        it1, ll1, (_, mu1, cov1) = fem.train(ctx, w0, mu0, cov0, maxsweeps=20, eps=1e-5)
+    for rep in range(4):
+        print(f"consecutive batch {rep}", flush=True)
+        b = fem.Batch([(n, d, K)] * fits)
+        for i in range(fits):
+            b.upload(i, X)
+            b.set_params(i, w0, mu0, cov0)
+        b.run(20, eps=1e-5)
+        res = [b.result(i) for i in range(fits)]
+        b.close()
+        for i, r in enumerate(res):
+            assert r["status"] != fem.Batch.FAILED, (rep, i)
+            assert r["ll"] == res[0]["ll"] and r["iter_counter"] == res[0]["iter_counter"], (rep, i)
+            np.testing.assert_array_equal(r["mu"], res[0]["mu"])
+        assert res[0]["iter_counter"] == it1
+        assert abs(res[0]["ll"] - ll1) <= 1e-12 * abs(ll1)
+        assert np.max(np.abs(res[0]["cov"] - cov1)) <= 1e-12 * np.max(np.abs(cov1))
