@@ -44,13 +44,15 @@ def balanced(n, d, K, lo, hi, scale, seed):
     return X.astype(np.float32), np.bincount(lab, minlength=K) / n, mu, cov
 
 
-# (n, d, K, lo, hi, scale): the last two reach the largest K the form takes
-# at d = 48 and d = 64
+# (n, d, K, lo, hi, scale): two reach the largest K the form takes at d = 48
+# and d = 64; the last two have a single component (one reference, one seed
+# row) and row counts that are not multiples of the 256-point workgroup
 SHAPES = [(100000, 16, 8, -10.0, 10.0, 0.1), (50000, 64, 16, -20.0, 20.0, 0.05),
           (20000, 17, 5, -10.0, 10.0, 0.1), (50000, 32, 8, -1.5, 1.5, 0.5),
           (40000, 8, 80, -20.0, 20.0, 0.05), (20000, 3, 4, -10.0, 10.0, 0.1),
           (20000, 1, 2, -10.0, 10.0, 0.1), (30000, 64, 64, -20.0, 20.0, 0.05),
-          (30000, 48, 118, -20.0, 20.0, 0.05), (30000, 64, 75, -20.0, 20.0, 0.05)]
+          (30000, 48, 118, -20.0, 20.0, 0.05), (30000, 64, 75, -20.0, 20.0, 0.05),
+          (5000, 5, 1, -1.0, 1.0, 1.0), (12345, 64, 1, -1.0, 1.0, 1.0)]
 
 
 @pytest.mark.parametrize("n,d,K,lo,hi,scale", SHAPES)

@@ -57,14 +57,15 @@ def grid_points(n, d, seed):
     return rng.integers(-2, 3, size=(n, d)).astype(np.float32)
 
 
-@pytest.mark.parametrize("case", ["exact_ties", "near_ties", "random_d64_k64", "separated"])
+@pytest.mark.parametrize("case", ["exact_ties", "near_ties", "random_d64_k64", "separated", "single"])
 def test_assignment_matches_oracle_classification(orc, case):
     """The assignment pass (distances' cross terms on FP64 MFMA, the exact
     sequential distance wherever the MFMA gap does not prove the argmin)
     against the oracle's classification by the same centroids: exact ties
     (lowest index wins), near-ties 1e-15 apart (below the MFMA error bound,
     decided by the exact distances), a random d = 64, K = 64 case, and
-    the generating means of well-separated clusters.  Zero Lloyd sweeps: the mixture is that of the
+    the generating means of well-separated clusters, and a single centroid.
+    Zero Lloyd sweeps: the mixture is that of the
     classification by C0; counts must be equal and covariances within
     1e-12."""
     print(f"assignment vs oracle: {case}", flush=True)
@@ -81,10 +82,14 @@ def test_assignment_matches_oracle_classification(orc, case):
         n, d, K = 100000, 64, 64
         X = fem.gengmm(n, d, K, -5.0, 5.0, 0.5, 41)[0]
         C0 = X[np.random.default_rng(2).choice(n, K, replace=False)].astype(np.float64)
-    else:
+    elif case == "separated":
         n, d, K = 30000, 17, 9
         X, _, C0, _ = fem.gengmm(n, d, K, -10.0, 10.0, 0.1, 42)
         C0 = np.ascontiguousarray(C0, dtype=np.float64)
+    else:  # one centroid: no second distance, every argmin proven
+        n, d = 3001, 3
+        X = fem.gengmm(n, d, 2, -10.0, 10.0, 0.1, 43)[0]
+        C0 = np.ascontiguousarray(X[:1], dtype=np.float64)
     K = C0.shape[0]
     cls = orc.classify(X, C0)
     assert np.bincount(cls, minlength=K).min() >= 2
