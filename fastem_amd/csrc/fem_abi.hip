@@ -199,7 +199,6 @@ int fem_create(fem_ctx **out, int64_t n, int d, int K, int precision, int device
     c->KP = 16 * ((K + 15) / 16);
     c->prec = precision;
     c->npad = ((n + fem::kRowAlign - 1) / fem::kRowAlign) * fem::kRowAlign;
-    const int npairs = fem::estep_pairs(c->NB);
     const int ub = fem::upper_blocks(c->NB);
     c->est_blocks = (int)(c->npad / 64);  // >= workgroups of any E-step variant
     // FP64 and FP32 E-steps: four 16-point groups per wave measured faster
@@ -246,7 +245,7 @@ int fem_create(fem_ctx **out, int64_t n, int d, int K, int precision, int device
         (e = dmalloc(&c->info, K)) != hipSuccess || (e = dmalloc(&c->coef, K)) != hipSuccess)
         return bail(e, "hipMalloc(params)");
     if ((e = dmalloc(&c->U, (size_t)K * c->dpad * c->dpad)) != hipSuccess ||
-        (e = dmalloc(&c->stage64, (size_t)K * (npairs * 64 + c->dpad))) != hipSuccess)
+        (e = dmalloc(&c->stage64, (size_t)K * fem::stage64_words(c->NB))) != hipSuccess)
         return bail(e, "hipMalloc(factors)");
     if ((e = dmalloc(&c->ll_part, c->est_blocks)) != hipSuccess ||
         (e = dmalloc(&c->ll_dev, 1)) != hipSuccess ||

@@ -457,7 +457,7 @@ int fem_batch_create(fem_batch **out, int nfits, const int64_t *n, const int *d,
         f.logdet = take(f.K);
         f.coef = take(f.K);
         f.U = take((int64_t)f.K * f.dpad * f.dpad);
-        f.stage64 = take((int64_t)f.K * (f.npairs * 64 + f.dpad));
+        f.stage64 = take((int64_t)f.K * fem::stage64_words(f.NB));
         f.llp = take(f.npad / 64);
         f.mpart = take((int64_t)f.m_chunks * f.KP * (f.dpad + 1));
         f.cpart = take((int64_t)f.c_chunks * f.KP * f.ub * 256);

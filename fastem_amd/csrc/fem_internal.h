@@ -25,6 +25,10 @@ inline int estep_pairs(int NB) {
 inline int upper_blocks(int NB) { return NB * (NB + 1) / 2; }
 // FP32 E-step stage block per component: fragments, mu_hi, -U^T mu_lo.
 inline int stage32_words(int NB) { return estep_pairs(NB) * 64 + 2 * 16 * NB; }
+// FP64 E-step stage block per component (fem_tiles.h stage64_words_of, checked
+// equal there): 2 NB (NB - 1) 16x16x4 fragments of 64 doubles, 10 NB 4x4x4
+// quads of 16 doubles, -U^T mu.
+constexpr int stage64_words(int NB) { return 2 * NB * (NB - 1) * 64 + 10 * NB * 16 + 16 * NB; }
 
 struct Launch {
     hipStream_t stream;

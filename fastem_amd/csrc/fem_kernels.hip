@@ -16,6 +16,10 @@
 
 namespace fem {
 
+static_assert(stage64_words_of(1) == stage64_words(1) && stage64_words_of(4) == stage64_words(4) &&
+                  stage64_words_of(5) == stage64_words(5) && stage64_words_of(8) == stage64_words(8),
+              "FP64 stage block size: host and device formulas");
+
 __global__ __launch_bounds__(256) void k_factor(int K, int d, int dpad, int NB, int npairs,
                                                 const double *__restrict__ cov,
                                                 const double *__restrict__ w,
@@ -48,6 +52,19 @@ hipError_t launch_factor(fem_ctx *c) {
 }
 
 // ---------------------------------------------------------------------------
+// d <= 64: registers capped for three waves per SIMD (the compiler's own
+// allocation, 206 VGPRs at d = 64, leaves two; capped, a few values spill
+// to scratch and the E-step measured 2-3% faster at d = 32 and 64).  Wider
+// d keeps the compiler's allocation (the cap spilled heavily at d = 128).
+template <int NB, int P>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_estep64_w3(
+    const float *__restrict__ X, int64_t n, int64_t npad, int K, const double *__restrict__ stage,
+    const double *__restrict__ coef, double *__restrict__ Rt, double *__restrict__ ll_part) {
+    extern __shared__ double lds[];  // estep64_lds<NB>()
+    __shared__ double wsum[4];
+    estep64_unit<NB, P>(blockIdx.x, X, n, npad, K, stage, coef, Rt, ll_part + blockIdx.x, lds, wsum);
+}
+
 template <int NB, int P>
 __global__ __launch_bounds__(256) void k_estep64(const float *__restrict__ X, int64_t n,
                                                  int64_t npad, int K,
@@ -84,14 +101,18 @@ template <int NB, int P>
 static hipError_t estep64_t(fem_ctx *c) {
     const size_t lds = (size_t)estep64_lds<NB>() * sizeof(double);
     const void *kern;
-    if constexpr (NB == 5) kern = (const void *)k_estep64_w2<NB, P>;
+    if constexpr (NB <= 4) kern = (const void *)k_estep64_w3<NB, P>;
+    else if constexpr (NB == 5) kern = (const void *)k_estep64_w2<NB, P>;
     else kern = (const void *)k_estep64<NB, P>;
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
     const int blocks = (int)(c->npad / (64 * P));
-    if constexpr (NB == 5)
+    if constexpr (NB <= 4)
+        hipLaunchKernelGGL((k_estep64_w3<NB, P>), dim3(blocks), dim3(256), lds, c->stream, c->X, c->n, c->npad,
+                           c->K, c->stage64, c->coef, c->Rt, c->ll_part);
+    else if constexpr (NB == 5)
         hipLaunchKernelGGL((k_estep64_w2<NB, P>), dim3(blocks), dim3(256), lds, c->stream, c->X, c->n, c->npad,
                            c->K, c->stage64, c->coef, c->Rt, c->ll_part);
     else

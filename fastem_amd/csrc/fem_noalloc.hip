@@ -98,7 +98,6 @@ extern "C" int fem_estep_noalloc(const float *X, int64_t n, int64_t ldx, int d, 
 
     // layout of the array: small per-component vectors, factor products,
     // then the row tile and its posteriors (sized by B)
-    const size_t npairs = fem::estep_pairs(v.NB);
     size_t off = 0;
     auto take = [&](size_t doubles) {
         double *p = base + off;
@@ -111,7 +110,7 @@ extern "C" int fem_estep_noalloc(const float *X, int64_t n, int64_t ldx, int d, 
     double *ll_acc = take(1);
     v.ll_dev = take(1);
     v.U = take((size_t)K * v.dpad * v.dpad);
-    v.stage64 = take((size_t)K * (npairs * 64 + v.dpad));
+    v.stage64 = take((size_t)K * fem::stage64_words(v.NB));
     const size_t fixed = off;
     int64_t B = std::min<int64_t>(65536, (n + fem::kRowAlign - 1) / fem::kRowAlign * fem::kRowAlign);
     auto tile_doubles = [&](int64_t b) {

@@ -6,8 +6,12 @@
 //
 // MFMA 16x16x4 operand layout (wave64, lane l): A[i][k] from lane i + 16k,
 // B[k][j] from lane j + 16k.  Accumulator element v of lane l holds
-// D[4*v + l/16][l%16] for v_mfma_f64_16x16x4f64 (checked on gfx950 with a
-// lane-layout probe).
+// D[4*v + l/16][l%16] for v_mfma_f64_16x16x4f64.  v_mfma_f64_4x4x4f64 (four
+// 4x4x4 blocks b): A_b[i][k] from lane i + 4b + 16k, B_b[k][j] from lane
+// j + 4b + 16k, D_b[i][j] in lane j + 4b + 16i.  So with the same A in every
+// block, B is the 16x16x4 B operand (column c = j + 4b) and the result is
+// rows 4v..4v+3 of the 16x16x4 product with a 16-row A in element v of the
+// same lanes.  (Both checked on gfx950 with lane-layout probes.)
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -51,6 +55,17 @@ __device__ __host__ constexpr int pairs_of(int NB) {
     int p = 0;
     for (int lb = 0; lb < 4 * NB; lb++) p += NB - lb / 4;
     return p;
+}
+
+// FP64 E-step operands of U_k^T (lower triangular) for NB 16-feature blocks:
+// the (lb, jb) pairs of a 4-feature column strip lb and a 16-row block
+// jb > lb/4 are whole 16x16x4 products; in the diagonal block jb = lb/4 only
+// the 4-row quads v >= lb%4 touch the triangle, one 4x4x4 product each.
+__device__ __host__ constexpr int offd_pairs_of(int NB) { return 2 * NB * (NB - 1); }
+__device__ __host__ constexpr int diag_quads_of(int NB) { return 10 * NB; }
+// Doubles of one component's FP64 stage block: fragments, quads, -U^T mu.
+__device__ __host__ constexpr int stage64_words_of(int NB) {
+    return offd_pairs_of(NB) * 64 + diag_quads_of(NB) * 16 + 16 * NB;
 }
 
 // Sum over the four 16-lane groups (lanes l, l^16, l^32, l^48) as
@@ -140,8 +155,9 @@ __device__ inline double chunk_sum4(const double *__restrict__ p, size_t stride,
 // Lower-symmetrise, right-looking Cholesky in LDS A[d*d] (the same operation
 // sequence per element as column Cholesky-Crout), log-det, the columns of
 // L^-1 by forward substitution stored as rows of U = L^-T, then the E-step
-// operands as one stage block per component, [fragments of U in MFMA
-// A-operand order (npairs x 64) | -U^T mu (dpad)], and, when stage32 is
+// operands as one stage block per component, [16x16x4 fragments of the
+// off-diagonal pairs | 4x4x4 fragments of the diagonal quads | -U^T mu (dpad)]
+// (stage64_words_of), and, when stage32 is
 // given, the FP32 form's per-component stage block
 // [fragments (npairs x 64) | mu_hi (dpad) | -U^T mu_lo (dpad)] with
 // mu_hi = float(mu), mu_lo = mu - mu_hi; mu_hi is stored lane-group major
@@ -208,21 +224,48 @@ __device__ inline void factor_unit(int k, int d, int dpad, int NB, int npairs, c
         }
     }
     __syncthreads();
-    for (int e = tid; e < npairs * 64; e += blockDim.x) {
+    // FP32 stage: every (lb, jb >= lb/4) pair as a 16x16x4 fragment, four
+    // pairs interleaved per lane per 16 bytes (npairs = 2 NB (NB + 1) is a
+    // multiple of 4) so the E-step reads four with one 16-byte LDS load
+    if (stage32)
+        for (int e = tid; e < npairs * 64; e += blockDim.x) {
+            int p = e >> 6, lane = e & 63;
+            int lb = 0, rem = p;
+            while (rem >= NB - lb / 4) {
+                rem -= NB - lb / 4;
+                lb++;
+            }
+            int jb = lb / 4 + rem;
+            double v = Uk[(size_t)(4 * lb + (lane >> 4)) * dpad + 16 * jb + (lane & 15)];
+            stage32[(size_t)k * (npairs * 64 + 2 * dpad) + (p >> 2) * 256 + lane * 4 + (p & 3)] = (float)v;
+        }
+    // FP64 stage (stage64_words_of(NB) doubles): the off-diagonal pairs
+    // (jb > lb/4) as 16x16x4 fragments, two interleaved per lane per 16 bytes;
+    // then the diagonal blocks' quads (lb, v >= lb%4) as 4x4x4 fragments of 16
+    // slots s = i + 4 kk holding U^T[16 (lb/4) + 4v + i][4 lb + kk], two
+    // interleaved per slot; then -U^T mu
+    const int noff = offd_pairs_of(NB), ndiag = diag_quads_of(NB);
+    double *sk = stage64 + (size_t)k * stage64_words_of(NB);
+    for (int e = tid; e < noff * 64; e += blockDim.x) {
         int p = e >> 6, lane = e & 63;
         int lb = 0, rem = p;
-        while (rem >= NB - lb / 4) {
-            rem -= NB - lb / 4;
+        while (rem >= NB - 1 - lb / 4) {
+            rem -= NB - 1 - lb / 4;
             lb++;
         }
-        int jb = lb / 4 + rem;
-        double v = Uk[(size_t)(4 * lb + (lane >> 4)) * dpad + 16 * jb + (lane & 15)];
-        // pairs interleaved per lane (2 per 16 bytes FP64, 4 FP32) so the
-        // E-step reads several consecutive pairs with one 16-byte LDS load;
-        // npairs = 2 NB (NB + 1) is a multiple of 4
-        stage64[(size_t)k * (npairs * 64 + dpad) + (p >> 1) * 128 + lane * 2 + (p & 1)] = v;
-        if (stage32)
-            stage32[(size_t)k * (npairs * 64 + 2 * dpad) + (p >> 2) * 256 + lane * 4 + (p & 3)] = (float)v;
+        int jb = lb / 4 + 1 + rem;
+        sk[(p >> 1) * 128 + lane * 2 + (p & 1)] = Uk[(size_t)(4 * lb + (lane >> 4)) * dpad + 16 * jb + (lane & 15)];
+    }
+    for (int e = tid; e < ndiag * 16; e += blockDim.x) {
+        int q = e >> 4, s = e & 15;
+        int lb = 0, rem = q;
+        while (rem >= 4 - lb % 4) {
+            rem -= 4 - lb % 4;
+            lb++;
+        }
+        const int v = lb % 4 + rem;
+        sk[noff * 64 + (q >> 1) * 32 + s * 2 + (q & 1)] =
+            Uk[(size_t)(4 * lb + (s >> 2)) * dpad + 16 * (lb / 4) + 4 * v + (s & 3)];
     }
     for (int j = tid; j < dpad; j += blockDim.x) {
         double s = 0.0, slo = 0.0;
@@ -232,7 +275,7 @@ __device__ inline void factor_unit(int k, int d, int dpad, int NB, int npairs, c
                 s += Uk[(size_t)f * dpad + j] * m;
                 slo += Uk[(size_t)f * dpad + j] * (m - (double)(float)m);
             }
-        stage64[(size_t)k * (npairs * 64 + dpad) + npairs * 64 + j] = -s;
+        sk[noff * 64 + ndiag * 16 + j] = -s;
         if (stage32) {
             float *st = stage32 + (size_t)k * (npairs * 64 + 2 * dpad) + npairs * 64;
             st[(j & 3) * (dpad >> 2) + (j >> 2)] = (j < d) ? (float)mu[(size_t)k * d + j] : 0.0f;
@@ -297,11 +340,19 @@ __device__ __attribute__((always_inline)) inline T lse_posteriors(T *__restrict_
 // Each wave owns P groups of 16 points.  Lane (g = l/16, c = l%16) holds, for
 // point c of each group, features 4*lb + g (lb = 0..4NB-1), loaded as FP32
 // and widened to FP64 once (the compiler keeps them widened across the
-// component loop): the B operand of every MFMA.  Component k's stage block
-// (fragments of U_k^T, then -m_k = -U_k^T mu_k) is double-buffered in LDS
-// (estep64_lds doubles): component k+1's block is copied from global memory
-// straight into the idle buffer (global_load_lds_dwordx4) while component k
-// is processed, and the one barrier per component publishes it.  Copied
+// component loop): the B operand of every MFMA.  Per component and strip lb
+// of four input features, the 16-row blocks jb > lb/4 below the diagonal
+// block are whole v_mfma_f64_16x16x4f64 products, and in the diagonal block
+// jb = lb/4 the 4-row quads v >= lb%4 (the only ones the triangle reaches)
+// are v_mfma_f64_4x4x4f64 products with the quad's 4x4 piece of U_k^T in
+// every block, accumulated into element v of the same accumulators: 10 NB
+// quads of 512 flops and 2 NB (NB - 1) pairs of 2048 flops, 1.06x the
+// N K d^2 flops of the triangle at d = 64 where whole 16x16x4 products of
+// every pair touching it are 1.25x.  Component k's stage block (fragments,
+// quads, then -m_k = -U_k^T mu_k) is double-buffered in LDS (estep64_lds
+// doubles): component k+1's block is copied from global memory straight
+// into the idle buffer (global_load_lds_dwordx4) while component k is
+// processed, and the one barrier per component publishes it.  Copied
 // through registers instead, the compiler sank those loads behind the MFMAs
 // and waited on each, which one wave per SIMD (d > 80, where the buffers fill
 // the LDS) cannot hide: the direct copy measured 30% faster at d = 96..128
@@ -311,7 +362,7 @@ __device__ __attribute__((always_inline)) inline T lse_posteriors(T *__restrict_
 // forms the log-sum-exp and the posteriors (lse_posteriors).  The tile's LL is
 // written to *ll_out by thread 0.  wsum: 4 LDS doubles.
 template <int NB>
-__device__ __host__ constexpr int estep64_stage() { return pairs_of(NB) * 64 + 16 * NB; }
+__device__ __host__ constexpr int estep64_stage() { return stage64_words_of(NB); }
 // LDS doubles of the two stage buffers, each padded to whole rounds of 256
 // 16-byte pieces (every lane of a direct copy writes one piece per round)
 template <int NB>
@@ -323,13 +374,15 @@ __device__ inline void estep64_unit(int64_t tile, const float *__restrict__ X, i
                                     double *__restrict__ Rt,
                                     double *__restrict__ ll_out, double *lds, double *wsum) {
     constexpr int DP = 16 * NB;
-    constexpr int NPAIR = pairs_of(NB);
-    constexpr int STAGE = NPAIR * 64 + DP;  // even
+    constexpr int NOFF = offd_pairs_of(NB);
+    constexpr int NDIAG = diag_quads_of(NB);
+    constexpr int STAGE = estep64_stage<NB>();  // even
     constexpr int STAGE2 = STAGE / 2;
     constexpr int PER = (STAGE2 + 255) / 256;
     constexpr int BUF2 = PER * 256;  // 16-byte pieces per LDS buffer
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int g = lane >> 4, c = lane & 15;
+    const int slot = (lane & 3) + 4 * g;  // this lane's 4x4x4 A slot i + 4 kk
     const int64_t base = tile * (64 * P) + wave * (16 * P);
 
     float xr[P][4 * NB];
@@ -355,7 +408,8 @@ __device__ inline void estep64_unit(int64_t tile, const float *__restrict__ X, i
 
     for (int k = 0; k < K; k++) {
         const double *lu = lds + (k & 1) * (2 * BUF2);
-        const double *lm = lu + NPAIR * 64;
+        const double *lq = lu + NOFF * 64;
+        const double *lm = lq + NDIAG * 16;
         if (k + 1 < K) {
             // component k+1's block straight into the idle buffer (each lane
             // one 16-byte piece per round: global_load_lds_dwordx4), so no
@@ -380,15 +434,25 @@ __device__ inline void estep64_unit(int64_t tile, const float *__restrict__ X, i
 #pragma unroll
             for (int p = 0; p < P; p++) acc[p][jb] = init;
         }
-        int pr = 0;
-        double2 a2 = {0.0, 0.0};
+        int pr = 0, qd = 0;
+        double2 a2 = {0.0, 0.0}, q2 = {0.0, 0.0};
 #pragma unroll
         for (int lb = 0; lb < 4 * NB; lb++) {
             double b[P];
 #pragma unroll
             for (int p = 0; p < P; p++) b[p] = (double)xr[p][lb];
+            const int jd = lb / 4;
 #pragma unroll
-            for (int jb = lb / 4; jb < NB; jb++) {
+            for (int v = lb % 4; v < 4; v++) {
+                if ((qd & 1) == 0) q2 = *reinterpret_cast<const double2 *>(lq + (qd >> 1) * 32 + slot * 2);
+                const double a = (qd & 1) ? q2.y : q2.x;
+                qd++;
+#pragma unroll
+                for (int p = 0; p < P; p++)
+                    acc[p][jd][v] = __builtin_amdgcn_mfma_f64_4x4x4f64(a, b[p], acc[p][jd][v], 0, 0, 0);
+            }
+#pragma unroll
+            for (int jb = jd + 1; jb < NB; jb++) {
                 if ((pr & 1) == 0) a2 = *reinterpret_cast<const double2 *>(lu + (pr >> 1) * 128 + lane * 2);
                 const double a = (pr & 1) ? a2.y : a2.x;
                 pr++;
