@@ -413,10 +413,10 @@ def main():
         "data": "synthetic (seeded gengmm mixture, k-means++ init)",
         "config": {"workload": "gmm-em-full-cov", "n": N, "d": D, "k": K,
                    "precision": "fp64", "init": "kmeans++ + 29 Lloyd sweeps (device)"},
-        "roofline": {"bound": "mfma", "kernel": "k_estep64<4,2>", "achieved": achieved / 1e12,
+        "roofline": {"bound": "mfma", "kernel": "k_estep64_w3<4,2>", "achieved": achieved / 1e12,
                      "peak": PEAK_FP64_MFMA / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_MFMA,
-                     "traffic": measured_traffic("fem::k_estep64<4, 2>"),
+                     "traffic": measured_traffic("fem::k_estep64_w3<4, 2>"),
                      "traffic_note": "HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE "
                                      "(profiles/hbm_traffic_n4194304_d64_k64.txt); algorithmic "
                                      "bytes: X read 1.07e9 + FP64 posteriors written 2.15e9"},

@@ -363,10 +363,13 @@ __device__ __attribute__((always_inline)) inline T lse_posteriors(T *__restrict_
 // written to *ll_out by thread 0.  wsum: 4 LDS doubles.
 template <int NB>
 __device__ __host__ constexpr int estep64_stage() { return stage64_words_of(NB); }
-// LDS doubles of the two stage buffers, each padded to whole rounds of 256
-// 16-byte pieces (every lane of a direct copy writes one piece per round)
+// LDS doubles of the two stage buffers, each padded to whole waves of 64
+// 16-byte pieces: a direct copy round writes one piece per lane, 256 per
+// round, and in the last round only the waves whose pieces start inside the
+// block copy (at d = 96 the two buffers then take 78 KB instead of 80 KB and
+// two workgroups fit the 160 KB of LDS)
 template <int NB>
-__device__ __host__ constexpr int estep64_lds() { return 2 * ((estep64_stage<NB>() / 2 + 255) / 256) * 512; }
+__device__ __host__ constexpr int estep64_lds() { return 2 * ((estep64_stage<NB>() / 2 + 63) / 64) * 128; }
 
 template <int NB, int P>
 __device__ inline void estep64_unit(int64_t tile, const float *__restrict__ X, int64_t n, int64_t npad, int K,
@@ -379,8 +382,9 @@ __device__ inline void estep64_unit(int64_t tile, const float *__restrict__ X, i
     constexpr int STAGE = estep64_stage<NB>();  // even
     constexpr int STAGE2 = STAGE / 2;
     constexpr int PER = (STAGE2 + 255) / 256;
-    constexpr int BUF2 = PER * 256;  // 16-byte pieces per LDS buffer
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    constexpr int BUF2 = (STAGE2 + 63) / 64 * 64;  // 16-byte pieces per LDS buffer
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int g = lane >> 4, c = lane & 15;
     const int slot = (lane & 3) + 4 * g;  // this lane's 4x4x4 A slot i + 4 kk
     const int64_t base = tile * (64 * P) + wave * (16 * P);
@@ -418,9 +422,10 @@ __device__ inline void estep64_unit(int64_t tile, const float *__restrict__ X, i
             double2 *dst = lds2 + ((k + 1) & 1) * BUF2 + 64 * wave;
 #pragma unroll
             for (int q = 0; q < PER; q++)
-                __builtin_amdgcn_global_load_lds(
-                    (const __attribute__((address_space(1))) void *)(next2 + min(tid + 256 * q, STAGE2 - 1)),
-                    (__attribute__((address_space(3))) void *)(dst + 256 * q), 16, 0, 0);
+                if (q + 1 < PER || 256 * q + 64 * wave < STAGE2)
+                    __builtin_amdgcn_global_load_lds(
+                        (const __attribute__((address_space(1))) void *)(next2 + min(tid + 256 * q, STAGE2 - 1)),
+                        (__attribute__((address_space(3))) void *)(dst + 256 * q), 16, 0, 0);
         }
         const double ck = coef[k];
         v4d acc[P][NB];
