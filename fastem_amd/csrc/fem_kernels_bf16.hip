@@ -763,8 +763,10 @@ hipError_t launch_meansb_partials(fem_ctx *c) {
 // own slice of LDS (written to the partial slice once at the end).  Grid and
 // XCD remapping as the FP32 form, with the part as the fastest-varying index;
 // strided feature blocks as the FP32 form (a lane's NB features of a point in
-// wide loads); the loads of the next 32 points are issued once the current 32
-// are split, before their MFMAs.
+// wide loads).  For d <= 64 the 32-point steps are software-pipelined (each
+// step's MFMAs beside the next step's split); for d > 64 the loads of the
+// next 32 points are issued once the current 32 are split, before their
+// MFMAs.
 __device__ __host__ constexpr int cov_parts(int NB) { return NB > 4 ? 2 : 1; }
 
 // Part of upper block u (row-major over ib <= jb) among NPART parts, the
@@ -809,56 +811,8 @@ __device__ inline void covb_wave(const float *__restrict__ X, const float *__res
         // the wave's FP64 accumulators live in its own LDS slice [NU][4][64]
         for (int e = lane; e < NU * 256; e += 64) lacc[e] = 0.0;
     }
-    // H register buffers of 32 points used in turn (two for d <= 64, so the
-    // prefetched values need no loop-carried copies)
-    constexpr int H = NB <= 4 ? 2 : 1;
-    float rv[H][8], xv[H][NB][8];
-    auto load32 = [&](int h, int64_t i) {
-        const float *rp = rrow + i + 8 * g;
-        const float4 r0 = *reinterpret_cast<const float4 *>(rp);
-        const float4 r1 = *reinterpret_cast<const float4 *>(rp + 4);
-        rv[h][0] = r0.x; rv[h][1] = r0.y; rv[h][2] = r0.z; rv[h][3] = r0.w;
-        rv[h][4] = r1.x; rv[h][5] = r1.y; rv[h][6] = r1.z; rv[h][7] = r1.w;
-        const float *xp = X + (i + 8 * g) * DP + NB * c;
-#pragma unroll
-        for (int fb = 0; fb < NB; fb++)
-            if (cov_needs_fb(NB, PART, fb))
-#pragma unroll
-                for (int e = 0; e < 8; e++) xv[h][fb][e] = xp[e * DP + fb];
-    };
-    // chunk bounds are multiples of 4096 points: every step has its partners
-    if (i0 < i1) load32(0, i0);
-    for (int64_t s0 = i0; s0 < i1; s0 += FLUSH) {
-        const int64_t s1 = (s0 + FLUSH < i1) ? s0 + FLUSH : i1;
-        v4f acc[NU];
-#pragma unroll
-        for (int u = 0; u < NU; u++) acc[u] = (v4f){0.f, 0.f, 0.f, 0.f};
-        for (int64_t i = s0; i < s1; i += 32 * H) {
-#pragma unroll
-            for (int h = 0; h < H; h++) {
-                float sw[8];
-#pragma unroll
-                for (int e = 0; e < 8; e++) sw[e] = __builtin_amdgcn_sqrtf(rv[h][e]) * inv;
-                v8bf th[NB], tl[NB];
-#pragma unroll
-                for (int fb = 0; fb < NB; fb++) {
-                    if (!cov_needs_fb(NB, PART, fb)) continue;
-                    float t[8];
-#pragma unroll
-                    for (int e = 0; e < 8; e++) t[e] = (xv[h][fb][e] - mh[fb]) * sw[e];
-                    split8(t, th[fb], tl[fb]);
-                }
-                if (i + 32 * (h + 1) < i1) load32((h + 1) % H, i + 32 * (h + 1));
-                int u = 0;
-#pragma unroll
-                for (int ib = 0; ib < NB; ib++)
-#pragma unroll
-                    for (int jb = ib; jb < NB; jb++, u++) {
-                        if (cov_part_of(u, UB, NPART) != PART) continue;
-                        acc[u - U0] = mfma3(th[ib], tl[ib], th[jb], tl[jb], acc[u - U0]);
-                    }
-            }
-        }
+    // FP32 block sums into the FP64 accumulators
+    auto flush = [&](const v4f (&acc)[NU]) {
         if (REG) {
 #pragma unroll
             for (int u = 0; u < (REG ? NU : 1); u++)
@@ -873,6 +827,139 @@ __device__ inline void covb_wave(const float *__restrict__ X, const float *__res
                 for (int v = 0; v < 4; v++) lacc[(u * 4 + v) * 64 + lane] += (double)acc[u][v];
                 if ((u & 3) == 3) asm volatile("" ::: "memory");
             }
+        }
+    };
+    // chunk bounds are multiples of 4096 points: every step has its partners
+    if constexpr (NB <= 4) {
+        // Software pipeline over 32-point steps: step s's MFMAs run from the
+        // split operands T[s & 1] while the same instruction stream forms
+        // and splits step s + 1 (independent registers, so the VALU work
+        // issues beside the MFMAs); the point buffer that split frees is
+        // refilled with step s + 3, and the posteriors, read from HBM once
+        // per component, run RD steps ahead in a ring of RD (four at d = 64,
+        // 10% faster there than two; two below, 4-11% faster there than
+        // four; the unpipelined loop was 7-11% slower than either).  Loads
+        // past the chunk re-read its last step (in bounds, never multiplied).
+        constexpr int RD = NB == 4 ? 4 : 2;
+        v8bf th[2][NB], tl[2][NB];
+        float xv[2][NB][8], rb[RD][8];
+        auto loadx = [&](int h, int64_t i) {
+            const float *xp = X + (i + 8 * g) * DP + NB * c;
+#pragma unroll
+            for (int fb = 0; fb < NB; fb++)
+                if (cov_needs_fb(NB, PART, fb))
+#pragma unroll
+                    for (int e = 0; e < 8; e++) xv[h][fb][e] = xp[e * DP + fb];
+        };
+        auto loadr = [&](int q, int64_t i) {
+            const float *rp = rrow + i + 8 * g;
+            const float4 r0 = *reinterpret_cast<const float4 *>(rp);
+            const float4 r1 = *reinterpret_cast<const float4 *>(rp + 4);
+            rb[q][0] = r0.x; rb[q][1] = r0.y; rb[q][2] = r0.z; rb[q][3] = r0.w;
+            rb[q][4] = r1.x; rb[q][5] = r1.y; rb[q][6] = r1.z; rb[q][7] = r1.w;
+        };
+        auto split32 = [&](int h, int q, int o) {
+            float sw[8];
+#pragma unroll
+            for (int e = 0; e < 8; e++) sw[e] = __builtin_amdgcn_sqrtf(rb[q][e]) * inv;
+#pragma unroll
+            for (int fb = 0; fb < NB; fb++) {
+                if (!cov_needs_fb(NB, PART, fb)) continue;
+                float t[8];
+#pragma unroll
+                for (int e = 0; e < 8; e++) t[e] = (xv[h][fb][e] - mh[fb]) * sw[e];
+                split8(t, th[o][fb], tl[o][fb]);
+            }
+        };
+        auto products = [&](int o, v4f (&acc)[NU]) {
+            int u = 0;
+#pragma unroll
+            for (int ib = 0; ib < NB; ib++)
+#pragma unroll
+                for (int jb = ib; jb < NB; jb++, u++) {
+                    if (cov_part_of(u, UB, NPART) != PART) continue;
+                    acc[u - U0] = mfma3(th[o][ib], tl[o][ib], th[o][jb], tl[o][jb], acc[u - U0]);
+                }
+        };
+        const int64_t last = i1 - 32;
+        auto at = [&](int64_t i) { return i < i1 ? i : last; };
+        // invariant at step s: T[s & 1] holds step s, point buffer (s + 1) & 1
+        // step s + 1, buffer s & 1 step s + 2, ring slot (s + j) % RD the
+        // posteriors of step s + j for j = 1..RD
+        if (i0 < i1) {
+            loadx(0, i0);
+            loadx(1, at(i0 + 32));
+#pragma unroll
+            for (int q = 0; q < RD; q++) loadr(q, at(i0 + 32 * q));
+            split32(0, 0, 0);
+            loadx(0, at(i0 + 64));
+            loadr(0, at(i0 + 32 * RD));
+        }
+        for (int64_t s0 = i0; s0 < i1; s0 += FLUSH) {
+            const int64_t s1 = (s0 + FLUSH < i1) ? s0 + FLUSH : i1;
+            v4f acc[NU];
+#pragma unroll
+            for (int u = 0; u < NU; u++) acc[u] = (v4f){0.f, 0.f, 0.f, 0.f};
+            // four steps per trip (FLUSH and the chunk are multiples of 128)
+            for (int64_t i = s0; i < s1; i += 128) {
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int64_t is = i + 32 * j;  // step s
+                    products(j & 1, acc);
+                    split32((j + 1) & 1, (j + 1) % RD, (j + 1) & 1);
+                    loadx((j + 1) & 1, at(is + 96));
+                    loadr((j + 1) % RD, at(is + 32 * (RD + 1)));
+                }
+            }
+            flush(acc);
+        }
+    } else {
+        // d > 64 (one wave per SIMD): one 32-point register buffer, the next
+        // step's loads issued once the current step is split
+        float rv[8], xv[NB][8];
+        auto load32 = [&](int64_t i) {
+            const float *rp = rrow + i + 8 * g;
+            const float4 r0 = *reinterpret_cast<const float4 *>(rp);
+            const float4 r1 = *reinterpret_cast<const float4 *>(rp + 4);
+            rv[0] = r0.x; rv[1] = r0.y; rv[2] = r0.z; rv[3] = r0.w;
+            rv[4] = r1.x; rv[5] = r1.y; rv[6] = r1.z; rv[7] = r1.w;
+            const float *xp = X + (i + 8 * g) * DP + NB * c;
+#pragma unroll
+            for (int fb = 0; fb < NB; fb++)
+                if (cov_needs_fb(NB, PART, fb))
+#pragma unroll
+                    for (int e = 0; e < 8; e++) xv[fb][e] = xp[e * DP + fb];
+        };
+        if (i0 < i1) load32(i0);
+        for (int64_t s0 = i0; s0 < i1; s0 += FLUSH) {
+            const int64_t s1 = (s0 + FLUSH < i1) ? s0 + FLUSH : i1;
+            v4f acc[NU];
+#pragma unroll
+            for (int u = 0; u < NU; u++) acc[u] = (v4f){0.f, 0.f, 0.f, 0.f};
+            for (int64_t i = s0; i < s1; i += 32) {
+                float sw[8];
+#pragma unroll
+                for (int e = 0; e < 8; e++) sw[e] = __builtin_amdgcn_sqrtf(rv[e]) * inv;
+                v8bf th[NB], tl[NB];
+#pragma unroll
+                for (int fb = 0; fb < NB; fb++) {
+                    if (!cov_needs_fb(NB, PART, fb)) continue;
+                    float t[8];
+#pragma unroll
+                    for (int e = 0; e < 8; e++) t[e] = (xv[fb][e] - mh[fb]) * sw[e];
+                    split8(t, th[fb], tl[fb]);
+                }
+                if (i + 32 < i1) load32(i + 32);
+                int u = 0;
+#pragma unroll
+                for (int ib = 0; ib < NB; ib++)
+#pragma unroll
+                    for (int jb = ib; jb < NB; jb++, u++) {
+                        if (cov_part_of(u, UB, NPART) != PART) continue;
+                        acc[u - U0] = mfma3(th[ib], tl[ib], th[jb], tl[jb], acc[u - U0]);
+                    }
+            }
+            flush(acc);
         }
     }
 #pragma unroll
