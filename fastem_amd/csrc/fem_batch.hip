@@ -73,14 +73,17 @@ struct fem_batch {
     int64_t nx = 0, nd = 0, ni = 0;
     // work lists
     // E-step tiles of the fits with 2 (estA) and 4 (estB) point groups per
-    // wave, covariance (fit, component group, chunk) items, and the
-    // shape-independent lists
+    // wave, each list ordered by the fits' 16-feature blocks NB (the items of
+    // NB = nb at [off[nb - 1], off[nb])), covariance (fit, component group,
+    // chunk) items, and the shape-independent lists
     Item *estA = nullptr, *estB = nullptr, *means = nullptr, *cov = nullptr, *comp = nullptr, *covred = nullptr;
     int n_estA = 0, n_estB = 0, n_means = 0, n_cov = 0, n_comp = 0, n_covred = 0;
+    int offA[5] = {0, 0, 0, 0, 0}, offB[5] = {0, 0, 0, 0, 0};
     int maxNB = 1;
     int cus = 256;
-    // tuned launch grids
-    int grid_estA = 0, grid_estB = 0, grid_means = 0, grid_cov = 0;
+    // tuned launch grids (the E-step ones as multiples of the CU count, per
+    // NB range of a list)
+    int mult_estA = 2, mult_estB = 2, grid_means = 0, grid_cov = 0;
     // run parameters
     int maxsweeps = 100;
     double eps = 1e-5, g = 0.0;
@@ -133,26 +136,26 @@ __global__ __launch_bounds__(256) void kb_regularize(const BFit *__restrict__ F,
     }
 }
 
-template <int P>
-__global__ __launch_bounds__(256) void kb_estep(const BFit *__restrict__ F, const BState *__restrict__ S,
-                                                const float *__restrict__ xp, double *__restrict__ dp,
-                                                const Item *__restrict__ items, int nitems) {
-    extern __shared__ double lds[];
+// One persistent E-step kernel per (NB, P): its items are the tiles of fits
+// with NB 16-feature blocks, so each instance gets the registers and LDS of
+// its own shape.  One kernel switching over NB took the largest shape's 255
+// registers (plus AGPR spills) and one wave per SIMD for every fit; per NB,
+// capped at two waves per SIMD (169-250 registers, no spill), the 256-fit
+// benchmark batch ran 10% faster (three waves spilled 8-92 registers in the
+// persistent loop and gained half as much).
+template <int NB, int P>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void kb_estep(
+    const BFit *__restrict__ F, const BState *__restrict__ S, const float *__restrict__ xp, double *__restrict__ dp,
+    const Item *__restrict__ items, int nitems) {
+    extern __shared__ double lds[];  // estep64_lds<NB>()
     __shared__ double wsum[4];
     for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
         const Item itm = items[it];
         if (!S[itm.fit].active) continue;
         const BFit &f = F[itm.fit];
         const int64_t tile = itm.a;
-        const float *X = xp + f.x;
-        double *Rt = dp + f.r;
-        double *llo = dp + f.llp + tile;
-        switch (f.NB) {
-            case 1: estep64_unit<1, P>(tile, X, f.n, f.npad, f.K, dp + f.stage64, dp + f.coef, Rt, llo, lds, wsum); break;
-            case 2: estep64_unit<2, P>(tile, X, f.n, f.npad, f.K, dp + f.stage64, dp + f.coef, Rt, llo, lds, wsum); break;
-            case 3: estep64_unit<3, P>(tile, X, f.n, f.npad, f.K, dp + f.stage64, dp + f.coef, Rt, llo, lds, wsum); break;
-            case 4: estep64_unit<4, P>(tile, X, f.n, f.npad, f.K, dp + f.stage64, dp + f.coef, Rt, llo, lds, wsum); break;
-        }
+        estep64_unit<NB, P>(tile, xp + f.x, f.n, f.npad, f.K, dp + f.stage64, dp + f.coef, dp + f.r,
+                            dp + f.llp + tile, lds, wsum);
     }
 }
 
@@ -303,16 +306,43 @@ int berr(fem_batch *b, int code, const std::string &m) {
 
 int grid_of(int mult, int cus, int nitems) { return std::max(1, std::min(nitems, mult * cus)); }
 
+// The E-step of the n items of fits with nb 16-feature blocks under P point
+// groups per wave, as one persistent launch of mult x CUs workgroups.
+template <int P>
+void launch_estep_nb(fem_batch *b, int nb, const Item *items, int n, int mult) {
+    if (n <= 0) return;
+    const int grid = grid_of(mult, b->cus, n);
+    // four point groups exist only for NB <= 2 (build_shape_lists keeps the
+    // lists so), so those instances are not built for NB = 3, 4
+    switch (nb) {
+#define FEM_KB_ESTEP(NBV)                                                                                           \
+    case NBV:                                                                                                     \
+        if constexpr (P == 2 || NBV <= 2)                                                                         \
+            hipLaunchKernelGGL((fem::kb_estep<NBV, P>), dim3(grid), dim3(256),                                    \
+                               (size_t)fem::estep64_lds<NBV>() * sizeof(double), b->stream, b->dfits, b->dstate,   \
+                               b->xpool, b->dpool, items, n);                                                      \
+        break;
+        FEM_KB_ESTEP(1)
+        FEM_KB_ESTEP(2)
+        FEM_KB_ESTEP(3)
+        FEM_KB_ESTEP(4)
+#undef FEM_KB_ESTEP
+    }
+}
+
+// A whole E-step list (ordered by NB, ranges off[0..4]): one launch per NB.
+void launch_estep_list(fem_batch *b, int P, const Item *items, const int *off, int mult) {
+    for (int nb = 1; nb <= 4; nb++) {
+        if (P == 4) launch_estep_nb<4>(b, nb, items + off[nb - 1], off[nb] - off[nb - 1], mult);
+        else launch_estep_nb<2>(b, nb, items + off[nb - 1], off[nb] - off[nb - 1], mult);
+    }
+}
+
 // one EM iteration of all active fits (the captured sequence)
 hipError_t launch_iteration(fem_batch *b) {
-    const size_t lds_e = (size_t)fem::estep64_lds<4>() * sizeof(double);
     const size_t lds_f = (size_t)64 * 64 * sizeof(double);
-    if (b->n_estA > 0)
-        hipLaunchKernelGGL(fem::kb_estep<2>, dim3(b->grid_estA), dim3(256), lds_e, b->stream, b->dfits, b->dstate,
-                           b->xpool, b->dpool, b->estA, b->n_estA);
-    if (b->n_estB > 0)
-        hipLaunchKernelGGL(fem::kb_estep<4>, dim3(b->grid_estB), dim3(256), lds_e, b->stream, b->dfits, b->dstate,
-                           b->xpool, b->dpool, b->estB, b->n_estB);
+    launch_estep_list(b, 2, b->estA, b->offA, b->mult_estA);
+    launch_estep_list(b, 4, b->estB, b->offB, b->mult_estB);
     hipLaunchKernelGGL(fem::kb_ll, dim3(b->F), dim3(256), 0, b->stream, b->dfits, b->dstate, b->dpool);
     hipLaunchKernelGGL(fem::kb_means, dim3(b->grid_means), dim3(256), 0, b->stream, b->dfits, b->dstate, b->xpool,
                        b->dpool, b->means, b->n_means);
@@ -369,6 +399,19 @@ void cov_items(std::vector<Item> &v, int i, const BFit &f) {
             if (4 * cg < f.K) v.push_back({i, cg, ch, 0});
 }
 
+// Orders an E-step list by the fits' NB (stable: the interleaving of fits is
+// kept within an NB) and records the range of each NB in off[0..4].
+void by_nb(const fem_batch *b, std::vector<Item> &v, int *off) {
+    std::stable_sort(v.begin(), v.end(),
+                     [&](const Item &x, const Item &y) { return b->fits[x.fit].NB < b->fits[y.fit].NB; });
+    off[0] = 0;
+    for (int nb = 1; nb <= 4; nb++) {
+        int n = 0;
+        for (const Item &it : v) n += b->fits[it.fit].NB == nb;
+        off[nb] = off[nb - 1] + n;
+    }
+}
+
 void set_chunks(BFit &f) {
     f.c_chunk_rows = f.c_chunk_base * f.cmul;
     f.c_chunks = (int)((f.npad + f.c_chunk_rows - 1) / f.c_chunk_rows);
@@ -389,11 +432,14 @@ hipError_t build_shape_lists(fem_batch *b) {
     for (int i = 0; i < b->F; i++) {
         BFit &f = b->fits[i];
         set_chunks(f);
+        if (f.NB > 2) f.eg = 2;  // four point groups only for d <= 32
         estep_items(f.eg == 4 ? c4 : a, i, f, f.eg);
         cov_items(cv, i, f);
     }
     interleave(a);
     interleave(c4);
+    by_nb(b, a, b->offA);
+    by_nb(b, c4, b->offB);
     hipError_t e;
     if ((e = replace_list(&b->estA, &b->n_estA, a)) != hipSuccess) return e;
     if ((e = replace_list(&b->estB, &b->n_estB, c4)) != hipSuccess) return e;
@@ -491,11 +537,7 @@ int fem_batch_create(fem_batch **out, int nfits, const int64_t *n, const int *d,
         (e = hipMemsetAsync(b->ipool, 0, sizeof(int) * std::max<int64_t>(1, b->ni), b->stream)) != hipSuccess ||
         (e = hipStreamSynchronize(b->stream)) != hipSuccess)
         return bail(FEM_ERR_HIP, hipGetErrorString(e));
-    const size_t lds_e = (size_t)fem::estep64_lds<4>() * sizeof(double);
-    (void)hipFuncSetAttribute((const void *)fem::kb_estep<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_e);
-    (void)hipFuncSetAttribute((const void *)fem::kb_estep<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_e);
-    b->grid_estA = grid_of(2, cus, b->n_estA);
-    b->grid_estB = grid_of(2, cus, b->n_estB);
+    b->mult_estA = b->mult_estB = 2;
     b->grid_means = grid_of(2, cus, b->n_means);
     b->grid_cov = grid_of(2, cus, b->n_cov);
     *out = b;
@@ -658,16 +700,6 @@ int fem_batch_tune(fem_batch *b, char *report, int report_len) {
         (void)hipEventElapsedTime(&ms, e0, e1);
         return ms / 3.0f;
     };
-    const size_t lds_e = (size_t)fem::estep64_lds<4>() * sizeof(double);
-    auto run_estep = [&](int P, const Item *items, int n, int grid) {
-        if (n <= 0) return;
-        if (P == 4)
-            hipLaunchKernelGGL(fem::kb_estep<4>, dim3(grid), dim3(256), lds_e, b->stream, b->dfits, b->dstate,
-                               b->xpool, b->dpool, items, n);
-        else
-            hipLaunchKernelGGL(fem::kb_estep<2>, dim3(grid), dim3(256), lds_e, b->stream, b->dfits, b->dstate,
-                               b->xpool, b->dpool, items, n);
-    };
     std::string rep;
     char line[256];
     std::map<std::pair<int, int>, std::vector<int>> classes;
@@ -680,17 +712,23 @@ int fem_batch_tune(fem_batch *b, char *report, int report_len) {
             if (*p) (void)hipFree(*p);
         }
     } free_tmp{&tmp};
-    // E-step point groups per class
+    // E-step point groups per class (four only for d <= 32, as the
+    // single-fit E-step: at d = 48, 64 four groups' accumulators and point
+    // tiles do not fit the registers of three waves per SIMD)
     for (auto &cl : classes) {
         float best = 1e30f;
         int bestP = 2;
+        const int nb = cl.first.first;
         for (int P : {2, 4}) {
+            if (P == 4 && nb > 2) continue;
             std::vector<Item> v;
             for (int i : cl.second) estep_items(v, i, b->fits[i], P);
             interleave(v);
             BH(b, replace_list(&tmp, &ntmp, v));
-            const int grid = grid_of(2, b->cus, ntmp);
-            const float ms = timeit([&]() { run_estep(P, tmp, ntmp, grid); });
+            const float ms = timeit([&]() {
+                if (P == 4) launch_estep_nb<4>(b, nb, tmp, ntmp, 2);
+                else launch_estep_nb<2>(b, nb, tmp, ntmp, 2);
+            });
             snprintf(line, sizeof line, "class d<=%d K<=%d (%zu fits): estep groups=%d %.3f ms\n", 16 * cl.first.first,
                      cl.first.second, cl.second.size(), P, ms);
             rep += line;
@@ -704,8 +742,8 @@ int fem_batch_tune(fem_batch *b, char *report, int report_len) {
     BH(b, build_shape_lists(b));
     // one E-step and means pass with the chosen shapes, so that the timed
     // covariance passes see this iteration's means
-    run_estep(2, b->estA, b->n_estA, grid_of(2, b->cus, b->n_estA));
-    run_estep(4, b->estB, b->n_estB, grid_of(2, b->cus, b->n_estB));
+    launch_estep_list(b, 2, b->estA, b->offA, 2);
+    launch_estep_list(b, 4, b->estB, b->offB, 2);
     hipLaunchKernelGGL(fem::kb_ll, dim3(b->F), dim3(256), 0, b->stream, b->dfits, b->dstate, b->dpool);
     hipLaunchKernelGGL(fem::kb_means, dim3(grid_of(2, b->cus, b->n_means)), dim3(256), 0, b->stream, b->dfits,
                        b->dstate, b->xpool, b->dpool, b->means, b->n_means);
@@ -757,12 +795,22 @@ int fem_batch_tune(fem_batch *b, char *report, int report_len) {
             }
         }
     };
-    if (b->n_estA > 0)
-        pick_grid("estep groups=2", {1, 2, 4}, b->n_estA, &b->grid_estA,
-                  [&](int g) { run_estep(2, b->estA, b->n_estA, g); });
-    if (b->n_estB > 0)
-        pick_grid("estep groups=4", {1, 2, 4}, b->n_estB, &b->grid_estB,
-                  [&](int g) { run_estep(4, b->estB, b->n_estB, g); });
+    // E-step lists: one multiple of the CU count for all of a list's NB ranges
+    auto pick_mult = [&](const char *name, int P, const Item *items, const int *off, int n, int *mult) {
+        if (n <= 0) return;
+        float best = 1e30f;
+        for (int m : {1, 2, 4}) {
+            const float ms = timeit([&]() { launch_estep_list(b, P, items, off, m); });
+            snprintf(line, sizeof line, "%s grid=%dxCU per NB: %.3f ms\n", name, m, ms);
+            rep += line;
+            if (ms < best) {
+                best = ms;
+                *mult = m;
+            }
+        }
+    };
+    pick_mult("estep groups=2", 2, b->estA, b->offA, b->n_estA, &b->mult_estA);
+    pick_mult("estep groups=4", 4, b->estB, b->offB, b->n_estB, &b->mult_estB);
     hipLaunchKernelGGL(fem::kb_ll, dim3(b->F), dim3(256), 0, b->stream, b->dfits, b->dstate, b->dpool);
     pick_grid("means", {1, 2, 4, 8}, b->n_means, &b->grid_means, [&](int g) {
         hipLaunchKernelGGL(fem::kb_means, dim3(g), dim3(256), 0, b->stream, b->dfits, b->dstate, b->xpool, b->dpool,
@@ -780,7 +828,7 @@ int fem_batch_tune(fem_batch *b, char *report, int report_len) {
                  16 * cl.first.first, cl.first.second, f.eg, (long long)f.c_chunk_rows);
         rep += line;
     }
-    snprintf(line, sizeof line, "chosen grids: estep %d/%d, means %d, cov %d (CUs=%d)\n", b->grid_estA, b->grid_estB,
+    snprintf(line, sizeof line, "chosen grids: estep %dxCU/%dxCU per NB, means %d, cov %d (CUs=%d)\n", b->mult_estA, b->mult_estB,
              b->grid_means, b->grid_cov, b->cus);
     rep += line;
     (void)hipEventDestroy(e0);
