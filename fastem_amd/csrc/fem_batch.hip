@@ -78,12 +78,12 @@ struct fem_batch {
     // chunk) items, and the shape-independent lists
     Item *estA = nullptr, *estB = nullptr, *means = nullptr, *cov = nullptr, *comp = nullptr, *covred = nullptr;
     int n_estA = 0, n_estB = 0, n_means = 0, n_cov = 0, n_comp = 0, n_covred = 0;
-    int offA[5] = {0, 0, 0, 0, 0}, offB[5] = {0, 0, 0, 0, 0};
+    int offA[5] = {0, 0, 0, 0, 0}, offB[5] = {0, 0, 0, 0, 0}, offC[5] = {0, 0, 0, 0, 0};
     int maxNB = 1;
     int cus = 256;
-    // tuned launch grids (the E-step ones as multiples of the CU count, per
-    // NB range of a list)
-    int mult_estA = 2, mult_estB = 2, grid_means = 0, grid_cov = 0;
+    // tuned launch grids (the E-step and covariance ones as multiples of the
+    // CU count, per NB range of a list)
+    int mult_estA = 2, mult_estB = 2, grid_means = 0, mult_cov = 2;
     // run parameters
     int maxsweeps = 100;
     double eps = 1e-5, g = 0.0;
@@ -200,6 +200,10 @@ __global__ __launch_bounds__(256) void kb_means_finalize(const BFit *__restrict_
     }
 }
 
+// Per NB, as the E-step: one kernel switching over NB took the largest
+// shape's 196 registers and two waves per SIMD for every fit; the single-NB
+// instances need 41-104 registers below d = 49.
+template <int NB>
 __global__ __launch_bounds__(256) void kb_cov(const BFit *__restrict__ F, const BState *__restrict__ S,
                                               const float *__restrict__ xp, double *__restrict__ dp,
                                               const Item *__restrict__ items, int nitems) {
@@ -207,12 +211,8 @@ __global__ __launch_bounds__(256) void kb_cov(const BFit *__restrict__ F, const 
         const Item itm = items[it];
         if (!S[itm.fit].active) continue;
         const BFit &f = F[itm.fit];
-        switch (f.NB) {
-            case 1: cov64_unit<1>(itm.a, itm.b, xp + f.x, dp + f.r, f.npad, f.K, f.KP, f.d, dp + f.mu, dp + f.nk, f.c_chunk_rows, dp + f.cpart); break;
-            case 2: cov64_unit<2>(itm.a, itm.b, xp + f.x, dp + f.r, f.npad, f.K, f.KP, f.d, dp + f.mu, dp + f.nk, f.c_chunk_rows, dp + f.cpart); break;
-            case 3: cov64_unit<3>(itm.a, itm.b, xp + f.x, dp + f.r, f.npad, f.K, f.KP, f.d, dp + f.mu, dp + f.nk, f.c_chunk_rows, dp + f.cpart); break;
-            case 4: cov64_unit<4>(itm.a, itm.b, xp + f.x, dp + f.r, f.npad, f.K, f.KP, f.d, dp + f.mu, dp + f.nk, f.c_chunk_rows, dp + f.cpart); break;
-        }
+        cov64_unit<NB>(itm.a, itm.b, xp + f.x, dp + f.r, f.npad, f.K, f.KP, f.d, dp + f.mu, dp + f.nk,
+                       f.c_chunk_rows, dp + f.cpart);
     }
 }
 
@@ -330,6 +330,23 @@ void launch_estep_nb(fem_batch *b, int nb, const Item *items, int n, int mult) {
     }
 }
 
+// The covariance items of fits with nb 16-feature blocks, and a whole
+// covariance list (ordered by NB) as one launch per NB.
+void launch_cov_nb(fem_batch *b, int nb, const Item *items, int n, int mult) {
+    if (n <= 0) return;
+    const int grid = grid_of(mult, b->cus, n);
+    switch (nb) {
+        case 1: hipLaunchKernelGGL(fem::kb_cov<1>, dim3(grid), dim3(256), 0, b->stream, b->dfits, b->dstate, b->xpool, b->dpool, items, n); break;
+        case 2: hipLaunchKernelGGL(fem::kb_cov<2>, dim3(grid), dim3(256), 0, b->stream, b->dfits, b->dstate, b->xpool, b->dpool, items, n); break;
+        case 3: hipLaunchKernelGGL(fem::kb_cov<3>, dim3(grid), dim3(256), 0, b->stream, b->dfits, b->dstate, b->xpool, b->dpool, items, n); break;
+        case 4: hipLaunchKernelGGL(fem::kb_cov<4>, dim3(grid), dim3(256), 0, b->stream, b->dfits, b->dstate, b->xpool, b->dpool, items, n); break;
+    }
+}
+
+void launch_cov_list(fem_batch *b, const Item *items, const int *off, int mult) {
+    for (int nb = 1; nb <= 4; nb++) launch_cov_nb(b, nb, items + off[nb - 1], off[nb] - off[nb - 1], mult);
+}
+
 // A whole E-step list (ordered by NB, ranges off[0..4]): one launch per NB.
 void launch_estep_list(fem_batch *b, int P, const Item *items, const int *off, int mult) {
     for (int nb = 1; nb <= 4; nb++) {
@@ -348,8 +365,7 @@ hipError_t launch_iteration(fem_batch *b) {
                        b->dpool, b->means, b->n_means);
     hipLaunchKernelGGL(fem::kb_means_finalize, dim3(grid_of(4, b->cus, b->n_comp)), dim3(256), 0, b->stream,
                        b->dfits, b->dstate, b->dpool, b->comp, b->n_comp);
-    hipLaunchKernelGGL(fem::kb_cov, dim3(b->grid_cov), dim3(256), 0, b->stream, b->dfits, b->dstate, b->xpool,
-                       b->dpool, b->cov, b->n_cov);
+    launch_cov_list(b, b->cov, b->offC, b->mult_cov);
     hipLaunchKernelGGL(fem::kb_cov_reduce, dim3(grid_of(8, b->cus, b->n_covred)), dim3(256), 0, b->stream,
                        b->dfits, b->dstate, b->dpool, b->covred, b->n_covred);
     hipLaunchKernelGGL(fem::kb_cov_finish, dim3(grid_of(4, b->cus, b->n_comp)), dim3(256), 0, b->stream,
@@ -440,6 +456,7 @@ hipError_t build_shape_lists(fem_batch *b) {
     interleave(c4);
     by_nb(b, a, b->offA);
     by_nb(b, c4, b->offB);
+    by_nb(b, cv, b->offC);
     hipError_t e;
     if ((e = replace_list(&b->estA, &b->n_estA, a)) != hipSuccess) return e;
     if ((e = replace_list(&b->estB, &b->n_estB, c4)) != hipSuccess) return e;
@@ -539,7 +556,7 @@ int fem_batch_create(fem_batch **out, int nfits, const int64_t *n, const int *d,
         return bail(FEM_ERR_HIP, hipGetErrorString(e));
     b->mult_estA = b->mult_estB = 2;
     b->grid_means = grid_of(2, cus, b->n_means);
-    b->grid_cov = grid_of(2, cus, b->n_cov);
+    b->mult_cov = 2;
     *out = b;
     return FEM_OK;
 }
@@ -765,11 +782,7 @@ int fem_batch_tune(fem_batch *b, char *report, int report_len) {
             }
             BH(b, hipMemcpy(b->dfits, b->fits.data(), sizeof(BFit) * b->F, hipMemcpyHostToDevice));
             BH(b, replace_list(&tmp, &ntmp, v));
-            const int grid = grid_of(2, b->cus, ntmp);
-            const float ms = timeit([&]() {
-                hipLaunchKernelGGL(fem::kb_cov, dim3(grid), dim3(256), 0, b->stream, b->dfits, b->dstate, b->xpool,
-                                   b->dpool, tmp, ntmp);
-            });
+            const float ms = timeit([&]() { launch_cov_nb(b, cl.first.first, tmp, ntmp, 2); });
             snprintf(line, sizeof line, "class d<=%d K<=%d: cov chunk rows=%lld %.3f ms\n", 16 * cl.first.first,
                      cl.first.second, (long long)b->fits[cl.second[0]].c_chunk_rows, ms);
             rep += line;
@@ -818,18 +831,26 @@ int fem_batch_tune(fem_batch *b, char *report, int report_len) {
     });
     hipLaunchKernelGGL(fem::kb_means_finalize, dim3(grid_of(4, b->cus, b->n_comp)), dim3(256), 0, b->stream,
                        b->dfits, b->dstate, b->dpool, b->comp, b->n_comp);
-    pick_grid("cov", {1, 2, 4, 8}, b->n_cov, &b->grid_cov, [&](int g) {
-        hipLaunchKernelGGL(fem::kb_cov, dim3(g), dim3(256), 0, b->stream, b->dfits, b->dstate, b->xpool, b->dpool,
-                           b->cov, b->n_cov);
-    });
+    if (b->n_cov > 0) {
+        float best = 1e30f;
+        for (int m : {1, 2, 4, 8}) {
+            const float ms = timeit([&]() { launch_cov_list(b, b->cov, b->offC, m); });
+            snprintf(line, sizeof line, "cov grid=%dxCU per NB: %.3f ms\n", m, ms);
+            rep += line;
+            if (ms < best) {
+                best = ms;
+                b->mult_cov = m;
+            }
+        }
+    }
     for (auto &cl : classes) {
         const BFit &f = b->fits[cl.second[0]];
         snprintf(line, sizeof line, "chosen for class d<=%d K<=%d: estep groups=%d, cov chunk rows=%lld\n",
                  16 * cl.first.first, cl.first.second, f.eg, (long long)f.c_chunk_rows);
         rep += line;
     }
-    snprintf(line, sizeof line, "chosen grids: estep %dxCU/%dxCU per NB, means %d, cov %d (CUs=%d)\n", b->mult_estA, b->mult_estB,
-             b->grid_means, b->grid_cov, b->cus);
+    snprintf(line, sizeof line, "chosen grids: estep %dxCU/%dxCU per NB, means %d, cov %dxCU per NB (CUs=%d)\n",
+             b->mult_estA, b->mult_estB, b->grid_means, b->mult_cov, b->cus);
     rep += line;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
