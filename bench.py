@@ -106,7 +106,8 @@ def em2_modes(X, w0, mu0, cov0, iters=10, log=print):
     return out
 
 
-def batch_workload(nfits=256, seed=7, maxsweeps=100, eps=1e-6, g=1e-3, compare=False, log=print):
+def batch_workload(nfits=256, seed=7, maxsweeps=100, eps=1e-6, g=1e-3, compare=False, log=print,
+                   seedings=(1, 1)):
     """Many independent fits advanced together (fem_batch_*): nfits fits in
     groups of 8 sharing a dataset (N log-uniform in [1e4, 1e6], d in
     {8, 16, 32, 64}, true K uniform in [4, 64]); each group is a K sweep over
@@ -119,20 +120,21 @@ def batch_workload(nfits=256, seed=7, maxsweeps=100, eps=1e-6, g=1e-3, compare=F
     import fastem_amd as fem
     rng = np.random.default_rng(seed)
     groups = nfits // 8
-    fits, data = [], []
+    fits, data, truth = [], [], []
     t = time.perf_counter()
     for gi in range(groups):
         n = int(round(np.exp(rng.uniform(np.log(1e4), np.log(1e6)))))
         d = int(rng.choice([8, 16, 32, 64]))
         kt = int(rng.integers(4, 65))
-        X = fem.gengmm(n, d, kt, -10.0, 10.0, 0.1, 1000 + gi)[0]
+        X, w_true, mu_true, cov_true = fem.gengmm(n, d, kt, -10.0, 10.0, 0.1, 1000 + gi)
         data.append(X)
+        truth.append((w_true, mu_true, cov_true))
         for K in (max(4, kt - 2), kt, min(64, kt + 2), min(64, kt + 4)):
             with fem.Context(n, d, K) as ctx:
                 ctx.upload(X)
                 for r in range(2):
                     try:
-                        init = ctx.kmeans_init(seed * 1000003 + gi * 16 + K * 2 + r, seeding=1,
+                        init = ctx.kmeans_init(seed * 1000003 + gi * 16 + K * 2 + r, seeding=seedings[r],
                                                max_sweeps=29)
                     except fem.FemError:
                         init = ctx.get_params()   # mixture written before the error
@@ -153,9 +155,18 @@ def batch_workload(nfits=256, seed=7, maxsweeps=100, eps=1e-6, g=1e-3, compare=F
     t_run = time.perf_counter() - t
     res = [b.result(i) for i in range(len(fits))]
     b.close()
+    # diagnostic (untimed): each dataset's true-K fit started from the
+    # generating mixture, one batch of one fit per dataset
+    bt = fem.Batch([(data[gi].shape[0], data[gi].shape[1], truth[gi][0].shape[0]) for gi in range(groups)])
+    for gi in range(groups):
+        bt.upload(gi, data[gi])
+        bt.set_params(gi, *truth[gi])
+    bt.run(maxsweeps, eps=eps, g=g, use_graph=True)
+    res_true = [bt.result(gi) for gi in range(groups)]
+    bt.close()
     iters = np.array([r["iter_counter"] for r in res])
     flop = sum(it * f[1] * f[3] * (2 * f[2] * f[2] + 3 * f[2]) for it, f in zip(iters, fits))
-    bic_ok, scored, kdiff = 0, 0, []
+    bic_ok, scored, kdiff, bic_true_ok, kt_better = 0, 0, [], 0, 0
     for gi in range(groups):
         cand = [(fem.bic(r["ll"], f[1], f[2], f[3]), f[3]) for r, f in zip(res, fits)
                 if f[0] == gi and r["status"] != fem.Batch.FAILED]
@@ -164,6 +175,14 @@ def batch_workload(nfits=256, seed=7, maxsweeps=100, eps=1e-6, g=1e-3, compare=F
             scored += 1
             bic_ok += int(min(cand)[1] == kt)
             kdiff.append(min(cand)[1] - kt)
+            rt = res_true[gi]
+            if rt["status"] != fem.Batch.FAILED:
+                bt_true = fem.bic(rt["ll"], data[gi].shape[0], data[gi].shape[1], kt)
+                # true K from the generating mixture against the k-means++ fits of the other orders
+                others = [c for c in cand if c[1] != kt]
+                bic_true_ok += int(not others or bt_true < min(others)[0])
+                best_kt = [c[0] for c in cand if c[1] == kt]
+                kt_better += int(bool(best_kt) and bt_true < min(best_kt))
     out = {"fits": len(fits), "datasets": groups, "points_total": int(sum(f[1] for f in fits)),
            "seconds": t_run, "tune_seconds": t_tune, "init_seconds_untimed": t_init,
            "fit_iterations": int(iters.sum()),
@@ -171,6 +190,8 @@ def batch_workload(nfits=256, seed=7, maxsweeps=100, eps=1e-6, g=1e-3, compare=F
            "converged": int(sum(r["status"] == fem.Batch.CONVERGED for r in res)),
            "failed": int(sum(r["status"] == fem.Batch.FAILED for r in res)),
            "bic_true_k_recovered": f"{bic_ok}/{scored}",
+           "bic_true_k_recovered_true_k_fit_from_generating_mixture": f"{bic_true_ok}/{scored}",
+           "true_k_fit_from_generating_mixture_beats_kmeanspp_fits": f"{kt_better}/{scored}",
            "bic_k_minus_true_k": {str(v): int(sum(1 for x in kdiff if x == v)) for v in sorted(set(kdiff))},
            "eps": eps, "g": g,
            "autotuner": report.strip().splitlines()[-1] if report else None}

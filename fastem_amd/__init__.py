@@ -16,7 +16,7 @@ EMINIT_PATH = os.path.join(_PKG, "eminit")
 
 PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
 ESTEP_FORMS = ("fp64", "fp32", "bf16x3", "bf16x3-ref")  # fem_estep_form
-SEED_FORGY, SEED_KMEANSPP, SEED_KMEANSOROR = 0, 1, 2
+SEED_FORGY, SEED_KMEANSPP, SEED_KMEANSOROR, SEED_KMEANSPP_GREEDY = 0, 1, 2, 3
 FEM_OK, FEM_ERR_ARG, FEM_ERR_HIP, FEM_ERR_NOT_PD, FEM_ERR_STATE = 0, -1, -2, -3, -4
 
 _lib = None
@@ -211,9 +211,10 @@ class Context:
         return R
 
     def kmeans_init(self, seed, seeding=1, max_sweeps=29):
-        """Forgy (SEED_FORGY), k-means++ (SEED_KMEANSPP) or k-means||
-        (SEED_KMEANSOROR) seeding, Lloyd sweeps and the mixture of the
-        clusters on the device data; becomes the parameters."""
+        """Forgy (SEED_FORGY), k-means++ (SEED_KMEANSPP), k-means||
+        (SEED_KMEANSOROR) or greedy k-means++ (SEED_KMEANSPP_GREEDY)
+        seeding, Lloyd sweeps and the mixture of the clusters on the device
+        data; becomes the parameters."""
         w = np.empty(self.K)
         mu = np.empty((self.K, self.d))
         cov = np.empty((self.K, self.d, self.d))

@@ -131,7 +131,7 @@ hipError_t dmalloc(T **p, size_t count) {
 void free_all(fem_ctx *c) {
     void *ptrs[] = {c->X, c->Rt, c->Rt32, c->w, c->mu, c->cov, c->nk, c->logdet, c->info, c->coef,
                     c->U, c->stage64, c->stage32, c->stageb, c->stager, c->refc, c->ll_part, c->ll_dev,
-                    c->mpart, c->cpart, c->kC, c->dmin, c->cls, c->kfit, c->kpick};
+                    c->mpart, c->cpart, c->kC, c->dmin, c->cls, c->kfit, c->kpick, c->kcand, c->kpot};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &e : c->ev)

@@ -14,6 +14,9 @@ namespace fem {
 constexpr int64_t kRowAlign = 4096;
 // Largest supported dimension (the factor kernel keeps L in LDS).
 constexpr int kMaxDim = 128;
+// Candidates per seed of the greedy k-means++ seeding: 2 + floor(ln K) <= 6
+// for K <= 128.
+constexpr int kMaxTrials = 8;
 
 // Number of (jb, lb) fragment pairs the E-step consumes for NB 16-feature
 // blocks: lb runs over 4-feature blocks, jb >= lb/4 (U is upper triangular).
@@ -88,6 +91,8 @@ struct fem_ctx {
     int *cls = nullptr;         // npad  cluster of each point
     int64_t *kpick = nullptr;   // 2     k-means++ draw decided on the device (row, decided)
     double *kfit = nullptr;     // per-workgroup fit partials
+    double *kcand = nullptr;    // kMaxTrials x d greedy k-means++ candidates
+    double *kpot = nullptr;     // (npad / 256) x kMaxTrials candidate potential partials
     bool owns_rt64 = false;     // Rt allocated for k-means in a non-FP64 context
 
     hipEvent_t ev[8] = {};

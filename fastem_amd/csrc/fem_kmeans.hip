@@ -16,6 +16,7 @@
 // (products by 1.0 are exact; sums in FP64, fixed order).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -85,6 +86,42 @@ __global__ __launch_bounds__(256) void k_kpp_update(const float *__restrict__ X,
         __syncthreads();
     }
     if (threadIdx.x == 0) bsum[blockIdx.x] = red[0];
+}
+
+// Greedy k-means++ trial: for each of the L candidate centres (L x d in
+// cand), pot[block * L + l] = the workgroup's sum over its rows of
+// min(dmin_i, |x_i - c_l|^2) (fixed tree order), the potential the seeding
+// would have with c_l added.
+template <int DP>
+__global__ __launch_bounds__(256) void k_kpp_trial(const float *__restrict__ X, int64_t n, int dpad, int d,
+                                                   const double *__restrict__ cand, int L,
+                                                   const double *__restrict__ dmin, double *__restrict__ pot) {
+    __shared__ double lc[kMaxTrials][DP];
+    __shared__ double red[256];
+    for (int e = threadIdx.x; e < L * d; e += 256) lc[e / d][e % d] = cand[e];
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    float x[DP];
+    double dm = 0.0;
+    if (i < n) {
+        load_row<DP>(X, i, dpad, x);
+        dm = dmin[i];
+    }
+    for (int l = 0; l < L; l++) {
+        double v = 0.0;
+        if (i < n) {
+            const double s = sqdist_reg<DP>(lc[l], x, d);
+            v = dm > s ? s : dm;
+        }
+        red[threadIdx.x] = v;
+        __syncthreads();
+        for (int st = 128; st > 0; st >>= 1) {
+            if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) pot[(size_t)blockIdx.x * L + l] = red[0];
+        __syncthreads();
+    }
 }
 
 // The k-means++ D^2 draw (Clust/PlusPlusInitializer/PlusPlusInitializer.cpp:
@@ -454,6 +491,8 @@ void ensure_scratch(fem_ctx *c) {
     if (!c->cls) KM_HIP(hipMalloc((void **)&c->cls, sizeof(int) * c->npad));
     if (!c->kfit) KM_HIP(hipMalloc((void **)&c->kfit, sizeof(double) * (c->npad / 256)));
     if (!c->kpick) KM_HIP(hipMalloc((void **)&c->kpick, sizeof(int64_t) * 2));
+    if (!c->kcand) KM_HIP(hipMalloc((void **)&c->kcand, sizeof(double) * fem::kMaxTrials * c->d));
+    if (!c->kpot) KM_HIP(hipMalloc((void **)&c->kpot, sizeof(double) * (c->npad / 256) * fem::kMaxTrials));
     if (!c->Rt) {
         KM_HIP(hipMalloc((void **)&c->Rt, sizeof(double) * c->KP * c->npad));
         c->owns_rt64 = true;
@@ -705,7 +744,8 @@ extern "C" int fem_kmeans_init_rng(fem_ctx *c, double (*rnd)(void *), void *rnd_
                                    int max_sweeps, double *w, double *mu, double *cov) {
     if (!c) return FEM_ERR_ARG;
     if (!rnd || !w || !mu || !cov || max_sweeps < 0 ||
-        (seeding != FEM_SEED_FORGY && seeding != FEM_SEED_KMEANSPP && seeding != FEM_SEED_KMEANSOROR)) {
+        (seeding != FEM_SEED_FORGY && seeding != FEM_SEED_KMEANSPP && seeding != FEM_SEED_KMEANSOROR &&
+         seeding != FEM_SEED_KMEANSPP_GREEDY)) {
         c->err = "fem_kmeans_init: bad argument";
         return FEM_ERR_ARG;
     }
@@ -716,8 +756,12 @@ extern "C" int fem_kmeans_init_rng(fem_ctx *c, double (*rnd)(void *), void *rnd_
         const int64_t n = c->n;
         auto draw = [&]() { return rnd(rnd_state); };
         std::vector<double> C((size_t)K * d, 0.0);
-        if (seeding == FEM_SEED_KMEANSPP) {
-            // Clust/PlusPlusInitializer/PlusPlusInitializer.cpp:33-84
+        if (seeding == FEM_SEED_KMEANSPP || seeding == FEM_SEED_KMEANSPP_GREEDY) {
+            // Clust/PlusPlusInitializer/PlusPlusInitializer.cpp:33-84; the
+            // greedy form draws L = 2 + floor(ln K) candidates per seed by the
+            // same rule and keeps the one of least potential
+            const bool greedy = seeding == FEM_SEED_KMEANSPP_GREEDY;
+            const int L = greedy ? std::min(fem::kMaxTrials, 2 + (int)std::log((double)K)) : 1;
             const int64_t r0 = (int64_t)(draw() * (double)n);
             fetch_row(c, r0, C.data());
             double *dm = nullptr;
@@ -732,16 +776,11 @@ extern "C" int fem_kmeans_init_rng(fem_ctx *c, double (*rnd)(void *), void *rnd_
             // sequential sums (the path the device falls back to)
             const char *kd = std::getenv("FASTEM_KPP_DRAW");
             const bool host_draw = kd && std::strcmp(kd, "host") == 0;
-            for (int k = 1; k < K; k++) {
-                KM_HIP(hipMemcpyAsync(c->kC, C.data() + (size_t)(k - 1) * d, sizeof(double) * d,
-                                      hipMemcpyHostToDevice, c->stream));
-                KM_HIP(fem::by_dpad(c->dpad, [&](auto DPc) -> hipError_t {
-                    constexpr int DP = decltype(DPc)::value;
-                    hipLaunchKernelGGL(fem::k_kpp_update<DP>, dim3(blocks), dim3(256), 0, c->stream, c->X, n,
-                                       c->dpad, d, c->kC, c->dmin, k == 1 ? 1 : 0, c->kfit);
-                    return hipGetLastError();
-                }));
-                const double r = draw();
+            int agg_step = 0;  // seed step whose prefix sums agg holds (0: none)
+            double sum = 0.0;
+            // the row of the D^2 rule for the uniform variate r at seed step k
+            // (n: none)
+            auto pick_row = [&](int k, double r) -> int64_t {
                 if (!host_draw) {
                     hipLaunchKernelGGL(fem::k_kpp_pick, dim3(1), dim3(256), 0, c->stream, c->kfit, blocks, c->dmin,
                                        n, r, c->kpick);
@@ -749,18 +788,18 @@ extern "C" int fem_kmeans_init_rng(fem_ctx *c, double (*rnd)(void *), void *rnd_
                     int64_t pick[2];
                     KM_HIP(hipMemcpyAsync(pick, c->kpick, sizeof pick, hipMemcpyDeviceToHost, c->stream));
                     KM_HIP(hipStreamSynchronize(c->stream));
-                    if (pick[1] == 1) {
-                        if (pick[0] < n) fetch_row(c, pick[0], C.data() + (size_t)k * d);
-                        continue;
-                    }
+                    if (pick[1] == 1) return pick[0];
                 }
                 // undecided on the device: the sequential FP64 sums
-                KM_HIP(hipMemcpyAsync(dm, c->dmin, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
-                KM_HIP(hipStreamSynchronize(c->stream));
-                double sum = 0.0;
-                for (int64_t i = 0; i < n; i++) {
-                    sum += dm[i] * 1.0;
-                    agg[i] = sum;
+                if (agg_step != k) {
+                    KM_HIP(hipMemcpyAsync(dm, c->dmin, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+                    KM_HIP(hipStreamSynchronize(c->stream));
+                    sum = 0.0;
+                    for (int64_t i = 0; i < n; i++) {
+                        sum += dm[i] * 1.0;
+                        agg[i] = sum;
+                    }
+                    agg_step = k;
                 }
                 // first i with r <= agg[i] / sum: agg is non-decreasing and
                 // rounded division by sum > 0 keeps that order, so a binary
@@ -772,7 +811,53 @@ extern "C" int fem_kmeans_init_rng(fem_ctx *c, double (*rnd)(void *), void *rnd_
                     if (r <= agg[mid] / sum) hi = mid;
                     else lo = mid + 1;
                 }
-                if (lo < n) fetch_row(c, lo, C.data() + (size_t)k * d);
+                return lo;
+            };
+            std::vector<double> cand((size_t)fem::kMaxTrials * d), pot((size_t)blocks * fem::kMaxTrials);
+            for (int k = 1; k < K; k++) {
+                KM_HIP(hipMemcpyAsync(c->kC, C.data() + (size_t)(k - 1) * d, sizeof(double) * d,
+                                      hipMemcpyHostToDevice, c->stream));
+                KM_HIP(fem::by_dpad(c->dpad, [&](auto DPc) -> hipError_t {
+                    constexpr int DP = decltype(DPc)::value;
+                    hipLaunchKernelGGL(fem::k_kpp_update<DP>, dim3(blocks), dim3(256), 0, c->stream, c->X, n,
+                                       c->dpad, d, c->kC, c->dmin, k == 1 ? 1 : 0, c->kfit);
+                    return hipGetLastError();
+                }));
+                if (!greedy) {
+                    const int64_t i = pick_row(k, draw());
+                    if (i < n) fetch_row(c, i, C.data() + (size_t)k * d);
+                    continue;
+                }
+                // greedy: L draws (each consumes one variate, in order), the
+                // candidates' potentials sum(min(dmin, |x - c|^2)) on the
+                // device, summed over the workgroups in order on the host;
+                // the least potential wins, ties to the earlier draw
+                int nc = 0;
+                for (int l = 0; l < L; l++) {
+                    const int64_t i = pick_row(k, draw());
+                    if (i < n) fetch_row(c, i, cand.data() + (size_t)(nc++) * d);
+                }
+                if (nc == 0) continue;
+                KM_HIP(hipMemcpyAsync(c->kcand, cand.data(), sizeof(double) * nc * d, hipMemcpyHostToDevice,
+                                      c->stream));
+                KM_HIP(fem::by_dpad(c->dpad, [&](auto DPc) -> hipError_t {
+                    constexpr int DP = decltype(DPc)::value;
+                    hipLaunchKernelGGL(fem::k_kpp_trial<DP>, dim3(blocks), dim3(256), 0, c->stream, c->X, n,
+                                       c->dpad, d, c->kcand, nc, c->dmin, c->kpot);
+                    return hipGetLastError();
+                }));
+                KM_HIP(hipMemcpyAsync(pot.data(), c->kpot, sizeof(double) * blocks * nc, hipMemcpyDeviceToHost,
+                                      c->stream));
+                KM_HIP(hipStreamSynchronize(c->stream));
+                int best = 0;
+                double bestp = 0.0;
+                for (int l = 0; l < nc; l++) {
+                    double p = 0.0;
+                    for (int bl = 0; bl < blocks; bl++) p += pot[(size_t)bl * nc + l];
+                    if (l == 0 || p < bestp) best = l, bestp = p;
+                }
+                std::copy(cand.begin() + (size_t)best * d, cand.begin() + (size_t)(best + 1) * d,
+                          C.begin() + (size_t)k * d);
             }
         } else if (seeding == FEM_SEED_FORGY) {
             // Clust/KMeansInitializer.cpp:33-49: K rows at (long)(Rand()*N),

@@ -109,9 +109,14 @@ int fem_get_posteriors(fem_ctx *ctx, double *R, int64_t ld);
 enum fem_seeding {
     FEM_SEED_FORGY = 0,      /* K rows at (long)(Rand()*N), Clust/KMeansInitializer.cpp:33-49 */
     FEM_SEED_KMEANSPP = 1,   /* Clust/PlusPlusInitializer/PlusPlusInitializer.cpp:33-84 */
-    FEM_SEED_KMEANSOROR = 2  /* k-means||, oversampling 1.4, 5 rounds, then weighted k-means++
+    FEM_SEED_KMEANSOROR = 2, /* k-means||, oversampling 1.4, 5 rounds, then weighted k-means++
                                 over the candidates: Clust/KMeansOrOrInitializer.cpp:15-174,
                                 EM2/EMInitializer.cpp:158-161 (one-thread draw order) */
+    FEM_SEED_KMEANSPP_GREEDY = 3 /* no counterpart in the reference: greedy k-means++ (Arthur and
+                                Vassilvitskii 2007, section 6): per seed, L = 2 + floor(ln K)
+                                candidates drawn by the k-means++ rule, one variate each, and
+                                the one of least potential sum_i min(D_i^2, |x_i - c|^2) kept
+                                (ties to the earlier draw); the batched workload's restarts */
 };
 
 /* Class of each of the n rows (cls[n], host): the component of largest

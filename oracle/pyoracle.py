@@ -239,6 +239,47 @@ def kmeanspp(X, K, rng):
     return C
 
 
+def _seqdist(X64, c):
+    """Squared distances of the rows to c, FP64, ascending feature order,
+    no fused multiply-add (the device's and orc_kmeanspp's order)."""
+    s = np.zeros(X64.shape[0])
+    for j in range(X64.shape[1]):
+        t = c[j] - X64[:, j]
+        s = s + t * t
+    return s
+
+
+def kmeanspp_greedy(X, K, rng):
+    """Greedy k-means++ (Arthur and Vassilvitskii 2007, section 6; no
+    counterpart in the reference): the first seed as k-means++, then per seed
+    L = 2 + floor(ln K) draws by the k-means++ D^2 rule (the first row i with
+    r <= agg_i / agg_{n-1}, agg the sequential prefix sums), one variate
+    each, and the candidate of least potential sum_i min(D_i^2, |x_i - c|^2)
+    kept, ties to the earlier draw.  numpy restatement (test infrastructure)."""
+    X64 = np.asarray(X, dtype=np.float32).astype(np.float64)
+    n, d = X64.shape
+    L = min(8, 2 + int(np.log(K)))
+    C = np.zeros((K, d))
+    C[0] = X64[int(rng.canonical() * n)]
+    dmin = None
+    for k in range(1, K):
+        dist = _seqdist(X64, C[k - 1])
+        dmin = dist if dmin is None else np.where(dmin > dist, dist, dmin)
+        agg = np.cumsum(dmin)
+        S = agg[-1]
+        cands = []
+        for _ in range(L):
+            r = rng.canonical()
+            hit = np.nonzero(r <= agg / S)[0] if S > 0 else np.array([], dtype=np.int64)
+            if hit.size:
+                cands.append(int(hit[0]))
+        if not cands:
+            continue
+        pots = [np.minimum(dmin, _seqdist(X64, X64[i])).sum() for i in cands]
+        C[k] = X64[cands[int(np.argmin(pots))]]
+    return C
+
+
 def lloyd(X, C, rng, minrel=1e-4, maxiter=30):
     X = _c32(X)
     C = _c64(C).copy()

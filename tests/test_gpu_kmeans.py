@@ -115,3 +115,44 @@ def test_kmeanspp_device_draw_equals_host_draw(monkeypatch, n, d, K, seed):
     host = fem.kmeanspp_init(X, K, seed=seed, lloyd_iters=0)
     for a, b in zip(dev, host):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("n,d,K,seed", [(20000, 8, 12, 21), (30001, 17, 6, 22), (12000, 64, 40, 23)])
+def test_greedy_kmeanspp_seeds_match_replay(orc, n, d, K, seed):
+    """Greedy k-means++ (the batched workload's seeding): the seeds equal a
+    numpy replay on the same Rand() stream (candidate draws by the k-means++
+    rule, least potential kept); then Lloyd and the mixture as k-means++."""
+    print(f"greedy kmeans++ n={n} d={d} K={K}", flush=True)
+    X = fem.gengmm(n, d, K, -10.0, 10.0, 0.1, seed)[0]
+    with fem.Context(n, d, K) as ctx:
+        ctx.upload(X)
+        w, mu, cov = ctx.kmeans_init(seed, seeding=fem.SEED_KMEANSPP_GREEDY, max_sweeps=0)
+    C = orc.kmeanspp_greedy(X, K, orc.Rand(seed))
+    np.testing.assert_array_equal(mu, C)  # no Lloyd sweep: the means are the seeds
+    cls = orc.classify(X, C)
+    np.testing.assert_array_equal(np.rint(w * n), np.bincount(cls, minlength=K))  # w = n_k / n
+    _, _, cov_ref = orc.gmm_from_clusters(X, C, cls)
+    assert np.max(np.abs(cov - cov_ref)) <= 1e-10 * np.max(np.abs(cov_ref))
+
+
+def test_greedy_kmeanspp_lowers_the_potential():
+    """On well-separated mixtures the greedy seeding leaves a lower k-means
+    potential (sum of squared distances to the nearest seed) than k-means++
+    from the same seeds of the stream, over ten streams."""
+    print("greedy kmeans++ potential", flush=True)
+    n, d, K = 40000, 16, 32
+    X = fem.gengmm(n, d, K, -10.0, 10.0, 0.1, 31)[0]
+    X64 = X.astype(np.float64)
+
+    def potential(C):
+        dd = ((X64[:, None, :] - C[None, :, :]) ** 2).sum(-1)
+        return dd.min(1).sum()
+
+    wins = 0
+    with fem.Context(n, d, K) as ctx:
+        ctx.upload(X)
+        for s in range(10):
+            g = ctx.kmeans_init(100 + s, seeding=fem.SEED_KMEANSPP_GREEDY, max_sweeps=0)[1]
+            p = ctx.kmeans_init(100 + s, seeding=fem.SEED_KMEANSPP, max_sweeps=0)[1]
+            wins += potential(g) < potential(p)
+    assert wins >= 8, wins
