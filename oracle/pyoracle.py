@@ -249,7 +249,7 @@ def _seqdist(X64, c):
     return s
 
 
-def kmeanspp_greedy(X, K, rng):
+def kmeanspp_greedy(X, K, rng, L=None):
     """Greedy k-means++ (Arthur and Vassilvitskii 2007, section 6; no
     counterpart in the reference): the first seed as k-means++, then per seed
     L = 2 + floor(ln K) draws by the k-means++ D^2 rule (the first row i with
@@ -258,7 +258,8 @@ def kmeanspp_greedy(X, K, rng):
     kept, ties to the earlier draw.  numpy restatement (test infrastructure)."""
     X64 = np.asarray(X, dtype=np.float32).astype(np.float64)
     n, d = X64.shape
-    L = min(8, 2 + int(np.log(K)))
+    if L is None:
+        L = min(8, 2 + int(np.log(K)))
     C = np.zeros((K, d))
     C[0] = X64[int(rng.canonical() * n)]
     dmin = None

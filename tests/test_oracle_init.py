@@ -1,5 +1,5 @@
 """Pins the oracle's initializers (CPU): Forgy, Jain, the data covariance
-trace and k-means||, each against an independent numpy replay of the
+trace, k-means|| and greedy k-means++, each against an independent numpy replay of the
 reference's Rand() stream (Util/Rand.cpp:7-14, whose mt19937_64 and canonical
 mapping test_oracle.py pins with the C++ standard's known answer)."""
 import numpy as np
@@ -181,3 +181,42 @@ def test_kmeans_init_composite(orc):
     np.testing.assert_array_equal(mu, C)
     cls = orc.classify(X, C)
     np.testing.assert_array_equal(w * 5000, np.bincount(cls, minlength=4))
+
+
+def _gmm_rows(n, d, K, seed):
+    rng = np.random.default_rng(seed)
+    mu = rng.uniform(-10, 10, (K, d))
+    lab = rng.integers(0, K, n)
+    return (mu[lab] + rng.standard_normal((n, d))).astype(np.float32)
+
+
+@pytest.mark.parametrize("n,d,K,seed", [(3000, 4, 7, 1), (5001, 9, 12, 2)])
+def test_greedy_kmeanspp_one_trial_is_kmeanspp(orc, n, d, K, seed):
+    """With one candidate per seed the greedy replay is plain k-means++: the
+    same seeds as the C oracle's orc_kmeanspp on the same Rand() stream."""
+    X = _gmm_rows(n, d, K, seed)
+    np.testing.assert_array_equal(orc.kmeanspp_greedy(X, K, orc.Rand(seed), L=1),
+                                  orc.kmeanspp(X, K, orc.Rand(seed)))
+
+
+def test_greedy_kmeanspp_keeps_least_potential(orc):
+    """Brute force on a small set: replaying the draws independently (numpy
+    cumsum rule, every candidate's potential by direct summation), each
+    greedy seed is the candidate of least potential."""
+    n, d, K, L, seed = 400, 3, 6, 4, 5
+    X = _gmm_rows(n, d, K, seed)
+    C = orc.kmeanspp_greedy(X, K, orc.Rand(seed), L=L)
+    X64 = X.astype(np.float64)
+    rng = orc.Rand(seed)
+    first = int(rng.canonical() * n)
+    np.testing.assert_array_equal(C[0], X64[first])
+    for k in range(1, K):
+        D = np.min([((X64 - C[j]) ** 2).sum(1) for j in range(k)], axis=0)
+        agg = np.cumsum(D)
+        cands = [int(np.argmax(rng.canonical() <= agg / agg[-1])) for _ in range(L)]
+        pots = [np.minimum(D, ((X64 - X64[i]) ** 2).sum(1)).sum() for i in cands]
+        np.testing.assert_array_equal(C[k], X64[cands[int(np.argmin(pots))]])
+    # and the greedy potential is not above plain k-means++'s from the same stream
+    Cp = orc.kmeanspp(X, K, orc.Rand(seed))
+    pot = lambda S: np.min([((X64 - c) ** 2).sum(1) for c in S], axis=0).sum()
+    assert pot(C) <= pot(Cp) * (1 + 1e-12)
