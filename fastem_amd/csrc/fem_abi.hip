@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "fem_internal.h"
 
@@ -131,7 +132,7 @@ hipError_t dmalloc(T **p, size_t count) {
 void free_all(fem_ctx *c) {
     void *ptrs[] = {c->X, c->Rt, c->Rt32, c->w, c->mu, c->cov, c->nk, c->logdet, c->info, c->coef,
                     c->U, c->stage64, c->stage32, c->stageb, c->stager, c->refc, c->ll_part, c->ll_dev,
-                    c->mpart, c->cpart, c->kC, c->dmin, c->cls, c->kfit, c->kpick, c->kcand, c->kpot};
+                    c->mpart, c->cpart, c->kC, c->dmin, c->cls, c->kfit, c->kpick, c->kcand, c->kpot, c->korder};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &e : c->ev)
@@ -242,7 +243,8 @@ int fem_create(fem_ctx **out, int64_t n, int d, int K, int precision, int device
         (e = dmalloc(&c->mu, (size_t)K * d)) != hipSuccess ||
         (e = dmalloc(&c->cov, (size_t)K * d * d)) != hipSuccess ||
         (e = dmalloc(&c->nk, K)) != hipSuccess || (e = dmalloc(&c->logdet, K)) != hipSuccess ||
-        (e = dmalloc(&c->info, K)) != hipSuccess || (e = dmalloc(&c->coef, K)) != hipSuccess)
+        (e = dmalloc(&c->info, K)) != hipSuccess || (e = dmalloc(&c->coef, K)) != hipSuccess ||
+        (e = dmalloc(&c->korder, K)) != hipSuccess)
         return bail(e, "hipMalloc(params)");
     if ((e = dmalloc(&c->U, (size_t)K * c->dpad * c->dpad)) != hipSuccess ||
         (e = dmalloc(&c->stage64, (size_t)K * fem::stage64_words(c->NB))) != hipSuccess)
@@ -252,9 +254,15 @@ int fem_create(fem_ctx **out, int64_t n, int d, int K, int precision, int device
         (e = dmalloc(&c->mpart, (size_t)c->m_chunks * c->KP * (c->dpad + 1))) != hipSuccess ||
         (e = dmalloc(&c->cpart, (size_t)c->c_chunks * c->KP * ub * 256)) != hipSuccess)
         return bail(e, "hipMalloc(partials)");
+    // the E-step's component order: the index order until the first
+    // factorisation ranks the weights
+    std::vector<int> ident(K);
+    for (int k = 0; k < K; k++) ident[k] = k;
     if ((e = hipMemsetAsync(c->X, 0, nX * sizeof(float), c->stream)) != hipSuccess ||
         (f64 && (e = hipMemsetAsync(c->Rt, 0, nR * sizeof(double), c->stream)) != hipSuccess) ||
         (!f64 && (e = hipMemsetAsync(c->Rt32, 0, nR * sizeof(float), c->stream)) != hipSuccess) ||
+        (e = hipMemcpyAsync(c->korder, ident.data(), sizeof(int) * K, hipMemcpyHostToDevice, c->stream)) !=
+            hipSuccess ||
         (e = hipStreamSynchronize(c->stream)) != hipSuccess)
         return bail(e, "initial clear");
     *out = c;

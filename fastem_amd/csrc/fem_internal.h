@@ -62,6 +62,7 @@ struct fem_ctx {
     double *logdet = nullptr;  // K
     int *info = nullptr;       // K
     double *coef = nullptr;    // K   log w_k - d/2 log 2pi - logdet_k/2
+    int *korder = nullptr;     // K   components by descending weight: the FP64 E-step's order
     double *U = nullptr;       // K x dpad x dpad  U_k = L_k^-T (zero padded)
     double *stage64 = nullptr; // K x (pairs x 64 + dpad): FP64 E-step fragments, -U_k^T mu_k
     float *stage32 = nullptr;  // K x stage32_words: FP32 fragments, mu_hi, -U_k^T mu_lo

@@ -32,6 +32,21 @@ __global__ __launch_bounds__(256) void k_factor(int K, int d, int dpad, int NB, 
     factor_unit(blockIdx.x, d, dpad, NB, npairs, cov, w, mu, logdet, info, coef, U, stage64, stage32, A, &fail);
 }
 
+// Components by descending weight, ties to the lower index (a weight that
+// is not a non-negative number ranks last): the FP64 E-step's processing
+// order.  K <= 128: one workgroup, a rank count per component.
+__global__ __launch_bounds__(256) void k_order(int K, const double *__restrict__ w, int *__restrict__ order) {
+    for (int k = threadIdx.x; k < K; k += 256) {
+        const double wk = w[k] >= 0.0 ? w[k] : -1.0;
+        int r = 0;
+        for (int j = 0; j < K; j++) {
+            const double wj = w[j] >= 0.0 ? w[j] : -1.0;
+            r += (wj > wk) || (wj == wk && j < k);
+        }
+        order[r] = k;
+    }
+}
+
 hipError_t launch_factor(fem_ctx *c) {
     const int npairs = estep_pairs(c->NB);
     const size_t lds = (size_t)c->d * c->d * sizeof(double);
@@ -43,6 +58,8 @@ hipError_t launch_factor(fem_ctx *c) {
     hipLaunchKernelGGL(k_factor, dim3(c->K), dim3(256), lds, c->stream, c->K, c->d, c->dpad,
                        c->NB, npairs, c->cov, c->w, c->mu, c->logdet, c->info, c->coef, c->U,
                        c->stage64, c->stage32);
+    if (c->korder)  // contexts without it (fem_estep_noalloc's) keep the index order
+        hipLaunchKernelGGL(k_order, dim3(1), dim3(256), 0, c->stream, c->K, c->w, c->korder);
     if (c->prec == FEM_PREC_BF16X3) {
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
@@ -59,10 +76,11 @@ hipError_t launch_factor(fem_ctx *c) {
 template <int NB, int P>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_estep64_w3(
     const float *__restrict__ X, int64_t n, int64_t npad, int K, const double *__restrict__ stage,
-    const double *__restrict__ coef, double *__restrict__ Rt, double *__restrict__ ll_part) {
+    const double *__restrict__ coef, double *__restrict__ Rt, double *__restrict__ ll_part,
+    const int *__restrict__ order) {
     extern __shared__ double lds[];  // estep64_lds<NB>()
     __shared__ double wsum[4];
-    estep64_unit<NB, P>(blockIdx.x, X, n, npad, K, stage, coef, Rt, ll_part + blockIdx.x, lds, wsum);
+    estep64_unit<NB, P>(blockIdx.x, X, n, npad, K, stage, coef, Rt, ll_part + blockIdx.x, lds, wsum, order);
 }
 
 template <int NB, int P>
@@ -71,10 +89,11 @@ __global__ __launch_bounds__(256) void k_estep64(const float *__restrict__ X, in
                                                  const double *__restrict__ stage,
                                                  const double *__restrict__ coef,
                                                  double *__restrict__ Rt,
-                                                 double *__restrict__ ll_part) {
+                                                 double *__restrict__ ll_part,
+                                                 const int *__restrict__ order) {
     extern __shared__ double lds[];  // estep64_lds<NB>()
     __shared__ double wsum[4];
-    estep64_unit<NB, P>(blockIdx.x, X, n, npad, K, stage, coef, Rt, ll_part + blockIdx.x, lds, wsum);
+    estep64_unit<NB, P>(blockIdx.x, X, n, npad, K, stage, coef, Rt, ll_part + blockIdx.x, lds, wsum, order);
 }
 
 // d = 65..80: LDS leaves room for two workgroups per CU; registers capped
@@ -82,10 +101,11 @@ __global__ __launch_bounds__(256) void k_estep64(const float *__restrict__ X, in
 template <int NB, int P>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_estep64_w2(
     const float *__restrict__ X, int64_t n, int64_t npad, int K, const double *__restrict__ stage,
-    const double *__restrict__ coef, double *__restrict__ Rt, double *__restrict__ ll_part) {
+    const double *__restrict__ coef, double *__restrict__ Rt, double *__restrict__ ll_part,
+    const int *__restrict__ order) {
     extern __shared__ double lds[];  // estep64_lds<NB>()
     __shared__ double wsum[4];
-    estep64_unit<NB, P>(blockIdx.x, X, n, npad, K, stage, coef, Rt, ll_part + blockIdx.x, lds, wsum);
+    estep64_unit<NB, P>(blockIdx.x, X, n, npad, K, stage, coef, Rt, ll_part + blockIdx.x, lds, wsum, order);
 }
 
 // Deterministic sum of n doubles into *out: fixed strided partials, then a
@@ -111,13 +131,13 @@ static hipError_t estep64_t(fem_ctx *c) {
     const int blocks = (int)(c->npad / (64 * P));
     if constexpr (NB <= 4)
         hipLaunchKernelGGL((k_estep64_w3<NB, P>), dim3(blocks), dim3(256), lds, c->stream, c->X, c->n, c->npad,
-                           c->K, c->stage64, c->coef, c->Rt, c->ll_part);
+                           c->K, c->stage64, c->coef, c->Rt, c->ll_part, c->korder);
     else if constexpr (NB == 5)
         hipLaunchKernelGGL((k_estep64_w2<NB, P>), dim3(blocks), dim3(256), lds, c->stream, c->X, c->n, c->npad,
-                           c->K, c->stage64, c->coef, c->Rt, c->ll_part);
+                           c->K, c->stage64, c->coef, c->Rt, c->ll_part, c->korder);
     else
         hipLaunchKernelGGL((k_estep64<NB, P>), dim3(blocks), dim3(256), lds, c->stream, c->X, c->n, c->npad,
-                           c->K, c->stage64, c->coef, c->Rt, c->ll_part);
+                           c->K, c->stage64, c->coef, c->Rt, c->ll_part, c->korder);
     hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(256), 0, c->stream, c->ll_part,
                        (int64_t)blocks, c->ll_dev);
     return hipGetLastError();

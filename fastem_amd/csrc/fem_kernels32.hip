@@ -36,11 +36,17 @@ __device__ __host__ constexpr int pairs32(int NB) {
 // into registers while component k is processed and written to the idle
 // buffer before the one barrier per component.  The B operand is
 // x - mu_hi and the accumulators start at -U_k^T mu_lo, so D = U_k^T (x - mu).
+// Components are processed in the order `order` (by descending weight) and a
+// log-density is stored only while within alive_range of its point's
+// running maximum, as in the FP64 E-step (fem_tiles.h); the storing lane
+// keeps the bits of what it stored and the log-sum-exp sweep's lanes take
+// their components' bits from it.
 template <int NB, int P>
 __global__ __launch_bounds__(256) void k_estep32(const float *__restrict__ X, int64_t n, int64_t npad,
                                                  int K, const float *__restrict__ stage,
                                                  const double *__restrict__ coef,
-                                                 float *__restrict__ Rt, double *__restrict__ ll_part) {
+                                                 float *__restrict__ Rt, double *__restrict__ ll_part,
+                                                 const int *__restrict__ order) {
     constexpr int DP = 16 * NB;
     constexpr int NPAIR = pairs32(NB);
     constexpr int STAGE = NPAIR * 64 + 2 * DP;  // a multiple of 4
@@ -62,21 +68,26 @@ __global__ __launch_bounds__(256) void k_estep32(const float *__restrict__ X, in
     // clamped to its last piece (same value to the same address)
     const float4 *stage4 = reinterpret_cast<const float4 *>(stage);
     float4 *lds4 = reinterpret_cast<float4 *>(lds32);
+    {
+        const float4 *first4 = stage4 + (size_t)order[0] * STAGE4;
 #pragma unroll
-    for (int q = 0; q < PER; q++) {
-        const int e = min(tid + 256 * q, STAGE4 - 1);
-        lds4[e] = stage4[e];
+        for (int q = 0; q < PER; q++) {
+            const int e = min(tid + 256 * q, STAGE4 - 1);
+            lds4[e] = first4[e];
+        }
     }
     __syncthreads();
 
-    float mx = -INFINITY;  // of point (g mod P, c)
+    float mx = -INFINITY;   // of point (g mod P, c)
+    uint64_t stored = 0;    // bit k: its log-density stored (K <= 64)
 
-    for (int k = 0; k < K; k++) {
-        const float *lu = lds32 + (k & 1) * STAGE;
+    for (int it = 0; it < K; it++) {
+        const int k = order[it];
+        const float *lu = lds32 + (it & 1) * STAGE;
         const float *lmh = lu + NPAIR * 64;
         const float *lml = lmh + DP;
-        const bool more = k + 1 < K;
-        const float4 *next4 = stage4 + (size_t)(more ? k + 1 : k) * STAGE4;
+        const bool more = it + 1 < K;
+        const float4 *next4 = stage4 + (size_t)(more ? order[it + 1] : k) * STAGE4;
         float4 pre[PER];
 #pragma unroll
         for (int q = 0; q < PER; q++) pre[q] = next4[min(tid + 256 * q, STAGE4 - 1)];
@@ -132,11 +143,14 @@ __global__ __launch_bounds__(256) void k_estep32(const float *__restrict__ X, in
             // unconditional (after the last component the idle buffer is
             // dead) and ahead of the branches below, so that the loads of the
             // next block stay in this basic block, ahead of the MFMAs
-            float4 *dst = lds4 + ((k + 1) & 1) * STAGE4;
+            float4 *dst = lds4 + ((it + 1) & 1) * STAGE4;
 #pragma unroll
             for (int q = 0; q < PER; q++) dst[min(tid + 256 * q, STAGE4 - 1)] = pre[q];
         }
-        if (g < P) Rt[(size_t)k * npad + base + 16 * g + c] = sv;
+        if (g < P && (K > 64 || !(sv < mx - alive_range<float>()))) {
+            Rt[(size_t)k * npad + base + 16 * g + c] = sv;
+            stored |= 1ull << (k & 63);
+        }
         // (the barrier also orders these stores before the sweep below,
         // whose lanes read entries other lanes stored)
         __syncthreads();
@@ -148,7 +162,7 @@ __global__ __launch_bounds__(256) void k_estep32(const float *__restrict__ X, in
         const int64_t pt = base + 16 * p + c;
         const bool valid = pt < n;
         const float mxp = __shfl(mx, 16 * p + c);
-        const float lse = lse_posteriors(Rt, npad, pt, K, g, mxp, valid);
+        const float lse = lse_posteriors(Rt, npad, pt, K, g, mxp, valid, sweep_bits(stored, 16 * p + c, g));
         if (g == 0 && valid) llsum += (double)lse;
     }
 #pragma unroll
@@ -183,7 +197,7 @@ static hipError_t estep32_t(fem_ctx *c) {
     }
     const int blocks = (int)(c->npad / (64 * P));
     hipLaunchKernelGGL((k_estep32<NB, P>), dim3(blocks), dim3(256), lds, c->stream, c->X, c->n, c->npad,
-                       c->K, c->stage32, c->coef, c->Rt32, c->ll_part);
+                       c->K, c->stage32, c->coef, c->Rt32, c->ll_part, c->korder);
     hipLaunchKernelGGL(k_reduce_sum32, dim3(1), dim3(256), 0, c->stream, c->ll_part, (int64_t)blocks,
                        c->ll_dev);
     return hipGetLastError();

@@ -193,6 +193,7 @@ __global__ __launch_bounds__(256) void k_estepb(const float *__restrict__ X, int
     __syncthreads();
 
     float mx = -INFINITY;  // of point (g mod P, c)
+    uint64_t stored = 0;   // bit k: its log-density stored (K <= 64)
 
     for (int k = 0; k < K; k++) {
         const unsigned int *lf = ldsb + (k & 1) * STAGE;
@@ -259,7 +260,12 @@ __global__ __launch_bounds__(256) void k_estepb(const float *__restrict__ X, int
 #pragma unroll
             for (int q = 0; q < PER; q++) dst[min(tid + 256 * q, STAGE4 - 1)] = pre[q];
         }
-        if (g < P) Rt[(size_t)k * npad + base + 16 * g + c] = sv;
+        // stored only while within alive_range of the point's running maximum
+        // (fem_tiles.h lse_posteriors)
+        if (g < P && (K > 64 || !(sv < mx - alive_range<float>()))) {
+            Rt[(size_t)k * npad + base + 16 * g + c] = sv;
+            stored |= 1ull << (k & 63);
+        }
         // (the barrier also orders these stores before the sweep below,
         // whose lanes read entries other lanes stored)
         __syncthreads();
@@ -271,7 +277,7 @@ __global__ __launch_bounds__(256) void k_estepb(const float *__restrict__ X, int
         const int64_t pt = base + 16 * p + c;
         const bool valid = pt < n;
         const float mxp = __shfl(mx, 16 * p + c);
-        const float lse = lse_posteriors(Rt, npad, pt, K, g, mxp, valid);
+        const float lse = lse_posteriors(Rt, npad, pt, K, g, mxp, valid, sweep_bits(stored, 16 * p + c, g));
         if (g == 0 && valid) llsum += (double)lse;
     }
 #pragma unroll
@@ -509,7 +515,15 @@ __global__ __launch_bounds__(256) void k_estepr(const float *__restrict__ X, int
         }
     float mx = -INFINITY, sv = 0.f;
     float *rt = Rt + blk0 + myloc;
-    auto store = [&](int k) { rt[(size_t)k * npad] = sv; };
+    // stored only while within alive_range of the point's running maximum
+    // (fem_tiles.h lse_posteriors); bit k of `stored`: component k stored
+    uint64_t stored = 0;
+    auto store = [&](int k) {
+        if (K > 64 || !(sv < mx - alive_range<float>())) {
+            rt[(size_t)k * npad] = sv;
+            stored |= 1ull << (k & 63);
+        }
+    };
     // components in pairs: even ones read A and fill B, odd ones the reverse;
     // at each barrier this wave's copy has landed and the barrier publishes
     // every wave's; the log-densities are stored after the wait
@@ -539,7 +553,7 @@ __global__ __launch_bounds__(256) void k_estepr(const float *__restrict__ X, int
         const int64_t pt = blk0 + loc[p];
         const bool valid = pt < n;
         const float mxp = __shfl(mx, 16 * p + c);
-        const float lse = lse_posteriors(Rt, npad, pt, K, g, mxp, valid);
+        const float lse = lse_posteriors(Rt, npad, pt, K, g, mxp, valid, sweep_bits(stored, 16 * p + c, g));
         if (g == 0 && valid) llsum += (double)lse;
     }
 #pragma unroll

@@ -293,21 +293,32 @@ __device__ inline void factor_unit(int k, int d, int dpad, int NB, int npairs, c
 // one batch of independent loads, e_k = exp(s_k - mx) is formed once and the
 // posteriors are e_k / sum; larger K streams the entries twice,
 // r_k = exp(s_k - lse).  Invalid (padding) points get zero posteriors.
-// Returns lse = log(sum) + mx for every lane of the column.
+// With K <= 64, bit i of `stored` says whether the entry of component
+// g + 4i was stored: the E-steps store a log-density only while it is within
+// alive_range<T>() of the point's running maximum, and every other entry
+// lies at least that far below the final maximum, where exp underflows to
+// +0: it is taken as exp(-inf) = +0 without being read, the value its
+// exponential has, so the posteriors and the sum are those of reading every
+// entry.  Returns lse = log(sum) + mx for every lane of the column.
 __device__ inline float exp_of(float v) { return expf(v); }
 __device__ inline double exp_of(double v) { return exp(v); }
+// Below the maximum by more than this, exp underflows to +0 (FP64: below
+// -745.2; FP32: below -103.98, or -87.4 with denormals flushed).
+template <typename T>
+__device__ __host__ constexpr T alive_range() { return sizeof(T) == 8 ? T(760) : T(110); }
 __device__ inline float log_of(float v) { return logf(v); }
 __device__ inline double log_of(double v) { return log(v); }
 
 template <typename T>
-__device__ __attribute__((always_inline)) inline T lse_posteriors(T *__restrict__ Rt, int64_t npad, int64_t pt, int K, int g, T mx, bool valid) {
+__device__ __attribute__((always_inline)) inline T lse_posteriors(T *__restrict__ Rt, int64_t npad, int64_t pt, int K, int g, T mx, bool valid,
+                                                             unsigned stored = 0xFFFFu) {
     constexpr int KQ = 16;
     if (K <= 4 * KQ) {
         T e[KQ];
 #pragma unroll
         for (int i = 0; i < KQ; i++) {
             const int k = g + 4 * i;
-            e[i] = k < K ? Rt[(size_t)k * npad + pt] : T(0);
+            e[i] = (k < K && ((stored >> i) & 1u)) ? Rt[(size_t)k * npad + pt] : T(-INFINITY);
         }
         T se = T(0);
 #pragma unroll
@@ -333,6 +344,18 @@ __device__ __attribute__((always_inline)) inline T lse_posteriors(T *__restrict_
         *r = valid ? exp_of(*r - lse) : T(0);
     }
     return lse;
+}
+
+// The sweep's stored bits for the lane of group g (components g + 4i, bit i)
+// from the 64 bits (bit k: component k) kept by the lane src that stored the
+// point's log-densities (the FP32 and BF16x3 E-steps, K <= 64).
+__device__ inline unsigned sweep_bits(uint64_t stored, int src, int g) {
+    const unsigned lo = __shfl((unsigned)stored, src), hi = __shfl((unsigned)(stored >> 32), src);
+    const uint64_t sb = ((uint64_t)hi << 32) | lo;
+    unsigned st = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) st |= (unsigned)((sb >> (g + 4 * i)) & 1u) << i;
+    return st;
 }
 
 // ---------------------------------------------------------------------------
@@ -371,11 +394,18 @@ __device__ __host__ constexpr int estep64_stage() { return stage64_words_of(NB);
 template <int NB>
 __device__ __host__ constexpr int estep64_lds() { return 2 * ((estep64_stage<NB>() / 2 + 63) / 64) * 128; }
 
+// order (K entries, or null for 0..K-1) is the order the components are
+// processed in; it changes no result (the row max and the component-ordered
+// sums do not depend on it), only which log-densities are within
+// alive_range of the running maximum when they are formed, and so stored:
+// the context passes the components by descending weight, so most points
+// meet their own component early.
 template <int NB, int P>
 __device__ inline void estep64_unit(int64_t tile, const float *__restrict__ X, int64_t n, int64_t npad, int K,
                                     const double *__restrict__ stage, const double *__restrict__ coef,
                                     double *__restrict__ Rt,
-                                    double *__restrict__ ll_out, double *lds, double *wsum) {
+                                    double *__restrict__ ll_out, double *lds, double *wsum,
+                                    const int *__restrict__ order = nullptr) {
     constexpr int DP = 16 * NB;
     constexpr int NOFF = offd_pairs_of(NB);
     constexpr int NDIAG = diag_quads_of(NB);
@@ -399,27 +429,36 @@ __device__ inline void estep64_unit(int64_t tile, const float *__restrict__ X, i
     // clamped to its last piece (same value to the same address)
     const double2 *stage2 = reinterpret_cast<const double2 *>(stage);
     double2 *lds2 = reinterpret_cast<double2 *>(lds);
+    auto comp = [&](int it) { return order ? order[it] : it; };
+    {
+        const double2 *first2 = stage2 + (size_t)comp(0) * STAGE2;
 #pragma unroll
-    for (int q = 0; q < PER; q++) {
-        const int e = min(tid + 256 * q, STAGE2 - 1);
-        lds2[e] = stage2[e];
+        for (int q = 0; q < PER; q++) {
+            const int e = min(tid + 256 * q, STAGE2 - 1);
+            lds2[e] = first2[e];
+        }
     }
     __syncthreads();
 
     double mx[P];
+    unsigned stored[P];  // bit k/4: log-density of component k stored (K <= 64)
 #pragma unroll
-    for (int p = 0; p < P; p++) mx[p] = -INFINITY;
+    for (int p = 0; p < P; p++) {
+        mx[p] = -INFINITY;
+        stored[p] = 0u;
+    }
 
-    for (int k = 0; k < K; k++) {
-        const double *lu = lds + (k & 1) * (2 * BUF2);
+    for (int it = 0; it < K; it++) {
+        const int k = comp(it);
+        const double *lu = lds + (it & 1) * (2 * BUF2);
         const double *lq = lu + NOFF * 64;
         const double *lm = lq + NDIAG * 16;
-        if (k + 1 < K) {
-            // component k+1's block straight into the idle buffer (each lane
-            // one 16-byte piece per round: global_load_lds_dwordx4), so no
-            // registers hold it and the copy overlaps this component's MFMAs
-            const double2 *next2 = stage2 + (size_t)(k + 1) * STAGE2;
-            double2 *dst = lds2 + ((k + 1) & 1) * BUF2 + 64 * wave;
+        if (it + 1 < K) {
+            // the next component's block straight into the idle buffer (each
+            // lane one 16-byte piece per round: global_load_lds_dwordx4), so
+            // no registers hold it and the copy overlaps this component's MFMAs
+            const double2 *next2 = stage2 + (size_t)comp(it + 1) * STAGE2;
+            double2 *dst = lds2 + ((it + 1) & 1) * BUF2 + 64 * wave;
 #pragma unroll
             for (int q = 0; q < PER; q++)
                 if (q + 1 < PER || 256 * q + 64 * wave < STAGE2)
@@ -484,9 +523,15 @@ __device__ inline void estep64_unit(int64_t tile, const float *__restrict__ X, i
         // the entries it stored itself)
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
+        // only log-densities within alive_range of the running maximum
+        // (NaN included) can have a non-zero posterior (lse_posteriors)
         if ((k & 3) == g) {
 #pragma unroll
-            for (int p = 0; p < P; p++) Rt[(size_t)k * npad + base + 16 * p + c] = sv[p];
+            for (int p = 0; p < P; p++)
+                if (K > 64 || !(sv[p] < mx[p] - alive_range<double>())) {
+                    Rt[(size_t)k * npad + base + 16 * p + c] = sv[p];
+                    stored[p] |= 1u << ((k >> 2) & 15);
+                }
         }
     }
 
@@ -495,7 +540,7 @@ __device__ inline void estep64_unit(int64_t tile, const float *__restrict__ X, i
     for (int p = 0; p < P; p++) {
         const int64_t pt = base + 16 * p + c;
         const bool valid = pt < n;
-        const double lse = lse_posteriors(Rt, npad, pt, K, g, mx[p], valid);
+        const double lse = lse_posteriors(Rt, npad, pt, K, g, mx[p], valid, stored[p]);
         if (g == 0 && valid) llsum += lse;
     }
 #pragma unroll
