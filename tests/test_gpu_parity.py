@@ -225,3 +225,45 @@ def test_config2_full_size_vs_oracle(orc):
     say("config2 max rel errors vs oracle:", errs, "LL trace", list(ref.ll_trace))
     assert errs["ll"] < LL_RTOL
     assert errs["w"] < PARAM_RTOL and errs["mu"] < PARAM_RTOL and errs["cov"] < PARAM_RTOL
+
+
+@pytest.mark.parametrize("prec", [fem.PREC_FP64, fem.PREC_FP32])
+def test_unstored_log_densities_have_zero_posteriors(orc, prec):
+    """The E-steps store a log-density only while it is within the
+    exponential's underflow range of the point's running maximum (760 in
+    FP64, 110 in FP32).  A 1-d mixture puts components about 741 and 780
+    (FP64) or 100 and 120 (FP32) below a point's own: posteriors against the
+    FP64 oracle, exactly zero where the oracle's are, whatever order the
+    components come in (weights in both orders)."""
+    print(f"unstored log-densities precision={prec}", flush=True)
+    far = (38.5, 39.5) if prec == fem.PREC_FP64 else (14.1, 15.5)
+    rng = np.random.default_rng(3)
+    X = np.concatenate([rng.normal(0.0, 0.3, 3000), rng.normal(far[0], 0.3, 500),
+                        rng.normal(far[1], 0.3, 500), rng.normal(-far[0], 0.3, 96)])
+    X = X.astype(np.float32)[:, None]
+    mu = np.array([[0.0], [far[0]], [far[1]], [-far[0]]])
+    cov = np.ones((4, 1, 1))
+    for w in (np.array([0.7, 0.1, 0.1, 0.1]), np.array([0.05, 0.15, 0.3, 0.5])):
+        with fem.Context(X.shape[0], 1, 4, prec) as ctx:
+            ctx.upload(X)
+            ctx.set_params(w, mu, cov)
+            ctx.factor()
+            ll = ctx.estep()
+            R = ctx.posteriors()
+        U, ld = orc.factor_all(cov)
+        ll_ref, R_ref = orc.estep(X, w, mu, U, ld)
+        # log-density gaps to the point's log-sum-exp (1-d, unit variances)
+        s = np.log(w)[None, :] - 0.5 * np.log(2 * np.pi) - 0.5 * (X.astype(np.float64) - mu[:, 0][None, :]) ** 2
+        gap = s - np.logaddexp.reduce(s, axis=1)[:, None]
+        rng_alive = 760.0 if prec == fem.PREC_FP64 else 110.0
+        unstored = gap < -rng_alive - 1.0  # below the range even against the final maximum
+        assert unstored.any() and ((gap > -rng_alive + 1.0) & (gap < -rng_alive + 30.0)).any()
+        np.testing.assert_array_equal(R[unstored], 0.0)
+        if prec == fem.PREC_FP64:
+            np.testing.assert_array_equal(R_ref[unstored], 0.0)  # FP64 exp underflows there too
+        if prec == fem.PREC_FP64:
+            assert abs(ll - ll_ref) <= 1e-12 * abs(ll_ref)
+            np.testing.assert_allclose(R, R_ref, rtol=1e-9, atol=1e-320)
+        else:
+            assert abs(ll - ll_ref) <= 1e-6 * abs(ll_ref)
+            np.testing.assert_allclose(R, R_ref, rtol=1e-3, atol=1e-30)
