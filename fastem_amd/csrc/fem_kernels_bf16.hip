@@ -433,9 +433,11 @@ __device__ __attribute__((always_inline)) inline void estepr_component(
 // of the seed rows are mostly broadcasts over distinct LDS banks instead of
 // 16 rows that collide in the banks.  Log-densities, posteriors and
 // references stay in point order.  One barrier per component.  Lane (g, c)
-// stores the log-densities of point (g, c) of its wave; the log-sum-exp
-// sweep (4 lanes per point, components g, g + 4, ...) reads them after a
-// barrier.
+// stores the log-densities of point (g, c) of its wave (those within
+// alive_range of its running maximum); the log-sum-exp sweep (4 lanes per
+// point, components g, g + 4, ...) runs over the points in their own order,
+// so 16 lanes cover a whole row segment of Rt, taking each point's maximum
+// and stored bits from LDS (written by location before its barrier).
 template <int NB, int R>
 __global__ __launch_bounds__(256) void k_estepr(const float *__restrict__ X, int64_t n, int64_t npad, int K,
                                                 const unsigned int *__restrict__ stage,
@@ -448,7 +450,8 @@ __global__ __launch_bounds__(256) void k_estepr(const float *__restrict__ X, int
     __shared__ __align__(16) uint4 bufA[BUF4];
     __shared__ __align__(16) uint4 bufB[BUF4];
     __shared__ double wsum[4];
-    __shared__ __align__(16) int skey[256];
+    __shared__ __align__(16) int skey[256];  // ranking keys; then each point's max (as float)
+    __shared__ uint64_t sbits[256];          // each point's stored bits
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int g = lane >> 4, c = lane & 15;
@@ -543,17 +546,22 @@ __global__ __launch_bounds__(256) void k_estepr(const float *__restrict__ X, int
             store(k + 1);
         }
     }
-    // the sweep below reads entries other lanes stored
+    // the sweep below runs over the points in their own order (whole rows
+    // of Rt per 16 lanes) and reads entries other lanes stored: each point's
+    // max and stored bits go to LDS by its location
+    reinterpret_cast<float *>(skey)[myloc] = mx;
+    sbits[myloc] = stored;
     __syncthreads();
 
     ref[blk0 + myloc] = best;
     double llsum = 0.0;
 #pragma unroll
     for (int p = 0; p < P; p++) {
-        const int64_t pt = blk0 + loc[p];
+        const int l = wave * 64 + 16 * p + c;
+        const int64_t pt = blk0 + l;
         const bool valid = pt < n;
-        const float mxp = __shfl(mx, 16 * p + c);
-        const float lse = lse_posteriors(Rt, npad, pt, K, g, mxp, valid, sweep_bits(stored, 16 * p + c, g));
+        const float mxp = reinterpret_cast<const float *>(skey)[l];
+        const float lse = lse_posteriors(Rt, npad, pt, K, g, mxp, valid, group_bits(sbits[l], g));
         if (g == 0 && valid) llsum += (double)lse;
     }
 #pragma unroll

@@ -349,13 +349,16 @@ __device__ __attribute__((always_inline)) inline T lse_posteriors(T *__restrict_
 // The sweep's stored bits for the lane of group g (components g + 4i, bit i)
 // from the 64 bits (bit k: component k) kept by the lane src that stored the
 // point's log-densities (the FP32 and BF16x3 E-steps, K <= 64).
-__device__ inline unsigned sweep_bits(uint64_t stored, int src, int g) {
-    const unsigned lo = __shfl((unsigned)stored, src), hi = __shfl((unsigned)(stored >> 32), src);
-    const uint64_t sb = ((uint64_t)hi << 32) | lo;
+__device__ inline unsigned group_bits(uint64_t sb, int g) {
     unsigned st = 0;
 #pragma unroll
     for (int i = 0; i < 16; i++) st |= (unsigned)((sb >> (g + 4 * i)) & 1u) << i;
     return st;
+}
+
+__device__ inline unsigned sweep_bits(uint64_t stored, int src, int g) {
+    const unsigned lo = __shfl((unsigned)stored, src), hi = __shfl((unsigned)(stored >> 32), src);
+    return group_bits(((uint64_t)hi << 32) | lo, g);
 }
 
 // ---------------------------------------------------------------------------
