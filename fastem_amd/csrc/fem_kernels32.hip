@@ -62,7 +62,8 @@ __global__ __launch_bounds__(256) void k_estep32(const float *__restrict__ X, in
 #pragma unroll
     for (int p = 0; p < P; p++)
 #pragma unroll
-        for (int lb = 0; lb < 4 * NB; lb++) xr[p][lb] = X[(base + 16 * p + c) * DP + 4 * lb + g];
+        for (int lb = 0; lb < 4 * NB; lb++)  // read once: non-temporal (L2 keeps the stage blocks)
+            xr[p][lb] = __builtin_nontemporal_load(&X[(base + 16 * p + c) * DP + 4 * lb + g]);
 
     // stage blocks move in 16-byte pieces; piece indices past the block are
     // clamped to its last piece (same value to the same address)

@@ -495,7 +495,11 @@ __global__ __launch_bounds__(256) void k_estepr(const float *__restrict__ X, int
             if (f0 < DP) {
                 const float4 *src = reinterpret_cast<const float4 *>(X + pt * DP + f0);
                 const float4 *mh = reinterpret_cast<const float4 *>(muh + (size_t)a * DP + f0);
-                const float4 u0 = src[0], u1 = src[1], m0 = mh[0], m1 = mh[1];
+                // the points are read once: non-temporal (L2 keeps the stage blocks)
+                const v4f w0 = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(src));
+                const v4f w1 = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(src) + 1);
+                const float4 u0 = {w0[0], w0[1], w0[2], w0[3]}, u1 = {w1[0], w1[1], w1[2], w1[3]};
+                const float4 m0 = mh[0], m1 = mh[1];
                 t[0] = u0.x - m0.x; t[1] = u0.y - m0.y; t[2] = u0.z - m0.z; t[3] = u0.w - m0.w;
                 t[4] = u1.x - m1.x; t[5] = u1.y - m1.y; t[6] = u1.z - m1.z; t[7] = u1.w - m1.w;
             } else {
