@@ -208,6 +208,7 @@ int fem_create(fem_ctx **out, int64_t n, int d, int K, int precision, int device
     if (const char *e = std::getenv("FASTEM_ESTEP64_GROUPS")) c->estep_groups = std::atoi(e) == 4 ? 4 : 2;
     if (const char *e = std::getenv("FASTEM_ESTEP32_GROUPS")) c->estep_groups32 = std::atoi(e) == 4 ? 4 : 2;
     if (const char *e = std::getenv("FASTEM_ESTEPB_GROUPS")) c->estep_groupsb = std::atoi(e) == 2 ? 2 : 4;
+    if (const char *e = std::getenv("FASTEM_COV_DENSE")) c->cov_sparse = std::atoi(e) == 0;
     c->m_chunks = (int)(c->npad / 4096);
     // covariance chunks: at most 256 chunks of a multiple of 4096 rows
     int64_t units = c->npad / 4096;

@@ -83,6 +83,8 @@ struct fem_ctx {
     double *ll_dev = nullptr;   // 1
     int m_chunks = 0, c_chunks = 0;
     int64_t c_chunk_rows = 0;
+    // covariances over the nonzero posteriors only (FASTEM_COV_DENSE=1: every point)
+    bool cov_sparse = true;
     double *mpart = nullptr;    // m_chunks x KP x (dpad+1)
     double *cpart = nullptr;    // c_chunks x KP x ublocks x 256
 

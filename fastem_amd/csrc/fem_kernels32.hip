@@ -451,11 +451,191 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     else if constexpr (NPART > 1) cov32_body<NB, 1>(cg, chunk, X, Rt, npad, K, d, mu, nk, chunk_rows, KP, cpart);
 }
 
+// The same over the nonzero posteriors only (as k_cov64_sparse, whose exact-
+// zero argument holds here with FP32's underflow): per wave and segment of
+// kCovSeg32 points, lane l reads points 4l..4l+3 of each 256 (one 16-byte
+// load), the points with r != 0 are listed in point order in the wave's LDS
+// slice, and the products run over the list four entries per MFMA (lane
+// group g takes entry 4j + g; entries past the end take r = 0).  The FP32
+// accumulators are flushed into FP64 before a segment's list would take them
+// past 2048 terms, the dense form's bound.
+constexpr int kCovSeg32 = 1024;
+
+template <int NB, int PART>
+__device__ inline void cov32_sparse(int cg, int chunk, const float *__restrict__ X, const float *__restrict__ Rt,
+                                    int64_t npad, int K, int d, const double *__restrict__ mu,
+                                    const double *__restrict__ nk, int64_t chunk_rows, int KP,
+                                    double *__restrict__ cpart, int *lidx, float *lrv) {
+    constexpr int DP = 16 * NB;
+    constexpr int UB = NB * (NB + 1) / 2;
+    constexpr int NPART = cov32_parts(NB);
+    constexpr int U0 = upper_part_begin(NB, NPART, PART);
+    constexpr int NU = upper_part_size(NB, NPART, PART);
+    constexpr int FLUSH = 2048;
+    constexpr bool REG = NB <= 4;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int g = lane >> 4, c = lane & 15;
+    const int k = 4 * cg + wave;
+    if (k >= K) return;
+    const int64_t i0 = (int64_t)chunk * chunk_rows;
+    const int64_t i1 = (i0 + chunk_rows < npad) ? i0 + chunk_rows : npad;
+    float mh[NB];
+#pragma unroll
+    for (int fb = 0; fb < NB; fb++) mh[fb] = (NB * c + fb < d) ? (float)mu[(size_t)k * d + NB * c + fb] : 0.0f;
+    const float inv = (float)(1.0 / sqrt(nk[k]));
+    double *out = cpart + ((size_t)chunk * KP + k) * UB * 256;
+    const float *rrow = Rt + (size_t)k * npad;
+    int *wi = lidx + wave * kCovSeg32;
+    float *wr = lrv + wave * kCovSeg32;
+    double accd[REG ? NU : 1][4];
+    if (REG) {
+#pragma unroll
+        for (int u = 0; u < (REG ? NU : 1); u++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) accd[u][v] = 0.0;
+    }
+    bool first = true;
+    v4f acc[NU];
+#pragma unroll
+    for (int u = 0; u < NU; u++) acc[u] = (v4f){0.f, 0.f, 0.f, 0.f};
+    int pending = 0;  // terms in acc since the last flush
+    auto flush = [&]() {
+        if (REG) {
+#pragma unroll
+            for (int u = 0; u < (REG ? NU : 1); u++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) accd[u][v] += (double)acc[u][v];
+        } else {
+#pragma unroll
+            for (int u = 0; u < NU; u++) {
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    double *o = out + (size_t)(U0 + u) * 256 + (4 * g + v) * 16 + c;
+                    *o = (first ? 0.0 : *o) + (double)acc[u][v];
+                }
+                if ((u & 3) == 3) asm volatile("" ::: "memory");
+            }
+        }
+        first = false;
+#pragma unroll
+        for (int u = 0; u < NU; u++) acc[u] = (v4f){0.f, 0.f, 0.f, 0.f};
+        pending = 0;
+    };
+
+    constexpr int NJ = kCovSeg32 / 256;
+    v4f rs[NJ];
+    auto loadseg = [&](int64_t s) {
+#pragma unroll
+        for (int j = 0; j < NJ; j++)
+            rs[j] = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(rrow + s + 256 * j + 4 * lane));
+    };
+    // entries e0 + 4 q + g (q < 2; called with e0 < m): weight and features
+    auto fetch = [&](int64_t s, int m, int e0, float (&sw)[2], float (&xv)[2][NB]) {
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const int e = e0 + 4 * q + g;
+            const bool ok = e < m;
+            const int ec = ok ? e : 0;
+            const int off = wi[ec];
+            sw[q] = ok ? __builtin_amdgcn_sqrtf(wr[ec]) * inv : 0.0f;
+            load_consecutive<NB>(X + (size_t)(s + off) * DP + NB * c, xv[q]);
+        }
+    };
+    auto products = [&](const float (&sw)[2], const float (&xv)[2][NB]) {
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            float t[NB];
+#pragma unroll
+            for (int fb = 0; fb < NB; fb++)
+                if (upper_part_uses(NB, NPART, PART, fb, false)) t[fb] = (xv[q][fb] - mh[fb]) * sw[q];
+            int u = 0;
+#pragma unroll
+            for (int ib = 0; ib < NB; ib++)
+#pragma unroll
+                for (int jb = ib; jb < NB; jb++, u++) {
+                    if (upper_part_of(u, UB, NPART) != PART) continue;
+                    acc[u - U0] = __builtin_amdgcn_mfma_f32_16x16x4f32(t[ib], t[jb], acc[u - U0], 0, 0, 0);
+                }
+        }
+    };
+
+    // chunk bounds are multiples of 4096 points: every segment is whole
+    if (i0 < i1) loadseg(i0);
+    for (int64_t s = i0; s < i1; s += kCovSeg32) {
+        int m = 0;
+#pragma unroll
+        for (int j = 0; j < NJ; j++) {
+            bool nz[4];
+            uint64_t b[4];
+            int below = 0;
+#pragma unroll
+            for (int w = 0; w < 4; w++) {
+                nz[w] = rs[j][w] != 0.0f;
+                b[w] = __ballot(nz[w]);
+                below += (int)__builtin_amdgcn_mbcnt_hi((unsigned)(b[w] >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo((unsigned)b[w], 0u));
+            }
+            int at = m + below;
+            const int off = 256 * j + 4 * lane;
+#pragma unroll
+            for (int w = 0; w < 4; w++) {
+                if (nz[w]) {
+                    wi[at] = off + w;
+                    wr[at] = rs[j][w];
+                }
+                at += nz[w];
+                m += __builtin_popcountll(b[w]);
+            }
+        }
+        if (s + kCovSeg32 < i1) loadseg(s + kCovSeg32);
+        // the list pads to whole groups of 4 terms per MFMA
+        if (pending + ((m + 3) & ~3) > FLUSH) flush();
+        pending += (m + 3) & ~3;
+        float swA[2], swB[2], xvA[2][NB], xvB[2][NB];
+        if (m > 0) fetch(s, m, 0, swA, xvA);
+        for (int e0 = 0; e0 < m; e0 += 16) {
+            if (e0 + 8 < m) fetch(s, m, e0 + 8, swB, xvB);
+            products(swA, xvA);
+            if (e0 + 16 < m) fetch(s, m, e0 + 16, swA, xvA);
+            if (e0 + 8 < m) products(swB, xvB);
+        }
+    }
+    flush();
+    if (REG) {
+#pragma unroll
+        for (int u = 0; u < (REG ? NU : 1); u++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) out[(size_t)(U0 + u) * 256 + (4 * g + v) * 16 + c] = accd[u][v];
+    }
+}
+
+template <int NB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_cov32_sparse(
+    const float *__restrict__ X, const float *__restrict__ Rt, int64_t npad, int K, int KP, int d,
+    const double *__restrict__ mu, const double *__restrict__ nk, int64_t chunk_rows, double *__restrict__ cpart) {
+    __shared__ int lidx[4 * kCovSeg32];
+    __shared__ float lrv[4 * kCovSeg32];
+    const int KG = KP / 4;
+    const int nblk = gridDim.x;
+    const int b = blockIdx.x;
+    const int logical = (nblk % 8 == 0) ? (b % 8) * (nblk / 8) + b / 8 : b;
+    constexpr int NPART = cov32_parts(NB);
+    const int part = logical % NPART, rest = logical / NPART;
+    const int cg = rest % KG, chunk = rest / KG;
+    if (part == 0) cov32_sparse<NB, 0>(cg, chunk, X, Rt, npad, K, d, mu, nk, chunk_rows, KP, cpart, lidx, lrv);
+    else if constexpr (NPART > 1)
+        cov32_sparse<NB, 1>(cg, chunk, X, Rt, npad, K, d, mu, nk, chunk_rows, KP, cpart, lidx, lrv);
+}
+
 template <int NB>
 static hipError_t cov32_t(fem_ctx *c) {
     const int blocks = (c->KP / 4) * c->c_chunks * cov32_parts(NB);
-    hipLaunchKernelGGL(k_cov32<NB>, dim3(blocks), dim3(256), 0, c->stream, c->X, c->Rt32, c->npad, c->K,
-                       c->KP, c->d, c->mu, c->nk, c->c_chunk_rows, c->cpart);
+    if (c->cov_sparse)
+        hipLaunchKernelGGL(k_cov32_sparse<NB>, dim3(blocks), dim3(256), 0, c->stream, c->X, c->Rt32, c->npad,
+                           c->K, c->KP, c->d, c->mu, c->nk, c->c_chunk_rows, c->cpart);
+    else
+        hipLaunchKernelGGL(k_cov32<NB>, dim3(blocks), dim3(256), 0, c->stream, c->X, c->Rt32, c->npad, c->K,
+                           c->KP, c->d, c->mu, c->nk, c->c_chunk_rows, c->cpart);
     return hipGetLastError();
 }
 

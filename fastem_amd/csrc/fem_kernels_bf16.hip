@@ -1042,9 +1042,220 @@ __global__ __launch_bounds__(256) void k_covb_wide(const float *__restrict__ X, 
     covb_body<NB>(X, Rt, npad, K, KP, d, mu, nk, chunk_rows, cpart);
 }
 
+// The same over the nonzero posteriors only (the exact-zero argument of
+// k_cov64_sparse): per wave and segment of kCovSegB points the nonzero
+// posteriors are listed in point order in the wave's LDS slice (lane l
+// reads points 4l..4l+3 of each 256), and each 32-entry group of the list is
+// one K = 32 step of the dense form (lane group g takes entries 8g..8g+7 of
+// the group; entries past the end take r = 0), the next group's points
+// loaded during the current group's products.  FP32 accumulators are
+// flushed into FP64 (registers for d <= 48, else the wave's partial slice)
+// before a segment's list would take them past 2048 terms.
+constexpr int kCovSegB = 1024;
+
+template <int NB, int PART>
+__device__ inline void covb_sparse(const float *__restrict__ X, const float *__restrict__ rrow, int64_t i0,
+                                   int64_t i1, const float (&mh)[NB], float inv, int g, int c,
+                                   double *__restrict__ out, int *wi, float *wr) {
+    constexpr int DP = 16 * NB;
+    constexpr int UB = NB * (NB + 1) / 2;
+    constexpr int NPART = cov_parts(NB);
+    constexpr int U0 = cov_part_begin(NB, PART);
+    constexpr int NU = cov_part_size(NB, PART);
+    constexpr int FLUSH = 2048;
+    constexpr bool REG = NB <= 3;
+    const int lane = 16 * g + c;
+    double accd[REG ? NU : 1][4];
+    if (REG) {
+#pragma unroll
+        for (int u = 0; u < (REG ? NU : 1); u++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) accd[u][v] = 0.0;
+    }
+    bool first = true;
+    v4f acc[NU];
+#pragma unroll
+    for (int u = 0; u < NU; u++) acc[u] = (v4f){0.f, 0.f, 0.f, 0.f};
+    int pending = 0;
+    auto flush = [&]() {
+        if (REG) {
+#pragma unroll
+            for (int u = 0; u < (REG ? NU : 1); u++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) accd[u][v] += (double)acc[u][v];
+        } else {
+#pragma unroll
+            for (int u = 0; u < NU; u++) {
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    double *o = out + (size_t)(U0 + u) * 256 + (4 * g + v) * 16 + c;
+                    *o = (first ? 0.0 : *o) + (double)acc[u][v];
+                }
+                if ((u & 3) == 3) asm volatile("" ::: "memory");
+            }
+        }
+        first = false;
+#pragma unroll
+        for (int u = 0; u < NU; u++) acc[u] = (v4f){0.f, 0.f, 0.f, 0.f};
+        pending = 0;
+    };
+
+    constexpr int NJ = kCovSegB / 256;
+    v4f rs[NJ];
+    auto loadseg = [&](int64_t s) {
+#pragma unroll
+        for (int j = 0; j < NJ; j++)
+            rs[j] = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(rrow + s + 256 * j + 4 * lane));
+    };
+    // entries e0 + 8 g + e of the list (called with e0 < m): weights and the
+    // lane's NB features of each point
+    auto fetch = [&](int64_t s, int m, int e0, float (&sw)[8], float (&xv)[NB][8]) {
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int en = e0 + 8 * g + e;
+            const bool ok = en < m;
+            const int ec = ok ? en : 0;
+            const int off = wi[ec];
+            sw[e] = ok ? __builtin_amdgcn_sqrtf(wr[ec]) * inv : 0.0f;
+            float xr[NB];
+            load_consecutive<NB>(X + (size_t)(s + off) * DP + NB * c, xr);
+#pragma unroll
+            for (int fb = 0; fb < NB; fb++) xv[fb][e] = xr[fb];
+        }
+    };
+    auto products = [&](const float (&sw)[8], const float (&xv)[NB][8]) {
+        v8bf th[NB], tl[NB];
+#pragma unroll
+        for (int fb = 0; fb < NB; fb++) {
+            if (!cov_needs_fb(NB, PART, fb)) continue;
+            float t[8];
+#pragma unroll
+            for (int e = 0; e < 8; e++) t[e] = (xv[fb][e] - mh[fb]) * sw[e];
+            split8(t, th[fb], tl[fb]);
+        }
+        int u = 0;
+#pragma unroll
+        for (int ib = 0; ib < NB; ib++)
+#pragma unroll
+            for (int jb = ib; jb < NB; jb++, u++) {
+                if (cov_part_of(u, UB, NPART) != PART) continue;
+                acc[u - U0] = mfma3(th[ib], tl[ib], th[jb], tl[jb], acc[u - U0]);
+            }
+    };
+
+    // chunk bounds are multiples of 4096 points: every segment is whole
+    if (i0 < i1) loadseg(i0);
+    for (int64_t s = i0; s < i1; s += kCovSegB) {
+        int m = 0;
+#pragma unroll
+        for (int j = 0; j < NJ; j++) {
+            bool nz[4];
+            uint64_t b[4];
+            int below = 0;
+#pragma unroll
+            for (int w = 0; w < 4; w++) {
+                nz[w] = rs[j][w] != 0.0f;
+                b[w] = __ballot(nz[w]);
+                below += (int)__builtin_amdgcn_mbcnt_hi((unsigned)(b[w] >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo((unsigned)b[w], 0u));
+            }
+            int at = m + below;
+            const int off = 256 * j + 4 * lane;
+#pragma unroll
+            for (int w = 0; w < 4; w++) {
+                if (nz[w]) {
+                    wi[at] = off + w;
+                    wr[at] = rs[j][w];
+                }
+                at += nz[w];
+                m += __builtin_popcountll(b[w]);
+            }
+        }
+        if (s + kCovSegB < i1) loadseg(s + kCovSegB);
+        const int padded = (m + 31) & ~31;
+        if (pending + padded > FLUSH) flush();
+        pending += padded;
+        float swA[8], swB[8], xvA[NB][8], xvB[NB][8];
+        if (m > 0) fetch(s, m, 0, swA, xvA);
+        for (int e0 = 0; e0 < m; e0 += 64) {
+            if (e0 + 32 < m) fetch(s, m, e0 + 32, swB, xvB);
+            products(swA, xvA);
+            if (e0 + 64 < m) fetch(s, m, e0 + 64, swA, xvA);
+            if (e0 + 32 < m) products(swB, xvB);
+        }
+    }
+    flush();
+    if (REG) {
+#pragma unroll
+        for (int u = 0; u < (REG ? NU : 1); u++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) out[(size_t)(U0 + u) * 256 + (4 * g + v) * 16 + c] = accd[u][v];
+    }
+}
+
+template <int NB>
+__device__ __attribute__((always_inline)) inline void covb_sparse_body(
+    const float *__restrict__ X, const float *__restrict__ Rt, int64_t npad, int K, int KP, int d,
+    const double *__restrict__ mu, const double *__restrict__ nk, int64_t chunk_rows, double *__restrict__ cpart,
+    int *lidx, float *lrv) {
+    constexpr int UB = NB * (NB + 1) / 2;
+    constexpr int NPART = cov_parts(NB);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int g = lane >> 4, c = lane & 15;
+    const int KG = KP / 4;
+    const int nblk = gridDim.x;
+    const int b = blockIdx.x;
+    const int logical = (nblk % 8 == 0) ? (b % 8) * (nblk / 8) + b / 8 : b;
+    const int part = logical % NPART, rest = logical / NPART;
+    const int cg = rest % KG, chunk = rest / KG;
+    const int k = 4 * cg + wave;
+    if (k >= K) return;
+    const int64_t i0 = (int64_t)chunk * chunk_rows;
+    const int64_t i1 = (i0 + chunk_rows < npad) ? i0 + chunk_rows : npad;
+    float mh[NB];
+#pragma unroll
+    for (int fb = 0; fb < NB; fb++) mh[fb] = (NB * c + fb < d) ? (float)mu[(size_t)k * d + NB * c + fb] : 0.0f;
+    const float inv = (float)(1.0 / sqrt(nk[k]));
+    double *out = cpart + ((size_t)chunk * KP + k) * UB * 256;
+    const float *rrow = Rt + (size_t)k * npad;
+    int *wi = lidx + wave * kCovSegB;
+    float *wr = lrv + wave * kCovSegB;
+    if (part == 0) covb_sparse<NB, 0>(X, rrow, i0, i1, mh, inv, g, c, out, wi, wr);
+    else if constexpr (NPART > 1) covb_sparse<NB, 1>(X, rrow, i0, i1, mh, inv, g, c, out, wi, wr);
+}
+
+// registers capped for two waves per SIMD for d <= 64, the compiler's
+// allocation above (as k_covb and k_covb_wide)
+template <int NB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_covb_sparse(
+    const float *__restrict__ X, const float *__restrict__ Rt, int64_t npad, int K, int KP, int d,
+    const double *__restrict__ mu, const double *__restrict__ nk, int64_t chunk_rows, double *__restrict__ cpart) {
+    __shared__ int lidx[4 * kCovSegB];
+    __shared__ float lrv[4 * kCovSegB];
+    covb_sparse_body<NB>(X, Rt, npad, K, KP, d, mu, nk, chunk_rows, cpart, lidx, lrv);
+}
+
+template <int NB>
+__global__ __launch_bounds__(256) void k_covb_sparse_wide(
+    const float *__restrict__ X, const float *__restrict__ Rt, int64_t npad, int K, int KP, int d,
+    const double *__restrict__ mu, const double *__restrict__ nk, int64_t chunk_rows, double *__restrict__ cpart) {
+    __shared__ int lidx[4 * kCovSegB];
+    __shared__ float lrv[4 * kCovSegB];
+    covb_sparse_body<NB>(X, Rt, npad, K, KP, d, mu, nk, chunk_rows, cpart, lidx, lrv);
+}
+
 template <int NB>
 static hipError_t covb_t(fem_ctx *c) {
     const int blocks = (c->KP / 4) * c->c_chunks * cov_parts(NB);
+    if (c->cov_sparse) {
+        if constexpr (NB <= 4)
+            hipLaunchKernelGGL(k_covb_sparse<NB>, dim3(blocks), dim3(256), 0, c->stream, c->X, c->Rt32, c->npad, c->K,
+                               c->KP, c->d, c->mu, c->nk, c->c_chunk_rows, c->cpart);
+        else
+            hipLaunchKernelGGL(k_covb_sparse_wide<NB>, dim3(blocks), dim3(256), 0, c->stream, c->X, c->Rt32, c->npad,
+                               c->K, c->KP, c->d, c->mu, c->nk, c->c_chunk_rows, c->cpart);
+        return hipGetLastError();
+    }
     const size_t lds = (size_t)4 * cov_lds_doubles(NB) * sizeof(double);
     auto kern = NB <= 4 ? k_covb<NB> : k_covb_wide<NB>;
     if (lds > 64 * 1024) {

@@ -99,6 +99,58 @@ def test_estep_mstep_parity(orc, n, d, K):
         assert np.array_equal(cov2[k], cov2[k].T)
 
 
+def mstep_with(X, w, mu, cov, dense):
+    """Posteriors and M-step of one FP64 iteration; dense=True sums the
+    covariances over every point (FASTEM_COV_DENSE=1), else over the points
+    with a nonzero posterior only (the default)."""
+    import os
+    n, d = X.shape
+    if dense:
+        os.environ["FASTEM_COV_DENSE"] = "1"
+    try:
+        with fem.Context(n, d, len(w)) as ctx:
+            ctx.upload(X)
+            ctx.set_params(w, mu, cov)
+            ctx.factor()
+            ctx.estep()
+            R = ctx.posteriors()
+            return R, ctx.mstep()
+    finally:
+        os.environ.pop("FASTEM_COV_DENSE", None)
+
+
+@pytest.mark.parametrize("n,d,K,order", [(30000, 16, 8, "random"), (30000, 64, 12, "random"),
+                                         (20000, 100, 6, "random"), (12000, 128, 4, "random"),
+                                         (30000, 64, 12, "by_class"), (9000, 33, 5, "by_class")])
+def test_cov_over_nonzero_posteriors(orc, n, d, K, order):
+    """Separated mixtures (means U[-40, 40], started from the generating
+    parameters): most posteriors are exactly zero and the covariance pass
+    sums over the rest.  Covariances within 1e-10 of the oracle and 1e-13 of
+    the dense sum over every point.  by_class orders the points by their
+    largest posterior, so most 1024-point segments list no point of a given
+    component and the rest list every point."""
+    say(f"cov over nonzero posteriors n={n} d={d} K={K} order={order}")
+    X, w, mu, cov = fem.gengmm(n, d, K, -40.0, 40.0, 0.05, 700 + d + K)
+    U, ld = orc.factor_all(cov)
+    if order == "by_class":
+        _, R0 = orc.estep(X, w, mu, U, ld)
+        X = np.ascontiguousarray(X[np.argsort(R0.argmax(1), kind="stable")])
+    ll_ref, R_ref = orc.estep(X, w, mu, U, ld)
+    w_ref, mu_ref, cov_ref = orc.mstep(X, R_ref)
+    R, (w2, mu2, cov2) = mstep_with(X, w, mu, cov, dense=False)
+    _, (_, _, cov_dense) = mstep_with(X, w, mu, cov, dense=True)
+    zero = np.mean(R == 0)
+    say("zero posteriors", zero, "cov rel err vs oracle", rel_err(cov2, cov_ref),
+        "vs dense", rel_err(cov2, cov_dense))
+    assert zero > 0.6
+    assert rel_err(w2, w_ref) < PARAM_RTOL
+    assert rel_err(mu2, mu_ref) < PARAM_RTOL
+    assert rel_err(cov2, cov_ref) < PARAM_RTOL
+    assert rel_err(cov2, cov_dense) < 1e-13
+    for k in range(K):
+        assert np.array_equal(cov2[k], cov2[k].T)
+
+
 @pytest.mark.parametrize("g", [0.0, 0.01])
 def test_trajectory_config1(orc, g):
     """N=100,000, d=16, K=8, means U[-10,10], cov AA^T/d + 0.1 I, k-means++

@@ -88,6 +88,54 @@ def test_trajectory(orc, prec):
     assert err.max() <= TOL[prec]["traj"]
 
 
+def one_step(X, w, mu, cov, prec, dense):
+    """Posteriors and M-step of one step; dense=True sums the covariances
+    over every point (FASTEM_COV_DENSE=1), else over the nonzero posteriors."""
+    import os
+    n, d = X.shape
+    if dense:
+        os.environ["FASTEM_COV_DENSE"] = "1"
+    try:
+        with fem.Context(n, d, len(w), prec) as ctx:
+            ctx.upload(X)
+            ctx.set_params(w, mu, cov)
+            ctx.factor()
+            ctx.estep()
+            return ctx.posteriors(), ctx.mstep()
+    finally:
+        os.environ.pop("FASTEM_COV_DENSE", None)
+
+
+@pytest.mark.parametrize("prec", [fem.PREC_FP32, fem.PREC_BF16X3])
+@pytest.mark.parametrize("n,d,K,order", [(30000, 64, 12, "random"), (30000, 64, 12, "by_class"),
+                                         (20000, 100, 6, "random"), (12000, 128, 4, "by_class"),
+                                         (9000, 33, 5, "random"), (20000, 8, 40, "by_class")])
+def test_cov_over_nonzero_posteriors(orc, prec, n, d, K, order):
+    """Separated mixtures started from the generating parameters: most
+    posteriors are exactly zero and the covariance pass sums over the rest.
+    Covariances within the form's tolerance of the oracle and of the dense
+    sum over every point.  by_class orders the points by their largest
+    posterior, so most segments list no point of a given component."""
+    print(f"{NAME[prec]} cov over nonzero posteriors n={n} d={d} K={K} order={order}", flush=True)
+    X, w, mu, cov = fem.gengmm(n, d, K, -40.0, 40.0, 0.05, 900 + d + K)
+    U, ld = orc.factor_all(cov)
+    if order == "by_class":
+        _, R0 = orc.estep(X, w, mu, U, ld)
+        X = np.ascontiguousarray(X[np.argsort(R0.argmax(1), kind="stable")])
+    _, R_ref = orc.estep(X, w, mu, U, ld)
+    _, _, cov_ref = orc.mstep(X, R_ref)
+    R, (_, _, cov_s) = one_step(X, w, mu, cov, prec, dense=False)
+    _, (_, _, cov_d) = one_step(X, w, mu, cov, prec, dense=True)
+    zero = float(np.mean(R == 0))
+    print(NAME[prec], "zero posteriors", zero, "cov vs oracle", rel(cov_s, cov_ref), "vs dense", rel(cov_s, cov_d),
+          flush=True)
+    assert zero > 0.6
+    assert rel(cov_s, cov_ref) <= TOL[prec]["cov"]
+    assert rel(cov_s, cov_d) <= TOL[prec]["cov"]
+    for k in range(K):
+        assert np.array_equal(cov_s[k], cov_s[k].T)
+
+
 def test_bf16x3_d128_k128_per_iteration_errors(orc):
     """The d = 128, K = 128 precision case at a size the oracle finishes in
     seconds: both forms step from the same start (the generating mixture with
