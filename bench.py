@@ -187,6 +187,8 @@ def batch_workload(nfits=256, seed=7, maxsweeps=100, eps=1e-6, g=1e-3, compare=F
            "seconds": t_run, "tune_seconds": t_tune, "init_seconds_untimed": t_init,
            "fit_iterations": int(iters.sum()),
            "fit_iterations_per_s": float(iters.sum() / t_run), "gflops_algorithmic": flop / t_run / 1e9,
+           "gflops_note": "dense-equivalent: the contraction flops of every (point, component) pair, "
+                          "while the covariance pass sums over the nonzero posteriors only",
            "converged": int(sum(r["status"] == fem.Batch.CONVERGED for r in res)),
            "failed": int(sum(r["status"] == fem.Batch.FAILED for r in res)),
            "bic_true_k_recovered": f"{bic_ok}/{scored}",
@@ -357,7 +359,7 @@ def main():
     w0, mu0, cov0 = fem.kmeanspp_init(X, K, seed=1, lloyd_iters=29)
     t_init = time.perf_counter() - t_init
 
-    def run_form(prec, steps, warmup):
+    def run_form(prec, steps, warmup, nonzero=False):
         ctx = fem.Context(N, D, K, prec, 0)
         ctx.upload(X)
         ctx.set_params(w0, mu0, cov0)
@@ -379,16 +381,29 @@ def main():
         for _ in range(steps):
             recs.append(step())
         wall = time.perf_counter() - t0
+        # fraction of the last E-step's posteriors that are nonzero: the
+        # (point, component) pairs the covariance pass sums over
+        nz = float(np.count_nonzero(ctx.posteriors())) / (N * K) if nonzero else None
         ctx.close()
         ph = {name: float(np.mean([r[i] for r in recs]))
               for i, name in ((1, "estep"), (2, "means"), (3, "cov"), (4, "factor"))}
         return {"iters_per_s": steps / wall, "ms_per_step": 1e3 * wall / steps,
-                "phases_ms": ph, "ll": [recs[0][0], recs[-1][0]]}
+                "phases_ms": ph, "ll": [recs[0][0], recs[-1][0]], "nonzero": nz}
 
     log("fp64 form")
-    r64 = run_form(fem.PREC_FP64, args.steps, args.warmup)
+    r64 = run_form(fem.PREC_FP64, args.steps, args.warmup, nonzero=True)
     variants = {}
     if not args.no_variants:
+        # the same form with the covariances summed over every point (zero
+        # posteriors included) instead of over the nonzero ones
+        log("fp64 form, dense covariance sum")
+        os.environ["FASTEM_COV_DENSE"] = "1"
+        try:
+            rd = run_form(fem.PREC_FP64, args.steps, args.warmup)
+        finally:
+            del os.environ["FASTEM_COV_DENSE"]
+        variants["fp64_dense_cov"] = {"iters_per_s": rd["iters_per_s"], "ms_per_step": rd["ms_per_step"],
+                                      "phases_ms": rd["phases_ms"], "loglik_first_last": rd["ll"]}
         log("fp32 form")
         r32 = run_form(fem.PREC_FP32, args.steps, args.warmup)
         variants["fp32"] = {
@@ -442,7 +457,12 @@ def main():
                                      "(profiles/hbm_traffic_n4194304_d64_k64.txt); algorithmic "
                                      "bytes: X read 1.07e9 + FP64 posteriors written 2.15e9"},
         "phases_ms": ph,
-        "phase_tflops": {k2: fl[k2] / ph[k2] / 1e9 for k2 in ("estep", "means", "cov")},
+        "phase_tflops": {"estep": fl["estep"] / ph["estep"] / 1e9, "means": fl["means"] / ph["means"] / 1e9,
+                         "cov_over_nonzero_posteriors": fl["cov"] * r64["nonzero"] / ph["cov"] / 1e9},
+        "nonzero_posterior_fraction": r64["nonzero"],
+        "cov_note": "the covariance pass sums over the (point, component) pairs with a nonzero "
+                    "posterior only (a zero posterior adds exactly zero); variants.fp64_dense_cov "
+                    "is the same iteration summing over every pair",
         "loglik_first_last": r64["ll"],
         "variants": variants,
         "init_seconds": t_init,

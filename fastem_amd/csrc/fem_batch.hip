@@ -19,10 +19,12 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
 
+#include "fem_cov64_kernel.h"
 #include "fem_internal.h"
 #include "fem_tiles.h"
 
@@ -87,6 +89,8 @@ struct fem_batch {
     // run parameters
     int maxsweeps = 100;
     double eps = 1e-5, g = 0.0;
+    // covariances over the nonzero posteriors only (FASTEM_COV_DENSE=1: every point)
+    bool cov_sparse = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::string err;
@@ -216,6 +220,23 @@ __global__ __launch_bounds__(256) void kb_cov(const BFit *__restrict__ F, const 
     }
 }
 
+// The same over each component's nonzero posteriors (the single fit's
+// k_cov64_sparse body, d <= 64), with its 48 KB of LDS lists.
+template <int NB>
+__global__ __launch_bounds__(256) void kb_cov_sparse(const BFit *__restrict__ F, const BState *__restrict__ S,
+                                                     const float *__restrict__ xp, double *__restrict__ dp,
+                                                     const Item *__restrict__ items, int nitems) {
+    __shared__ int lidx[4 * kCovSeg];
+    __shared__ double lrv[4 * kCovSeg];
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const Item itm = items[it];
+        if (!S[itm.fit].active) continue;
+        const BFit &f = F[itm.fit];
+        cov64_sparse<NB, 1, 0>(itm.a, itm.b, xp + f.x, dp + f.r, f.npad, f.K, f.d, dp + f.mu, dp + f.nk,
+                               f.c_chunk_rows, f.KP, dp + f.cpart, lidx, lrv);
+    }
+}
+
 __global__ __launch_bounds__(256) void kb_cov_reduce(const BFit *__restrict__ F, const BState *__restrict__ S,
                                                      double *__restrict__ dp, const Item *__restrict__ items,
                                                      int nitems) {
@@ -335,6 +356,15 @@ void launch_estep_nb(fem_batch *b, int nb, const Item *items, int n, int mult) {
 void launch_cov_nb(fem_batch *b, int nb, const Item *items, int n, int mult) {
     if (n <= 0) return;
     const int grid = grid_of(mult, b->cus, n);
+    if (b->cov_sparse) {
+        switch (nb) {
+            case 1: hipLaunchKernelGGL(fem::kb_cov_sparse<1>, dim3(grid), dim3(256), 0, b->stream, b->dfits, b->dstate, b->xpool, b->dpool, items, n); break;
+            case 2: hipLaunchKernelGGL(fem::kb_cov_sparse<2>, dim3(grid), dim3(256), 0, b->stream, b->dfits, b->dstate, b->xpool, b->dpool, items, n); break;
+            case 3: hipLaunchKernelGGL(fem::kb_cov_sparse<3>, dim3(grid), dim3(256), 0, b->stream, b->dfits, b->dstate, b->xpool, b->dpool, items, n); break;
+            case 4: hipLaunchKernelGGL(fem::kb_cov_sparse<4>, dim3(grid), dim3(256), 0, b->stream, b->dfits, b->dstate, b->xpool, b->dpool, items, n); break;
+        }
+        return;
+    }
     switch (nb) {
         case 1: hipLaunchKernelGGL(fem::kb_cov<1>, dim3(grid), dim3(256), 0, b->stream, b->dfits, b->dstate, b->xpool, b->dpool, items, n); break;
         case 2: hipLaunchKernelGGL(fem::kb_cov<2>, dim3(grid), dim3(256), 0, b->stream, b->dfits, b->dstate, b->xpool, b->dpool, items, n); break;
@@ -486,6 +516,7 @@ int fem_batch_create(fem_batch **out, int nfits, const int64_t *n, const int *d,
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
     b->cus = cus;
+    if (const char *e = std::getenv("FASTEM_COV_DENSE")) b->cov_sparse = std::atoi(e) == 0;
     std::vector<Item> means, comp, covred;
     for (int i = 0; i < nfits; i++) {
         BFit f{};
