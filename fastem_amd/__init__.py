@@ -50,6 +50,7 @@ _SIGS = {
     "fem_get_posteriors": (ctypes.c_int, [_vp, _dp, _i64]),
     "fem_last_times": (ctypes.c_int, [_vp, _fp, _fp, _fp, _fp]),
     "fem_last_estep_form": (ctypes.c_int, [_vp, _ip]),
+    "fem_last_estep_skipped": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double)]),
     "fem_kmeans_init": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, _dp, _dp,
                                        _dp]),
     "fem_kmeans_init_rng": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _dp, _dp,
@@ -245,6 +246,14 @@ class Context:
         self._check(lib().fem_last_times(self._h, *[ctypes.byref(x) for x in t]))
         return {"estep_ms": t[0].value, "means_ms": t[1].value, "cov_ms": t[2].value,
                 "factor_ms": t[3].value}
+
+    def estep_skipped(self):
+        """Share of the (wave, component) evaluations the last FP64 E-step
+        skipped as provably beyond exp's underflow range (0 for other forms
+        and for contexts with K > 64, d <= 16 or FASTEM_ESTEP_DENSE=1)."""
+        f = ctypes.c_double()
+        self._check(lib().fem_last_estep_skipped(self._h, ctypes.byref(f)))
+        return f.value
 
     def estep_form(self):
         """Kernel of the last estep(): "fp64", "fp32", "bf16x3" (centred on

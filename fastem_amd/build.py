@@ -12,7 +12,7 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libfastem.so")
 
 HIP_SOURCES = ["fem_kernels.hip", "fem_cov64.hip", "fem_cov64_narrow.hip", "fem_kernels32.hip", "fem_kernels_bf16.hip", "fem_kmeans.hip",
-               "fem_batch.hip", "fem_setnet.hip", "fem_abi.hip", "fem_noalloc.hip"]
+               "fem_batch.hip", "fem_setnet.hip", "fem_abi.hip", "fem_noalloc.hip", "fem_sort.hip"]
 HOST_SOURCES = ["fem_tools.cpp", "host/linalg.cpp", "host/rand.cpp", "host/block_manager.cpp",
                 "host/normal_density.cpp", "host/gaussian_mixture.cpp", "host/dataset.cpp",
                 "host/em_algorithm.cpp", "host/em_initializer.cpp", "host/setnet_init.cpp"]

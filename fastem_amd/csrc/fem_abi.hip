@@ -132,7 +132,8 @@ hipError_t dmalloc(T **p, size_t count) {
 void free_all(fem_ctx *c) {
     void *ptrs[] = {c->X, c->Rt, c->Rt32, c->w, c->mu, c->cov, c->nk, c->logdet, c->info, c->coef,
                     c->U, c->stage64, c->stage32, c->stageb, c->stager, c->refc, c->ll_part, c->ll_dev,
-                    c->mpart, c->cpart, c->kC, c->dmin, c->cls, c->kfit, c->kpick, c->kcand, c->kpot, c->korder};
+                    c->mpart, c->cpart, c->kC, c->dmin, c->cls, c->kfit, c->kpick, c->kcand, c->kpot, c->korder,
+                    c->Xs, c->perm, c->perm2, c->ref, c->ref2, c->slots, c->rel, c->sort_tmp, c->nskip};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &e : c->ev)
@@ -209,6 +210,8 @@ int fem_create(fem_ctx **out, int64_t n, int d, int K, int precision, int device
     if (const char *e = std::getenv("FASTEM_ESTEP32_GROUPS")) c->estep_groups32 = std::atoi(e) == 4 ? 4 : 2;
     if (const char *e = std::getenv("FASTEM_ESTEPB_GROUPS")) c->estep_groupsb = std::atoi(e) == 2 ? 2 : 4;
     if (const char *e = std::getenv("FASTEM_COV_DENSE")) c->cov_sparse = std::atoi(e) == 0;
+    if (const char *e = std::getenv("FASTEM_ESTEP_DENSE")) c->prune = std::atoi(e) == 0;
+    c->prune = c->prune && K <= 64 && c->NB >= 2;
     c->m_chunks = (int)(c->npad / 4096);
     // covariance chunks: at most 256 chunks of a multiple of 4096 rows
     int64_t units = c->npad / 4096;
@@ -255,6 +258,12 @@ int fem_create(fem_ctx **out, int64_t n, int d, int K, int precision, int device
         (e = dmalloc(&c->mpart, (size_t)c->m_chunks * c->KP * (c->dpad + 1))) != hipSuccess ||
         (e = dmalloc(&c->cpart, (size_t)c->c_chunks * c->KP * ub * 256)) != hipSuccess)
         return bail(e, "hipMalloc(partials)");
+    // the pruned FP64 E-step's per-slot components (none yet: -1) and its
+    // skip count; the slot order's buffers come with the first sort
+    if (c->prune && ((e = dmalloc(&c->ref, (size_t)c->npad)) != hipSuccess ||
+                     (e = dmalloc(&c->nskip, 1)) != hipSuccess ||
+                     (e = hipMemsetAsync(c->ref, 0xFF, (size_t)c->npad * sizeof(int), c->stream)) != hipSuccess))
+        return bail(e, "hipMalloc(pruning)");
     // the E-step's component order: the index order until the first
     // factorisation ranks the weights
     std::vector<int> ident(K);
@@ -291,6 +300,7 @@ static int upload(fem_ctx *c, const float *X, int64_t ld, hipMemcpyKind kind) {
                                     (size_t)c->d * sizeof(float), (size_t)c->n, kind, c->stream));
     FEM_HIP(c, hipStreamSynchronize(c->stream));
     c->ref_ok = false;
+    fem::reset_order(c);
     return FEM_OK;
 }
 
@@ -360,6 +370,10 @@ int fem_set_params(fem_ctx *c, const double *w, const double *mu, const double *
     c->have_params = true;
     c->factored = false;
     c->ref_ok = false;
+    // the slot order stays (any order gives the same results); it is
+    // re-sorted after the next pruned E-steps
+    c->esteps_since_sort = 0;
+    c->sort_every = 1;
     return FEM_OK;
 }
 
@@ -478,10 +492,31 @@ int fem_get_classes(fem_ctx *c, int *cls) {
     int *tmp = nullptr;
     FEM_HIP(c, hipMalloc((void **)&tmp, (size_t)c->n * sizeof(int)));
     hipError_t e = fem::launch_classes(c, tmp);
-    if (e == hipSuccess) e = hipMemcpyAsync(cls, tmp, (size_t)c->n * sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    int *pt = nullptr;  // point order of the slots' classes
+    if (e == hipSuccess && c->sorted) {
+        e = hipMalloc((void **)&pt, (size_t)c->n * sizeof(int));
+        if (e == hipSuccess) e = fem::launch_unpermute_int(c, tmp, pt);
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(cls, pt ? pt : tmp, (size_t)c->n * sizeof(int), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(tmp);
+    if (pt) (void)hipFree(pt);
     if (e != hipSuccess) return fail(c, FEM_ERR_HIP, std::string("fem_get_classes: ") + hipGetErrorString(e));
+    return FEM_OK;
+}
+
+int fem_last_estep_skipped(fem_ctx *c, double *frac) {
+    if (int r = check_ctx(c)) return r;
+    if (!frac) return fail(c, FEM_ERR_ARG, "null pointer");
+    *frac = 0.0;
+    if (!c->nskip || !c->refs_valid || c->prec != FEM_PREC_FP64) return FEM_OK;
+    if (int r = set_device(c)) return r;
+    unsigned long long sk = 0;
+    FEM_HIP(c, hipMemcpyAsync(&sk, c->nskip, sizeof(sk), hipMemcpyDeviceToHost, c->stream));
+    FEM_HIP(c, hipStreamSynchronize(c->stream));
+    const double evals = (double)(c->npad / (16 * c->estep_groups)) * c->K;  // waves x components
+    *frac = (double)sk / evals;
     return FEM_OK;
 }
 

@@ -33,7 +33,8 @@ __global__ __launch_bounds__(256) void k_cov64(const float *__restrict__ X,
 // of component k is zero adds exactly zero to component k's sum (w t = 0 in
 // each of its products), and in a separated mixture most posteriors are zero:
 // exp underflows to +0 for log-density gaps beyond about 745.  Per wave
-// (component k) and segment of kCovSeg points of its chunk, the segment's
+// (component k) and segment of kCovSeg points of its chunk (segments
+// chunk, chunk + chunks, ... of all points), the segment's
 // posteriors are read once (lane l: points 2l and 2l + 1 of each 128, one
 // 16-byte load), the points with r != 0 are listed in the wave's LDS slice
 // in point order (ballots and mbcnt), and the products of the dense form run
@@ -42,7 +43,7 @@ __global__ __launch_bounds__(256) void k_cov64(const float *__restrict__ X,
 // that of the nonzero terms, grouped 4 at a time in list order.  The next
 // segment's posteriors are in flight during the current segment's products,
 // and the next step's points during the current step's.
-constexpr int kCovSeg = 1024;
+constexpr int kCovSeg = 256;
 
 typedef double v2d __attribute__((ext_vector_type(2)));
 
@@ -59,8 +60,11 @@ __device__ inline void cov64_sparse(int cg, int chunk, const float *__restrict__
     const int g = lane >> 4, c = lane & 15;
     const int k = 4 * cg + wave;
     if (k >= K) return;
-    const int64_t i0 = (int64_t)chunk * chunk_rows;
-    const int64_t i1 = (i0 + chunk_rows < npad) ? i0 + chunk_rows : npad;
+    // the chunk's segments are strided over the points (segments chunk,
+    // chunk + chunks, ...), so that in a sorted slot order, where each
+    // component's nonzero posteriors are contiguous, every chunk (and so
+    // every workgroup) takes an equal share of them
+    const int64_t nseg = npad / kCovSeg, nch = (npad + chunk_rows - 1) / chunk_rows;
     double muv[NB];
 #pragma unroll
     for (int fb = 0; fb < NB; fb++) muv[fb] = (NB * c + fb < d) ? mu[(size_t)k * d + NB * c + fb] : 0.0;
@@ -116,9 +120,10 @@ __device__ inline void cov64_sparse(int cg, int chunk, const float *__restrict__
         }
     };
 
-    // chunk bounds are multiples of 4096 points: every segment is whole
-    if (i0 < i1) loadseg(i0);
-    for (int64_t s = i0; s < i1; s += kCovSeg) {
+    // npad is a multiple of 4096 points: every segment is whole
+    if (chunk < nseg) loadseg((int64_t)chunk * kCovSeg);
+    for (int64_t sg = chunk; sg < nseg; sg += nch) {
+        const int64_t s = sg * kCovSeg;
         int m = 0;
 #pragma unroll
         for (int j = 0; j < NJ; j++) {
@@ -138,7 +143,7 @@ __device__ inline void cov64_sparse(int cg, int chunk, const float *__restrict__
             }
             m += __builtin_popcountll(b0) + __builtin_popcountll(b1);
         }
-        if (s + kCovSeg < i1) loadseg(s + kCovSeg);
+        if (sg + nch < nseg) loadseg((sg + nch) * kCovSeg);
         double rvA[2], rvB[2];
         float xvA[2][NB], xvB[2][NB];
         if (m > 0) fetch(s, m, 0, rvA, xvA);
@@ -188,10 +193,10 @@ template <int NB>
 inline hipError_t cov_t(fem_ctx *c) {
     const int blocks = (c->KP / 4) * c->c_chunks * cov64_parts(NB);
     if (c->cov_sparse)
-        hipLaunchKernelGGL(k_cov64_sparse<NB>, dim3(blocks), dim3(256), 0, c->stream, c->X, c->Rt, c->npad,
+        hipLaunchKernelGGL(k_cov64_sparse<NB>, dim3(blocks), dim3(256), 0, c->stream, c->Xe(), c->Rt, c->npad,
                            c->K, c->KP, c->d, c->mu, c->nk, c->c_chunk_rows, c->cpart);
     else
-        hipLaunchKernelGGL(k_cov64<NB>, dim3(blocks), dim3(256), 0, c->stream, c->X, c->Rt, c->npad,
+        hipLaunchKernelGGL(k_cov64<NB>, dim3(blocks), dim3(256), 0, c->stream, c->Xe(), c->Rt, c->npad,
                            c->K, c->KP, c->d, c->mu, c->nk, c->c_chunk_rows, c->cpart);
     return hipGetLastError();
 }

@@ -85,6 +85,30 @@ struct fem_ctx {
     int64_t c_chunk_rows = 0;
     // covariances over the nonzero posteriors only (FASTEM_COV_DENSE=1: every point)
     bool cov_sparse = true;
+
+    // FP64 E-step pruning (K <= 64, d > 16; FASTEM_ESTEP_DENSE=1 turns it
+    // off): the EM kernels run over the points in slot order, Xs[s] = X[perm[s]]
+    // once the points are sorted by the component of their largest
+    // log-density (ref, per slot), so a workgroup's points share few
+    // components; the E-step takes those first and skips, per wave, every
+    // other component whose first 16 whitened coordinates already put all of
+    // the wave's points beyond exp's underflow range.  Posteriors and classes
+    // leave the context in point order.
+    bool prune = true;
+    bool sorted = false;        // Xs and perm hold a sorted order
+    float *Xs = nullptr;        // npad x dpad, X in slot order
+    int *perm = nullptr;        // npad: point of each slot
+    int *perm2 = nullptr;       // npad: next permutation (sort output)
+    int *ref = nullptr;         // npad: component of largest log-density per slot (K: none)
+    int *ref2 = nullptr;        // npad: sorted keys
+    int *slots = nullptr;       // npad: 0..npad-1 (sort values in)
+    int *rel = nullptr;         // npad: old slot of each new slot (sort values out)
+    void *sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    unsigned long long *nskip = nullptr;  // wave-component evaluations skipped by the last FP64 E-step
+    int esteps_since_sort = 0, sort_every = 1;
+    bool refs_valid = false;    // ref was written by a pruned E-step on the current order
+    const float *Xe() const { return sorted ? Xs : X; }
     double *mpart = nullptr;    // m_chunks x KP x (dpad+1)
     double *cpart = nullptr;    // c_chunks x KP x ublocks x 256
 
@@ -115,6 +139,10 @@ hipError_t launch_cov(fem_ctx *c);
 hipError_t launch_zero(double *p, int64_t n, hipStream_t s);
 hipError_t launch_transpose_post(const fem_ctx *c, double *out_dev, int64_t ld);
 hipError_t launch_classes(const fem_ctx *c, int *cls_dev);
+// the pruned FP64 E-step's slot order (fem_sort.hip)
+hipError_t sort_points(fem_ctx *c);
+void reset_order(fem_ctx *c);
+hipError_t launch_unpermute_int(const fem_ctx *c, const int *in, int *out);
 // Row copies between pageable host memory and pitched device memory (bytes),
 // staged through two pinned buffers so the DMA engine moves large contiguous
 // chunks while the host packs or unpacks rows; device pad bytes beyond

@@ -73,39 +73,46 @@ hipError_t launch_factor(fem_ctx *c) {
 // allocation, 206 VGPRs at d = 64, leaves two; capped, a few values spill
 // to scratch and the E-step measured 2-3% faster at d = 32 and 64).  Wider
 // d keeps the compiler's allocation (the cap spilled heavily at d = 128).
-template <int NB, int P>
+template <int NB, int P, bool PRUNE = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_estep64_w3(
     const float *__restrict__ X, int64_t n, int64_t npad, int K, const double *__restrict__ stage,
     const double *__restrict__ coef, double *__restrict__ Rt, double *__restrict__ ll_part,
-    const int *__restrict__ order) {
+    const int *__restrict__ order, int *__restrict__ ref, unsigned long long *__restrict__ nskip) {
     extern __shared__ double lds[];  // estep64_lds<NB>()
     __shared__ double wsum[4];
-    estep64_unit<NB, P>(blockIdx.x, X, n, npad, K, stage, coef, Rt, ll_part + blockIdx.x, lds, wsum, order);
+    __shared__ int lint[128];
+    estep64_unit<NB, P, PRUNE>(blockIdx.x, X, n, npad, K, stage, coef, Rt, ll_part + blockIdx.x, lds, wsum, order,
+                               ref, lint, nskip);
 }
 
-template <int NB, int P>
+template <int NB, int P, bool PRUNE = false>
 __global__ __launch_bounds__(256) void k_estep64(const float *__restrict__ X, int64_t n,
                                                  int64_t npad, int K,
                                                  const double *__restrict__ stage,
                                                  const double *__restrict__ coef,
                                                  double *__restrict__ Rt,
                                                  double *__restrict__ ll_part,
-                                                 const int *__restrict__ order) {
+                                                 const int *__restrict__ order, int *__restrict__ ref,
+                                                 unsigned long long *__restrict__ nskip) {
     extern __shared__ double lds[];  // estep64_lds<NB>()
     __shared__ double wsum[4];
-    estep64_unit<NB, P>(blockIdx.x, X, n, npad, K, stage, coef, Rt, ll_part + blockIdx.x, lds, wsum, order);
+    __shared__ int lint[128];
+    estep64_unit<NB, P, PRUNE>(blockIdx.x, X, n, npad, K, stage, coef, Rt, ll_part + blockIdx.x, lds, wsum, order,
+                               ref, lint, nskip);
 }
 
 // d = 65..80: LDS leaves room for two workgroups per CU; registers capped
 // so that two waves per SIMD fit
-template <int NB, int P>
+template <int NB, int P, bool PRUNE = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_estep64_w2(
     const float *__restrict__ X, int64_t n, int64_t npad, int K, const double *__restrict__ stage,
     const double *__restrict__ coef, double *__restrict__ Rt, double *__restrict__ ll_part,
-    const int *__restrict__ order) {
+    const int *__restrict__ order, int *__restrict__ ref, unsigned long long *__restrict__ nskip) {
     extern __shared__ double lds[];  // estep64_lds<NB>()
     __shared__ double wsum[4];
-    estep64_unit<NB, P>(blockIdx.x, X, n, npad, K, stage, coef, Rt, ll_part + blockIdx.x, lds, wsum, order);
+    __shared__ int lint[128];
+    estep64_unit<NB, P, PRUNE>(blockIdx.x, X, n, npad, K, stage, coef, Rt, ll_part + blockIdx.x, lds, wsum, order,
+                               ref, lint, nskip);
 }
 
 // Deterministic sum of n doubles into *out: fixed strided partials, then a
@@ -117,30 +124,46 @@ __global__ __launch_bounds__(256) void k_reduce_sum(const double *__restrict__ p
     if (threadIdx.x == 0) *out = r;
 }
 
-template <int NB, int P>
-static hipError_t estep64_t(fem_ctx *c) {
+template <int NB, int P, bool PRUNE>
+static hipError_t estep64_launch(fem_ctx *c, int *ref) {
     const size_t lds = (size_t)estep64_lds<NB>() * sizeof(double);
     const void *kern;
-    if constexpr (NB <= 4) kern = (const void *)k_estep64_w3<NB, P>;
-    else if constexpr (NB == 5) kern = (const void *)k_estep64_w2<NB, P>;
-    else kern = (const void *)k_estep64<NB, P>;
+    if constexpr (NB <= 4) kern = (const void *)k_estep64_w3<NB, P, PRUNE>;
+    else if constexpr (NB == 5) kern = (const void *)k_estep64_w2<NB, P, PRUNE>;
+    else kern = (const void *)k_estep64<NB, P, PRUNE>;
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
     const int blocks = (int)(c->npad / (64 * P));
     if constexpr (NB <= 4)
-        hipLaunchKernelGGL((k_estep64_w3<NB, P>), dim3(blocks), dim3(256), lds, c->stream, c->X, c->n, c->npad,
-                           c->K, c->stage64, c->coef, c->Rt, c->ll_part, c->korder);
+        hipLaunchKernelGGL((k_estep64_w3<NB, P, PRUNE>), dim3(blocks), dim3(256), lds, c->stream, c->Xe(), c->n,
+                           c->npad, c->K, c->stage64, c->coef, c->Rt, c->ll_part, c->korder, ref, c->nskip);
     else if constexpr (NB == 5)
-        hipLaunchKernelGGL((k_estep64_w2<NB, P>), dim3(blocks), dim3(256), lds, c->stream, c->X, c->n, c->npad,
-                           c->K, c->stage64, c->coef, c->Rt, c->ll_part, c->korder);
+        hipLaunchKernelGGL((k_estep64_w2<NB, P, PRUNE>), dim3(blocks), dim3(256), lds, c->stream, c->Xe(), c->n,
+                           c->npad, c->K, c->stage64, c->coef, c->Rt, c->ll_part, c->korder, ref, c->nskip);
     else
-        hipLaunchKernelGGL((k_estep64<NB, P>), dim3(blocks), dim3(256), lds, c->stream, c->X, c->n, c->npad,
-                           c->K, c->stage64, c->coef, c->Rt, c->ll_part, c->korder);
+        hipLaunchKernelGGL((k_estep64<NB, P, PRUNE>), dim3(blocks), dim3(256), lds, c->stream, c->Xe(), c->n,
+                           c->npad, c->K, c->stage64, c->coef, c->Rt, c->ll_part, c->korder, ref, c->nskip);
     hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(256), 0, c->stream, c->ll_part,
                        (int64_t)blocks, c->ll_dev);
     return hipGetLastError();
+}
+
+// The pruned kernel (and the sorted slot order, sort_points) where it
+// applies: K <= 64, d > 16, not switched off; else the plain kernel.
+template <int NB, int P>
+static hipError_t estep64_t(fem_ctx *c) {
+    if constexpr (NB >= 2) {
+        if (c->prune && c->ref && c->K <= 64) {
+            if (hipError_t e = sort_points(c)) return e;
+            if (hipError_t e = hipMemsetAsync(c->nskip, 0, sizeof(unsigned long long), c->stream)) return e;
+            hipError_t e = estep64_launch<NB, P, true>(c, c->ref);
+            if (e == hipSuccess) c->refs_valid = true;
+            return e;
+        }
+    }
+    return estep64_launch<NB, P, false>(c, nullptr);
 }
 
 // Point groups per wave of the FP64 E-step: 2 or 4 (4 only for d <= 64).
@@ -179,7 +202,7 @@ __global__ __launch_bounds__(256) void k_means_finalize(int K, int d, int dpad, 
 
 template <int NB>
 static hipError_t means_t(fem_ctx *c) {
-    hipLaunchKernelGGL(k_means64<NB>, dim3(c->m_chunks), dim3(256), 0, c->stream, c->X, c->Rt,
+    hipLaunchKernelGGL(k_means64<NB>, dim3(c->m_chunks), dim3(256), 0, c->stream, c->Xe(), c->Rt,
                        c->npad, c->KP, c->mpart);
     return hipGetLastError();
 }
@@ -254,11 +277,13 @@ hipError_t launch_zero(double *p, int64_t n, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Slot i's posteriors to row perm[i] (i before the points were sorted).
 __global__ void k_transpose_post(const double *__restrict__ Rt, int64_t npad, int64_t n, int K,
-                                 double *__restrict__ out, int64_t ld) {
+                                 double *__restrict__ out, int64_t ld, const int *__restrict__ perm) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    for (int k = 0; k < K; k++) out[i * ld + k] = Rt[(size_t)k * npad + i];
+    const int64_t row = perm ? perm[i] : i;
+    for (int k = 0; k < K; k++) out[row * ld + k] = Rt[(size_t)k * npad + i];
 }
 
 // Class of each row: the first component of largest posterior, compared with
@@ -291,7 +316,7 @@ hipError_t launch_classes(const fem_ctx *c, int *cls_dev) {
 hipError_t launch_transpose_post(const fem_ctx *c, double *out_dev, int64_t ld) {
     if (c->prec != FEM_PREC_FP64) return launch_transpose_post32(c, out_dev, ld);
     hipLaunchKernelGGL(k_transpose_post, dim3((unsigned)((c->n + 255) / 256)), dim3(256), 0,
-                       c->stream, c->Rt, c->npad, c->n, c->K, out_dev, ld);
+                       c->stream, c->Rt, c->npad, c->n, c->K, out_dev, ld, c->sorted ? c->perm : nullptr);
     return hipGetLastError();
 }
 

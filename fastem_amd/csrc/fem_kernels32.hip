@@ -197,7 +197,7 @@ static hipError_t estep32_t(fem_ctx *c) {
         if (e != hipSuccess) return e;
     }
     const int blocks = (int)(c->npad / (64 * P));
-    hipLaunchKernelGGL((k_estep32<NB, P>), dim3(blocks), dim3(256), lds, c->stream, c->X, c->n, c->npad,
+    hipLaunchKernelGGL((k_estep32<NB, P>), dim3(blocks), dim3(256), lds, c->stream, c->Xe(), c->n, c->npad,
                        c->K, c->stage32, c->coef, c->Rt32, c->ll_part, c->korder);
     hipLaunchKernelGGL(k_reduce_sum32, dim3(1), dim3(256), 0, c->stream, c->ll_part, (int64_t)blocks,
                        c->ll_dev);
@@ -283,7 +283,7 @@ __global__ __launch_bounds__(256) void k_means32(const float *__restrict__ X, co
 
 template <int NB>
 static hipError_t means32_t(fem_ctx *c) {
-    hipLaunchKernelGGL(k_means32<NB>, dim3(c->m_chunks), dim3(256), 0, c->stream, c->X, c->Rt32, c->npad,
+    hipLaunchKernelGGL(k_means32<NB>, dim3(c->m_chunks), dim3(256), 0, c->stream, c->Xe(), c->Rt32, c->npad,
                        c->KP, c->mpart);
     return hipGetLastError();
 }
@@ -453,7 +453,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 
 // The same over the nonzero posteriors only (as k_cov64_sparse, whose exact-
 // zero argument holds here with FP32's underflow): per wave and segment of
-// kCovSeg32 points, lane l reads points 4l..4l+3 of each 256 (one 16-byte
+// kCovSeg32 points (segments strided over all points, as there), lane l reads points 4l..4l+3 of each 256 (one 16-byte
 // load), the points with r != 0 are listed in point order in the wave's LDS
 // slice, and the products run over the list four entries per MFMA (lane
 // group g takes entry 4j + g; entries past the end take r = 0).  The FP32
@@ -477,8 +477,8 @@ __device__ inline void cov32_sparse(int cg, int chunk, const float *__restrict__
     const int g = lane >> 4, c = lane & 15;
     const int k = 4 * cg + wave;
     if (k >= K) return;
-    const int64_t i0 = (int64_t)chunk * chunk_rows;
-    const int64_t i1 = (i0 + chunk_rows < npad) ? i0 + chunk_rows : npad;
+    // segments chunk, chunk + chunks, ... (as cov64_sparse)
+    const int64_t nseg = npad / kCovSeg32, nch = (npad + chunk_rows - 1) / chunk_rows;
     float mh[NB];
 #pragma unroll
     for (int fb = 0; fb < NB; fb++) mh[fb] = (NB * c + fb < d) ? (float)mu[(size_t)k * d + NB * c + fb] : 0.0f;
@@ -559,9 +559,10 @@ __device__ inline void cov32_sparse(int cg, int chunk, const float *__restrict__
         }
     };
 
-    // chunk bounds are multiples of 4096 points: every segment is whole
-    if (i0 < i1) loadseg(i0);
-    for (int64_t s = i0; s < i1; s += kCovSeg32) {
+    // npad is a multiple of 4096 points: every segment is whole
+    if (chunk < nseg) loadseg((int64_t)chunk * kCovSeg32);
+    for (int64_t sg = chunk; sg < nseg; sg += nch) {
+        const int64_t s = sg * kCovSeg32;
         int m = 0;
 #pragma unroll
         for (int j = 0; j < NJ; j++) {
@@ -587,7 +588,7 @@ __device__ inline void cov32_sparse(int cg, int chunk, const float *__restrict__
                 m += __builtin_popcountll(b[w]);
             }
         }
-        if (s + kCovSeg32 < i1) loadseg(s + kCovSeg32);
+        if (sg + nch < nseg) loadseg((sg + nch) * kCovSeg32);
         // the list pads to whole groups of 4 terms per MFMA
         if (pending + ((m + 3) & ~3) > FLUSH) flush();
         pending += (m + 3) & ~3;
@@ -631,10 +632,10 @@ template <int NB>
 static hipError_t cov32_t(fem_ctx *c) {
     const int blocks = (c->KP / 4) * c->c_chunks * cov32_parts(NB);
     if (c->cov_sparse)
-        hipLaunchKernelGGL(k_cov32_sparse<NB>, dim3(blocks), dim3(256), 0, c->stream, c->X, c->Rt32, c->npad,
+        hipLaunchKernelGGL(k_cov32_sparse<NB>, dim3(blocks), dim3(256), 0, c->stream, c->Xe(), c->Rt32, c->npad,
                            c->K, c->KP, c->d, c->mu, c->nk, c->c_chunk_rows, c->cpart);
     else
-        hipLaunchKernelGGL(k_cov32<NB>, dim3(blocks), dim3(256), 0, c->stream, c->X, c->Rt32, c->npad, c->K,
+        hipLaunchKernelGGL(k_cov32<NB>, dim3(blocks), dim3(256), 0, c->stream, c->Xe(), c->Rt32, c->npad, c->K,
                            c->KP, c->d, c->mu, c->nk, c->c_chunk_rows, c->cpart);
     return hipGetLastError();
 }
@@ -654,15 +655,16 @@ hipError_t launch_cov32_partials(fem_ctx *c) {
 }
 
 __global__ void k_transpose_post32(const float *__restrict__ Rt, int64_t npad, int64_t n, int K,
-                                   double *__restrict__ out, int64_t ld) {
+                                   double *__restrict__ out, int64_t ld, const int *__restrict__ perm) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    for (int k = 0; k < K; k++) out[i * ld + k] = (double)Rt[(size_t)k * npad + i];
+    const int64_t row = perm ? perm[i] : i;
+    for (int k = 0; k < K; k++) out[row * ld + k] = (double)Rt[(size_t)k * npad + i];
 }
 
 hipError_t launch_transpose_post32(const fem_ctx *c, double *out_dev, int64_t ld) {
     hipLaunchKernelGGL(k_transpose_post32, dim3((unsigned)((c->n + 255) / 256)), dim3(256), 0, c->stream,
-                       c->Rt32, c->npad, c->n, c->K, out_dev, ld);
+                       c->Rt32, c->npad, c->n, c->K, out_dev, ld, c->sorted ? c->perm : nullptr);
     return hipGetLastError();
 }
 

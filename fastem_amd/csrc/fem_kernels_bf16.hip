@@ -619,7 +619,7 @@ static hipError_t estepb_t(fem_ctx *c) {
         if (e != hipSuccess) return e;
     }
     const int blocks = (int)(c->npad / (64 * P));
-    hipLaunchKernelGGL((k_estepb<NB, P>), dim3(blocks), dim3(256), lds, c->stream, c->X, c->n, c->npad, c->K,
+    hipLaunchKernelGGL((k_estepb<NB, P>), dim3(blocks), dim3(256), lds, c->stream, c->Xe(), c->n, c->npad, c->K,
                        c->stageb, c->coef, c->Rt32, c->ll_part);
     hipLaunchKernelGGL(k_reduce_sumb, dim3(1), dim3(256), 0, c->stream, c->ll_part, (int64_t)blocks, c->ll_dev);
     return hipGetLastError();
@@ -629,7 +629,7 @@ static hipError_t estepb_t(fem_ctx *c) {
 template <int NB, int R>
 static hipError_t estepr_t(fem_ctx *c) {
     const int blocks = (int)(c->npad / 256);
-    hipLaunchKernelGGL((k_estepr<NB, R>), dim3(blocks), dim3(256), 0, c->stream, c->X, c->n, c->npad, c->K,
+    hipLaunchKernelGGL((k_estepr<NB, R>), dim3(blocks), dim3(256), 0, c->stream, c->Xe(), c->n, c->npad, c->K,
                        c->stager, c->coef, c->refc, c->Rt32, c->ll_part);
     hipLaunchKernelGGL(k_reduce_sumb, dim3(1), dim3(256), 0, c->stream, c->ll_part, (int64_t)blocks, c->ll_dev);
     return hipGetLastError();
@@ -755,7 +755,7 @@ __global__ __launch_bounds__(256) void k_meansb(const float *__restrict__ X, con
 
 template <int NB>
 static hipError_t meansb_t(fem_ctx *c) {
-    hipLaunchKernelGGL(k_meansb<NB>, dim3(c->m_chunks), dim3(256), 0, c->stream, c->X, c->Rt32, c->npad, c->KP,
+    hipLaunchKernelGGL(k_meansb<NB>, dim3(c->m_chunks), dim3(256), 0, c->stream, c->Xe(), c->Rt32, c->npad, c->KP,
                        c->mpart);
     return hipGetLastError();
 }
@@ -1043,7 +1043,8 @@ __global__ __launch_bounds__(256) void k_covb_wide(const float *__restrict__ X, 
 }
 
 // The same over the nonzero posteriors only (the exact-zero argument of
-// k_cov64_sparse): per wave and segment of kCovSegB points the nonzero
+// k_cov64_sparse): per wave and segment of kCovSegB points (segments
+// strided over all points, as there) the nonzero
 // posteriors are listed in point order in the wave's LDS slice (lane l
 // reads points 4l..4l+3 of each 256), and each 32-entry group of the list is
 // one K = 32 step of the dense form (lane group g takes entries 8g..8g+7 of
@@ -1054,8 +1055,8 @@ __global__ __launch_bounds__(256) void k_covb_wide(const float *__restrict__ X, 
 constexpr int kCovSegB = 1024;
 
 template <int NB, int PART>
-__device__ inline void covb_sparse(const float *__restrict__ X, const float *__restrict__ rrow, int64_t i0,
-                                   int64_t i1, const float (&mh)[NB], float inv, int g, int c,
+__device__ inline void covb_sparse(const float *__restrict__ X, const float *__restrict__ rrow, int64_t seg0,
+                                   int64_t nseg, int64_t nch, const float (&mh)[NB], float inv, int g, int c,
                                    double *__restrict__ out, int *wi, float *wr) {
     constexpr int DP = 16 * NB;
     constexpr int UB = NB * (NB + 1) / 2;
@@ -1143,9 +1144,11 @@ __device__ inline void covb_sparse(const float *__restrict__ X, const float *__r
             }
     };
 
-    // chunk bounds are multiples of 4096 points: every segment is whole
-    if (i0 < i1) loadseg(i0);
-    for (int64_t s = i0; s < i1; s += kCovSegB) {
+    // segments seg0, seg0 + nch, ... (as cov64_sparse); npad is a multiple
+    // of 4096 points, so every segment is whole
+    if (seg0 < nseg) loadseg(seg0 * kCovSegB);
+    for (int64_t sg = seg0; sg < nseg; sg += nch) {
+        const int64_t s = sg * kCovSegB;
         int m = 0;
 #pragma unroll
         for (int j = 0; j < NJ; j++) {
@@ -1171,7 +1174,7 @@ __device__ inline void covb_sparse(const float *__restrict__ X, const float *__r
                 m += __builtin_popcountll(b[w]);
             }
         }
-        if (s + kCovSegB < i1) loadseg(s + kCovSegB);
+        if (sg + nch < nseg) loadseg((sg + nch) * kCovSegB);
         const int padded = (m + 31) & ~31;
         if (pending + padded > FLUSH) flush();
         pending += padded;
@@ -1210,8 +1213,6 @@ __device__ __attribute__((always_inline)) inline void covb_sparse_body(
     const int cg = rest % KG, chunk = rest / KG;
     const int k = 4 * cg + wave;
     if (k >= K) return;
-    const int64_t i0 = (int64_t)chunk * chunk_rows;
-    const int64_t i1 = (i0 + chunk_rows < npad) ? i0 + chunk_rows : npad;
     float mh[NB];
 #pragma unroll
     for (int fb = 0; fb < NB; fb++) mh[fb] = (NB * c + fb < d) ? (float)mu[(size_t)k * d + NB * c + fb] : 0.0f;
@@ -1220,8 +1221,9 @@ __device__ __attribute__((always_inline)) inline void covb_sparse_body(
     const float *rrow = Rt + (size_t)k * npad;
     int *wi = lidx + wave * kCovSegB;
     float *wr = lrv + wave * kCovSegB;
-    if (part == 0) covb_sparse<NB, 0>(X, rrow, i0, i1, mh, inv, g, c, out, wi, wr);
-    else if constexpr (NPART > 1) covb_sparse<NB, 1>(X, rrow, i0, i1, mh, inv, g, c, out, wi, wr);
+    const int64_t nseg = npad / kCovSegB, nch = (npad + chunk_rows - 1) / chunk_rows;
+    if (part == 0) covb_sparse<NB, 0>(X, rrow, chunk, nseg, nch, mh, inv, g, c, out, wi, wr);
+    else if constexpr (NPART > 1) covb_sparse<NB, 1>(X, rrow, chunk, nseg, nch, mh, inv, g, c, out, wi, wr);
 }
 
 // registers capped for two waves per SIMD for d <= 64, the compiler's
@@ -1249,10 +1251,10 @@ static hipError_t covb_t(fem_ctx *c) {
     const int blocks = (c->KP / 4) * c->c_chunks * cov_parts(NB);
     if (c->cov_sparse) {
         if constexpr (NB <= 4)
-            hipLaunchKernelGGL(k_covb_sparse<NB>, dim3(blocks), dim3(256), 0, c->stream, c->X, c->Rt32, c->npad, c->K,
+            hipLaunchKernelGGL(k_covb_sparse<NB>, dim3(blocks), dim3(256), 0, c->stream, c->Xe(), c->Rt32, c->npad, c->K,
                                c->KP, c->d, c->mu, c->nk, c->c_chunk_rows, c->cpart);
         else
-            hipLaunchKernelGGL(k_covb_sparse_wide<NB>, dim3(blocks), dim3(256), 0, c->stream, c->X, c->Rt32, c->npad,
+            hipLaunchKernelGGL(k_covb_sparse_wide<NB>, dim3(blocks), dim3(256), 0, c->stream, c->Xe(), c->Rt32, c->npad,
                                c->K, c->KP, c->d, c->mu, c->nk, c->c_chunk_rows, c->cpart);
         return hipGetLastError();
     }
@@ -1262,7 +1264,7 @@ static hipError_t covb_t(fem_ctx *c) {
         hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), lds, c->stream, c->X, c->Rt32, c->npad, c->K, c->KP, c->d,
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), lds, c->stream, c->Xe(), c->Rt32, c->npad, c->K, c->KP, c->d,
                        c->mu, c->nk, c->c_chunk_rows, c->cpart);
     return hipGetLastError();
 }

@@ -560,6 +560,9 @@ int lloyd_and_mixture(fem_ctx *c, std::vector<double> &C, Draw &&draw, int max_s
                       double *cov) {
     const int K = c->K, d = c->d;
     const int64_t n = c->n;
+    // the assignments and the mixture's sums are per point: back to point
+    // order (the pruned E-step's slot order is rebuilt afterwards)
+    fem::reset_order(c);
     double best = std::numeric_limits<double>::max();
     std::vector<double> sums((size_t)K * d), counts(K);
     for (int it = 1; it <= max_sweeps; it++) {

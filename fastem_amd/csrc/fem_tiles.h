@@ -405,12 +405,29 @@ __device__ __host__ constexpr int estep64_lds() { return 2 * ((estep64_stage<NB>
 // alive_range of the running maximum when they are formed, and so stored:
 // the context passes the components by descending weight, so most points
 // meet their own component early.
-template <int NB, int P>
+//
+// Pruning (PRUNE, with ref non-null, K <= 64, NB >= 2; lds_int: 128 ints;
+// the instance without it is the plain kernel): ref holds per
+// point the component of its largest log-density in the previous E-step (K:
+// none).  The components some point of the tile names come first (ascending),
+// then the others in `order`.  Per wave and component, the 10 quads of the
+// first 16-row block are formed first; their squared norm q0 is a lower bound
+// of the point's q, so ub = coef_k - q0 / 2 bounds its log-density from above
+// (the rounded sums are monotone: q's in-lane sum starts with q0's terms in
+// the same order and adds non-negative ones, and the cross-group sums are the
+// same additions).  When ub < running max - alive_range for every point of
+// the wave, the unpruned kernel would neither store this log-density nor
+// change the maximum, so the rest of the component is skipped and the
+// outputs stay bit for bit those of the same order unpruned.  The component
+// of largest log-density (the first in processing order) goes back to ref,
+// and *nskip counts the skipped (wave, component) pairs.
+template <int NB, int P, bool PRUNE = false>
 __device__ inline void estep64_unit(int64_t tile, const float *__restrict__ X, int64_t n, int64_t npad, int K,
                                     const double *__restrict__ stage, const double *__restrict__ coef,
                                     double *__restrict__ Rt,
                                     double *__restrict__ ll_out, double *lds, double *wsum,
-                                    const int *__restrict__ order = nullptr) {
+                                    const int *__restrict__ order = nullptr, int *__restrict__ ref = nullptr,
+                                    int *lds_int = nullptr, unsigned long long *nskip = nullptr) {
     constexpr int DP = 16 * NB;
     constexpr int NOFF = offd_pairs_of(NB);
     constexpr int NDIAG = diag_quads_of(NB);
@@ -438,7 +455,31 @@ __device__ inline void estep64_unit(int64_t tile, const float *__restrict__ X, i
     // clamped to its last piece (same value to the same address)
     const double2 *stage2 = reinterpret_cast<const double2 *>(stage);
     double2 *lds2 = reinterpret_cast<double2 *>(lds);
-    auto comp = [&](int it) { return order ? order[it] : it; };
+    const bool prune = PRUNE && NB >= 2 && ref != nullptr && K <= 64;
+    if (prune) {
+        // the tile's named components first (ascending), then the others
+        int *flag = lds_int, *sorder = lds_int + 64;
+        if (tid < 64) flag[tid] = 0;
+        __syncthreads();
+        if (tid < 64 * P) {
+            const int r = ref[tile * (64 * P) + tid];
+            if (r >= 0 && r < K) flag[r] = 1;
+        }
+        __syncthreads();
+        if (wave == 0) {
+            const bool f = lane < K && flag[lane];
+            const uint64_t fm = __ballot(f);
+            if (f) sorder[__builtin_amdgcn_mbcnt_hi((unsigned)(fm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)fm, 0u))] = lane;
+            const int nf = __builtin_popcountll(fm);
+            const int kk = lane < K ? (order ? order[lane] : lane) : 0;
+            const bool u = lane < K && !flag[kk];
+            const uint64_t um = __ballot(u);
+            if (u) sorder[nf + __builtin_amdgcn_mbcnt_hi((unsigned)(um >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)um, 0u))] = kk;
+        }
+        __syncthreads();
+    }
+    const int *wo = prune ? lds_int + 64 : order;
+    auto comp = [&](int it) { return wo ? wo[it] : it; };
     {
         const double2 *first2 = stage2 + (size_t)comp(0) * STAGE2;
 #pragma unroll
@@ -451,11 +492,14 @@ __device__ inline void estep64_unit(int64_t tile, const float *__restrict__ X, i
 
     double mx[P];
     unsigned stored[P];  // bit k/4: log-density of component k stored (K <= 64)
+    int best[P];
 #pragma unroll
     for (int p = 0; p < P; p++) {
         mx[p] = -INFINITY;
         stored[p] = 0u;
+        best[p] = K;
     }
+    unsigned skipped = 0;
 
     for (int it = 0; it < K; it++) {
         const int k = comp(it);
@@ -487,44 +531,127 @@ __device__ inline void estep64_unit(int64_t tile, const float *__restrict__ X, i
 #pragma unroll
             for (int p = 0; p < P; p++) acc[p][jb] = init;
         }
-        int pr = 0, qd = 0;
-        double2 a2 = {0.0, 0.0}, q2 = {0.0, 0.0};
-#pragma unroll
-        for (int lb = 0; lb < 4 * NB; lb++) {
+        // quad qd and pair pr of strip lb, in the order the stage block holds
+        // them (strip by strip: the strip's quads, then its pairs)
+        auto quads = [&](int lb, int qd) {
             double b[P];
 #pragma unroll
             for (int p = 0; p < P; p++) b[p] = (double)xr[p][lb];
             const int jd = lb / 4;
+            double2 q2 = {0.0, 0.0};
 #pragma unroll
-            for (int v = lb % 4; v < 4; v++) {
-                if ((qd & 1) == 0) q2 = *reinterpret_cast<const double2 *>(lq + (qd >> 1) * 32 + slot * 2);
+            for (int v = lb % 4; v < 4; v++, qd++) {
+                if ((qd & 1) == 0 || v == lb % 4)
+                    q2 = *reinterpret_cast<const double2 *>(lq + (qd >> 1) * 32 + slot * 2);
                 const double a = (qd & 1) ? q2.y : q2.x;
-                qd++;
 #pragma unroll
                 for (int p = 0; p < P; p++)
                     acc[p][jd][v] = __builtin_amdgcn_mfma_f64_4x4x4f64(a, b[p], acc[p][jd][v], 0, 0, 0);
             }
+        };
+        auto pairs = [&](int lb, int pr) {
+            double b[P];
 #pragma unroll
-            for (int jb = jd + 1; jb < NB; jb++) {
-                if ((pr & 1) == 0) a2 = *reinterpret_cast<const double2 *>(lu + (pr >> 1) * 128 + lane * 2);
+            for (int p = 0; p < P; p++) b[p] = (double)xr[p][lb];
+            const int jd = lb / 4;
+            double2 a2 = {0.0, 0.0};
+#pragma unroll
+            for (int jb = jd + 1; jb < NB; jb++, pr++) {
+                if ((pr & 1) == 0 || jb == jd + 1)
+                    a2 = *reinterpret_cast<const double2 *>(lu + (pr >> 1) * 128 + lane * 2);
                 const double a = (pr & 1) ? a2.y : a2.x;
-                pr++;
 #pragma unroll
                 for (int p = 0; p < P; p++)
                     acc[p][jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[p], acc[p][jb], 0, 0, 0);
             }
+        };
+        // quads before strip lb: 4 per block for the blocks before lb / 4,
+        // then 4 - v0 for each strip v0 < lb % 4 of its block; pairs before
+        // strip lb: NB - 1 - lb' / 4 for each strip lb' < lb
+        auto quad0 = [](int lb) { return 10 * (lb / 4) + (lb % 4) * 4 - (lb % 4) * (lb % 4 - 1) / 2; };
+        auto pair0 = [](int lb) {
+            int r = 0;
+            for (int l = 0; l < lb; l++) r += NB - 1 - l / 4;
+            return r;
+        };
+        bool skip = false;
+        if constexpr (PRUNE) {
+          if (prune) {
+            // the first block's quads, then the bound
+#pragma unroll
+            for (int lb = 0; lb < 4; lb++) quads(lb, quad0(lb));
+            bool out = true;
+#pragma unroll
+            for (int p = 0; p < P; p++) {
+                double q0 = 0.0;
+#pragma unroll
+                for (int v = 0; v < 4; v++) q0 += acc[p][0][v] * acc[p][0][v];
+                q0 = wave_xor_sum_g(q0);
+                const double ub = -0.5 * q0 + ck;
+                out = out && (base + 16 * p + c >= n || ub < mx[p] - alive_range<double>());
+            }
+            skip = __all(out);
+          } else {
+#pragma unroll
+            for (int lb = 0; lb < 4; lb++) quads(lb, quad0(lb));
+          }
         }
         double sv[P];
+        if (!skip) {
+            if constexpr (PRUNE) {
+                // the first block's quads are done
 #pragma unroll
-        for (int p = 0; p < P; p++) {
-            double q = 0.0;
+                for (int lb = 0; lb < 4; lb++) pairs(lb, pair0(lb));
 #pragma unroll
-            for (int jb = 0; jb < NB; jb++)
+                for (int lb = 4; lb < 4 * NB; lb++) {
+                    quads(lb, quad0(lb));
+                    pairs(lb, pair0(lb));
+                }
+            } else {
+                // strip by strip, two quads or pairs per 16-byte LDS read
+                int pr = 0, qd = 0;
+                double2 a2 = {0.0, 0.0}, q2 = {0.0, 0.0};
 #pragma unroll
-                for (int v = 0; v < 4; v++) q += acc[p][jb][v] * acc[p][jb][v];
-            q = wave_xor_sum_g(q);
-            sv[p] = -0.5 * q + ck;
-            mx[p] = fmax(mx[p], sv[p]);
+                for (int lb = 0; lb < 4 * NB; lb++) {
+                    double b[P];
+#pragma unroll
+                    for (int p = 0; p < P; p++) b[p] = (double)xr[p][lb];
+                    const int jd = lb / 4;
+#pragma unroll
+                    for (int v = lb % 4; v < 4; v++) {
+                        if ((qd & 1) == 0) q2 = *reinterpret_cast<const double2 *>(lq + (qd >> 1) * 32 + slot * 2);
+                        const double a = (qd & 1) ? q2.y : q2.x;
+                        qd++;
+#pragma unroll
+                        for (int p = 0; p < P; p++)
+                            acc[p][jd][v] = __builtin_amdgcn_mfma_f64_4x4x4f64(a, b[p], acc[p][jd][v], 0, 0, 0);
+                    }
+#pragma unroll
+                    for (int jb = jd + 1; jb < NB; jb++) {
+                        if ((pr & 1) == 0) a2 = *reinterpret_cast<const double2 *>(lu + (pr >> 1) * 128 + lane * 2);
+                        const double a = (pr & 1) ? a2.y : a2.x;
+                        pr++;
+#pragma unroll
+                        for (int p = 0; p < P; p++)
+                            acc[p][jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[p], acc[p][jb], 0, 0, 0);
+                    }
+                }
+            }
+#pragma unroll
+            for (int p = 0; p < P; p++) {
+                double q = 0.0;
+#pragma unroll
+                for (int jb = 0; jb < NB; jb++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) q += acc[p][jb][v] * acc[p][jb][v];
+                q = wave_xor_sum_g(q);
+                sv[p] = -0.5 * q + ck;
+                if constexpr (PRUNE)
+                    if (sv[p] > mx[p]) best[p] = k;
+                mx[p] = fmax(mx[p], sv[p]);
+            }
+        } else {
+            skipped++;
         }
         // this wave's pieces of the next block have landed and the barrier
         // publishes all of them; the log-densities are stored after it, so
@@ -534,7 +661,7 @@ __device__ inline void estep64_unit(int64_t tile, const float *__restrict__ X, i
         __syncthreads();
         // only log-densities within alive_range of the running maximum
         // (NaN included) can have a non-zero posterior (lse_posteriors)
-        if ((k & 3) == g) {
+        if (!skip && (k & 3) == g) {
 #pragma unroll
             for (int p = 0; p < P; p++)
                 if (K > 64 || !(sv[p] < mx[p] - alive_range<double>())) {
@@ -542,6 +669,16 @@ __device__ inline void estep64_unit(int64_t tile, const float *__restrict__ X, i
                     stored[p] |= 1u << ((k >> 2) & 15);
                 }
         }
+    }
+    if (PRUNE && prune) {
+        if (g == 0) {
+#pragma unroll
+            for (int p = 0; p < P; p++) {
+                const int64_t pt = base + 16 * p + c;
+                ref[pt] = pt < n ? min(best[p], K - 1) : K;  // valid points sort first
+            }
+        }
+        if (nskip && lane == 0 && skipped) atomicAdd(nskip, (unsigned long long)skipped);
     }
 
     double llsum = 0.0;

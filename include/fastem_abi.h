@@ -236,6 +236,20 @@ enum fem_estep_form {
 };
 int fem_last_estep_form(fem_ctx *ctx, int *form);
 
+/* No counterpart in the reference (which evaluates every component for every
+ * point, EM2/MatrixEM.cpp:69-87).  The FP64 E-step of a context with K <= 64
+ * and d > 16 keeps the points, inside the context, sorted by the component
+ * of their largest log-density in the last E-step, takes each workgroup's
+ * components first and skips a component for a wave of 32 points when the
+ * first 16 rows of U_k^T (x - mu_k) already put every point's log-density
+ * more than 760 below its running maximum: that log-density's posterior is
+ * exactly zero, and the E-step's outputs are bit for bit those of the same
+ * order without the skip (posteriors and classes leave the context in point
+ * order).  *frac = the share of (wave, component) evaluations the last FP64
+ * E-step skipped (0 for other forms).  FASTEM_ESTEP_DENSE=1 at fem_create
+ * evaluates every component in point order. */
+int fem_last_estep_skipped(fem_ctx *ctx, double *frac);
+
 /* Message of the last failed call on ctx (or of the last failed fem_create
  * when ctx is NULL). */
 const char *fem_last_error(const fem_ctx *ctx);

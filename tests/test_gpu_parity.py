@@ -176,6 +176,89 @@ def test_trajectory_config1(orc, g):
     assert rel_err(cov, ref.cov) < PARAM_RTOL
 
 
+def run_dense_or_pruned(X, w0, mu0, cov0, iters, dense):
+    """fem.train for `iters` iterations with the FP64 E-step evaluating every
+    component in point order (dense=True, FASTEM_ESTEP_DENSE=1) or pruned
+    over the sorted slot order (the default); returns the LL trace, the
+    parameters, the last E-step's posteriors and classes (point order) and
+    the share of (wave, component) evaluations the last E-step skipped."""
+    import os
+    n, d = X.shape
+    K = len(w0)
+    if dense:
+        os.environ["FASTEM_ESTEP_DENSE"] = "1"
+    try:
+        trace = []
+        with fem.Context(n, d, K) as ctx:
+            ctx.upload(X)
+            _, _, params = fem.train(ctx, w0, mu0, cov0, maxsweeps=iters, eps=1e-300, ll_trace=trace)
+            return trace, params, ctx.posteriors(), ctx.classes(), ctx.estep_skipped()
+    finally:
+        os.environ.pop("FASTEM_ESTEP_DENSE", None)
+
+
+@pytest.mark.parametrize("n,d,K,lo,hi,iters", [(60000, 40, 12, -20.0, 20.0, 25), (60000, 64, 16, -20.0, 20.0, 8),
+                                               (30000, 17, 5, -10.0, 10.0, 20), (20000, 100, 6, -20.0, 20.0, 10),
+                                               (80000, 32, 40, -20.0, 20.0, 6)])
+def test_pruned_estep_trajectory(orc, n, d, K, lo, hi, iters):
+    """The pruned FP64 E-step (points re-sorted by their component after
+    E-steps 1, 2, 4, ...; components skipped per wave on the 16-row bound)
+    against the oracle's TrainNoCopy from the generating mixture, perturbed: per-iteration LL
+    and final parameters within 1e-10; and the LL trace, last posteriors and
+    classes (which leave the context in point order) against the same run
+    with every component evaluated in point order."""
+    say(f"pruned trajectory n={n} d={d} K={K} iters={iters}")
+    X, w, mu, cov = fem.gengmm(n, d, K, lo, hi, 0.05, 500 + d + K)
+    rng = np.random.default_rng(d + K)
+    w0, mu0, cov0 = w, mu + 0.3 * rng.standard_normal(mu.shape), 1.2 * cov
+    ref = orc.train(X, w0, mu0, cov0, maxsweeps=iters, eps=1e-300, svd=False)
+    trace, (w, mu, cov), R, cls, skipped = run_dense_or_pruned(X, w0, mu0, cov0, iters, dense=False)
+    trace_d, (w_d, mu_d, cov_d), R_d, cls_d, skipped_d = run_dense_or_pruned(X, w0, mu0, cov0, iters, dense=True)
+    errs = np.abs(np.array(trace) - ref.ll_trace) / np.abs(ref.ll_trace)
+    say("skipped", skipped, "max LL rel err vs oracle", errs.max(),
+        "vs dense", np.max(np.abs(np.array(trace) - trace_d) / np.abs(trace_d)))
+    assert skipped_d == 0.0
+    assert errs.max() < LL_RTOL
+    assert rel_err(w, ref.w) < PARAM_RTOL
+    assert rel_err(mu, ref.mu) < PARAM_RTOL
+    assert rel_err(cov, ref.cov) < PARAM_RTOL
+    assert np.max(np.abs(np.array(trace) - trace_d) / np.abs(trace_d)) < 1e-12
+    np.testing.assert_allclose(R, R_d, rtol=1e-9, atol=1e-13)
+    assert np.mean(cls == cls_d) > 0.9999
+    np.testing.assert_allclose(R.sum(1), 1.0, rtol=1e-13)
+    if d >= 32:
+        assert skipped > 0.5
+
+
+def test_pruned_classes_follow_the_points(orc):
+    """Points in a random order and in class order give the same per-point
+    posteriors and classes through the sorted slot order, and the classes
+    are the argmax of the oracle's posteriors."""
+    n, d, K = 30000, 48, 10
+    say("pruned classes in point order")
+    X, w, mu, cov = fem.gengmm(n, d, K, -20.0, 20.0, 0.05, 77)
+    perm = np.random.default_rng(5).permutation(n)
+    outs = []
+    for Xo in (X, np.ascontiguousarray(X[perm])):
+        with fem.Context(n, d, K) as ctx:
+            ctx.upload(Xo)
+            ctx.set_params(w, mu, cov)
+            for _ in range(3):  # sorted after the first E-step
+                ctx.factor()
+                ctx.estep()
+            outs.append((ctx.posteriors(), ctx.classes(), ctx.estep_skipped()))
+    (R0, c0, s0), (R1, c1, s1) = outs
+    say("skipped", s0, s1)
+    inv = np.argsort(perm)
+    np.testing.assert_allclose(R1[inv], R0, rtol=1e-12, atol=1e-15)
+    assert np.array_equal(c1[inv], c0)
+    U, ld = orc.factor_all(cov)
+    _, R_ref = orc.estep(X, w, mu, U, ld)
+    np.testing.assert_allclose(R0, R_ref, rtol=1e-9, atol=1e-13)
+    assert np.array_equal(c0, R_ref.argmax(1))
+    assert s0 > 0.5 and s1 > 0.5
+
+
 def test_convergence_stop_matches(orc):
     n, d, K = 20000, 5, 4
     say("convergence stop")
