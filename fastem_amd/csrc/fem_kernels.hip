@@ -115,6 +115,220 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                                ref, lint, nskip);
 }
 
+// ---------------------------------------------------------------------------
+// The pruned FP64 E-step, one wave at a time (K <= 64, NB >= 2).  The points
+// are in the sorted slot order (fem_sort.hip), so a wave's 16P points mostly
+// share one component.  Each wave takes the components its points named in
+// the last E-step (ref) first, ascending, then the others by descending
+// weight, and for each forms only the first 16-row block from the global
+// stage block (10 quads; accumulators start at -m_k): its squared norm q0
+// bounds q from below (q's in-lane sum starts with q0's terms in the same
+// order and adds non-negative ones; the cross-group sums are the same
+// additions), so ub = coef_k - q0 / 2 bounds the log-density from above.
+// When ub < running max - alive_range for every point of the wave, the
+// plain kernel would neither store this log-density nor change the maximum
+// (its posterior is exactly zero), and the component is skipped; otherwise
+// the rest of U_k^T (x - mu_k) is formed with the plain kernel's products in
+// its order (the pairs of the first four strips, then strip by strip), its
+// fragments read from the global stage block.  Every log-density formed is
+// the plain kernel's bit for bit, and the outputs do not depend on the
+// order.  No LDS stage and no barrier per component: the waves run
+// independently until the tile's LL.  ref receives each point's component
+// of largest log-density (K for padding rows); *nskip counts the skipped
+// (wave, component) pairs.  No reference counterpart (the reference
+// evaluates every component, EM2/MatrixEM.cpp:69-87).
+template <int NB, int P>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_estep64_pruned(
+    const float *__restrict__ X, int64_t n, int64_t npad, int K, const double *__restrict__ stage,
+    const double *__restrict__ coef, double *__restrict__ Rt, double *__restrict__ ll_part,
+    const int *__restrict__ order, int *__restrict__ ref, unsigned long long *__restrict__ nskip) {
+    constexpr int DP = 16 * NB;
+    constexpr int NOFF = offd_pairs_of(NB);
+    constexpr int NDIAG = diag_quads_of(NB);
+    constexpr int STAGE = estep64_stage<NB>();
+    __shared__ double wsum[4];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int g = lane >> 4, c = lane & 15;
+    const int slot = (lane & 3) + 4 * g;  // this lane's 4x4x4 A slot
+    const int64_t base = (int64_t)blockIdx.x * (64 * P) + wave * (16 * P);
+
+    float xr[P][4 * NB];
+#pragma unroll
+    for (int p = 0; p < P; p++)
+#pragma unroll
+        for (int lb = 0; lb < 4 * NB; lb++)
+            xr[p][lb] = __builtin_nontemporal_load(&X[(base + 16 * p + c) * DP + 4 * lb + g]);
+
+    // the components this wave's points named: a mask over lanes
+    uint64_t named = 0;
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        const int r = ref[base + 16 * p + c];
+        if (r >= 0 && r < K) named |= 1ull << r;
+    }
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned lo = (unsigned)named, hi = (unsigned)(named >> 32);
+        named |= ((uint64_t)(unsigned)__shfl_xor((int)hi, o) << 32) | (unsigned)__shfl_xor((int)lo, o);
+    }
+    // (readfirstlane returns int: widened through unsigned, so that bit 31
+    // does not spread into the high word)
+    named = ((uint64_t)(unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(named >> 32)) << 32) |
+            (uint64_t)(unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)named);
+    named &= K >= 64 ? ~0ull : (1ull << K) - 1;
+
+    double mx[P];
+    unsigned stored[P];  // bit k/4: log-density of component k stored
+    int best[P];
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        mx[p] = -INFINITY;
+        stored[p] = 0u;
+        best[p] = K;
+    }
+    unsigned skipped = 0;
+
+    auto process = [&](int k) {
+        const double *blk = stage + (size_t)k * STAGE;
+        const double *lu = blk;
+        const double *lq = blk + NOFF * 64;
+        const double *lm = lq + NDIAG * 16;
+        const double ck = coef[k];
+        v4d acc[P][NB];
+        {
+            v4d init;
+            init[0] = lm[0 + g];
+            init[1] = lm[4 + g];
+            init[2] = lm[8 + g];
+            init[3] = lm[12 + g];
+#pragma unroll
+            for (int p = 0; p < P; p++) acc[p][0] = init;
+        }
+        // quads of strip lb (quad qd0 first) and pairs of strip lb (pair pr0
+        // first), from the global stage block
+        auto quads = [&](int lb, int qd0) {
+            double b[P];
+#pragma unroll
+            for (int p = 0; p < P; p++) b[p] = (double)xr[p][lb];
+            const int jd = lb / 4;
+            int qd = qd0;
+#pragma unroll
+            for (int v = lb % 4; v < 4; v++, qd++) {
+                const double a = lq[(qd >> 1) * 32 + slot * 2 + (qd & 1)];
+#pragma unroll
+                for (int p = 0; p < P; p++)
+                    acc[p][jd][v] = __builtin_amdgcn_mfma_f64_4x4x4f64(a, b[p], acc[p][jd][v], 0, 0, 0);
+            }
+        };
+        auto pairs = [&](int lb, int pr0) {
+            double b[P];
+#pragma unroll
+            for (int p = 0; p < P; p++) b[p] = (double)xr[p][lb];
+            int pr = pr0;
+#pragma unroll
+            for (int jb = lb / 4 + 1; jb < NB; jb++, pr++) {
+                const double a = lu[(pr >> 1) * 128 + lane * 2 + (pr & 1)];
+#pragma unroll
+                for (int p = 0; p < P; p++)
+                    acc[p][jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[p], acc[p][jb], 0, 0, 0);
+            }
+        };
+        auto quad0 = [](int lb) { return 10 * (lb / 4) + (lb % 4) * 4 - (lb % 4) * (lb % 4 - 1) / 2; };
+        auto pair0 = [](int lb) {
+            int r = 0;
+            for (int l = 0; l < lb; l++) r += NB - 1 - l / 4;
+            return r;
+        };
+#pragma unroll
+        for (int lb = 0; lb < 4; lb++) quads(lb, quad0(lb));
+        bool out = true;
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            double q0 = 0.0;
+#pragma unroll
+            for (int v = 0; v < 4; v++) q0 += acc[p][0][v] * acc[p][0][v];
+            q0 = wave_xor_sum_g(q0);
+            out = out && (base + 16 * p + c >= n || -0.5 * q0 + ck < mx[p] - alive_range<double>());
+        }
+        if (__all(out)) {
+            skipped++;
+            return;
+        }
+#pragma unroll
+        for (int jb = 1; jb < NB; jb++) {
+            v4d init;
+            init[0] = lm[16 * jb + 0 + g];
+            init[1] = lm[16 * jb + 4 + g];
+            init[2] = lm[16 * jb + 8 + g];
+            init[3] = lm[16 * jb + 12 + g];
+#pragma unroll
+            for (int p = 0; p < P; p++) acc[p][jb] = init;
+        }
+#pragma unroll
+        for (int lb = 0; lb < 4; lb++) pairs(lb, pair0(lb));
+#pragma unroll
+        for (int lb = 4; lb < 4 * NB; lb++) {
+            quads(lb, quad0(lb));
+            pairs(lb, pair0(lb));
+        }
+        double sv[P];
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            double q = 0.0;
+#pragma unroll
+            for (int jb = 0; jb < NB; jb++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) q += acc[p][jb][v] * acc[p][jb][v];
+            q = wave_xor_sum_g(q);
+            sv[p] = -0.5 * q + ck;
+            if (sv[p] > mx[p]) best[p] = k;
+            mx[p] = fmax(mx[p], sv[p]);
+        }
+        if ((k & 3) == g) {
+#pragma unroll
+            for (int p = 0; p < P; p++)
+                if (!(sv[p] < mx[p] - alive_range<double>())) {
+                    Rt[(size_t)k * npad + base + 16 * p + c] = sv[p];
+                    stored[p] |= 1u << ((k >> 2) & 15);
+                }
+        }
+    };
+    // the named components ascending, then the others in `order`
+    uint64_t rest = named;
+    int it = 0;
+    auto next = [&]() {
+        if (rest) {
+            const int k = __builtin_ctzll(rest);
+            rest &= rest - 1;
+            return k;
+        }
+        while (it < K) {
+            const int k = order ? order[it] : it;
+            it++;
+            if (!((named >> k) & 1)) return k;
+        }
+        return -1;
+    };
+    for (int k = next(); k >= 0; k = next()) process(k);
+
+    double llsum = 0.0;
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        const int64_t pt = base + 16 * p + c;
+        const bool valid = pt < n;
+        const double lse = lse_posteriors(Rt, npad, pt, K, g, mx[p], valid, stored[p]);
+        if (g == 0 && valid) llsum += lse;
+        if (g == 0) ref[pt] = valid ? min(best[p], K - 1) : K;  // valid points sort first
+    }
+    if (lane == 0 && skipped) atomicAdd(nskip, (unsigned long long)skipped);
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) llsum += __shfl_xor(llsum, o);
+    if (lane == 0) wsum[wave] = llsum;
+    __syncthreads();
+    if (tid == 0) ll_part[blockIdx.x] = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
+}
+
 // Deterministic sum of n doubles into *out: fixed strided partials, then a
 // fixed tree.  One workgroup of 256 threads.
 __global__ __launch_bounds__(256) void k_reduce_sum(const double *__restrict__ parts, int64_t n,
@@ -158,7 +372,20 @@ static hipError_t estep64_t(fem_ctx *c) {
         if (c->prune && c->ref && c->K <= 64) {
             if (hipError_t e = sort_points(c)) return e;
             if (hipError_t e = hipMemsetAsync(c->nskip, 0, sizeof(unsigned long long), c->stream)) return e;
-            hipError_t e = estep64_launch<NB, P, true>(c, c->ref);
+            hipError_t e;
+            if constexpr (NB <= 4) {
+                // one wave at a time, fragments from the global stage blocks
+                const int blocks = (int)(c->npad / (64 * P));
+                hipLaunchKernelGGL((k_estep64_pruned<NB, P>), dim3(blocks), dim3(256), 0, c->stream, c->Xe(), c->n,
+                                   c->npad, c->K, c->stage64, c->coef, c->Rt, c->ll_part, c->korder, c->ref,
+                                   c->nskip);
+                hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(256), 0, c->stream, c->ll_part, (int64_t)blocks,
+                                   c->ll_dev);
+                e = hipGetLastError();
+            } else {
+                // the LDS-staged kernel with the workgroup's components first
+                e = estep64_launch<NB, P, true>(c, c->ref);
+            }
             if (e == hipSuccess) c->refs_valid = true;
             return e;
         }

@@ -42,13 +42,14 @@ static hipError_t alloc_sort(fem_ctx *c) {
     if (e == hipSuccess) e = hipMalloc((void **)&c->rel, np * sizeof(int));
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_iota, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, c->stream, c->slots, (int64_t)np);
-    size_t bytes = 0;
-    e = hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, reinterpret_cast<const unsigned *>(c->ref),
-                                           reinterpret_cast<unsigned *>(c->ref2), c->slots, c->rel, (int)np, 0, 8,
-                                           c->stream);
-    if (e == hipSuccess) e = hipMalloc(&c->sort_tmp, bytes);
-    c->sort_tmp_bytes = bytes;
-    return e;
+    return hipGetLastError();
+}
+
+// Key bits of 0..K (K: padding rows, sorted last).
+static int key_bits(int K) {
+    int bits = 1;
+    while ((1 << bits) <= K) bits++;
+    return bits;
 }
 
 // Called before a pruned FP64 E-step: re-sorts the slots on the components
@@ -59,9 +60,22 @@ hipError_t sort_points(fem_ctx *c) {
     if (++c->esteps_since_sort < c->sort_every) return hipSuccess;
     if (!c->Xs)
         if (hipError_t e = alloc_sort(c)) return e;
-    int bits = 1;
-    while ((1 << bits) <= c->K) bits++;  // keys 0..K (K: padding rows, sorted last)
-    hipError_t e = hipcub::DeviceRadixSort::SortPairs(c->sort_tmp, c->sort_tmp_bytes,
+    const int bits = key_bits(c->K);
+    // the temporary storage for exactly this call's arguments (the sort's
+    // passes, and so its scratch, depend on the key bits and the count)
+    size_t need = 0;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, need, reinterpret_cast<const unsigned *>(c->ref),
+                                                      reinterpret_cast<unsigned *>(c->ref2), c->slots, c->rel,
+                                                      (int)c->npad, 0, bits, c->stream);
+    if (e != hipSuccess) return e;
+    if (need > c->sort_tmp_bytes) {
+        if (c->sort_tmp) (void)hipFree(c->sort_tmp);
+        c->sort_tmp = nullptr;
+        c->sort_tmp_bytes = 0;
+        if ((e = hipMalloc(&c->sort_tmp, need)) != hipSuccess) return e;
+        c->sort_tmp_bytes = need;
+    }
+    e = hipcub::DeviceRadixSort::SortPairs(c->sort_tmp, c->sort_tmp_bytes,
                                                       reinterpret_cast<const unsigned *>(c->ref),
                                                       reinterpret_cast<unsigned *>(c->ref2), c->slots, c->rel,
                                                       (int)c->npad, 0, bits, c->stream);
