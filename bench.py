@@ -384,26 +384,41 @@ def main():
         # fraction of the last E-step's posteriors that are nonzero: the
         # (point, component) pairs the covariance pass sums over
         nz = float(np.count_nonzero(ctx.posteriors())) / (N * K) if nonzero else None
+        skipped = ctx.estep_skipped()  # of the last E-step (0 unless pruned)
         ctx.close()
         ph = {name: float(np.mean([r[i] for r in recs]))
               for i, name in ((1, "estep"), (2, "means"), (3, "cov"), (4, "factor"))}
         return {"iters_per_s": steps / wall, "ms_per_step": 1e3 * wall / steps,
-                "phases_ms": ph, "ll": [recs[0][0], recs[-1][0]], "nonzero": nz}
+                "phases_ms": ph, "ll": [recs[0][0], recs[-1][0]], "nonzero": nz, "skipped": skipped}
+
+    def with_env(env, fn):
+        old = {k2: os.environ.get(k2) for k2 in env}
+        os.environ.update(env)
+        try:
+            return fn()
+        finally:
+            for k2, v in old.items():
+                if v is None:
+                    os.environ.pop(k2, None)
+                else:
+                    os.environ[k2] = v
 
     log("fp64 form")
     r64 = run_form(fem.PREC_FP64, args.steps, args.warmup, nonzero=True)
-    variants = {}
+    # the FP64 E-step evaluating every component for every point (no pruning,
+    # point order): the dense kernel's roofline and the iteration without the
+    # skip; and the iteration with both the E-step and the covariances dense,
+    # the reference's shape of work
+    log("fp64 form, dense E-step")
+    re_d = with_env({"FASTEM_ESTEP_DENSE": "1"}, lambda: run_form(fem.PREC_FP64, args.steps, args.warmup))
+    variants = {"fp64_dense_estep": {"iters_per_s": re_d["iters_per_s"], "ms_per_step": re_d["ms_per_step"],
+                                     "phases_ms": re_d["phases_ms"], "loglik_first_last": re_d["ll"]}}
     if not args.no_variants:
-        # the same form with the covariances summed over every point (zero
-        # posteriors included) instead of over the nonzero ones
-        log("fp64 form, dense covariance sum")
-        os.environ["FASTEM_COV_DENSE"] = "1"
-        try:
-            rd = run_form(fem.PREC_FP64, args.steps, args.warmup)
-        finally:
-            del os.environ["FASTEM_COV_DENSE"]
-        variants["fp64_dense_cov"] = {"iters_per_s": rd["iters_per_s"], "ms_per_step": rd["ms_per_step"],
-                                      "phases_ms": rd["phases_ms"], "loglik_first_last": rd["ll"]}
+        log("fp64 form, dense E-step and covariance sum")
+        rd = with_env({"FASTEM_ESTEP_DENSE": "1", "FASTEM_COV_DENSE": "1"},
+                      lambda: run_form(fem.PREC_FP64, args.steps, args.warmup))
+        variants["fp64_dense"] = {"iters_per_s": rd["iters_per_s"], "ms_per_step": rd["ms_per_step"],
+                                  "phases_ms": rd["phases_ms"], "loglik_first_last": rd["ll"]}
         log("fp32 form")
         r32 = run_form(fem.PREC_FP32, args.steps, args.warmup)
         variants["fp32"] = {
@@ -433,7 +448,15 @@ def main():
 
     fl = algorithmic_flops(N, D, K)
     ph = r64["phases_ms"]
-    achieved = fl["estep"] / (ph["estep"] * 1e-3)
+    # the pruned E-step's MFMA work per launch: every (wave, component) forms
+    # its first 16-row block (10 4x4x4 quads per 16-point group), and the
+    # evaluations not skipped the rest (1.0625 N K d^2 in all at d = 64)
+    waves, groups = N // 32, 2
+    first_block = 10 * groups * 512.0
+    full = 1.0625 * 32 * D * D
+    executed = waves * K * (first_block + (1.0 - r64["skipped"]) * (full - first_block))
+    achieved = executed / (ph["estep"] * 1e-3)
+    achieved_dense = fl["estep"] / (re_d["phases_ms"]["estep"] * 1e-3)
     out = {
         "metric": "EM iters/s (plain FP64-statistics GMM EM, N=4194304 d=64 K=64)",
         "value": r64["iters_per_s"],
@@ -449,20 +472,35 @@ def main():
         "data": "synthetic (seeded gengmm mixture, k-means++ init)",
         "config": {"workload": "gmm-em-full-cov", "n": N, "d": D, "k": K,
                    "precision": "fp64", "init": "kmeans++ + 29 Lloyd sweeps (device)"},
-        "roofline": {"bound": "mfma", "kernel": "k_estep64_w3<4,2>", "achieved": achieved / 1e12,
+        "roofline": {"bound": "mfma", "kernel": "k_estep64_pruned<4,2>", "achieved": achieved / 1e12,
                      "peak": PEAK_FP64_MFMA / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_MFMA,
-                     "traffic": measured_traffic("fem::k_estep64_w3<4, 2>"),
+                     "flops_per_launch": executed,
+                     "flops_note": "MFMA flops the pruned E-step executes per launch: the first 16-row "
+                                   "block of every (32-point wave, component) pair and the whole "
+                                   "triangle for the pairs not skipped (estep_skipped_fraction)",
+                     "traffic": measured_traffic("fem::k_estep64_pruned<4, 2>"),
                      "traffic_note": "HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE "
                                      "(profiles/hbm_traffic_n4194304_d64_k64.txt); algorithmic "
                                      "bytes: X read 1.07e9 + FP64 posteriors written 2.15e9"},
+        "roofline_dense_estep": {"bound": "mfma", "kernel": "k_estep64_w3<4,2>",
+                                 "achieved": achieved_dense / 1e12, "peak": PEAK_FP64_MFMA / 1e12,
+                                 "unit": "TFLOP/s", "frac": achieved_dense / PEAK_FP64_MFMA,
+                                 "flops_per_launch": fl["estep"],
+                                 "traffic": measured_traffic("fem::k_estep64_w3<4, 2>"),
+                                 "note": "the same E-step evaluating every component for every point "
+                                         "(variants.fp64_dense_estep, FASTEM_ESTEP_DENSE=1): N K d^2 flops"},
         "phases_ms": ph,
-        "phase_tflops": {"estep": fl["estep"] / ph["estep"] / 1e9, "means": fl["means"] / ph["means"] / 1e9,
+        "phase_tflops": {"estep_executed": achieved / 1e12, "means": fl["means"] / ph["means"] / 1e9,
                          "cov_over_nonzero_posteriors": fl["cov"] * r64["nonzero"] / ph["cov"] / 1e9},
         "nonzero_posterior_fraction": r64["nonzero"],
-        "cov_note": "the covariance pass sums over the (point, component) pairs with a nonzero "
-                    "posterior only (a zero posterior adds exactly zero); variants.fp64_dense_cov "
-                    "is the same iteration summing over every pair",
+        "estep_skipped_fraction": r64["skipped"],
+        "work_note": "exact skips of work that adds exactly zero: the E-step skips a component for a "
+                     "wave of 32 points when the first 16 rows of U_k^T (x - mu_k) put every point's "
+                     "log-density more than 760 below its running maximum (posterior exactly 0; "
+                     "outputs bit for bit those of the dense E-step), and the covariance pass sums "
+                     "over the nonzero posteriors only; variants.fp64_dense_estep and fp64_dense "
+                     "are the same iteration without the first and without both",
         "loglik_first_last": r64["ll"],
         "variants": variants,
         "init_seconds": t_init,
