@@ -5,6 +5,11 @@
 #include <omp.h>
 
 #include <cmath>
+#include <condition_variable>
+#include <exception>
+#include <memory>
+#include <mutex>
+#include <thread>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -140,6 +145,12 @@ void EMAlgorithm::MStep() {
 
 OPTFLOAT EMAlgorithm::FindLargestConditionNumber() {
     FetchParams();
+    return LargestHostConditionNumber();
+}
+
+// The largest condition number of the host copies of the covariances (as
+// fetched by FetchParams); throws when an eigenvalue iteration fails.
+OPTFLOAT EMAlgorithm::LargestHostConditionNumber() const {
     std::vector<double> conds(K, 0.0);
 #pragma omp parallel for schedule(dynamic, 1)
     for (int i = 0; i < K; i++) {
@@ -187,30 +198,106 @@ OPTFLOAT EMAlgorithm::TrainNoCopy(GaussianMixture &G, int Verbosity, bool SVD) {
         MaxCond = FindLargestConditionNumber();
         if (MaxCond > AbortThr) throw EMException("Condition number of a covariance matrix is too high");
     }
+    // With SVD the condition numbers of an iteration's new covariances are
+    // computed on the host while the device runs the next iteration's
+    // E-step; the iteration's check, abort and line follow once both are
+    // done.  The output is the reference's: an abort still comes before the
+    // iteration's line, and the next E-step's result is then never used.
+    // one host thread for the whole run (its OpenMP team persists between
+    // iterations, where a thread per iteration started a new team each time)
+    struct CondWorker {
+        const EMAlgorithm *em;
+        std::mutex m;
+        std::condition_variable cv;
+        bool work = false, done = false, stop = false;
+        OPTFLOAT result = 0.0;
+        std::exception_ptr err;
+        std::thread th;
+        explicit CondWorker(const EMAlgorithm *e) : em(e) {
+            th = std::thread([this]() {
+                std::unique_lock<std::mutex> lk(m);
+                for (;;) {
+                    cv.wait(lk, [this]() { return work || stop; });
+                    if (stop) return;
+                    lk.unlock();
+                    OPTFLOAT r = 0.0;
+                    std::exception_ptr ex;
+                    try {
+                        r = em->LargestHostConditionNumber();
+                    } catch (...) {
+                        ex = std::current_exception();
+                    }
+                    lk.lock();
+                    result = r;
+                    err = ex;
+                    work = false;
+                    done = true;
+                    cv.notify_all();
+                }
+            });
+        }
+        void start() {
+            std::lock_guard<std::mutex> lk(m);
+            work = true;
+            done = false;
+            cv.notify_all();
+        }
+        OPTFLOAT get() {
+            std::unique_lock<std::mutex> lk(m);
+            cv.wait(lk, [this]() { return done; });
+            done = false;
+            if (err) std::rethrow_exception(err);
+            return result;
+        }
+        ~CondWorker() {
+            {
+                std::unique_lock<std::mutex> lk(m);
+                cv.wait(lk, [this]() { return !work; });
+                stop = true;
+                cv.notify_all();
+            }
+            th.join();
+        }
+    };
+    std::unique_ptr<CondWorker> cond;
+    if (SVD) cond = std::make_unique<CondWorker>(this);
+    struct Line {
+        int iter;
+        OPTFLOAT LogL, e, MinTrace;
+        double ms;
+    } pending{};
+    bool have_pending = false;
+    auto finish = [&]() {  // the check and line of the pending iteration
+        MaxCond = cond->get();
+        if (MaxCond > AbortThr) throw EMException("Condition number of a covariance matrix is too high");
+        if (Verbosity > 1)
+            printf("iter: %d LogL: %10.6f e: %g Cond: %g Trace: %g Time[ms]: %g\n", pending.iter, pending.LogL,
+                   pending.e, MaxCond, pending.MinTrace, pending.ms);
+        if (Verbosity > 2) printf("iter: %d LogL(17g): %.17g\n", pending.iter, pending.LogL);
+        have_pending = false;
+    };
     for (IterCounter = 1; IterCounter <= MaxSweeps; IterCounter++) {
         PrecomputeDensityMatrix();
+        if (have_pending) finish();
         LogL = ComputeLogLike();
         EStep();
         MStep();
         RegularizeCovariances();
         InitDensities();
-        if (SVD) {
-            MaxCond = FindLargestConditionNumber();
-            if (MaxCond > AbortThr) throw EMException("Condition number of a covariance matrix is too high");
-        }
         OPTFLOAT e = std::fabs((LogL - PrevLogL) / LogL);
         if (LogL < PrevLogL) e = 0.0;
         const double t1 = monotonic_seconds();
         if (SVD) {
-            const OPTFLOAT MinTrace = FindMinTrace();
-            if (Verbosity > 1)
-                printf("iter: %d LogL: %10.6f e: %g Cond: %g Trace: %g Time[ms]: %g\n", IterCounter, LogL, e,
-                       MaxCond, MinTrace, 1000.0 * (t1 - t0));
+            FetchParams();
+            pending = {IterCounter, LogL, e, FindMinTrace(), 1000.0 * (t1 - t0)};
+            have_pending = true;
+            cond->start();
+            if (e < Eps || IterCounter == MaxSweeps) finish();  // no further E-step to overlap
         } else {
             if (Verbosity > 1)
                 printf("iter: %d LogL: %10.6f e: %g Time[ms]: %g\n", IterCounter, LogL, e, 1000.0 * (t1 - t0));
+            if (Verbosity > 2) printf("iter: %d LogL(17g): %.17g\n", IterCounter, LogL);
         }
-        if (Verbosity > 2) printf("iter: %d LogL(17g): %.17g\n", IterCounter, LogL);
         t0 = t1;
         PrevLogL = LogL;
         if (e < Eps) {

@@ -44,6 +44,9 @@ protected:
 
     virtual void MPIReduceLogL(OPTFLOAT &) {}
     OPTFLOAT FindLargestConditionNumber();
+    // the same on the host copies of the covariances alone (no fetch): run on
+    // a host thread while the device computes the next E-step
+    OPTFLOAT LargestHostConditionNumber() const;
     OPTFLOAT FindMinTrace();
     void DecodeMixture(const GaussianMixture &G);
     void ComputePLLs() {}
