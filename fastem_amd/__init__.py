@@ -44,6 +44,8 @@ _SIGS = {
     "fem_set_regularization": (ctypes.c_int, [_vp, ctypes.c_double]),
     "fem_set_params": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
     "fem_get_params": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
+    "fem_get_params_start": (ctypes.c_int, [_vp]),
+    "fem_get_params_finish": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
     "fem_factor": (ctypes.c_int, [_vp, _dp, _ip]),
     "fem_estep": (ctypes.c_int, [_vp, _dp]),
     "fem_mstep": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
@@ -177,6 +179,18 @@ class Context:
         mu = np.empty((self.K, self.d))
         cov = np.empty((self.K, self.d, self.d))
         self._check(lib().fem_get_params(self._h, _d(w), _d(mu), _d(cov)))
+        return w, mu, cov
+
+    def get_params_start(self):
+        """Queues the parameter download into the context's pinned staging."""
+        self._check(lib().fem_get_params_start(self._h))
+
+    def get_params_finish(self):
+        """Waits for the download get_params_start queued; (w, mu, cov)."""
+        w = np.empty(self.K)
+        mu = np.empty((self.K, self.d))
+        cov = np.empty((self.K, self.d, self.d))
+        self._check(lib().fem_get_params_finish(self._h, _d(w), _d(mu), _d(cov)))
         return w, mu, cov
 
     def factor(self):

@@ -138,6 +138,8 @@ void free_all(fem_ctx *c) {
         if (p) (void)hipFree(p);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    if (c->ev_params) (void)hipEventDestroy(c->ev_params);
+    if (c->params_host) (void)hipHostFree(c->params_host);
     if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
@@ -388,6 +390,38 @@ int fem_get_params(fem_ctx *c, double *w, double *mu, double *cov) {
         FEM_HIP(c, hipMemcpyAsync(cov, c->cov, K * d * d * sizeof(double), hipMemcpyDeviceToHost,
                                   c->stream));
     FEM_HIP(c, hipStreamSynchronize(c->stream));
+    return FEM_OK;
+}
+
+int fem_get_params_start(fem_ctx *c) {
+    if (int r = check_ctx(c)) return r;
+    if (int r = set_device(c)) return r;
+    const size_t K = c->K, d = c->d;
+    if (!c->params_host)
+        FEM_HIP(c, hipHostMalloc((void **)&c->params_host, (K + K * d + K * d * d) * sizeof(double),
+                                 hipHostMallocDefault));
+    if (!c->ev_params) FEM_HIP(c, hipEventCreateWithFlags(&c->ev_params, hipEventDisableTiming));
+    double *h = c->params_host;
+    FEM_HIP(c, hipMemcpyAsync(h, c->w, K * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    FEM_HIP(c, hipMemcpyAsync(h + K, c->mu, K * d * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    FEM_HIP(c, hipMemcpyAsync(h + K + K * d, c->cov, K * d * d * sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream));
+    FEM_HIP(c, hipEventRecord(c->ev_params, c->stream));
+    c->params_pending = true;
+    return FEM_OK;
+}
+
+int fem_get_params_finish(fem_ctx *c, double *w, double *mu, double *cov) {
+    if (int r = check_ctx(c)) return r;
+    if (!c->params_pending) return fail(c, FEM_ERR_STATE, "fem_get_params_finish without fem_get_params_start");
+    if (int r = set_device(c)) return r;
+    FEM_HIP(c, hipEventSynchronize(c->ev_params));
+    c->params_pending = false;
+    const size_t K = c->K, d = c->d;
+    const double *h = c->params_host;
+    if (w) std::memcpy(w, h, K * sizeof(double));
+    if (mu) std::memcpy(mu, h + K, K * d * sizeof(double));
+    if (cov) std::memcpy(cov, h + K + K * d, K * d * d * sizeof(double));
     return FEM_OK;
 }
 

@@ -123,6 +123,9 @@ struct fem_ctx {
     bool owns_rt64 = false;     // Rt allocated for k-means in a non-FP64 context
 
     hipEvent_t ev[8] = {};
+    double *params_host = nullptr;     // pinned staging of fem_get_params_start: w, mu, cov
+    hipEvent_t ev_params = nullptr;    // recorded after the staging copies
+    bool params_pending = false;       // a start whose finish has not run
     float t_estep = 0, t_means = 0, t_cov = 0, t_factor = 0;
     std::string err;
 };

@@ -4,6 +4,7 @@
 // messages follow EM2/EMAlgorithm.cpp:139-191; the heavy hooks call the C ABI.
 #include <omp.h>
 
+#include <algorithm>
 #include <cmath>
 #include <condition_variable>
 #include <exception>
@@ -149,7 +150,7 @@ OPTFLOAT EMAlgorithm::FindLargestConditionNumber() {
 }
 
 // The largest condition number of the host copies of the covariances (as
-// fetched by FetchParams); throws when an eigenvalue iteration fails.
+// fetched by FetchParams or FinishStagedFetch); throws when an eigenvalue iteration fails.
 OPTFLOAT EMAlgorithm::LargestHostConditionNumber() const {
     std::vector<double> conds(K, 0.0);
 #pragma omp parallel for schedule(dynamic, 1)
@@ -166,6 +167,12 @@ OPTFLOAT EMAlgorithm::LargestHostConditionNumber() const {
         if (conds[i] > m) m = conds[i];
     }
     return m;
+}
+
+void EMAlgorithm::FinishStagedFetch(double *cov) {
+    Check(fem_get_params_finish(Ctx, Probs.data(), Means.data(), cov), "fem_get_params_finish");
+    for (int k = 0; k < K; k++)
+        std::memcpy(Covs[k].data(), cov + (size_t)k * nCols * nCols, sizeof(double) * nCols * nCols);
 }
 
 OPTFLOAT EMAlgorithm::FindMinTrace() {
@@ -205,30 +212,37 @@ OPTFLOAT EMAlgorithm::TrainNoCopy(GaussianMixture &G, int Verbosity, bool SVD) {
     // iteration's line, and the next E-step's result is then never used.
     // one host thread for the whole run (its OpenMP team persists between
     // iterations, where a thread per iteration started a new team each time)
+    // The worker also brings the parameters to the host (queued by
+    // fem_get_params_start after the iteration's factorisation) and scans the
+    // traces, so the device goes on to the next E-step without a host wait.
     struct CondWorker {
-        const EMAlgorithm *em;
+        EMAlgorithm *em;
+        std::vector<double> staged;
         std::mutex m;
         std::condition_variable cv;
         bool work = false, done = false, stop = false;
-        OPTFLOAT result = 0.0;
+        OPTFLOAT result = 0.0, trace = 0.0;
         std::exception_ptr err;
         std::thread th;
-        explicit CondWorker(const EMAlgorithm *e) : em(e) {
+        explicit CondWorker(EMAlgorithm *e) : em(e), staged((size_t)e->K * e->nCols * e->nCols) {
             th = std::thread([this]() {
                 std::unique_lock<std::mutex> lk(m);
                 for (;;) {
                     cv.wait(lk, [this]() { return work || stop; });
                     if (stop) return;
                     lk.unlock();
-                    OPTFLOAT r = 0.0;
+                    OPTFLOAT r = 0.0, t = std::numeric_limits<double>::max();
                     std::exception_ptr ex;
                     try {
+                        em->FinishStagedFetch(staged.data());
+                        for (int i = 0; i < em->K; i++) t = std::min<OPTFLOAT>(t, em->Covs[i].trace());
                         r = em->LargestHostConditionNumber();
                     } catch (...) {
                         ex = std::current_exception();
                     }
                     lk.lock();
                     result = r;
+                    trace = t;
                     err = ex;
                     work = false;
                     done = true;
@@ -242,11 +256,12 @@ OPTFLOAT EMAlgorithm::TrainNoCopy(GaussianMixture &G, int Verbosity, bool SVD) {
             done = false;
             cv.notify_all();
         }
-        OPTFLOAT get() {
+        OPTFLOAT get(OPTFLOAT *min_trace) {
             std::unique_lock<std::mutex> lk(m);
             cv.wait(lk, [this]() { return done; });
             done = false;
             if (err) std::rethrow_exception(err);
+            *min_trace = trace;
             return result;
         }
         ~CondWorker() {
@@ -268,7 +283,7 @@ OPTFLOAT EMAlgorithm::TrainNoCopy(GaussianMixture &G, int Verbosity, bool SVD) {
     } pending{};
     bool have_pending = false;
     auto finish = [&]() {  // the check and line of the pending iteration
-        MaxCond = cond->get();
+        MaxCond = cond->get(&pending.MinTrace);
         if (MaxCond > AbortThr) throw EMException("Condition number of a covariance matrix is too high");
         if (Verbosity > 1)
             printf("iter: %d LogL: %10.6f e: %g Cond: %g Trace: %g Time[ms]: %g\n", pending.iter, pending.LogL,
@@ -288,8 +303,9 @@ OPTFLOAT EMAlgorithm::TrainNoCopy(GaussianMixture &G, int Verbosity, bool SVD) {
         if (LogL < PrevLogL) e = 0.0;
         const double t1 = monotonic_seconds();
         if (SVD) {
-            FetchParams();
-            pending = {IterCounter, LogL, e, FindMinTrace(), 1000.0 * (t1 - t0)};
+            Check(fem_get_params_start(Ctx), "fem_get_params_start");
+            DeviceParamsNewer = false;  // on the host once the worker has run
+            pending = {IterCounter, LogL, e, 0.0, 1000.0 * (t1 - t0)};
             have_pending = true;
             cond->start();
             if (e < Eps || IterCounter == MaxSweeps) finish();  // no further E-step to overlap

@@ -47,6 +47,9 @@ protected:
     // the same on the host copies of the covariances alone (no fetch): run on
     // a host thread while the device computes the next E-step
     OPTFLOAT LargestHostConditionNumber() const;
+    // the parameters fem_get_params_start queued, into Probs/Means/Covs
+    // (cov: K d^2 doubles of staging)
+    void FinishStagedFetch(double *cov);
     OPTFLOAT FindMinTrace();
     void DecodeMixture(const GaussianMixture &G);
     void ComputePLLs() {}

@@ -77,6 +77,17 @@ int fem_set_regularization(fem_ctx *ctx, double g);
 int fem_set_params(fem_ctx *ctx, const double *w, const double *mu, const double *cov);
 /* Downloads the current parameters; any pointer may be NULL. */
 int fem_get_params(fem_ctx *ctx, double *w, double *mu, double *cov);
+/* fem_get_params in two halves, for a host that works on the parameters
+ * while the device runs on: _start queues the copies into pinned staging the
+ * context owns (no synchronisation), _finish waits for them and copies out
+ * (any pointer may be NULL).  _finish may run on another host thread than
+ * the one queueing further work on the context; a pair must not overlap
+ * another pair or fem_destroy.  No reference counterpart (the reference's
+ * parameters never leave host memory); it lets `em2` overlap
+ * EMAlgorithm::FindLargestConditionNumber (EM2/EMAlgorithm.cpp:159-163) with
+ * the next E-step without a host-side wait between the two. */
+int fem_get_params_start(fem_ctx *ctx);
+int fem_get_params_finish(fem_ctx *ctx, double *w, double *mu, double *cov);
 
 /* Stage (d): batched FP64 Cholesky, log-determinant and U_k = (L_k^-1)^T for
  * every component.  Replaces EMAlgorithm::InitDensities ->

@@ -71,6 +71,36 @@ def test_factor_not_pd(orc):
         assert ei.value.code == fem.FEM_ERR_STATE
 
 
+def test_staged_params_download():
+    """fem_get_params_start / _finish return the parameters as they were when
+    the copies were queued, with later device work already behind them."""
+    say("staged parameter download")
+    n, d, K = 20000, 10, 5
+    X, _, _, _ = mixture(n, d, K, seed=77)
+    w, mu, cov = jittered_init(X, K, seed=1)
+    with fem.Context(n, d, K) as ctx:
+        ctx.upload(X)
+        ctx.set_params(w, mu, cov)
+        with pytest.raises(fem.FemError):
+            ctx.get_params_finish()
+        ctx.factor()
+        ctx.estep()
+        p1 = ctx.mstep()
+        ctx.get_params_start()
+        ctx.factor()
+        ctx.estep()
+        p2 = ctx.mstep()
+        s1 = ctx.get_params_finish()
+        with pytest.raises(fem.FemError):
+            ctx.get_params_finish()
+        ctx.get_params_start()
+        s2 = ctx.get_params_finish()
+    for a, b, c, e in zip(p1, s1, p2, s2):
+        np.testing.assert_array_equal(b, a)
+        np.testing.assert_array_equal(e, c)
+    assert not np.array_equal(p1[1], p2[1])
+
+
 @pytest.mark.parametrize("n,d,K", [(1000, 3, 2), (5003, 17, 7), (4096, 16, 8), (3000, 64, 5),
                                    (2500, 128, 3), (777, 1, 1), (9000, 8, 64), (4099, 2, 16),
                                    (3001, 90, 4), (2600, 100, 3), (2200, 112, 2), (6000, 12, 100),
